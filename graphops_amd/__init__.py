@@ -1,0 +1,494 @@
+"""graphops_amd -- graphops' PageRank family on AMD MI355X (gfx950).
+
+Python mirror of the reference's operator API (crate graphops v0.4.1,
+src/pagerank.rs and src/ppr.rs): same function names, the same
+PageRankConfig / PageRankRun, the same checked/unchecked split and error
+messages.  Every call goes through the C ABI of libgraphops_amd.so
+(include/graphops_amd.h) to hand-written HIP kernels; there is no CPU
+fallback.
+
+Adapters (src/graph.rs):
+  AdjacencyMatrix(matrix)   graph.rs:122-142 (edge iff entry > 0.0)
+  DiGraph                   the petgraph adapter, graph.rs:144-158, 199-212
+  any object with node_count() / neighbors(u) [/ edge_weight(u, v)]
+  DeviceGraph               a graph already resident on the GPU
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from . import generators  # noqa: F401
+
+__all__ = [
+    "PageRankConfig", "PageRankRun", "GraphopsError", "InvalidParameter", "IndexOutOfBounds",
+    "DimensionMismatch", "DeviceError", "AdjacencyMatrix", "DiGraph", "DeviceGraph", "Context",
+    "default_context", "pagerank", "pagerank_run", "pagerank_checked", "pagerank_checked_run",
+    "pagerank_weighted", "pagerank_weighted_run", "pagerank_weighted_checked",
+    "pagerank_weighted_checked_run", "personalized_pagerank", "personalized_pagerank_run",
+    "personalized_pagerank_checked", "personalized_pagerank_checked_run", "ppr_batch_run",
+    "top_k", "normalize",
+]
+
+
+# ---------------------------------------------------------------- errors
+class GraphopsError(Exception):
+    """Base of the errors below (graphops::Error, lib.rs:108-123)."""
+
+
+class InvalidParameter(GraphopsError, ValueError):
+    """Error::InvalidParameter; str() is the reference's Display text."""
+
+
+class IndexOutOfBounds(GraphopsError, IndexError):
+    """Error::IndexOutOfBounds."""
+
+
+class DimensionMismatch(GraphopsError, ValueError):
+    """Error::DimensionMismatch."""
+
+
+class DeviceError(GraphopsError, RuntimeError):
+    """A HIP failure (no reference counterpart)."""
+
+
+class OutOfMemory(DeviceError):
+    pass
+
+
+class NotSupported(GraphopsError):
+    pass
+
+
+_ERRORS = {
+    L.GOPS_INVALID_PARAMETER: InvalidParameter,
+    L.GOPS_INDEX_OUT_OF_BOUNDS: IndexOutOfBounds,
+    L.GOPS_DIMENSION_MISMATCH: DimensionMismatch,
+    L.GOPS_DEVICE_ERROR: DeviceError,
+    L.GOPS_OUT_OF_MEMORY: OutOfMemory,
+    L.GOPS_NOT_SUPPORTED: NotSupported,
+}
+
+
+def _check(rc, ctx=None, what=""):
+    if rc == L.GOPS_OK:
+        return
+    msg = ""
+    if ctx is not None:
+        msg = (L.lib().gops_last_error(ctx) or b"").decode()
+    raise _ERRORS.get(rc, GraphopsError)(msg or f"{what} failed with status {rc}")
+
+
+# ---------------------------------------------------------------- config/run
+@dataclass
+class PageRankConfig:
+    """PageRankConfig, pagerank.rs:37-59 (defaults 0.85 / 100 / 1e-6)."""
+
+    damping: float = 0.85
+    max_iterations: int = 100
+    tolerance: float = 1e-6
+
+    def validate(self) -> None:
+        """PageRankConfig::validate, pagerank.rs:63-85."""
+        d, t = float(self.damping), float(self.tolerance)
+        if not np.isfinite(d):
+            raise InvalidParameter("invalid parameter: damping must be finite")
+        if not (0.0 <= d <= 1.0):
+            raise InvalidParameter("invalid parameter: damping must be in [0,1]")
+        if int(self.max_iterations) == 0:
+            raise InvalidParameter("invalid parameter: max_iterations must be > 0")
+        if not np.isfinite(t) or t <= 0.0:
+            raise InvalidParameter("invalid parameter: tolerance must be finite and > 0")
+
+    def _c(self):
+        return L.PageRankConfigC(float(self.damping), int(self.max_iterations),
+                                 float(self.tolerance))
+
+
+@dataclass
+class PageRankRun:
+    """PageRankRun, pagerank.rs:14-29."""
+
+    scores: np.ndarray
+    iterations: int
+    diff_l1: float
+    converged: bool
+    stats: dict = field(default_factory=dict, repr=False)
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One HIP device + stream + the memory it allocates (gops_context)."""
+
+    def __init__(self, device: int = 0, precision: str = "mixed", anchor_period: int = 32,
+                 normalize: bool = False, timing: bool = False):
+        h = ctypes.c_void_p()
+        rc = L.lib().gops_context_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise DeviceError(f"gops_context_create(device={device}) failed with status {rc}; "
+                              "a HIP-capable GPU is required")
+        self._h = h
+        self.device = device
+        self.set_precision(precision)
+        self.set_option(L.OPT_ANCHOR_PERIOD, anchor_period)
+        self.set_option(L.OPT_NORMALIZE, int(bool(normalize)))
+        self.set_option(L.OPT_TIMING, int(bool(timing)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_option(self, opt, value):
+        _check(L.lib().gops_context_set_option(self._h, opt, float(value)), self._h, "set_option")
+
+    def set_precision(self, precision: str):
+        self.set_option(L.OPT_PRECISION,
+                        {"mixed": L.PRECISION_MIXED, "fp64": L.PRECISION_FP64}[precision])
+
+    def last_stats(self) -> dict:
+        s = L.RunStatsC()
+        _check(L.lib().gops_last_run_stats(self._h, ctypes.byref(s)), self._h, "stats")
+        return {k: getattr(s, k) for k, _ in L.RunStatsC._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gops_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+# ---------------------------------------------------------------- adapters
+class AdjacencyMatrix:
+    """Row-major adjacency-matrix adapter, graph.rs:122-142: matrix[i][j] is
+    the weight of edge i -> j; an entry is an edge iff > 0.0."""
+
+    def __init__(self, matrix):
+        self.matrix = np.ascontiguousarray(np.asarray(matrix, dtype=np.float64))
+        if self.matrix.ndim != 2 or self.matrix.shape[0] != self.matrix.shape[1]:
+            if self.matrix.size == 0:
+                self.matrix = self.matrix.reshape(0, 0)
+            else:
+                raise ValueError("AdjacencyMatrix needs a square matrix")
+
+    def node_count(self) -> int:
+        return self.matrix.shape[0]
+
+    def neighbors(self, node: int):
+        return [int(j) for j in np.nonzero(self.matrix[node] > 0.0)[0]]
+
+    def out_degree(self, node: int) -> int:
+        return len(self.neighbors(node))
+
+    def edge_weight(self, source: int, target: int) -> float:
+        return float(self.matrix[source, target])
+
+
+class DiGraph:
+    """Edge-list graph with petgraph adapter semantics (graph.rs:144-158,
+    199-212): neighbors(u) lists every out-edge, parallel edges repeated, most
+    recently added first (petgraph 0.6 Graph keeps outgoing edges in a
+    linked list with new edges at the head); edge_weight(u, v) is the weight
+    of the first edge u -> v met in that order (find_edge), 0.0 if none."""
+
+    def __init__(self, n: int = 0):
+        self._out = [[] for _ in range(n)]  # (target, weight) in insertion order
+        self._version = 0
+
+    def add_node(self) -> int:
+        self._out.append([])
+        self._version += 1
+        return len(self._out) - 1
+
+    def add_edge(self, u: int, v: int, weight: float = 1.0) -> None:
+        self._out[u].append((int(v), float(weight)))
+        self._version += 1
+
+    def node_count(self) -> int:
+        return len(self._out)
+
+    def neighbors(self, node: int):
+        return [v for v, _ in reversed(self._out[node])]
+
+    def out_degree(self, node: int) -> int:
+        return len(self._out[node])
+
+    def edge_weight(self, source: int, target: int) -> float:
+        for v, w in reversed(self._out[source]):
+            if v == target:
+                return w
+        return 0.0
+
+    def _lists(self, weighted):
+        n = len(self._out)
+        off = np.zeros(n + 1, dtype=np.uint64)
+        for u in range(n):
+            off[u + 1] = off[u] + len(self._out[u])
+        nb = np.zeros(int(off[-1]), dtype=np.uint32)
+        ew = np.zeros(int(off[-1]), dtype=np.float64) if weighted else None
+        for u in range(n):
+            nbrs = self.neighbors(u)
+            nb[off[u]:off[u + 1]] = nbrs
+            if weighted:
+                ew[off[u]:off[u + 1]] = [self.edge_weight(u, v) for v in nbrs]
+        return off, nb, ew
+
+
+_STORAGE = {"none": L.WEIGHT_NONE, "f64": L.WEIGHT_F64, "f32": L.WEIGHT_F32,
+            "bf16": L.WEIGHT_BF16}
+
+
+class DeviceGraph:
+    """A graph resident on the GPU (gops_graph): the CSR of in-edges built
+    once from the adapter's neighbour lists (pagerank.rs:136, 226-234)."""
+
+    def __init__(self, handle, ctx: Context, weighted: bool):
+        self._h = handle
+        self.ctx = ctx
+        self.weighted = weighted
+        n = ctypes.c_uint64()
+        m = ctypes.c_uint64()
+        L.lib().gops_graph_info(handle, ctypes.byref(n), ctypes.byref(m))
+        self.n = int(n.value)
+        self.nnz = int(m.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def node_count(self) -> int:
+        return self.n
+
+    @classmethod
+    def from_lists(cls, n, offsets, targets, edge_weights=None, weight_storage="f64",
+                   ctx: Context = None):
+        """Neighbour lists in adapter order: targets[offsets[u]:offsets[u+1]]
+        = Graph::neighbors(u); edge_weights[j] = edge_weight(u, targets[j])."""
+        ctx = ctx or default_context()
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        tg = np.ascontiguousarray(targets, dtype=np.uint32)
+        if off.size != n + 1:
+            raise DimensionMismatch(f"dimension mismatch: {off.size} vs {n + 1}")
+        storage = L.WEIGHT_NONE
+        ew = None
+        if edge_weights is not None:
+            storage = _STORAGE[weight_storage]
+            ew = np.ascontiguousarray(edge_weights, dtype=np.float64)
+            if ew.size != tg.size:
+                raise DimensionMismatch(f"dimension mismatch: {ew.size} vs {tg.size}")
+        h = ctypes.c_void_p()
+        rc = L.lib().gops_graph_create(ctx.handle, n, L.ptr(off, L._u64p), L.ptr(tg, L._u32p),
+                                       L.ptr(ew, L._f64p), storage, ctypes.byref(h))
+        _check(rc, ctx.handle, "gops_graph_create")
+        return cls(h, ctx, storage != L.WEIGHT_NONE)
+
+    @classmethod
+    def from_dense(cls, matrix, weighted=True, weight_storage="f64", ctx: Context = None):
+        """AdjacencyMatrix semantics (graph.rs:122-142)."""
+        ctx = ctx or default_context()
+        mat = np.ascontiguousarray(np.asarray(matrix, dtype=np.float64))
+        n = mat.shape[0] if mat.ndim == 2 else 0
+        storage = _STORAGE[weight_storage] if weighted else L.WEIGHT_NONE
+        h = ctypes.c_void_p()
+        rc = L.lib().gops_graph_from_dense(ctx.handle, n, L.ptr(mat, L._f64p), storage,
+                                           ctypes.byref(h))
+        _check(rc, ctx.handle, "gops_graph_from_dense")
+        return cls(h, ctx, storage != L.WEIGHT_NONE)
+
+    @classmethod
+    def from_edges(cls, edges, weight_storage="f64", ctx: Context = None):
+        """From a generators.Edges (weights used when present)."""
+        return cls.from_lists(edges.n, edges.offsets, edges.targets, edges.weights,
+                              weight_storage, ctx)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gops_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _device_graph(graph, weighted: bool, ctx: Context = None) -> DeviceGraph:
+    if isinstance(graph, DeviceGraph):
+        return graph
+    ctx = ctx or default_context()
+    if isinstance(graph, AdjacencyMatrix):
+        return DeviceGraph.from_dense(graph.matrix, weighted=weighted, ctx=ctx)
+    if isinstance(graph, DiGraph):
+        key = (id(ctx), weighted, graph._version)
+        cache = getattr(graph, "_gops_cache", None)
+        if cache is not None and cache[0] == key:
+            return cache[1]
+        off, nb, ew = graph._lists(weighted)
+        dg = DeviceGraph.from_lists(graph.node_count(), off, nb, ew, ctx=ctx)
+        graph._gops_cache = (key, dg)
+        return dg
+    # generic Graph / WeightedGraph adapter (graph.rs:26-39, 67-72)
+    n = int(graph.node_count())
+    lists = [list(graph.neighbors(u)) for u in range(n)]
+    off = np.zeros(n + 1, dtype=np.uint64)
+    off[1:] = np.cumsum([len(x) for x in lists]) if n else []
+    nb = np.array([v for x in lists for v in x], dtype=np.uint32)
+    ew = None
+    if weighted:
+        ew = np.array([graph.edge_weight(u, v) for u in range(n) for v in lists[u]],
+                      dtype=np.float64)
+    return DeviceGraph.from_lists(n, off, nb, ew, ctx=ctx)
+
+
+# ---------------------------------------------------------------- operators
+def _run(fn_name, graph, config, weighted, pers=None, checked=False, ctx=None,
+         with_scores=True):
+    config = config if config is not None else PageRankConfig()
+    dg = _device_graph(graph, weighted, ctx)
+    c = dg.ctx.handle
+    n = dg.n
+    scores = np.zeros(max(n, 1), dtype=np.float64) if with_scores else None
+    info = L.RunInfoC()
+    cfg = config._c()
+    fn = getattr(L.lib(), fn_name)
+    if pers is not None:
+        p = np.ascontiguousarray(pers, dtype=np.float64)
+        rc = fn(c, dg.handle, ctypes.byref(cfg), L.ptr(p, L._f64p), p.size,
+                L.ptr(scores, L._f64p), ctypes.byref(info))
+    else:
+        rc = fn(c, dg.handle, ctypes.byref(cfg), L.ptr(scores, L._f64p), ctypes.byref(info))
+    _check(rc, c, fn_name)
+    return PageRankRun(scores[:n] if with_scores else None, int(info.iterations),
+                       float(info.diff_l1), bool(info.converged), dg.ctx.last_stats())
+
+
+def pagerank_run(graph, config: PageRankConfig = None, ctx: Context = None) -> PageRankRun:
+    """pagerank_run, pagerank.rs:122-187."""
+    return _run("gops_pagerank_run", graph, config, False, ctx=ctx)
+
+
+def pagerank(graph, config: PageRankConfig = None, ctx: Context = None) -> np.ndarray:
+    """pagerank, pagerank.rs:114-116."""
+    return pagerank_run(graph, config, ctx).scores
+
+
+def pagerank_checked_run(graph, config: PageRankConfig, ctx: Context = None) -> PageRankRun:
+    """pagerank_checked_run, pagerank.rs:321-324."""
+    return _run("gops_pagerank_checked_run", graph, config, False, ctx=ctx)
+
+
+def pagerank_checked(graph, config: PageRankConfig, ctx: Context = None) -> np.ndarray:
+    """pagerank_checked, pagerank.rs:91-94."""
+    return pagerank_checked_run(graph, config, ctx).scores
+
+
+def pagerank_weighted_run(graph, config: PageRankConfig = None,
+                          ctx: Context = None) -> PageRankRun:
+    """pagerank_weighted_run, pagerank.rs:210-289."""
+    return _run("gops_pagerank_weighted_run", graph, config, True, ctx=ctx)
+
+
+def pagerank_weighted(graph, config: PageRankConfig = None, ctx: Context = None) -> np.ndarray:
+    """pagerank_weighted, pagerank.rs:201-203."""
+    return pagerank_weighted_run(graph, config, ctx).scores
+
+
+def pagerank_weighted_checked_run(graph, config: PageRankConfig,
+                                  ctx: Context = None) -> PageRankRun:
+    """pagerank_weighted_checked_run, pagerank.rs:328-336 (computes once)."""
+    return _run("gops_pagerank_weighted_checked_run", graph, config, True, ctx=ctx)
+
+
+def pagerank_weighted_checked(graph, config: PageRankConfig, ctx: Context = None) -> np.ndarray:
+    """pagerank_weighted_checked, pagerank.rs:294-316."""
+    return pagerank_weighted_checked_run(graph, config, ctx).scores
+
+
+def personalized_pagerank_run(graph, config: PageRankConfig, personalization,
+                              ctx: Context = None) -> PageRankRun:
+    """personalized_pagerank_run, ppr.rs:90-156."""
+    return _run("gops_personalized_pagerank_run", graph, config, False, pers=personalization,
+                ctx=ctx)
+
+
+def personalized_pagerank(graph, config: PageRankConfig, personalization,
+                          ctx: Context = None) -> np.ndarray:
+    """personalized_pagerank, ppr.rs:66-72."""
+    return personalized_pagerank_run(graph, config, personalization, ctx).scores
+
+
+def personalized_pagerank_checked_run(graph, config: PageRankConfig, personalization,
+                                      ctx: Context = None) -> PageRankRun:
+    """personalized_pagerank_checked_run, ppr.rs:77-85 (computes once)."""
+    return _run("gops_personalized_pagerank_checked_run", graph, config, False,
+                pers=personalization, ctx=ctx)
+
+
+def personalized_pagerank_checked(graph, config: PageRankConfig, personalization,
+                                  ctx: Context = None) -> np.ndarray:
+    """personalized_pagerank_checked, ppr.rs:11-44."""
+    return personalized_pagerank_checked_run(graph, config, personalization, ctx).scores
+
+
+def ppr_batch_run(graph, configs, seed_offsets, seeds, seed_weights=None, ctx: Context = None,
+                  with_scores=True):
+    """Many personalized_pagerank_run calls (ppr.rs:90-156) in one device run.
+    Query q's personalization is 1.0 (or seed_weights[j]) on
+    seeds[seed_offsets[q]:seed_offsets[q+1]] and 0 elsewhere.  Returns
+    (scores[num_queries, n] or None, list of PageRankRun without scores)."""
+    dg = _device_graph(graph, False, ctx)
+    c = dg.ctx.handle
+    q = len(seed_offsets) - 1
+    if isinstance(configs, PageRankConfig):
+        configs = [configs] * q
+    cfgs = (L.PageRankConfigC * max(q, 1))(*[cf._c() for cf in configs])
+    off = np.ascontiguousarray(seed_offsets, dtype=np.uint64)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+    sw = None if seed_weights is None else np.ascontiguousarray(seed_weights, dtype=np.float64)
+    scores = np.zeros((q, dg.n), dtype=np.float64) if with_scores else None
+    infos = (L.RunInfoC * max(q, 1))()
+    rc = L.lib().gops_ppr_batch_run(c, dg.handle, q, cfgs, L.ptr(off, L._u64p),
+                                    L.ptr(sd, L._u32p), L.ptr(sw, L._f64p),
+                                    L.ptr(scores, L._f64p), infos)
+    _check(rc, c, "gops_ppr_batch_run")
+    runs = [PageRankRun(None, int(infos[i].iterations), float(infos[i].diff_l1),
+                        bool(infos[i].converged)) for i in range(q)]
+    return scores, runs
+
+
+# ---------------------------------------------------------------- ranking
+def top_k(scores, k):
+    """top_k, topk.rs:17-42: finite, > 0 scores, descending (ties in
+    unspecified order, as the reference's unstable sort)."""
+    s = np.asarray(scores, dtype=np.float64)
+    if k == 0 or s.size == 0:
+        return []
+    idx = np.nonzero(np.isfinite(s) & (s > 0.0))[0]
+    order = idx[np.argsort(-s[idx], kind="stable")][:k]
+    return [(int(i), float(s[i])) for i in order]
+
+
+def normalize(scores):
+    """normalize, topk.rs:54-61 (in place; no-op unless the sum is > 0)."""
+    total = float(np.sum(scores))
+    if total > 0.0:
+        scores /= total
+    return scores

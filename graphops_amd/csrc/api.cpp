@@ -1,0 +1,633 @@
+// api.cpp -- extern "C" entry points of libgraphops_amd (see
+// include/graphops_amd.h for the reference functions each one replaces).
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <new>
+
+#include "internal.h"
+
+using namespace gops;
+
+namespace {
+
+gops_status fail(gops_context *ctx, gops_status st, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  return st;
+}
+
+gops_status invalid(gops_context *ctx, const std::string &msg) {
+  // Display text of Error::InvalidParameter, lib.rs:117-118
+  return fail(ctx, GOPS_INVALID_PARAMETER, "invalid parameter: " + msg);
+}
+
+gops_status hip_fail(gops_context *ctx, hipError_t e, const char *what) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return fail(ctx, GOPS_OUT_OF_MEMORY, std::string(what) + ": out of device memory");
+  return fail(ctx, GOPS_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ctx, expr)                                 \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, #expr); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void **)p, count * sizeof(T));
+}
+
+template <typename T>
+hipError_t upload(gops_context *ctx, T **p, const T *src, size_t count) {
+  hipError_t e = dalloc(p, count);
+  if (e != hipSuccess) return e;
+  if (count == 0) return hipSuccess;
+  return hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+}
+
+// PageRankConfig::validate, pagerank.rs:63-85 (messages verbatim)
+gops_status validate_config(gops_context *ctx, const gops_pagerank_config *cfg) {
+  if (!std::isfinite(cfg->damping)) return invalid(ctx, "damping must be finite");
+  if (!(cfg->damping >= 0.0 && cfg->damping <= 1.0)) return invalid(ctx, "damping must be in [0,1]");
+  if (cfg->max_iterations == 0) return invalid(ctx, "max_iterations must be > 0");
+  if (!std::isfinite(cfg->tolerance) || cfg->tolerance <= 0.0)
+    return invalid(ctx, "tolerance must be finite and > 0");
+  return GOPS_OK;
+}
+
+void free_run(RunBuffers &rb) {
+  hipFree(rb.xbar);
+  hipFree(rb.e);
+  hipFree(rb.yd[0]);
+  hipFree(rb.yd[1]);
+  hipFree(rb.y64[0]);
+  hipFree(rb.y64[1]);
+  hipFree(rb.q32);
+  hipFree(rb.q64);
+  hipFree(rb.out);
+  hipFree(rb.partials);
+  hipFree(rb.hub_partial);
+  hipFree(rb.hub_count);
+  hipFree(rb.state);
+  rb = RunBuffers{};
+}
+
+void free_dev(DevGraph &dg) {
+  hipFree(dg.row_ptr);
+  hipFree(dg.col);
+  hipFree(dg.w);
+  hipFree(dg.inv_deg32);
+  hipFree(dg.inv_deg64);
+  hipFree(dg.inv_ws32);
+  hipFree(dg.inv_ws64);
+  hipFree(dg.rank);
+  hipFree(dg.order);
+  hipFree(dg.desc);
+  hipFree(dg.hub_row);
+  hipFree(dg.hub_nchunks);
+  hipFree(dg.hub_first);
+  dg = DevGraph{};
+}
+
+gops_status alloc_run(gops_context *ctx, gops_graph *g) {
+  RunBuffers &rb = g->run;
+  if (rb.state) return GOPS_OK;
+  const size_t n = g->dev.n;
+  rb.n = n;
+  HIP_TRY(ctx, dalloc(&rb.xbar, n));
+  HIP_TRY(ctx, dalloc(&rb.e, n));
+  HIP_TRY(ctx, dalloc(&rb.yd[0], n));
+  HIP_TRY(ctx, dalloc(&rb.yd[1], n));
+  HIP_TRY(ctx, dalloc(&rb.y64[0], n));
+  HIP_TRY(ctx, dalloc(&rb.y64[1], n));
+  HIP_TRY(ctx, dalloc(&rb.out, n));
+  rb.partial_slots = std::max<size_t>((size_t)g->dev.num_desc, 4096);
+  HIP_TRY(ctx, dalloc(&rb.partials, rb.partial_slots * 4));
+  HIP_TRY(ctx, dalloc(&rb.hub_partial, std::max(g->dev.num_hub_chunks, 1)));
+  HIP_TRY(ctx, dalloc(&rb.hub_count, std::max(g->dev.num_hubs, 1)));
+  HIP_TRY(ctx, hipMemsetAsync(rb.hub_count, 0, sizeof(uint32_t) * std::max(g->dev.num_hubs, 1),
+                              ctx->stream));
+  HIP_TRY(ctx, dalloc(&rb.state, 1));
+  HIP_TRY(ctx, hipMemsetAsync(rb.yd[0], 0, n * sizeof(float), ctx->stream));
+  HIP_TRY(ctx, hipMemsetAsync(rb.yd[1], 0, n * sizeof(float), ctx->stream));
+  return GOPS_OK;
+}
+
+gops_status upload_graph(gops_context *ctx, gops_graph *g, const HostCSRT &h, const Plan &p,
+                         int weight_storage) {
+  DevGraph &dg = g->dev;
+  dg.n = h.n;
+  dg.m = h.m;
+  dg.weight_storage = h.has_weights ? weight_storage : GOPS_WEIGHT_NONE;
+  HIP_TRY(ctx, upload(ctx, &dg.row_ptr, h.row_ptr.data(), h.row_ptr.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.col, h.col.data(), h.col.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.rank, h.rank.data(), h.rank.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.order, h.order.data(), h.order.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.desc, p.desc.data(), p.desc.size()));
+  dg.num_desc = (int)p.desc.size();
+  HIP_TRY(ctx, upload(ctx, &dg.hub_row, p.hub_row.data(), p.hub_row.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.hub_nchunks, p.hub_nchunks.data(), p.hub_nchunks.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.hub_first, p.hub_first.data(), p.hub_first.size()));
+  dg.num_hubs = p.num_hubs;
+  dg.num_hub_chunks = 0;
+  for (int32_t c : p.hub_nchunks) dg.num_hub_chunks += c;
+
+  // per-node 1/out-degree (and 1/out_wsum): f64 for anchor sweeps, f32 for
+  // delta sweeps.  0 marks a dangling node; a non-dangling value that
+  // underflows in f32 is kept at the smallest subnormal so it stays
+  // non-dangling.
+  auto to32 = [](const std::vector<double> &v) {
+    std::vector<float> f(v.size());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)v.size(); i++) {
+      float x = (float)v[i];
+      if (v[i] != 0.0 && x == 0.0f) x = std::numeric_limits<float>::denorm_min();
+      f[i] = x;
+    }
+    return f;
+  };
+  {
+    std::vector<float> f = to32(h.inv_deg);
+    HIP_TRY(ctx, upload(ctx, &dg.inv_deg64, h.inv_deg.data(), h.inv_deg.size()));
+    HIP_TRY(ctx, upload(ctx, &dg.inv_deg32, f.data(), f.size()));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  if (h.has_weights) {
+    std::vector<float> f = to32(h.inv_ws);
+    HIP_TRY(ctx, upload(ctx, &dg.inv_ws64, h.inv_ws.data(), h.inv_ws.size()));
+    HIP_TRY(ctx, upload(ctx, &dg.inv_ws32, f.data(), f.size()));
+    const size_t m = h.m;
+    if (dg.weight_storage == GOPS_WEIGHT_F64) {
+      HIP_TRY(ctx, upload(ctx, (double **)&dg.w, h.w.data(), m));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    } else if (dg.weight_storage == GOPS_WEIGHT_F32) {
+      std::vector<float> w32(m);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < (int64_t)m; i++) w32[i] = (float)h.w[i];
+      HIP_TRY(ctx, upload(ctx, (float **)&dg.w, w32.data(), m));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    } else {
+      std::vector<uint16_t> w16(m);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < (int64_t)m; i++) w16[i] = f32_to_bf16((float)h.w[i]);
+      HIP_TRY(ctx, upload(ctx, (uint16_t **)&dg.w, w16.data(), m));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GOPS_OK;
+}
+
+gops_status ensure_events(gops_context *ctx, size_t count) {
+  while (ctx->events.size() < count) {
+    hipEvent_t ev;
+    HIP_TRY(ctx, hipEventCreate(&ev));
+    ctx->events.push_back(ev);
+  }
+  return GOPS_OK;
+}
+
+}  // namespace
+
+namespace gops {
+
+// Shared driver of pagerank_run / pagerank_weighted_run /
+// personalized_pagerank_run: host-side setup, the device sweep chain, the
+// result in the caller's ids.
+gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_config *cfg,
+                       RunKind kind, const double *pers, uint64_t pers_len, double *scores,
+                       gops_run_info *info) {
+  if (!ctx || !g || !cfg) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  if (g->ctx != ctx) return fail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  ctx->last_error.clear();
+  const uint64_t n = g->dev.n;
+  gops_run_info local{};
+  if (!info) info = &local;
+  ctx->stats = gops_run_stats{};
+  ctx->stats.n = n;
+  ctx->stats.nnz = g->dev.m;
+  if (kind == kWeighted && !g->weighted)
+    return fail(ctx, GOPS_INVALID_PARAMETER, "graph was created without edge weights");
+  if (kind == kPersonalized && pers_len != n) {
+    if (pers_len > n || !pers)
+      return fail(ctx, GOPS_DIMENSION_MISMATCH,
+                  "dimension mismatch: " + std::to_string(pers_len) + " vs " + std::to_string(n));
+    return fail(ctx, GOPS_DIMENSION_MISMATCH,
+                "dimension mismatch: " + std::to_string(pers_len) + " vs " + std::to_string(n));
+  }
+  if (n == 0) {  // pagerank.rs:124-131, ppr.rs:97-103
+    info->iterations = 0;
+    info->diff_l1 = 0.0;
+    info->converged = 1;
+    return GOPS_OK;
+  }
+  gops_status st = alloc_run(ctx, g);
+  if (st != GOPS_OK) return st;
+  RunBuffers &rb = g->run;
+  SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0};
+
+  if (kind == kPersonalized) {
+    // ppr.rs:104-109: p / sum(p), or uniform when the sum is not > 0
+    double psum = 0.0;
+    for (uint64_t i = 0; i < n; i++) psum += pers[i];
+    std::vector<double> q64(n);
+    std::vector<float> q32(n);
+    const uint32_t *order = g->order_host.data();
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < (int64_t)n; r++) {
+      const double v = psum > 0.0 ? pers[order[r]] / psum : 1.0 / (double)n;
+      q64[r] = v;
+      q32[r] = (float)v;
+    }
+    if (!rb.q64) {
+      HIP_TRY(ctx, dalloc(&rb.q64, n));
+      HIP_TRY(ctx, dalloc(&rb.q32, n));
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(rb.q64, q64.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(rb.q32, q32.data(), n * sizeof(float), hipMemcpyHostToDevice,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+
+  RunState hs{};
+  hs.damping = cfg->damping;
+  hs.tolerance = cfg->tolerance;
+  hs.n_f64 = (double)n;
+  hs.inv_n = 1.0 / (double)n;
+  hs.max_iter = cfg->max_iterations;
+  hs.mirror = ctx->mirror_dev;
+  __atomic_store_n(ctx->mirror_host, (uint64_t)0, __ATOMIC_RELEASE);
+  ctx->host_state[0] = hs;
+  HIP_TRY(ctx, hipMemcpyAsync(rb.state, &ctx->host_state[0], sizeof(RunState),
+                              hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+
+  const bool fp64 = ctx->precision == GOPS_PRECISION_FP64;
+  const uint64_t R = (uint64_t)std::max(1, ctx->anchor_period);
+  const uint64_t max_it = cfg->max_iterations;
+  const bool timing = ctx->timing != 0;
+  const size_t sweep_events = timing ? 2 * (size_t)std::min<uint64_t>(max_it, 100000) : 0;
+  st = ensure_events(ctx, 2 + sweep_events);
+  if (st != GOPS_OK) return st;
+  hipEvent_t ev_begin = ctx->events[0], ev_end = ctx->events[1];
+  std::vector<uint8_t> sweep_kind;  // 0 f32, 1 f64 (per enqueued sweep)
+
+  HIP_TRY(ctx, hipEventRecord(ev_begin, ctx->stream));
+  HIP_TRY(ctx, launch_init(ctx, g, setup));
+
+  // The device decides when to stop (RunState::done; later sweeps return at
+  // once).  The host keeps a few sweeps queued beyond the last one it has
+  // seen finish, reading the progress word the last workgroup of every sweep
+  // writes to mapped host memory, so no sweep waits on the host and at most
+  // kAhead launches are wasted after convergence.
+  const uint64_t kAhead = 4;
+  uint64_t j = 0, seen = 0;
+  while (j < max_it) {
+    for (; j < max_it && j < seen + kAhead; j++) {
+      const bool anchor = fp64 || (j % R == 0);
+      if (anchor && !fp64 && j > 0) HIP_TRY(ctx, launch_fold(ctx, g, setup));
+      if (timing && j < 100000) HIP_TRY(ctx, hipEventRecord(ctx->events[2 + 2 * j], ctx->stream));
+      if (fp64) {
+        HIP_TRY(ctx, launch_sweep_f64(ctx, g, setup, (int)(j & 1), 0, true, false));
+      } else if (anchor) {
+        HIP_TRY(ctx, launch_sweep_f64(ctx, g, setup, 0, (int)((j + 1) & 1), false, true));
+      } else {
+        HIP_TRY(ctx, launch_sweep_f32(ctx, g, setup, (int)(j & 1)));
+      }
+      if (timing && j < 100000) HIP_TRY(ctx, hipEventRecord(ctx->events[3 + 2 * j], ctx->stream));
+      sweep_kind.push_back(anchor ? 1 : 0);
+    }
+    if (j >= max_it) break;
+    bool done = false;
+    for (uint64_t spin = 0;; spin++) {
+      const uint64_t w = __atomic_load_n(ctx->mirror_host, __ATOMIC_ACQUIRE);
+      if (w & 1) {
+        done = true;
+        break;
+      }
+      if ((w >> 2) > seen) {
+        seen = w >> 2;
+        break;
+      }
+      if ((spin & 1023) == 1023) {
+        const hipError_t q = hipStreamQuery(ctx->stream);
+        if (q == hipSuccess) {  // queue drained: whatever the word says is final
+          const uint64_t w2 = __atomic_load_n(ctx->mirror_host, __ATOMIC_ACQUIRE);
+          done = (w2 & 1) != 0;
+          if (!done && (w2 >> 2) <= seen)
+            return fail(ctx, GOPS_DEVICE_ERROR, "sweep chain stalled without progress");
+          seen = w2 >> 2;
+          break;
+        }
+        if (q != hipErrorNotReady) return hip_fail(ctx, q, "hipStreamQuery");
+      }
+      __builtin_ia32_pause();
+    }
+    if (done) break;
+  }
+  HIP_TRY(ctx, launch_finalize(ctx, g, ctx->normalize));
+  HIP_TRY(ctx, hipEventRecord(ev_end, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(&ctx->host_state[0], rb.state, sizeof(RunState),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  RunState last = ctx->host_state[0];
+  // the finalize kernel reuses dsum_x for the output sum; iteration fields are intact
+  if (max_it == 0) {  // unchecked max_iterations == 0 (pagerank.rs:139-141, 182-186)
+    info->iterations = 0;
+    info->diff_l1 = std::numeric_limits<double>::infinity();
+    info->converged = 0;
+  } else {
+    info->iterations = last.iter;
+    info->diff_l1 = last.diff;
+    info->converged = last.converged;
+  }
+  float ms = 0.0f;
+  HIP_TRY(ctx, hipEventElapsedTime(&ms, ev_begin, ev_end));
+  ctx->stats.device_ms = ms;
+  for (uint64_t k = 0; k < last.iter && k < sweep_kind.size(); k++) {
+    if (sweep_kind[k]) ctx->stats.sweeps_fp64++;
+    else ctx->stats.sweeps_fp32++;
+    if (timing && k < 100000) {
+      float t = 0.0f;
+      HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->events[2 + 2 * k], ctx->events[3 + 2 * k]));
+      if (sweep_kind[k]) ctx->stats.fp64_sweep_ms += t;
+      else ctx->stats.fp32_sweep_ms += t;
+    }
+  }
+  if (scores) {
+    HIP_TRY(ctx, hipMemcpy(scores, rb.out, n * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return GOPS_OK;
+}
+
+}  // namespace gops
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+gops_status gops_context_create(int device, gops_context **out) {
+  if (!out) return GOPS_INVALID_PARAMETER;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return GOPS_DEVICE_ERROR;
+  if (device < 0 || device >= count) return GOPS_INVALID_PARAMETER;
+  if (hipSetDevice(device) != hipSuccess) return GOPS_DEVICE_ERROR;
+  auto *ctx = new (std::nothrow) gops_context;
+  if (!ctx) return GOPS_OUT_OF_MEMORY;
+  ctx->device = device;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return GOPS_DEVICE_ERROR;
+  }
+  if (hipHostMalloc((void **)&ctx->host_state, 2 * sizeof(RunState), hipHostMallocDefault) !=
+      hipSuccess) {
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GOPS_OUT_OF_MEMORY;
+  }
+  if (hipHostMalloc((void **)&ctx->mirror_host, 64, hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&ctx->mirror_dev, ctx->mirror_host, 0) != hipSuccess) {
+    hipHostFree(ctx->host_state);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GOPS_OUT_OF_MEMORY;
+  }
+  *ctx->mirror_host = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  *out = ctx;
+  return GOPS_OK;
+}
+
+void gops_context_destroy(gops_context *ctx) {
+  if (!ctx) return;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+  hipHostFree(ctx->host_state);
+  hipHostFree(ctx->mirror_host);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *gops_last_error(const gops_context *ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+gops_status gops_context_set_option(gops_context *ctx, int option, double value) {
+  if (!ctx) return GOPS_INVALID_PARAMETER;
+  switch (option) {
+    case GOPS_OPT_PRECISION:
+      if (value != GOPS_PRECISION_MIXED && value != GOPS_PRECISION_FP64)
+        return fail(ctx, GOPS_INVALID_PARAMETER, "unknown precision");
+      ctx->precision = (int)value;
+      return GOPS_OK;
+    case GOPS_OPT_ANCHOR_PERIOD:
+      if (!(value >= 1 && value <= 1e6)) return fail(ctx, GOPS_INVALID_PARAMETER, "anchor period");
+      ctx->anchor_period = (int)value;
+      return GOPS_OK;
+    case GOPS_OPT_NORMALIZE:
+      ctx->normalize = value != 0;
+      return GOPS_OK;
+    case GOPS_OPT_TIMING:
+      ctx->timing = value != 0;
+      return GOPS_OK;
+    case GOPS_OPT_PPR_TILE:
+      if (!(value == 0 || value == 8 || value == 16 || value == 32 || value == 64))
+        return fail(ctx, GOPS_INVALID_PARAMETER, "ppr tile must be 0, 8, 16, 32 or 64");
+      ctx->ppr_tile = (int)value;
+      return GOPS_OK;
+    default:
+      return fail(ctx, GOPS_INVALID_PARAMETER, "unknown option");
+  }
+}
+
+gops_status gops_last_run_stats(const gops_context *ctx, gops_run_stats *out) {
+  if (!ctx || !out) return GOPS_INVALID_PARAMETER;
+  *out = ctx->stats;
+  return GOPS_OK;
+}
+
+gops_status gops_graph_create(gops_context *ctx, uint64_t n, const uint64_t *offsets,
+                              const uint32_t *targets, const double *edge_weights,
+                              int weight_storage, gops_graph **out) {
+  if (!ctx || !out) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  *out = nullptr;
+  if (!offsets) return fail(ctx, GOPS_INVALID_PARAMETER, "offsets must not be null");
+  if (offsets[n] > 0 && !targets) return fail(ctx, GOPS_INVALID_PARAMETER, "targets must not be null");
+  if (weight_storage < GOPS_WEIGHT_NONE || weight_storage > GOPS_WEIGHT_BF16)
+    return fail(ctx, GOPS_INVALID_PARAMETER, "unknown weight storage");
+  if (weight_storage != GOPS_WEIGHT_NONE && !edge_weights && offsets[n] > 0)
+    return fail(ctx, GOPS_INVALID_PARAMETER, "weight storage given without edge weights");
+  // a weighted graph without edges still has to be flagged as weighted
+  static const double kNoWeights[1] = {0.0};
+  const double *ew = nullptr;
+  if (weight_storage != GOPS_WEIGHT_NONE) ew = edge_weights ? edge_weights : kNoWeights;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HostCSRT h;
+  std::string err;
+  gops_status st = build_csrt(n, offsets, targets, ew, h, err);
+  if (st != GOPS_OK) return fail(ctx, st, err);
+  Plan p;
+  plan_blocks(h, p);
+  auto *g = new (std::nothrow) gops_graph;
+  if (!g) return fail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
+  g->ctx = ctx;
+  g->weighted = weight_storage != GOPS_WEIGHT_NONE;
+  g->first_bad_weight = h.first_bad_weight;
+  g->order_host = h.order;
+  g->rank_host = h.rank;
+  st = upload_graph(ctx, g, h, p, weight_storage);
+  if (st != GOPS_OK) {
+    free_dev(g->dev);
+    delete g;
+    return st;
+  }
+  *out = g;
+  return GOPS_OK;
+}
+
+gops_status gops_graph_from_dense(gops_context *ctx, uint64_t n, const double *matrix,
+                                  int weight_storage, gops_graph **out) {
+  if (!ctx || !out) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  if (n > 0 && !matrix) return fail(ctx, GOPS_INVALID_PARAMETER, "matrix must not be null");
+  if (n >= (1ull << 31)) return fail(ctx, GOPS_NOT_SUPPORTED, "node count must be < 2^31");
+  // AdjacencyMatrix::neighbors (graph.rs:128-135): columns with entry > 0.0
+  std::vector<uint64_t> off(n + 1, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t u = 0; u < (int64_t)n; u++) {
+    uint64_t c = 0;
+    const double *row = matrix + (uint64_t)u * n;
+    for (uint64_t v = 0; v < n; v++) c += row[v] > 0.0;
+    off[u + 1] = c;
+  }
+  for (uint64_t u = 0; u < n; u++) off[u + 1] += off[u];
+  std::vector<uint32_t> nb(off[n]);
+  std::vector<double> ew(weight_storage != GOPS_WEIGHT_NONE ? off[n] : 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t u = 0; u < (int64_t)n; u++) {
+    uint64_t k = off[u];
+    const double *row = matrix + (uint64_t)u * n;
+    for (uint64_t v = 0; v < n; v++) {
+      if (row[v] > 0.0) {
+        nb[k] = (uint32_t)v;
+        if (!ew.empty()) ew[k] = row[v];  // edge_weight = raw entry, graph.rs:139-141
+        k++;
+      }
+    }
+  }
+  return gops_graph_create(ctx, n, off.data(), nb.data(), ew.empty() ? nullptr : ew.data(),
+                           weight_storage, out);
+}
+
+void gops_graph_destroy(gops_graph *g) {
+  if (!g) return;
+  if (g->ctx) {
+    hipSetDevice(g->ctx->device);
+    hipStreamSynchronize(g->ctx->stream);
+  }
+  free_run(g->run);
+  free_dev(g->dev);
+  delete g;
+}
+
+gops_status gops_graph_info(const gops_graph *g, uint64_t *n, uint64_t *nnz) {
+  if (!g) return GOPS_INVALID_PARAMETER;
+  if (n) *n = g->dev.n;
+  if (nnz) *nnz = g->dev.m;
+  return GOPS_OK;
+}
+
+gops_status gops_pagerank_run(gops_context *ctx, gops_graph *g, const gops_pagerank_config *cfg,
+                              double *scores, gops_run_info *info) {
+  return run_single(ctx, g, cfg, kPlain, nullptr, 0, scores, info);
+}
+
+gops_status gops_pagerank_checked_run(gops_context *ctx, gops_graph *g,
+                                      const gops_pagerank_config *cfg, double *scores,
+                                      gops_run_info *info) {
+  if (!ctx || !cfg) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  gops_status st = validate_config(ctx, cfg);
+  if (st != GOPS_OK) return st;
+  return run_single(ctx, g, cfg, kPlain, nullptr, 0, scores, info);
+}
+
+gops_status gops_pagerank_weighted_run(gops_context *ctx, gops_graph *g,
+                                       const gops_pagerank_config *cfg, double *scores,
+                                       gops_run_info *info) {
+  return run_single(ctx, g, cfg, kWeighted, nullptr, 0, scores, info);
+}
+
+gops_status gops_pagerank_weighted_checked_run(gops_context *ctx, gops_graph *g,
+                                               const gops_pagerank_config *cfg, double *scores,
+                                               gops_run_info *info) {
+  if (!ctx || !cfg || !g) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  gops_status st = validate_config(ctx, cfg);
+  if (st != GOPS_OK) return st;
+  // pagerank.rs:300-314: first offending neighbour weight decides the message
+  if (g->first_bad_weight == 1) return invalid(ctx, "edge weights must be finite");
+  if (g->first_bad_weight == 2) return invalid(ctx, "edge weights must be non-negative");
+  return run_single(ctx, g, cfg, kWeighted, nullptr, 0, scores, info);
+}
+
+gops_status gops_personalized_pagerank_run(gops_context *ctx, gops_graph *g,
+                                           const gops_pagerank_config *cfg,
+                                           const double *personalization, uint64_t len,
+                                           double *scores, gops_run_info *info) {
+  return run_single(ctx, g, cfg, kPersonalized, personalization, len, scores, info);
+}
+
+gops_status gops_personalized_pagerank_checked_run(gops_context *ctx, gops_graph *g,
+                                                   const gops_pagerank_config *cfg,
+                                                   const double *p, uint64_t len,
+                                                   double *scores, gops_run_info *info) {
+  if (!ctx || !cfg || !g) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  gops_status st = validate_config(ctx, cfg);
+  if (st != GOPS_OK) return st;
+  const uint64_t n = g->dev.n;
+  if (len != n)  // ppr.rs:18-24
+    return invalid(ctx, "personalization length must equal node_count (len=" +
+                            std::to_string(len) + " node_count=" + std::to_string(n) + ")");
+  if (len > 0 && !p) return fail(ctx, GOPS_INVALID_PARAMETER, "personalization must not be null");
+  for (uint64_t i = 0; i < len; i++) {  // ppr.rs:25-36
+    if (!std::isfinite(p[i])) return invalid(ctx, "personalization entries must be finite");
+    if (p[i] < 0.0) return invalid(ctx, "personalization entries must be non-negative");
+  }
+  double sum = 0.0;
+  for (uint64_t i = 0; i < len; i++) sum += p[i];
+  if (sum <= 0.0) return invalid(ctx, "personalization sum must be > 0");  // ppr.rs:37-42
+  return run_single(ctx, g, cfg, kPersonalized, p, len, scores, info);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched personalized PageRank, one query per device run (the plain path).
+namespace gops {
+gops_status ppr_batch_looped(gops_context *ctx, gops_graph *g, uint32_t nq,
+                             const gops_pagerank_config *cfgs, const uint64_t *seed_off,
+                             const uint32_t *seeds, const double *seed_w, double *scores,
+                             gops_run_info *infos) {
+  const uint64_t n = g->dev.n;
+  std::vector<double> p(n, 0.0);
+  for (uint32_t q = 0; q < nq; q++) {
+    for (uint64_t j = seed_off[q]; j < seed_off[q + 1]; j++) {
+      if (seeds[j] >= n)
+        return fail(ctx, GOPS_INDEX_OUT_OF_BOUNDS, "index out of bounds: " + std::to_string(seeds[j]));
+      p[seeds[j]] += seed_w ? seed_w[j] : 1.0;
+    }
+    gops_run_info info{};
+    gops_status st = run_single(ctx, g, &cfgs[q], kPersonalized, p.data(), n,
+                                scores ? scores + (size_t)q * n : nullptr, &info);
+    if (st != GOPS_OK) return st;
+    if (infos) infos[q] = info;
+    for (uint64_t j = seed_off[q]; j < seed_off[q + 1]; j++) p[seeds[j]] = 0.0;
+  }
+  return GOPS_OK;
+}
+}  // namespace gops

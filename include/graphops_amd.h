@@ -1,0 +1,247 @@
+/*
+ * graphops_amd.h -- C ABI of the MI355X (gfx950) PageRank family.
+ *
+ * Drop-in boundary for graphops v0.4.1's PageRank operators.  The reference
+ * has no FFI of its own; each entry point below is what a thin Rust
+ * `extern "C"` crate binds in place of the generic Rust function cited beside
+ * it (file:line in the reference crate).  Plain pointers and sizes only; no
+ * C++ exception or abort crosses this boundary.
+ *
+ * Conventions (mirroring the reference, pagerank.rs / ppr.rs / lib.rs):
+ *   - Unchecked `*_run` calls never reject inputs; they compute on what they
+ *     are given and reproduce the reference's edge cases (n == 0 returns
+ *     {iterations 0, diff 0.0, converged}; max_iterations == 0 returns the
+ *     initial vector with {0, +inf, not converged}).
+ *   - `*_checked_run` calls validate first and return GOPS_INVALID_PARAMETER
+ *     with the reference's message text, readable via gops_last_error().
+ *   - Scores are written to a caller-owned double[n] (the reference returns an
+ *     owned Vec<f64> of length n), indexed by the caller's node ids.  A NULL
+ *     scores pointer leaves the result on the device (for timing).
+ *   - A context owns one HIP stream and all device memory it allocates; it is
+ *     not thread-safe.  Separate contexts are independent; nothing is global.
+ */
+#ifndef GRAPHOPS_AMD_H
+#define GRAPHOPS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The first four mirror graphops::Error (lib.rs:108-123):
+ * InvalidParameter(String) lib.rs:117-118, IndexOutOfBounds(usize)
+ * lib.rs:113-114, DimensionMismatch(usize, usize) lib.rs:121-122. */
+typedef enum gops_status {
+    GOPS_OK = 0,
+    GOPS_INVALID_PARAMETER = 1,
+    GOPS_INDEX_OUT_OF_BOUNDS = 2,
+    GOPS_DIMENSION_MISMATCH = 3,
+    GOPS_DEVICE_ERROR = 4,
+    GOPS_OUT_OF_MEMORY = 5,
+    GOPS_NOT_SUPPORTED = 6
+} gops_status;
+
+typedef struct gops_context gops_context;
+typedef struct gops_graph gops_graph;
+typedef struct gops_edges gops_edges;
+typedef struct gops_server gops_server;
+
+/* PageRankConfig, pagerank.rs:37-49, field for field (usize -> uint64_t).
+ * Default (pagerank.rs:51-59): {0.85, 100, 1e-6}. */
+typedef struct gops_pagerank_config {
+    double damping;
+    uint64_t max_iterations;
+    double tolerance;
+} gops_pagerank_config;
+
+/* PageRankRun, pagerank.rs:14-29, without `scores` (written to the caller's
+ * buffer).  iterations == max_iterations when converged == 0. */
+typedef struct gops_run_info {
+    uint64_t iterations;
+    double diff_l1;
+    int32_t converged;
+    int32_t reserved;
+} gops_run_info;
+
+/* Device storage of edge weights (the caller always passes f64 weights, the
+ * type WeightedGraph::edge_weight returns, graph.rs:67-72). */
+enum {
+    GOPS_WEIGHT_NONE = 0, /* unweighted graph (edge weights ignored / absent) */
+    GOPS_WEIGHT_F64 = 1,
+    GOPS_WEIGHT_F32 = 2,  /* rounded to nearest f32 on upload */
+    GOPS_WEIGHT_BF16 = 3  /* rounded to f32, then to nearest-even bf16 */
+};
+
+/* Context options (gops_context_set_option).  No reference counterpart:
+ * the reference is f64 throughout. */
+enum {
+    GOPS_OPT_PRECISION = 1,     /* GOPS_PRECISION_MIXED (default) or _FP64 */
+    GOPS_OPT_ANCHOR_PERIOD = 2, /* sweeps between f64 anchor sweeps (mixed), default 32 */
+    GOPS_OPT_NORMALIZE = 3,     /* 1: rescale the result to sum to 1 in f64 (default 0,
+                                   the reference never renormalises, lib.rs:11-12) */
+    GOPS_OPT_TIMING = 4,        /* 1: time every sweep kernel with HIP events */
+    GOPS_OPT_PPR_TILE = 5       /* batched PPR query tile width (0 = autotuned) */
+};
+enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
+
+/* Statistics of the most recent run on a context (no reference counterpart). */
+typedef struct gops_run_stats {
+    uint64_t n;             /* nodes */
+    uint64_t nnz;           /* edges (adapter neighbour entries) */
+    uint64_t sweeps_fp32;   /* f32 delta sweeps executed */
+    uint64_t sweeps_fp64;   /* f64 anchor sweeps executed */
+    double device_ms;       /* first sweep launch .. result ready, HIP events */
+    double fp32_sweep_ms;   /* summed duration of f32 sweep kernels (GOPS_OPT_TIMING) */
+    double fp64_sweep_ms;   /* summed duration of f64 sweep kernels (GOPS_OPT_TIMING) */
+    double other_ms;        /* init / fold / finalize kernels (GOPS_OPT_TIMING) */
+} gops_run_stats;
+
+/* ---- context ---------------------------------------------------------- */
+gops_status gops_context_create(int device, gops_context **out);
+void gops_context_destroy(gops_context *ctx);
+/* Message of the last failing call on ctx ("" if none).  For
+ * GOPS_INVALID_PARAMETER it is Error's Display text, e.g.
+ * "invalid parameter: damping must be in [0,1]" (lib.rs:117). */
+const char *gops_last_error(const gops_context *ctx);
+gops_status gops_context_set_option(gops_context *ctx, int option, double value);
+gops_status gops_last_run_stats(const gops_context *ctx, gops_run_stats *out);
+
+/* ---- graphs ----------------------------------------------------------- */
+/* Graph from adapter output: for u in 0..n, targets[offsets[u]..offsets[u+1]]
+ * are Graph::neighbors(u) (graph.rs:26-39) in adapter order, duplicates kept
+ * (they count toward out_degree, pagerank.rs:137); edge_weights (nullable) are
+ * WeightedGraph::edge_weight(u, v) per entry (graph.rs:67-72).  Replaces the
+ * reference's per-call neighbour materialisation (pagerank.rs:136, 226-234;
+ * ppr.rs:112, 133).  The CSR transpose is built on the host and uploaded
+ * once; the caller's arrays may be freed on return.  Ids must be < n
+ * (GOPS_INDEX_OUT_OF_BOUNDS otherwise); n and offsets[n] must be < 2^31. */
+gops_status gops_graph_create(gops_context *ctx, uint64_t n, const uint64_t *offsets,
+                              const uint32_t *targets, const double *edge_weights,
+                              int weight_storage, gops_graph **out);
+/* AdjacencyMatrix adapter (graph.rs:122-142): row-major n*n matrix; edge
+ * u->v iff matrix[u*n+v] > 0.0 (NaN and negatives are not edges, +inf and
+ * self-loops are); edge weight = the raw entry. */
+gops_status gops_graph_from_dense(gops_context *ctx, uint64_t n, const double *matrix,
+                                  int weight_storage, gops_graph **out);
+void gops_graph_destroy(gops_graph *g);
+gops_status gops_graph_info(const gops_graph *g, uint64_t *n, uint64_t *nnz);
+
+/* ---- PageRank family -------------------------------------------------- */
+/* pagerank_run, pagerank.rs:122-187 (pagerank, pagerank.rs:114-116, is this
+ * call with the info ignored). */
+gops_status gops_pagerank_run(gops_context *ctx, gops_graph *g,
+                              const gops_pagerank_config *cfg, double *scores,
+                              gops_run_info *info);
+/* pagerank_checked_run, pagerank.rs:321-324 (and pagerank_checked,
+ * pagerank.rs:91-94): PageRankConfig::validate (pagerank.rs:63-85) first. */
+gops_status gops_pagerank_checked_run(gops_context *ctx, gops_graph *g,
+                                      const gops_pagerank_config *cfg, double *scores,
+                                      gops_run_info *info);
+/* pagerank_weighted_run, pagerank.rs:210-289 (pagerank_weighted,
+ * pagerank.rs:201-203).  Needs a graph created with edge weights. */
+gops_status gops_pagerank_weighted_run(gops_context *ctx, gops_graph *g,
+                                       const gops_pagerank_config *cfg, double *scores,
+                                       gops_run_info *info);
+/* pagerank_weighted_checked_run, pagerank.rs:328-336 (and
+ * pagerank_weighted_checked, pagerank.rs:294-316): validate the config and
+ * that every neighbour entry's weight is finite and >= 0, then run once. */
+gops_status gops_pagerank_weighted_checked_run(gops_context *ctx, gops_graph *g,
+                                               const gops_pagerank_config *cfg,
+                                               double *scores, gops_run_info *info);
+/* personalized_pagerank_run, ppr.rs:90-156 (personalized_pagerank,
+ * ppr.rs:66-72).  personalization has len entries (the reference indexes it
+ * for i < n, ppr.rs:127; a shorter vector is GOPS_DIMENSION_MISMATCH here
+ * instead of a panic). */
+gops_status gops_personalized_pagerank_run(gops_context *ctx, gops_graph *g,
+                                           const gops_pagerank_config *cfg,
+                                           const double *personalization, uint64_t len,
+                                           double *scores, gops_run_info *info);
+/* personalized_pagerank_checked_run, ppr.rs:77-85 (and
+ * personalized_pagerank_checked, ppr.rs:11-44). */
+gops_status gops_personalized_pagerank_checked_run(gops_context *ctx, gops_graph *g,
+                                                   const gops_pagerank_config *cfg,
+                                                   const double *personalization,
+                                                   uint64_t len, double *scores,
+                                                   gops_run_info *info);
+
+/* Batched personalized PageRank: num_queries independent calls of
+ * personalized_pagerank_run (ppr.rs:90-156) in one device run over the shared
+ * graph.  Query q's personalization is sparse: seeds[seed_offsets[q] ..
+ * seed_offsets[q+1]] with weights seed_weights[...] (NULL: all 1.0), i.e. the
+ * dense vector the reference would take with zeros elsewhere.  cfgs[q] is
+ * query q's config.  scores (nullable) receives num_queries rows of n
+ * doubles; infos (nullable) receives num_queries infos. */
+gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t num_queries,
+                               const gops_pagerank_config *cfgs,
+                               const uint64_t *seed_offsets, const uint32_t *seeds,
+                               const double *seed_weights, double *scores,
+                               gops_run_info *infos);
+
+/* ---- serving loop (continuous batching of PPR queries) ---------------- */
+typedef struct gops_server_config {
+    uint32_t max_slots;      /* query slots resident on the device (0 = from HBM budget) */
+    double hbm_budget_bytes; /* device bytes the slot pool may use (0 = 90% of free) */
+    uint32_t tile;           /* queries per tile (0 = autotuned) */
+    uint32_t topk;           /* >0: results are top-k (id, score) instead of n scores */
+} gops_server_config;
+
+typedef struct gops_server_result {
+    uint64_t ticket;
+    gops_run_info info;
+    double submit_ms;  /* host clock, ms since server creation */
+    double finish_ms;
+} gops_server_result;
+
+/* No reference counterpart (the reference is a library call per query). */
+gops_status gops_server_create(gops_context *ctx, gops_graph *g,
+                               const gops_server_config *cfg, gops_server **out);
+void gops_server_destroy(gops_server *s);
+/* Queue one personalized PageRank query; returns its ticket. */
+gops_status gops_server_submit(gops_server *s, const gops_pagerank_config *cfg,
+                               const uint32_t *seeds, uint64_t num_seeds,
+                               const double *seed_weights, uint64_t *ticket);
+/* Advance the device by up to `sweeps` sweeps, admitting queued queries into
+ * free slots first; returns the number of active queries via *active. */
+gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active);
+/* Collect a finished query.  Returns GOPS_OK and fills out / scores (n
+ * doubles, nullable) or topk arrays (nullable), or GOPS_NOT_SUPPORTED-free
+ * status GOPS_OK with *ready = 0 when nothing has finished. */
+gops_status gops_server_poll(gops_server *s, int *ready, gops_server_result *out,
+                             double *scores, uint32_t *topk_ids, double *topk_scores);
+gops_status gops_server_stats(const gops_server *s, uint64_t *queued, uint64_t *active,
+                              uint64_t *finished, uint32_t *slots);
+
+/* ---- synthetic inputs (no reference counterpart: the reference has no
+ *      generator) -------------------------------------------------------- */
+enum { GOPS_GEN_UNIT = 0, GOPS_GEN_UNIFORM = 1, GOPS_GEN_LOGNORMAL = 2 };
+enum { GOPS_ROUND_F64 = 0, GOPS_ROUND_F32 = 1, GOPS_ROUND_BF16 = 2 };
+
+/* R-MAT: 2^scale nodes, edge_factor * 2^scale drawn edges, quadrant
+ * probabilities (a, b, c, 1-a-b-c); self-loops and duplicates removed;
+ * neighbour lists sorted by target.  Weights per unique edge (u, v) are drawn
+ * from a counter RNG keyed by (seed, u, v): GOPS_GEN_UNIFORM in [p0, p1),
+ * GOPS_GEN_LOGNORMAL(mu = p0, sigma = p1), then rounded per `rounding`. */
+gops_status gops_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                          uint64_t seed, int weight_kind, double p0, double p1,
+                          int rounding, gops_edges **out);
+/* Directed Erdos-Renyi G(n, p) over ordered pairs u != v, unit weights. */
+gops_status gops_gen_erdos_renyi(uint64_t n, double p, uint64_t seed, gops_edges **out);
+/* Seed sets for batched PPR: sizes uniform in [min_seeds, max_seeds]
+ * (log-uniform if log_uniform != 0), distinct nodes drawn proportional to
+ * out-degree (the source of a uniformly drawn edge). */
+gops_status gops_gen_seed_sets(const gops_edges *e, uint32_t num_queries, uint32_t min_seeds,
+                               uint32_t max_seeds, int log_uniform, uint64_t seed,
+                               uint64_t *offsets /* num_queries+1 */, uint32_t *seeds,
+                               uint64_t seeds_capacity);
+gops_status gops_edges_view(const gops_edges *e, uint64_t *n, uint64_t *m,
+                            const uint64_t **offsets, const uint32_t **targets,
+                            const double **weights);
+void gops_edges_destroy(gops_edges *e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAPHOPS_AMD_H */

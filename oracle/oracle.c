@@ -1,0 +1,478 @@
+/*
+ * oracle.c -- CPU f64 restatement of graphops' PageRank family.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: tests/, the
+ * smoke check and the benchmark's cpu_baseline leg call it to check (and, for
+ * the baseline, to time) the HIP product path.  Nothing in graphops_amd/
+ * links, loads or calls it; the product fails loudly without its own HIP
+ * library instead of falling back here.
+ *
+ * Reference: crate graphops v0.4.1 (Rust).  Every function cites the lines it
+ * restates.  The restatement keeps the reference's operation order so that the
+ * f64 results are bit-for-bit what the reference computes:
+ *   share     = (d * x_u) / deg_u                      pagerank.rs:157
+ *   share_w   = (d * x_u) * (w / ws_u)                 pagerank.rs:259
+ *   fill (PR) = (1 - d)/n + (d * dsum)/n               pagerank.rs:150-152
+ *   fill (PPR)= (1 - d) * p_i + (d * dsum) * p_i       ppr.rs:126-127
+ *   dsum, diff: sequential folds in index order         pagerank.rs:144-149,169-173
+ *   stop      : diff < tol (strict), checked after the swap  pagerank.rs:174-179
+ * Compile with -O2 -ffp-contract=off (no FMA contraction, no fast-math).
+ *
+ * Parity status (pins live in tests/test_oracle.py):
+ *   orc_pagerank_run, orc_pagerank_weighted_run, orc_ppr_run: pinned by exact
+ *     rational fixed points, exact-rational iteration trajectories on small
+ *     graphs, dense linear solves and networkx fixed points.
+ *   orc_*_pull_run: pinned bit-for-bit against the push forms above.
+ *   The reference's optional `simd` L1 residual (third-party innr 0.4,
+ *     pagerank.rs:166-167) is NOT restated: parity unpinned for that feature;
+ *     the default scalar residual is what is restated here.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_OK 0
+#define ORC_INVALID 1
+
+static void put_msg(char *msg, size_t cap, const char *text) {
+    if (msg && cap) {
+        snprintf(msg, cap, "%s", text);
+    }
+}
+
+/* PageRankConfig::validate, pagerank.rs:63-85.  Returns ORC_INVALID with the
+ * reference's message text (the Error::InvalidParameter payload). */
+int orc_validate_config(double damping, uint64_t max_iterations, double tolerance,
+                        char *msg, size_t cap) {
+    if (!isfinite(damping)) {
+        put_msg(msg, cap, "damping must be finite");
+        return ORC_INVALID;
+    }
+    if (!(damping >= 0.0 && damping <= 1.0)) {
+        put_msg(msg, cap, "damping must be in [0,1]");
+        return ORC_INVALID;
+    }
+    if (max_iterations == 0) {
+        put_msg(msg, cap, "max_iterations must be > 0");
+        return ORC_INVALID;
+    }
+    if (!isfinite(tolerance) || tolerance <= 0.0) {
+        put_msg(msg, cap, "tolerance must be finite and > 0");
+        return ORC_INVALID;
+    }
+    return ORC_OK;
+}
+
+/* Weight checks of pagerank_weighted_checked, pagerank.rs:300-314: every
+ * neighbour entry's edge_weight must be finite and >= 0. */
+int orc_check_weights(uint64_t m, const double *ew, char *msg, size_t cap) {
+    for (uint64_t j = 0; j < m; j++) {
+        double w = ew[j];
+        if (!isfinite(w)) {
+            put_msg(msg, cap, "edge weights must be finite");
+            return ORC_INVALID;
+        }
+        if (w < 0.0) {
+            put_msg(msg, cap, "edge weights must be non-negative");
+            return ORC_INVALID;
+        }
+    }
+    return ORC_OK;
+}
+
+/* Personalization checks of personalized_pagerank_checked, ppr.rs:17-42. */
+int orc_check_personalization(uint64_t n, const double *p, uint64_t p_len,
+                              char *msg, size_t cap) {
+    if (p_len != n) {
+        if (msg && cap) {
+            snprintf(msg, cap,
+                     "personalization length must equal node_count (len=%llu node_count=%llu)",
+                     (unsigned long long)p_len, (unsigned long long)n);
+        }
+        return ORC_INVALID;
+    }
+    for (uint64_t i = 0; i < p_len; i++) {
+        if (!isfinite(p[i])) {
+            put_msg(msg, cap, "personalization entries must be finite");
+            return ORC_INVALID;
+        }
+        if (p[i] < 0.0) {
+            put_msg(msg, cap, "personalization entries must be non-negative");
+            return ORC_INVALID;
+        }
+    }
+    double sum = 0.0;
+    for (uint64_t i = 0; i < p_len; i++) sum += p[i];
+    if (sum <= 0.0) {
+        put_msg(msg, cap, "personalization sum must be > 0");
+        return ORC_INVALID;
+    }
+    return ORC_OK;
+}
+
+/* AdjacencyMatrix adapter, graph.rs:122-142: neighbours of u are the columns
+ * v with m[u][v] > 0.0 (NaN and negatives excluded), in column order;
+ * edge_weight(u, v) is the raw entry.  Fills off[n+1]; when nb/ew are
+ * non-NULL also the neighbour ids and their edge weights.  Returns m. */
+uint64_t orc_dense_neighbors(uint64_t n, const double *mat, uint64_t *off,
+                             uint32_t *nb, double *ew) {
+    uint64_t m = 0;
+    for (uint64_t u = 0; u < n; u++) {
+        off[u] = m;
+        const double *row = mat + u * n;
+        for (uint64_t v = 0; v < n; v++) {
+            if (row[v] > 0.0) {
+                if (nb) nb[m] = (uint32_t)v;
+                if (ew) ew[m] = row[v];
+                m++;
+            }
+        }
+    }
+    off[n] = m;
+    return m;
+}
+
+static void run_empty(double *diff, uint64_t *iters, int *conv) {
+    /* n == 0 early return, pagerank.rs:124-131 / ppr.rs:97-103 */
+    *iters = 0;
+    *diff = 0.0;
+    *conv = 1;
+}
+
+static double l1_seq(uint64_t n, const double *a, const double *b) {
+    /* default (non-simd) residual, pagerank.rs:169-173 */
+    double s = 0.0;
+    for (uint64_t i = 0; i < n; i++) s += fabs(a[i] - b[i]);
+    return s;
+}
+
+/* pagerank_run, pagerank.rs:122-187 (push form, neighbour lists in adapter
+ * order).  scores[n] receives the returned PageRankRun::scores. */
+void orc_pagerank_run(uint64_t n, const uint64_t *off, const uint32_t *nb,
+                      double damping, uint64_t max_iterations, double tolerance,
+                      double *scores, uint64_t *iters_out, double *diff_out,
+                      int *conv_out) {
+    if (n == 0) {
+        run_empty(diff_out, iters_out, conv_out);
+        return;
+    }
+    double n_f64 = (double)n;
+    double *cur = scores;
+    double *nxt = (double *)malloc(n * sizeof(double));
+    for (uint64_t i = 0; i < n; i++) cur[i] = 1.0 / n_f64;         /* :133 */
+    uint64_t iters = 0;
+    double last_diff = INFINITY;
+    int converged = 0;
+    for (uint64_t it = 0; it < max_iterations; it++) {             /* :142 */
+        iters++;
+        double dsum = 0.0;                                         /* :144-149 */
+        for (uint64_t i = 0; i < n; i++)
+            if (off[i + 1] - off[i] == 0) dsum += cur[i];
+        double dangling_contrib = damping * dsum / n_f64;          /* :150 */
+        double teleport = (1.0 - damping) / n_f64;                 /* :151 */
+        double fill = teleport + dangling_contrib;                 /* :152 */
+        for (uint64_t i = 0; i < n; i++) nxt[i] = fill;
+        for (uint64_t u = 0; u < n; u++) {                         /* :154-162 */
+            uint64_t deg = off[u + 1] - off[u];
+            if (deg > 0) {
+                double share = damping * cur[u] / (double)deg;
+                for (uint64_t j = off[u]; j < off[u + 1]; j++) nxt[nb[j]] += share;
+            }
+        }
+        double diff = l1_seq(n, cur, nxt);                         /* :169-173 */
+        last_diff = diff;
+        double *t = cur; cur = nxt; nxt = t;                       /* :175 */
+        if (diff < tolerance) {                                    /* :176-179 */
+            converged = 1;
+            break;
+        }
+    }
+    if (cur != scores) {
+        memcpy(scores, cur, n * sizeof(double));
+        free(cur);
+    } else {
+        free(nxt);
+    }
+    *iters_out = iters;
+    *diff_out = last_diff;
+    *conv_out = converged;
+}
+
+/* out_wsum, pagerank.rs:227-234: ws_u = sum over neighbour entries of
+ * max(edge_weight(u, v), 0) in neighbour order (f64::max returns the non-NaN
+ * operand, as fmax does). */
+void orc_out_wsum(uint64_t n, const uint64_t *off, const double *ew, double *ws) {
+    for (uint64_t u = 0; u < n; u++) {
+        double s = 0.0;
+        for (uint64_t j = off[u]; j < off[u + 1]; j++) s += fmax(ew[j], 0.0);
+        ws[u] = s;
+    }
+}
+
+/* pagerank_weighted_run, pagerank.rs:210-289 (push form).  ew[j] is
+ * edge_weight(u, nb[j]) for the j-th neighbour entry. */
+void orc_pagerank_weighted_run(uint64_t n, const uint64_t *off, const uint32_t *nb,
+                               const double *ew, double damping,
+                               uint64_t max_iterations, double tolerance,
+                               double *scores, uint64_t *iters_out,
+                               double *diff_out, int *conv_out) {
+    if (n == 0) {
+        run_empty(diff_out, iters_out, conv_out);
+        return;
+    }
+    double n_f64 = (double)n;
+    double *cur = scores;
+    double *nxt = (double *)malloc(n * sizeof(double));
+    double *ws = (double *)malloc(n * sizeof(double));
+    for (uint64_t i = 0; i < n; i++) cur[i] = 1.0 / n_f64;         /* :220 */
+    orc_out_wsum(n, off, ew, ws);                                  /* :227-234 */
+    uint64_t iters = 0;
+    double last_diff = INFINITY;
+    int converged = 0;
+    for (uint64_t it = 0; it < max_iterations; it++) {             /* :239 */
+        iters++;
+        double dsum = 0.0;                                         /* :241-246 */
+        for (uint64_t i = 0; i < n; i++)
+            if (ws[i] == 0.0) dsum += cur[i];
+        double dangling_contrib = damping * dsum / n_f64;          /* :248 */
+        double teleport = (1.0 - damping) / n_f64;                 /* :249 */
+        double fill = teleport + dangling_contrib;
+        for (uint64_t i = 0; i < n; i++) nxt[i] = fill;            /* :250 */
+        for (uint64_t u = 0; u < n; u++) {                         /* :252-263 */
+            double w_s = ws[u];
+            if (w_s > 0.0) {
+                for (uint64_t j = off[u]; j < off[u + 1]; j++) {
+                    double w = fmax(ew[j], 0.0);
+                    if (w > 0.0) nxt[nb[j]] += damping * cur[u] * (w / w_s);
+                }
+            }
+        }
+        double diff = l1_seq(n, cur, nxt);                         /* :270-274 */
+        last_diff = diff;
+        double *t = cur; cur = nxt; nxt = t;
+        if (diff < tolerance) {                                    /* :277-280 */
+            converged = 1;
+            break;
+        }
+    }
+    if (cur != scores) {
+        memcpy(scores, cur, n * sizeof(double));
+        free(cur);
+    } else {
+        free(nxt);
+    }
+    free(ws);
+    *iters_out = iters;
+    *diff_out = last_diff;
+    *conv_out = converged;
+}
+
+/* personalized_pagerank_run, ppr.rs:90-156 (push form). */
+void orc_ppr_run(uint64_t n, const uint64_t *off, const uint32_t *nb,
+                 const double *personalization, double damping,
+                 uint64_t max_iterations, double tolerance, double *scores,
+                 uint64_t *iters_out, double *diff_out, int *conv_out) {
+    if (n == 0) {
+        run_empty(diff_out, iters_out, conv_out);
+        return;
+    }
+    double *p_vec = (double *)malloc(n * sizeof(double));
+    double p_sum = 0.0;                                            /* :104 */
+    for (uint64_t i = 0; i < n; i++) p_sum += personalization[i];
+    if (p_sum > 0.0) {                                             /* :105-109 */
+        for (uint64_t i = 0; i < n; i++) p_vec[i] = personalization[i] / p_sum;
+    } else {
+        for (uint64_t i = 0; i < n; i++) p_vec[i] = 1.0 / (double)n;
+    }
+    double *cur = scores;
+    double *nxt = (double *)malloc(n * sizeof(double));
+    memcpy(cur, p_vec, n * sizeof(double));                        /* :110 */
+    uint64_t iters = 0;
+    double last_diff = INFINITY;
+    int converged = 0;
+    for (uint64_t it = 0; it < max_iterations; it++) {             /* :117 */
+        iters++;
+        double dsum = 0.0;                                         /* :119-124 */
+        for (uint64_t i = 0; i < n; i++)
+            if (off[i + 1] - off[i] == 0) dsum += cur[i];
+        for (uint64_t i = 0; i < n; i++)                           /* :125-128 */
+            nxt[i] = (1.0 - damping) * p_vec[i] + damping * dsum * p_vec[i];
+        for (uint64_t u = 0; u < n; u++) {                         /* :129-137 */
+            uint64_t deg = off[u + 1] - off[u];
+            if (deg > 0) {
+                double share = damping * cur[u] / (double)deg;
+                for (uint64_t j = off[u]; j < off[u + 1]; j++) nxt[nb[j]] += share;
+            }
+        }
+        double diff = l1_seq(n, cur, nxt);                         /* :138-142 */
+        last_diff = diff;
+        double *t = cur; cur = nxt; nxt = t;
+        if (diff < tolerance) {                                    /* :145-148 */
+            converged = 1;
+            break;
+        }
+    }
+    if (cur != scores) {
+        memcpy(scores, cur, n * sizeof(double));
+        free(cur);
+    } else {
+        free(nxt);
+    }
+    free(p_vec);
+    *iters_out = iters;
+    *diff_out = last_diff;
+    *conv_out = converged;
+}
+
+/*
+ * Pull forms over the CSR transpose (in-edges of v sorted by ascending source,
+ * duplicates adjacent).  For a fixed v the push loop above adds the shares of
+ * u = 0, 1, ... in increasing u, so pulling them in ascending-source order
+ * reproduces the same f64 addition sequence bit-for-bit; rows are independent,
+ * so OpenMP over rows keeps that property.  dsum and diff stay sequential.
+ *   kind 0: unweighted PageRank (pagerank_run)      scale[u] = deg_u (as f64)
+ *   kind 1: weighted PageRank (pagerank_weighted_run) scale[u] = ws_u, w[] given
+ *   kind 2: personalized PageRank (personalized_pagerank_run), p[] given
+ * w[e] must already be max(edge_weight, 0).  threads <= 0 means all.
+ */
+void orc_pull_run(int kind, uint64_t n, const uint64_t *in_off, const uint32_t *in_src,
+                  const double *w, const double *scale, const double *personalization,
+                  double damping, uint64_t max_iterations, double tolerance,
+                  int threads, double *scores, uint64_t *iters_out,
+                  double *diff_out, int *conv_out) {
+    if (n == 0) {
+        run_empty(diff_out, iters_out, conv_out);
+        return;
+    }
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#else
+    threads = 1;
+#endif
+    double n_f64 = (double)n;
+    double *cur = scores;
+    double *nxt = (double *)malloc(n * sizeof(double));
+    double *p_vec = NULL;
+    if (kind == 2) {
+        p_vec = (double *)malloc(n * sizeof(double));
+        double p_sum = 0.0;
+        for (uint64_t i = 0; i < n; i++) p_sum += personalization[i];
+        if (p_sum > 0.0) {
+            for (uint64_t i = 0; i < n; i++) p_vec[i] = personalization[i] / p_sum;
+        } else {
+            for (uint64_t i = 0; i < n; i++) p_vec[i] = 1.0 / n_f64;
+        }
+        memcpy(cur, p_vec, n * sizeof(double));
+    } else {
+        for (uint64_t i = 0; i < n; i++) cur[i] = 1.0 / n_f64;
+    }
+    uint64_t iters = 0;
+    double last_diff = INFINITY;
+    int converged = 0;
+    for (uint64_t it = 0; it < max_iterations; it++) {
+        iters++;
+        double dsum = 0.0;
+        for (uint64_t i = 0; i < n; i++)
+            if (scale[i] == 0.0) dsum += cur[i];
+        double fill = (1.0 - damping) / n_f64 + damping * dsum / n_f64;
+        double ddsum = damping * dsum;
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(threads)
+        for (int64_t vv = 0; vv < (int64_t)n; vv++) {
+            uint64_t v = (uint64_t)vv;
+            double acc;
+            if (kind == 2)
+                acc = (1.0 - damping) * p_vec[v] + ddsum * p_vec[v];
+            else
+                acc = fill;
+            for (uint64_t e = in_off[v]; e < in_off[v + 1]; e++) {
+                uint32_t u = in_src[e];
+                if (kind == 1) {
+                    double we = w[e];
+                    if (we > 0.0) acc += damping * cur[u] * (we / scale[u]);
+                } else {
+                    acc += damping * cur[u] / scale[u];
+                }
+            }
+            nxt[v] = acc;
+        }
+        double diff = l1_seq(n, cur, nxt);
+        last_diff = diff;
+        double *t = cur; cur = nxt; nxt = t;
+        if (diff < tolerance) {
+            converged = 1;
+            break;
+        }
+    }
+    if (cur != scores) {
+        memcpy(scores, cur, n * sizeof(double));
+        free(cur);
+    } else {
+        free(nxt);
+    }
+    free(p_vec);
+    *iters_out = iters;
+    *diff_out = last_diff;
+    *conv_out = converged;
+}
+
+/* One push sweep of pagerank_weighted_run / pagerank_run (pagerank.rs:239-263,
+ * 142-162) from a given vector, for timing a bounded sample of the CPU path.
+ * Returns the sweep's L1 residual; writes the new vector into out[n]. */
+double orc_push_sweep(int weighted, uint64_t n, const uint64_t *off, const uint32_t *nb,
+                      const double *ew, const double *ws, double damping,
+                      const double *cur, double *out) {
+    double n_f64 = (double)n;
+    double dsum = 0.0;
+    if (weighted) {
+        for (uint64_t i = 0; i < n; i++)
+            if (ws[i] == 0.0) dsum += cur[i];
+    } else {
+        for (uint64_t i = 0; i < n; i++)
+            if (off[i + 1] - off[i] == 0) dsum += cur[i];
+    }
+    double fill = (1.0 - damping) / n_f64 + damping * dsum / n_f64;
+    for (uint64_t i = 0; i < n; i++) out[i] = fill;
+    for (uint64_t u = 0; u < n; u++) {
+        if (weighted) {
+            double w_s = ws[u];
+            if (w_s > 0.0) {
+                for (uint64_t j = off[u]; j < off[u + 1]; j++) {
+                    double w = fmax(ew[j], 0.0);
+                    if (w > 0.0) out[nb[j]] += damping * cur[u] * (w / w_s);
+                }
+            }
+        } else {
+            uint64_t deg = off[u + 1] - off[u];
+            if (deg > 0) {
+                double share = damping * cur[u] / (double)deg;
+                for (uint64_t j = off[u]; j < off[u + 1]; j++) out[nb[j]] += share;
+            }
+        }
+    }
+    return l1_seq(n, cur, out);
+}
+
+/* Transpose neighbour lists (adapter order) into in-edge lists sorted by
+ * ascending source; duplicates keep their adapter order.  Sequential. */
+void orc_transpose(uint64_t n, const uint64_t *off, const uint32_t *nb, const double *ew,
+                   uint64_t *in_off, uint32_t *in_src, double *in_w) {
+    uint64_t m = off[n];
+    memset(in_off, 0, (n + 1) * sizeof(uint64_t));
+    for (uint64_t j = 0; j < m; j++) in_off[nb[j] + 1]++;
+    for (uint64_t v = 0; v < n; v++) in_off[v + 1] += in_off[v];
+    uint64_t *cur = (uint64_t *)malloc((n + 1) * sizeof(uint64_t));
+    memcpy(cur, in_off, (n + 1) * sizeof(uint64_t));
+    for (uint64_t u = 0; u < n; u++) {
+        for (uint64_t j = off[u]; j < off[u + 1]; j++) {
+            uint64_t pos = cur[nb[j]]++;
+            in_src[pos] = (uint32_t)u;
+            if (in_w) in_w[pos] = fmax(ew[j], 0.0);
+        }
+    }
+    free(cur);
+}

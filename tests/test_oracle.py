@@ -1,0 +1,245 @@
+"""Pins of the f64 oracle (CPU only).
+
+The oracle restates pagerank.rs:122-289 and ppr.rs:90-156.  These tests check
+it against what fixes those functions independently of the restatement:
+exact rational fixed points and trajectories, dense linear solves, networkx,
+the reference's own unit/property tests and special cases -- and that planted
+mistakes (in the oracle's inputs, on the CPU) are caught.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+D = 0.85
+BOUND = D / (1 - D)  # ||x_K - x*||_1 <= d/(1-d) * diff_K for this contraction
+
+
+def frac(s):
+    return float(Fraction(s))
+
+
+def lists_from(nbrs):
+    off = np.zeros(len(nbrs) + 1, dtype=np.uint64)
+    for u, lst in enumerate(nbrs):
+        off[u + 1] = off[u] + len(lst)
+    nb = np.array([v for lst in nbrs for v in lst], dtype=np.uint32)
+    return off, nb
+
+
+def run_case(oracle, case, tol, max_iterations=10000):
+    kind = case.get("kind")
+    if "matrix" in case:
+        off, nb, ew = oracle.dense_neighbors(np.array(case["matrix"], dtype=float))
+        weighted = case["weighted"]
+    else:
+        off, nb = lists_from(case["neighbors"])
+        ew = (np.array([w for lst in case["edge_weights"] for w in lst], dtype=float)
+              if case.get("edge_weights") is not None else None)
+        weighted = kind == "weighted"
+    p = case.get("personalization")
+    if p is not None:
+        return oracle.personalized_pagerank_run(off, nb, p, case["damping"], max_iterations, tol)
+    if weighted:
+        return oracle.pagerank_weighted_run(off, nb, ew, case["damping"], max_iterations, tol)
+    return oracle.pagerank_run(off, nb, case["damping"], max_iterations, tol)
+
+
+def test_closed_forms(golden, oracle):
+    for case in golden["closed_forms"]:
+        r = run_case(oracle, case, 1e-13)
+        x = np.array([frac(s) for s in case["fixed_point"]])
+        assert r.converged, case["name"]
+        err = np.abs(r.scores - x).sum()
+        assert err <= BOUND * r.diff_l1 + 1e-15, (case["name"], err)
+
+
+def test_exact_trajectories(golden, oracle):
+    """Iteration count, converged flag, last residual and scores equal the
+    reference loop run in exact arithmetic."""
+    for t in golden["trajectories"]:
+        off, nb, ew = oracle.dense_neighbors(np.array(t["matrix"], dtype=float))
+        if t["personalization"] is not None:
+            r = oracle.personalized_pagerank_run(off, nb, t["personalization"], t["damping"],
+                                                 t["max_iterations"], t["tolerance"])
+        elif t["weighted"]:
+            r = oracle.pagerank_weighted_run(off, nb, ew, t["damping"], t["max_iterations"],
+                                             t["tolerance"])
+        else:
+            r = oracle.pagerank_run(off, nb, t["damping"], t["max_iterations"], t["tolerance"])
+        assert r.iterations == t["iterations"], t["name"]
+        assert r.converged == t["converged"], t["name"]
+        # f64 rounding of the iterates leaves an absolute residual noise of a few ulp(1)
+        assert abs(r.diff_l1 - t["diff_l1"]) <= 1e-12 * t["diff_l1"] + 2e-15, t["name"]
+        np.testing.assert_allclose(r.scores, t["scores"], rtol=0, atol=1e-14, err_msg=t["name"])
+
+
+def test_dense_solves(golden, oracle):
+    for case in golden["dense_solves"]:
+        r = run_case(oracle, case, 1e-13)
+        x = np.array(case["fixed_point"])
+        assert r.converged
+        assert np.abs(r.scores - x).sum() <= BOUND * r.diff_l1 + 1e-12, case["name"]
+
+
+def test_networkx_fixed_points(golden, oracle):
+    for case in golden["networkx"]:
+        r = run_case(oracle, case, 1e-13)
+        x = np.array(case["fixed_point"])
+        assert np.abs(r.scores - x).sum() <= 1e-11, case["name"]
+
+
+def test_c1_workload_matches_dense_solve(golden, oracle):
+    """C1: G(2000, 0.005) seed 42, dense-adapter semantics, tol 1e-10."""
+    import hashlib
+    import os
+
+    from graphops_amd import generators
+
+    e = generators.erdos_renyi(2000, 0.005, 42)
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(e.offsets).tobytes())
+    h.update(np.ascontiguousarray(e.targets).tobytes())
+    assert h.hexdigest() == golden["c1"]["sha256"]
+    mat = np.zeros((2000, 2000))
+    for u in range(2000):
+        mat[u, e.targets[e.offsets[u]:e.offsets[u + 1]]] = 1.0
+    off, nb, _ = oracle.dense_neighbors(mat)
+    r = oracle.pagerank_run(off, nb, 0.85, 100, 1e-10)
+    x = np.load(os.path.join(os.path.dirname(__file__), "golden", "c1_er2000_fixed_point.npy"))
+    assert r.converged
+    assert np.abs(r.scores - x).sum() <= BOUND * r.diff_l1 + 1e-13
+
+
+def random_graph(rng, n, m, weighted):
+    nbrs = [[] for _ in range(n)]
+    for _ in range(m):
+        nbrs[int(rng.integers(0, n))].append(int(rng.integers(0, n)))
+    off, nb = lists_from(nbrs)
+    ew = rng.choice([0.0, -0.5, 0.3, 1.0, 2.5, np.nan], size=nb.size) if weighted else None
+    return off, nb, ew
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("threads", [1, 4])
+def test_pull_is_bitwise_push(oracle, kind, threads):
+    rng = np.random.default_rng(kind * 10 + threads)
+    for trial in range(5):
+        n = int(rng.integers(1, 300))
+        off, nb, ew = random_graph(rng, n, int(rng.integers(0, 6 * n)), kind == 1)
+        p = None
+        if kind == 0:
+            ref = oracle.pagerank_run(off, nb, 0.85, 300, 1e-12)
+        elif kind == 1:
+            ref = oracle.pagerank_weighted_run(off, nb, ew, 0.85, 300, 1e-12)
+        else:
+            p = rng.random(n) * (rng.random(n) < 0.3)
+            ref = oracle.personalized_pagerank_run(off, nb, p, 0.85, 300, 1e-12)
+        in_off, in_src, in_w = oracle.transpose(off, nb, ew)
+        scale = oracle.out_wsum(off, ew) if kind == 1 else np.diff(off).astype(float)
+        got = oracle.pull_run(kind, in_off, in_src, scale, w=in_w, p=p, damping=0.85,
+                              max_iterations=300, tolerance=1e-12, threads=threads)
+        assert got.iterations == ref.iterations
+        assert got.diff_l1 == ref.diff_l1 or (np.isnan(got.diff_l1) and np.isnan(ref.diff_l1))
+        assert np.array_equal(got.scores, ref.scores, equal_nan=True)
+
+
+def test_validation_messages(oracle):
+    """PageRankConfig::validate (pagerank.rs:63-85) and the checked variants'
+    messages (pagerank.rs:300-314, ppr.rs:17-42)."""
+    cases = [((float("nan"), 100, 1e-6), "damping must be finite"),
+             ((1.5, 100, 1e-6), "damping must be in [0,1]"),
+             ((-0.1, 100, 1e-6), "damping must be in [0,1]"),
+             ((0.85, 0, 1e-6), "max_iterations must be > 0"),
+             ((0.85, 100, 0.0), "tolerance must be finite and > 0"),
+             ((0.85, 100, float("inf")), "tolerance must be finite and > 0")]
+    for args, msg in cases:
+        with pytest.raises(oracle.OracleError, match="invalid parameter: " + msg.replace("[", r"\[").replace("]", r"\]")):
+            oracle.validate_config(*args)
+    oracle.validate_config(0.0, 1, 1e-300)
+    oracle.validate_config(1.0, 1, 1.0)
+    with pytest.raises(oracle.OracleError, match="edge weights must be finite"):
+        oracle.check_weights(np.array([1.0, np.inf, -1.0]))
+    with pytest.raises(oracle.OracleError, match="edge weights must be non-negative"):
+        oracle.check_weights(np.array([1.0, -1.0, np.nan]))
+    with pytest.raises(oracle.OracleError, match=r"personalization length must equal node_count \(len=1 node_count=2\)"):
+        oracle.check_personalization(2, np.array([1.0]))
+    with pytest.raises(oracle.OracleError, match="personalization entries must be finite"):
+        oracle.check_personalization(2, np.array([np.nan, 1.0]))
+    with pytest.raises(oracle.OracleError, match="personalization entries must be non-negative"):
+        oracle.check_personalization(2, np.array([-1.0, 1.0]))
+    with pytest.raises(oracle.OracleError, match="sum"):
+        oracle.check_personalization(2, np.array([0.0, 0.0]))
+
+
+def test_reference_unit_tests(oracle):
+    """pagerank.rs:344-377 and the proptest at 380-395."""
+    off, nb, ew = oracle.dense_neighbors(np.array([[0, 2, 1], [0, 0, 1], [0, 0, 0]], float))
+    s = oracle.pagerank_weighted_run(off, nb, ew).scores
+    assert abs(s.sum() - 1.0) < 1e-6
+    off, nb, ew = oracle.dense_neighbors(np.array([[0, 2, 1], [0, 0, 0], [0, 0, 0]], float))
+    s = oracle.pagerank_weighted_run(off, nb, ew).scores
+    assert s[1] >= s[2]
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        n = int(rng.integers(1, 10))
+        adj = np.zeros((n, n))
+        for _ in range(int(rng.integers(0, 40))):
+            u, v = int(rng.integers(0, 10)), int(rng.integers(0, 10))
+            if u < n and v < n and u != v:
+                adj[u, v] = 1.0
+        off, nb, _ = oracle.dense_neighbors(adj)
+        s = oracle.pagerank_run(off, nb).scores
+        assert s.size == n and abs(s.sum() - 1.0) < 1e-6 and (s >= -1e-12).all()
+
+
+def test_special_cases(oracle):
+    # n == 0 (pagerank.rs:124-131)
+    r = oracle.pagerank_run(np.zeros(1, np.uint64), np.zeros(0, np.uint32))
+    assert (r.iterations, r.diff_l1, r.converged, r.scores.size) == (0, 0.0, True, 0)
+    off, nb, _ = oracle.dense_neighbors(np.array([[0, 1, 1, 0], [0, 0, 1, 0], [1, 0, 0, 0],
+                                                  [0, 0, 1, 0]], float))
+    # unchecked max_iterations == 0: initial vector, 0 iterations, +inf, not converged
+    r = oracle.pagerank_run(off, nb, 0.85, 0, 1e-6)
+    assert r.iterations == 0 and r.diff_l1 == float("inf") and not r.converged
+    np.testing.assert_array_equal(r.scores, np.full(4, 0.25))
+    # d = 0: the first sweep lands on the teleport vector (uniform) -> diff 0
+    r = oracle.pagerank_run(off, nb, 0.0, 100, 1e-6)
+    assert r.iterations == 1 and r.converged and r.diff_l1 == 0.0
+    # PPR with an all-zero personalization falls back to uniform (ppr.rs:105-109)
+    a = oracle.personalized_pagerank_run(off, nb, np.zeros(4), 0.85, 100, 1e-10)
+    b = oracle.pagerank_run(off, nb, 0.85, 100, 1e-10)
+    np.testing.assert_allclose(a.scores, b.scores, atol=1e-15)
+    # L1 contraction of the residual: diff_{k+1} <= d * diff_k
+    prev = None
+    for k in range(1, 30):
+        r = oracle.pagerank_run(off, nb, 0.85, k, 1e-300)
+        if prev is not None:
+            # equality holds for this graph; allow the f64 noise of the iterates
+            assert r.diff_l1 <= 0.85 * prev + 1e-15
+        prev = r.diff_l1
+
+
+def test_planted_mistakes_are_caught(golden, oracle):
+    """Mistakes planted on the CPU side must fail the pins above: a transposed
+    graph, dangling mass sent uniformly instead of to p, and a dropped
+    dangling term."""
+    case = next(c for c in golden["closed_forms"] if c["name"] == "example4")
+    x = np.array([frac(s) for s in case["fixed_point"]])
+    mat = np.array(case["matrix"], float)
+    off, nb, _ = oracle.dense_neighbors(mat.T)  # transposed operand
+    r = oracle.pagerank_run(off, nb, 0.85, 1000, 1e-13)
+    assert np.abs(r.scores - x).sum() > 1e-3
+    # PPR on line3 with dangling mass redistributed uniformly = a different chain
+    ppr = next(c for c in golden["closed_forms"] if c["name"] == "line3_ppr_e0")
+    xp = np.array([frac(s) for s in ppr["fixed_point"]])
+    off, nb, _ = oracle.dense_neighbors(np.array(ppr["matrix"], float))
+    wrong = oracle.pagerank_run(off, nb, 0.85, 1000, 1e-13).scores  # uniform teleport+dangling
+    assert np.abs(wrong - xp).sum() > 1e-3
+    # dropping the dangling term: a chain with a self-loop at the dangling node instead
+    line = next(c for c in golden["closed_forms"] if c["name"] == "line3")
+    xl = np.array([frac(s) for s in line["fixed_point"]])
+    m2 = np.array(line["matrix"], float)
+    m2[2, 2] = 1.0
+    off, nb, _ = oracle.dense_neighbors(m2)
+    assert np.abs(oracle.pagerank_run(off, nb, 0.85, 1000, 1e-13).scores - xl).sum() > 1e-3
