@@ -63,7 +63,6 @@ gops_status validate_config(gops_context *ctx, const gops_pagerank_config *cfg) 
 
 void free_run(RunBuffers &rb) {
   hipFree(rb.xbar);
-  hipFree(rb.e);
   hipFree(rb.yd[0]);
   hipFree(rb.yd[1]);
   hipFree(rb.y64[0]);
@@ -73,8 +72,8 @@ void free_run(RunBuffers &rb) {
   hipFree(rb.out);
   hipFree(rb.partials);
   hipFree(rb.hub_partial);
-  hipFree(rb.hub_count);
   hipFree(rb.state);
+  hipFree(rb.sweep_log);
   rb = RunBuffers{};
 }
 
@@ -101,19 +100,17 @@ gops_status alloc_run(gops_context *ctx, gops_graph *g) {
   const size_t n = g->dev.n;
   rb.n = n;
   HIP_TRY(ctx, dalloc(&rb.xbar, n));
-  HIP_TRY(ctx, dalloc(&rb.e, n));
   HIP_TRY(ctx, dalloc(&rb.yd[0], n));
   HIP_TRY(ctx, dalloc(&rb.yd[1], n));
   HIP_TRY(ctx, dalloc(&rb.y64[0], n));
   HIP_TRY(ctx, dalloc(&rb.y64[1], n));
   HIP_TRY(ctx, dalloc(&rb.out, n));
-  rb.partial_slots = std::max<size_t>((size_t)g->dev.num_desc, 4096);
-  HIP_TRY(ctx, dalloc(&rb.partials, rb.partial_slots * 4));
+  // sweep grid <= kMaxSweepBlocks, elementwise grids <= 2048 workgroups
+  rb.partial_slots = std::max<size_t>(kMaxSweepBlocks, 2048);
+  HIP_TRY(ctx, dalloc(&rb.partials, rb.partial_slots * kMaxReduce));
   HIP_TRY(ctx, dalloc(&rb.hub_partial, std::max(g->dev.num_hub_chunks, 1)));
-  HIP_TRY(ctx, dalloc(&rb.hub_count, std::max(g->dev.num_hubs, 1)));
-  HIP_TRY(ctx, hipMemsetAsync(rb.hub_count, 0, sizeof(uint32_t) * std::max(g->dev.num_hubs, 1),
-                              ctx->stream));
   HIP_TRY(ctx, dalloc(&rb.state, 1));
+  HIP_TRY(ctx, dalloc(&rb.sweep_log, kSweepLogCap));
   HIP_TRY(ctx, hipMemsetAsync(rb.yd[0], 0, n * sizeof(float), ctx->stream));
   HIP_TRY(ctx, hipMemsetAsync(rb.yd[1], 0, n * sizeof(float), ctx->stream));
   return GOPS_OK;
@@ -263,46 +260,42 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   hs.inv_n = 1.0 / (double)n;
   hs.max_iter = cfg->max_iterations;
   hs.mirror = ctx->mirror_dev;
+  hs.all_f64 = ctx->precision == GOPS_PRECISION_FP64;
+  hs.mode = kModeAnchor;  // the first sweep gathers x0 in f64
+  hs.anchor_period = (uint64_t)std::max(2, ctx->anchor_period);
+  hs.anchor_drop = ctx->anchor_drop;
+  hs.x_scale = 1.0;
+  hs.sweep_log = rb.sweep_log;
+  hs.sweep_log_cap = kSweepLogCap;
   __atomic_store_n(ctx->mirror_host, (uint64_t)0, __ATOMIC_RELEASE);
   ctx->host_state[0] = hs;
   HIP_TRY(ctx, hipMemcpyAsync(rb.state, &ctx->host_state[0], sizeof(RunState),
                               hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
-  const bool fp64 = ctx->precision == GOPS_PRECISION_FP64;
-  const uint64_t R = (uint64_t)std::max(1, ctx->anchor_period);
   const uint64_t max_it = cfg->max_iterations;
   const bool timing = ctx->timing != 0;
-  const size_t sweep_events = timing ? 2 * (size_t)std::min<uint64_t>(max_it, 100000) : 0;
-  st = ensure_events(ctx, 2 + sweep_events);
+  const uint64_t timed_cap = timing ? std::min<uint64_t>(max_it, kSweepLogCap) : 0;
+  st = ensure_events(ctx, 2 + 2 * (size_t)timed_cap);
   if (st != GOPS_OK) return st;
   hipEvent_t ev_begin = ctx->events[0], ev_end = ctx->events[1];
-  std::vector<uint8_t> sweep_kind;  // 0 f32, 1 f64 (per enqueued sweep)
 
   HIP_TRY(ctx, hipEventRecord(ev_begin, ctx->stream));
   HIP_TRY(ctx, launch_init(ctx, g, setup));
 
   // The device decides when to stop (RunState::done; later sweeps return at
-  // once).  The host keeps a few sweeps queued beyond the last one it has
-  // seen finish, reading the progress word the last workgroup of every sweep
-  // writes to mapped host memory, so no sweep waits on the host and at most
-  // kAhead launches are wasted after convergence.
+  // once) and which kind each sweep is.  The host keeps a few identical
+  // launches queued beyond the last sweep it has seen finish, reading the
+  // progress word the last workgroup of every sweep writes to mapped host
+  // memory, so no sweep waits on the host and at most kAhead launches are
+  // wasted after convergence.
   const uint64_t kAhead = 4;
   uint64_t j = 0, seen = 0;
   while (j < max_it) {
     for (; j < max_it && j < seen + kAhead; j++) {
-      const bool anchor = fp64 || (j % R == 0);
-      if (anchor && !fp64 && j > 0) HIP_TRY(ctx, launch_fold(ctx, g, setup));
-      if (timing && j < 100000) HIP_TRY(ctx, hipEventRecord(ctx->events[2 + 2 * j], ctx->stream));
-      if (fp64) {
-        HIP_TRY(ctx, launch_sweep_f64(ctx, g, setup, (int)(j & 1), 0, true, false));
-      } else if (anchor) {
-        HIP_TRY(ctx, launch_sweep_f64(ctx, g, setup, 0, (int)((j + 1) & 1), false, true));
-      } else {
-        HIP_TRY(ctx, launch_sweep_f32(ctx, g, setup, (int)(j & 1)));
-      }
-      if (timing && j < 100000) HIP_TRY(ctx, hipEventRecord(ctx->events[3 + 2 * j], ctx->stream));
-      sweep_kind.push_back(anchor ? 1 : 0);
+      if (j < timed_cap) HIP_TRY(ctx, hipEventRecord(ctx->events[2 + 2 * j], ctx->stream));
+      HIP_TRY(ctx, launch_sweep(ctx, g, setup));
+      if (j < timed_cap) HIP_TRY(ctx, hipEventRecord(ctx->events[3 + 2 * j], ctx->stream));
     }
     if (j >= max_it) break;
     bool done = false;
@@ -338,7 +331,6 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
                               hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   RunState last = ctx->host_state[0];
-  // the finalize kernel reuses dsum_x for the output sum; iteration fields are intact
   if (max_it == 0) {  // unchecked max_iterations == 0 (pagerank.rs:139-141, 182-186)
     info->iterations = 0;
     info->diff_l1 = std::numeric_limits<double>::infinity();
@@ -351,13 +343,18 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   float ms = 0.0f;
   HIP_TRY(ctx, hipEventElapsedTime(&ms, ev_begin, ev_end));
   ctx->stats.device_ms = ms;
-  for (uint64_t k = 0; k < last.iter && k < sweep_kind.size(); k++) {
-    if (sweep_kind[k]) ctx->stats.sweeps_fp64++;
+  const uint64_t logged = std::min<uint64_t>(last.iter, kSweepLogCap);
+  std::vector<uint8_t> modes(logged);
+  if (logged)
+    HIP_TRY(ctx, hipMemcpy(modes.data(), rb.sweep_log, logged, hipMemcpyDeviceToHost));
+  for (uint64_t k = 0; k < logged; k++) {
+    const bool f64 = modes[k] == kModeAnchor;
+    if (f64) ctx->stats.sweeps_fp64++;
     else ctx->stats.sweeps_fp32++;
-    if (timing && k < 100000) {
+    if (k < timed_cap) {
       float t = 0.0f;
       HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->events[2 + 2 * k], ctx->events[3 + 2 * k]));
-      if (sweep_kind[k]) ctx->stats.fp64_sweep_ms += t;
+      if (f64) ctx->stats.fp64_sweep_ms += t;
       else ctx->stats.fp32_sweep_ms += t;
     }
   }
@@ -436,6 +433,10 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
       return GOPS_OK;
     case GOPS_OPT_TIMING:
       ctx->timing = value != 0;
+      return GOPS_OK;
+    case GOPS_OPT_ANCHOR_DROP:
+      if (!(value > 0.0 && value < 1.0)) return fail(ctx, GOPS_INVALID_PARAMETER, "anchor drop");
+      ctx->anchor_drop = value;
       return GOPS_OK;
     case GOPS_OPT_PPR_TILE:
       if (!(value == 0 || value == 8 || value == 16 || value == 32 || value == 64))

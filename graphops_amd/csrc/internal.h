@@ -25,24 +25,40 @@ struct BlockDesc {
   int32_t x, y, z, w;
 };
 
+// Sweep modes, chosen on the device by the last workgroup of the previous
+// sweep (see pagerank.hip).
+enum : int32_t { kModeDelta = 0, kModeDeltaFold = 1, kModeAnchor = 2 };
+
 // Scalars a sweep chain keeps in device memory.
 struct RunState {
-  double dsum_in;   // dangling sum of the vector the next sweep gathers
-  double dsum_x;    // dangling sum of the current iterate (for f64 sweeps)
-  double diff;      // L1 residual of the last sweep
+  double dsum_in;      // dangling sum of the delta the next f32 sweep gathers
+  double mass_in;      // sum of that delta (0 in exact arithmetic; removed by the sweep)
+  double dsum_x;       // dangling sum of the iterate the next anchor sweep gathers
+  double diff;         // L1 residual of the last sweep
   double damping;
   double tolerance;
-  double inv_n;     // 1.0 / n
-  double n_f64;     // n as f64
+  double inv_n;        // 1.0 / n
+  double n_f64;        // n as f64
+  double diff_anchor;  // residual of the last anchor sweep
+  double anchor_drop;  // re-anchor once diff <= anchor_drop * diff_anchor
+  double x_scale;      // the next anchor gathers x = x_scale * xbar
   uint64_t iter;
   uint64_t max_iter;
+  uint64_t since_anchor;
+  uint64_t anchor_period;  // re-anchor at the latest after this many sweeps
   int32_t done;
   int32_t converged;
+  int32_t mode;        // mode of the next sweep
+  int32_t all_f64;     // every sweep an f64 anchor sweep
+  int32_t yd_cur;      // yd[yd_cur] holds the delta the next sweep gathers
+  int32_t y64_cur;     // y64[y64_cur] holds the iterate the next anchor gathers
   uint32_t ticket;
   uint32_t pad;
   // host-mapped progress word written after every sweep:
   // (iterations << 2) | (converged << 1) | done
   uint64_t *mirror;
+  uint8_t *sweep_log;  // mode of every sweep (capacity sweep_log_cap)
+  uint64_t sweep_log_cap;
 };
 
 // Host copy of the preprocessed graph (new ids = rows sorted by in-degree,
@@ -91,8 +107,7 @@ struct DevGraph {
 // Per-graph scratch of a single-vector run (allocated on first use).
 struct RunBuffers {
   uint64_t n = 0;
-  double *xbar = nullptr;        // f64 anchor iterate
-  float *e = nullptr;            // f32 accumulated deltas since the anchor
+  double *xbar = nullptr;        // the iterate x (f64)
   float *yd[2] = {nullptr, nullptr};   // prescaled deltas d*delta/deg (f32)
   double *y64[2] = {nullptr, nullptr}; // prescaled iterate d*x/deg (f64)
   float *q32 = nullptr;          // personalization (new ids), f32
@@ -100,10 +115,14 @@ struct RunBuffers {
   double *out = nullptr;         // result in original ids
   double *partials = nullptr;    // per-workgroup partial sums
   double *hub_partial = nullptr; // per hub chunk partial sums
-  uint32_t *hub_count = nullptr; // per hub arrival counters
   RunState *state = nullptr;
+  uint8_t *sweep_log = nullptr;
   size_t partial_slots = 0;
 };
+
+constexpr uint64_t kSweepLogCap = 1 << 16;
+constexpr size_t kMaxReduce = 8;  // partial sums per workgroup, at most
+constexpr int kMaxSweepBlocks = 2048;  // grid of the sweep kernel (grid-stride over descriptors)
 
 }  // namespace gops
 
@@ -123,6 +142,7 @@ struct gops_context {
   std::string last_error;
   int precision = GOPS_PRECISION_MIXED;
   int anchor_period = 32;
+  double anchor_drop = 1e-2;
   int normalize = 0;
   int timing = 0;
   int ppr_tile = 0;
@@ -165,10 +185,7 @@ struct SweepSetup {
   int personalized;       // q vector instead of uniform
 };
 hipError_t launch_init(gops_context *ctx, gops_graph *g, const SweepSetup &s);
-hipError_t launch_fold(gops_context *ctx, gops_graph *g, const SweepSetup &s);
-hipError_t launch_sweep_f64(gops_context *ctx, gops_graph *g, const SweepSetup &s, int y64_in,
-                            int yd_out, bool write_y64, bool write_yd);
-hipError_t launch_sweep_f32(gops_context *ctx, gops_graph *g, const SweepSetup &s, int parity);
+hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize);
 
 }  // namespace gops

@@ -4,21 +4,21 @@
 // written as a pull over the CSR of in-edges (rows renumbered by in-degree).
 // Each sweep is one pass over HBM that fuses: the gather-multiply-reduce of
 // the row, the teleport and dangling-redistribution terms, the L1 residual
-// (f64) and the dangling sum the next sweep needs (f64).  The grid-wide f64
-// sums are made by the last workgroup to finish, in a fixed order, so the
-// results do not depend on scheduling.
+// (f64) and the dangling sums the next sweep needs (f64).
 //
-// Two sweep kinds (mixed precision, "delta" scheme):
-//   f64 anchor sweep:  t = fill + A x  in f64 (x = anchor iterate xbar),
-//                      delta = t - xbar, xbar <- t.
-//   f32 delta sweep:   delta' = A delta in f32 (A = d * P^T plus dangling
-//                      redistribution, a linear map), e += delta'.
-// The iterate is x = xbar + e and delta is exactly x_{k+1} - x_k, so the L1
-// residual sum |delta| tracks the f64 iteration's residual without the f32
-// rounding floor of storing x itself in f32.  A periodic f64 anchor sweep
-// (after folding e into xbar) recomputes delta from the iterate and bounds
-// the drift of the f32 chain.
+// No workgroup of a sweep waits for, or synchronises with, another: on gfx950
+// a device-scope fence writes back and invalidates the XCD's L2, which would
+// evict the gathered vector.  Each workgroup writes its partial sums with plain
+// stores; a one-workgroup finish kernel then combines the chunks of split hub
+// rows and reduces the partials in a fixed order (so results do not depend on
+// scheduling) and decides what the next sweep does.  The kernel boundary is
+// the only synchronisation.
+//
+// Mixed precision uses a "delta" scheme (sweeps over x_{k+1} - x_k that gather
+// it as f32, with periodic f64 anchor sweeps); see the sweep-chain section.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "internal.h"
 
@@ -29,6 +29,7 @@ struct Bf16W {};
 
 struct SweepArgs {
   const BlockDesc *desc;
+  int num_desc;
   const int32_t *row_ptr;
   const int32_t *col;
   const void *w;
@@ -38,18 +39,19 @@ struct SweepArgs {
   float *yout32;
   const double *yin64;
   double *yout64;
-  float *e;
   double *xbar;
   const float *q32;
   const double *q64;
-  double *partials;
-  double *hub_partial;
-  uint32_t *hub_count;
+  double *partials;     // kReduce per workgroup of the sweep kernel
+  double *hub_partial;  // one per hub chunk
   const int32_t *hub_row;
   const int32_t *hub_nchunks;
   const int32_t *hub_first;
+  int num_hubs;
   RunState *st;
 };
+
+constexpr int kReduce = 5;  // |delta|, dangling delta, dangling x, delta, x
 
 template <typename WT>
 __device__ __forceinline__ float wload32(const void *w, int k) {
@@ -85,10 +87,6 @@ __device__ __forceinline__ int dpp_i(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
 }
 template <int CTRL>
-__device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
-}
-template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
   const int lo = dpp_i<CTRL>(__double2loint(v));
   const int hi = dpp_i<CTRL>(__double2hiint(v));
@@ -97,20 +95,11 @@ __device__ __forceinline__ double dpp_d(double v) {
 // ds_swizzle bit mode: and_mask 0x1f, or_mask 0, xor_mask 0x10 -> lane ^ 16 within 32
 constexpr int kSwzXor16 = 0x1F | (0x10 << 10);
 
-__device__ __forceinline__ float group_sum(float v, int lg) {
-  if (lg >= 1) v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
-  if (lg >= 2) v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
-  if (lg >= 3) v += dpp_f<0x141>(v);  // row_half_mirror
-  if (lg >= 4) v += dpp_f<0x140>(v);  // row_mirror
-  if (lg >= 5) v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), kSwzXor16));
-  if (lg >= 6) v += __shfl_xor(v, 32);
-  return v;
-}
 __device__ __forceinline__ double group_sum(double v, int lg) {
-  if (lg >= 1) v += dpp_d<0xB1>(v);
-  if (lg >= 2) v += dpp_d<0x4E>(v);
-  if (lg >= 3) v += dpp_d<0x141>(v);
-  if (lg >= 4) v += dpp_d<0x140>(v);
+  if (lg >= 1) v += dpp_d<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (lg >= 2) v += dpp_d<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (lg >= 3) v += dpp_d<0x141>(v);  // row_half_mirror
+  if (lg >= 4) v += dpp_d<0x140>(v);  // row_mirror
   if (lg >= 5) {
     const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), kSwzXor16);
     const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), kSwzXor16);
@@ -120,19 +109,14 @@ __device__ __forceinline__ double group_sum(double v, int lg) {
   return v;
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-  return group_sum(v, 6);
-}
-
 // Sum of `v` over the workgroup, valid in thread 0.  Fixed order.
-template <typename T, int N>
-__device__ __forceinline__ void block_sum(T (&v)[N]) {
-  __shared__ T sh[N][kBlock / 64];
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N]) {
+  __shared__ double sh[N][kBlock / 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int i = 0; i < N; i++) {
-    T s = wave_sum(v[i]);
+    const double s = group_sum(v[i], 6);
     if (lane == 0) sh[i][wave] = s;
   }
   __syncthreads();
@@ -143,46 +127,63 @@ __device__ __forceinline__ void block_sum(T (&v)[N]) {
   __syncthreads();
 }
 
-// Writes this workgroup's partial sums; the last workgroup to arrive reduces
-// all of them (fixed order) into out[0..N) and returns true in thread 0.
-template <int N>
-__device__ bool grid_reduce(double (&v)[N], double *partials, uint32_t *ticket,
-                            double (&out)[N]) {
-  block_sum<double, N>(v);
-  __shared__ int last;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < N; i++) partials[(size_t)blockIdx.x * N + i] = v[i];
-    __threadfence();
-    const uint32_t t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double s[N];
-#pragma unroll
-  for (int i = 0; i < N; i++) s[i] = 0.0;
-  for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-#pragma unroll
-    for (int i = 0; i < N; i++) s[i] += __builtin_nontemporal_load(partials + (size_t)b * N + i);
-  }
-  block_sum<double, N>(s);
-  if (threadIdx.x == 0) {
-    *ticket = 0;
-#pragma unroll
-    for (int i = 0; i < N; i++) out[i] = s[i];
-    return true;
-  }
-  return false;
-}
+// ---------------------------------------------------------------------------
+// The sweep chain.  Every launch runs the same two kernels; the mode of the
+// sweep is chosen on the device by the finish kernel of the previous sweep
+// (RunState::mode):
+//   ANCHOR      f64: t = fill + sum w * y64[u] (y64 = d*x*inv of the current
+//               iterate), delta = t - x, x = t, yd = d*delta*inv stored in f32
+//   DELTA       delta' = sum w * yd[u] + d*dsum(delta)*q, f32 operands summed
+//               in f64; x += delta' (f64), yd' = d*delta'*inv stored in f32
+//   DELTA_FOLD  DELTA, and also y64 = d*x*inv and the sums of x for the
+//               ANCHOR sweep that follows.
+// delta is x_{k+1} - x_k, so sum |delta| is the reference's L1 residual
+// (pagerank.rs:164-173), and the only f32 rounding is the storage of the
+// gathered delta (4 bytes per node instead of 8).  A DELTA_FOLD + ANCHOR pair
+// is scheduled once the residual has dropped by anchor_drop since the last
+// anchor (or after anchor_period sweeps): the anchor recomputes delta from the
+// iterate in f64, so the f32 storage rounding never propagates over more than
+// one such segment.
 
 // Bookkeeping of pagerank.rs:163-179: iterations counts the sweep, diff_l1 is
-// its residual, stop on diff < tolerance (strict) or at the iteration cap.
-__device__ __forceinline__ void finish_sweep(RunState *st, double diff) {
+// its residual, stop on diff < tolerance (strict) or at the iteration cap;
+// then the choice of the next sweep's mode and buffers.
+__device__ void finish_sweep(RunState *st, int mode, const double (&s)[kReduce]) {
+  st->mass_in = s[3];
+  const double diff = s[0];
   const uint64_t it = st->iter + 1;
-  st->iter = it;
-  st->diff = diff;
+  int next;
+  if (mode == kModeAnchor) {
+    st->dsum_in = s[1];
+    st->dsum_x = s[2];
+    st->diff_anchor = diff;
+    st->since_anchor = 0;
+    st->x_scale = 1.0;
+    if (st->all_f64) {
+      st->y64_cur ^= 1;
+      next = kModeAnchor;
+    } else {
+      next = kModeDelta;
+    }
+  } else if (mode == kModeDeltaFold) {
+    st->dsum_in = s[1];
+    st->dsum_x = s[2];
+    // the folded iterate sums to 1 in exact arithmetic (pagerank.rs:133 starts
+    // at 1/n per node and every sweep conserves the total); the anchor rescales
+    // it by 1/sum so that rounding leaves no error along the stationary
+    // direction, which the iteration itself would remove only at rate d
+    st->x_scale = 1.0 / s[4];
+    next = kModeAnchor;
+  } else {
+    st->dsum_in = s[1];
+    const uint64_t since = st->since_anchor + 1;
+    st->since_anchor = since;
+    next = (diff <= st->anchor_drop * st->diff_anchor || since + 1 >= st->anchor_period)
+               ? kModeDeltaFold
+               : kModeDelta;
+  }
+  st->yd_cur ^= 1;
+  st->mode = next;
   int conv = 0, done = 0;
   if (diff < st->tolerance) {
     conv = 1;
@@ -190,130 +191,108 @@ __device__ __forceinline__ void finish_sweep(RunState *st, double diff) {
   } else if (it >= st->max_iter) {
     done = 1;
   }
+  st->iter = it;
+  st->diff = diff;
   st->converged = conv;
   st->done = done;
+  if (it - 1 < st->sweep_log_cap) st->sweep_log[it - 1] = (uint8_t)mode;
   if (st->mirror) {
     __hip_atomic_store(st->mirror, (it << 2) | (uint64_t)(conv << 1) | (uint64_t)done,
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __threadfence_system();
   }
 }
 
-// ---------------------------------------------------------------------------
-// f32 delta sweep:  delta'_v = sum_{u->v} w_uv * yin_u + d * dsum(delta) * q_v
-//                   e_v += delta'_v;  yout_v = d * delta'_v * inv_v
-template <typename WT>
-__global__ __launch_bounds__(kBlock) void sweep_f32_kernel(SweepArgs a) {
-  RunState *st = a.st;
-  if (st->done) return;
-  const float df = (float)st->damping;
-  const double ddsum = st->damping * st->dsum_in;
-  const float uni = (float)(ddsum * st->inv_n);
-  const float ddsum_f = (float)ddsum;
-  const BlockDesc bd = a.desc[blockIdx.x];
-  double red[2] = {0.0, 0.0};  // |delta| sum, dangling delta sum
-
-  auto epilogue = [&](int r, float acc) {
-    const float delta = acc + (a.q32 ? ddsum_f * a.q32[r] : uni);
+// Per-row epilogue of an f32 delta sweep.
+struct DeltaEpilogue {
+  const SweepArgs &a;
+  double d, ddsum, uni;
+  bool fold;
+  __device__ DeltaEpilogue(const SweepArgs &a_, const RunState *st, bool fold_) : a(a_), fold(fold_) {
+    d = st->damping;
+    // d * (dangling sum - total sum) of the input delta: the total is 0 in
+    // exact arithmetic; removing it keeps the f32 storage rounding from
+    // building up along the stationary direction (decaying only at rate d)
+    ddsum = d * (st->dsum_in - st->mass_in);
+    uni = ddsum * st->inv_n;
+  }
+  __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kReduce]) const {
+    const double delta = acc + (a.q32 ? ddsum * (double)a.q32[r] : uni);
     const float inv = __builtin_nontemporal_load(a.inv32 + r);
-    const float e_old = __builtin_nontemporal_load(a.e + r);
-    __builtin_nontemporal_store(e_old + delta, a.e + r);
-    a.yout32[r] = (df * delta) * inv;
-    red[0] += fabs((double)delta);
-    if (inv == 0.0f) red[1] += (double)delta;
-  };
-
-  if ((bd.x & kHubFlag) == 0) {
-    const int lg = bd.x;
-    const int G = 1 << lg;
-    const int lane = threadIdx.x & (G - 1);
-    const int grp = threadIdx.x >> lg;
-    const int ngrp = kBlock >> lg;
-    for (int base = bd.y; base < bd.z; base += ngrp) {
-      const int r = base + grp;
-      float acc = 0.0f;
-      if (r < bd.z) {
-        const int b = __builtin_nontemporal_load(a.row_ptr + r);
-        const int e = __builtin_nontemporal_load(a.row_ptr + r + 1);
-        for (int k = b + lane; k < e; k += G) {
-          const int c = __builtin_nontemporal_load(a.col + k);
-          acc += wload32<WT>(a.w, k) * a.yin32[c];
-        }
-      }
-      acc = group_sum(acc, lg);
-      if (r < bd.z && lane == 0) epilogue(r, acc);
-    }
-  } else {
-    const int h = bd.y;
-    const int chunk = bd.x & (kHubFlag - 1);
-    float acc = 0.0f;
-    for (int k = bd.z + (int)threadIdx.x; k < bd.w; k += kBlock) {
-      const int c = __builtin_nontemporal_load(a.col + k);
-      acc += wload32<WT>(a.w, k) * a.yin32[c];
-    }
-    float tot[1] = {acc};
-    block_sum<float, 1>(tot);
-    if (threadIdx.x == 0) {
-      const int first = a.hub_first[h], nch = a.hub_nchunks[h];
-      a.hub_partial[first + chunk] = (double)tot[0];
-      __threadfence();
-      const uint32_t prev = atomicAdd(a.hub_count + h, 1u);
-      if (prev == (uint32_t)(nch - 1)) {
-        __threadfence();
-        double s = 0.0;
-        for (int c = 0; c < nch; c++) s += __builtin_nontemporal_load(a.hub_partial + first + c);
-        a.hub_count[h] = 0;
-        epilogue(a.hub_row[h], (float)s);
-      }
+    const double x = __builtin_nontemporal_load(a.xbar + r) + delta;
+    __builtin_nontemporal_store(x, a.xbar + r);
+    a.yout32[r] = (float)(d * delta * (double)inv);
+    red[0] += fabs(delta);
+    red[3] += delta;
+    if (inv == 0.0f) red[1] += delta;
+    if (fold) {
+      const double inv64 = a.inv64[r];
+      a.yout64[r] = (d * x) * inv64;
+      if (inv64 == 0.0) red[2] += x;
+      red[4] += x;
     }
   }
+};
 
-  double out[2];
-  if (grid_reduce<2>(red, a.partials, &st->ticket, out)) {
-    st->dsum_in = out[1];
-    finish_sweep(st, out[0]);
+// Per-row epilogue of an f64 anchor sweep: x = sc * xbar is the iterate
+// (sc = 1 except after a mixed-precision fold).
+struct AnchorEpilogue {
+  const SweepArgs &a;
+  double d, sc, fill_u, ddsum;
+  bool write_y64;
+  __device__ AnchorEpilogue(const SweepArgs &a_, const RunState *st) : a(a_) {
+    d = st->damping;
+    sc = st->x_scale;
+    const double nf = st->n_f64;
+    const double dsum = st->dsum_x * sc;
+    fill_u = (1.0 - d) / nf + d * dsum / nf;  // pagerank.rs:150-152
+    ddsum = d * dsum;
+    write_y64 = st->all_f64 != 0;
   }
-}
-
-// ---------------------------------------------------------------------------
-// f64 anchor sweep:  t_v = fill_v + sum_{u->v} w_uv * yin64_u
-//   fill_v = (1-d)/n + (d*dsum)/n (pagerank.rs:150-152) or
-//            (1-d)*q_v + (d*dsum)*q_v (ppr.rs:126-127)
-//   delta = t - xbar_v; xbar_v = t; optional f32 yout = d*delta*inv (for a
-//   following delta sweep) and f64 y64out = d*t*inv (for a following anchor).
-template <typename WT, bool WRITE_YD, bool WRITE_Y64>
-__global__ __launch_bounds__(kBlock) void sweep_f64_kernel(SweepArgs a) {
-  RunState *st = a.st;
-  if (st->done) return;
-  const double d = st->damping;
-  const double nf = st->n_f64;
-  const double fill_u = (1.0 - d) / nf + d * st->dsum_x / nf;
-  const double ddsum = d * st->dsum_x;
-  const BlockDesc bd = a.desc[blockIdx.x];
-  double red[3] = {0.0, 0.0, 0.0};  // |delta|, dangling delta, dangling t
-
-  auto epilogue = [&](int r, double acc) {
-    double fill;
-    if (a.q64) {
+  __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kReduce]) const {
+    double fill = fill_u;
+    if (a.q64) {  // ppr.rs:126-127
       const double q = a.q64[r];
       fill = (1.0 - d) * q + ddsum * q;
-    } else {
-      fill = fill_u;
     }
-    const double t = acc + fill;
-    const double xo = a.xbar[r];
-    const double delta = t - xo;
+    const double t = sc * acc + fill;
+    const double delta = t - sc * a.xbar[r];
     a.xbar[r] = t;
     const double inv = a.inv64[r];
-    if (WRITE_YD) a.yout32[r] = (float)((d * delta) * inv);
-    if (WRITE_Y64) a.yout64[r] = (d * t) * inv;
+    a.yout32[r] = (float)((d * delta) * inv);
+    if (write_y64) a.yout64[r] = (d * t) * inv;
     red[0] += fabs(delta);
+    red[3] += delta;
     if (inv == 0.0) {
       red[1] += delta;
       red[2] += t;
     }
-  };
+  }
+};
 
+// f32 sweep: f32 operands; their product is exact in f64 and summed in f64
+template <typename WT>
+struct EdgeF32 {
+  const SweepArgs &a;
+  __device__ __forceinline__ double operator()(int k) const {
+    const int c = __builtin_nontemporal_load(a.col + k);
+    return (double)wload32<WT>(a.w, k) * (double)a.yin32[c];
+  }
+};
+
+template <typename WT>
+struct EdgeF64 {
+  const SweepArgs &a;
+  __device__ __forceinline__ double operator()(int k) const {
+    const int c = __builtin_nontemporal_load(a.col + k);
+    return wload64<WT>(a.w, k) * a.yin64[c];
+  }
+};
+
+// One work descriptor: a run of rows (2^lg lanes per row), or one chunk of a
+// hub row whose sum goes to hub_partial for the finish kernel.
+template <typename LOAD, typename EPI>
+__device__ __forceinline__ void do_desc(const SweepArgs &a, const BlockDesc &bd, const LOAD &load,
+                                        const EPI &epi, double (&red)[kReduce]) {
   if ((bd.x & kHubFlag) == 0) {
     const int lg = bd.x;
     const int G = 1 << lg;
@@ -324,88 +303,132 @@ __global__ __launch_bounds__(kBlock) void sweep_f64_kernel(SweepArgs a) {
       const int r = base + grp;
       double acc = 0.0;
       if (r < bd.z) {
-        const int b = a.row_ptr[r], e = a.row_ptr[r + 1];
-        for (int k = b + lane; k < e; k += G) {
-          const int c = __builtin_nontemporal_load(a.col + k);
-          acc += wload64<WT>(a.w, k) * a.yin64[c];
-        }
+        const int b = __builtin_nontemporal_load(a.row_ptr + r);
+        const int e = __builtin_nontemporal_load(a.row_ptr + r + 1);
+        for (int k = b + lane; k < e; k += G) acc += load(k);
       }
       acc = group_sum(acc, lg);
-      if (r < bd.z && lane == 0) epilogue(r, acc);
+      if (r < bd.z && lane == 0) epi(r, acc, red);
     }
   } else {
     const int h = bd.y;
     const int chunk = bd.x & (kHubFlag - 1);
-    double acc = 0.0;
-    for (int k = bd.z + (int)threadIdx.x; k < bd.w; k += kBlock) {
-      const int c = __builtin_nontemporal_load(a.col + k);
-      acc += wload64<WT>(a.w, k) * a.yin64[c];
-    }
-    double tot[1] = {acc};
-    block_sum<double, 1>(tot);
-    if (threadIdx.x == 0) {
-      const int first = a.hub_first[h], nch = a.hub_nchunks[h];
-      a.hub_partial[first + chunk] = tot[0];
-      __threadfence();
-      const uint32_t prev = atomicAdd(a.hub_count + h, 1u);
-      if (prev == (uint32_t)(nch - 1)) {
-        __threadfence();
-        double s = 0.0;
-        for (int c = 0; c < nch; c++) s += __builtin_nontemporal_load(a.hub_partial + first + c);
-        a.hub_count[h] = 0;
-        epilogue(a.hub_row[h], s);
-      }
-    }
-  }
-
-  double out[3];
-  if (grid_reduce<3>(red, a.partials, &st->ticket, out)) {
-    st->dsum_in = out[1];
-    st->dsum_x = out[2];
-    finish_sweep(st, out[0]);
+    double acc[1] = {0.0};
+    for (int k = bd.z + (int)threadIdx.x; k < bd.w; k += kBlock) acc[0] += load(k);
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) a.hub_partial[a.hub_first[h] + chunk] = acc[0];
   }
 }
 
-// ---------------------------------------------------------------------------
-// init (x0 = 1/n, pagerank.rs:133, or the normalised personalization,
-// ppr.rs:104-110) and fold (xbar += e, e = 0) both end with
-// y64 = d*x*inv and the dangling sum of x.
-template <bool INIT>
-__global__ __launch_bounds__(kBlock) void fold_kernel(SweepArgs a, int n) {
+__device__ __forceinline__ void pick_buffers(SweepArgs &a, const RunState *st, float *yd0,
+                                             float *yd1, double *y640, double *y641) {
+  const int ycur = st->yd_cur;
+  a.yin32 = ycur ? yd1 : yd0;
+  a.yout32 = ycur ? yd0 : yd1;
+  const int c64 = st->y64_cur;
+  if (st->mode == kModeAnchor) {
+    a.yin64 = c64 ? y641 : y640;
+    a.yout64 = c64 ? y640 : y641;
+  } else {
+    a.yout64 = c64 ? y641 : y640;  // the fold writes the iterate the anchor will gather
+  }
+}
+
+// The sweep: a fixed grid strides over the work descriptors.  Buffers are
+// picked from the device state, so every launch of a chain has identical
+// arguments.
+template <typename WT>
+__global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, float *yd1,
+                                                       double *y640, double *y641) {
+  const RunState *st = a.st;
+  if (st->done) return;
+  pick_buffers(a, st, yd0, yd1, y640, y641);
+  double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (st->mode == kModeAnchor) {
+    const AnchorEpilogue epi(a, st);
+    const EdgeF64<WT> load{a};
+    for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x) do_desc(a, a.desc[i], load, epi, red);
+  } else {
+    const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
+    const EdgeF32<WT> load{a};
+    for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x) do_desc(a, a.desc[i], load, epi, red);
+  }
+  block_sum<kReduce>(red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kReduce; i++) a.partials[(size_t)blockIdx.x * kReduce + i] = red[i];
+  }
+}
+
+// One workgroup: hub-row epilogues (chunk partials summed in chunk order),
+// the fixed-order reduction of the sweep's partials, the bookkeeping.
+__global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float *yd0, float *yd1,
+                                                              double *y640, double *y641,
+                                                              int sweep_blocks) {
   RunState *st = a.st;
-  if (!INIT && st->done) return;
+  if (st->done) return;
+  pick_buffers(a, st, yd0, yd1, y640, y641);
+  const int mode = st->mode;
+  double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  auto hubs = [&](const auto &epi) {
+    for (int h = threadIdx.x; h < a.num_hubs; h += kBlock) {
+      const int first = a.hub_first[h], nch = a.hub_nchunks[h];
+      double s = 0.0;
+      for (int c = 0; c < nch; c++) s += a.hub_partial[first + c];
+      epi(a.hub_row[h], s, red);
+    }
+  };
+  if (mode == kModeAnchor) {
+    hubs(AnchorEpilogue(a, st));
+  } else {
+    hubs(DeltaEpilogue(a, st, mode == kModeDeltaFold));
+  }
+  for (int b = threadIdx.x; b < sweep_blocks; b += kBlock) {
+#pragma unroll
+    for (int i = 0; i < kReduce; i++) red[i] += a.partials[(size_t)b * kReduce + i];
+  }
+  block_sum<kReduce>(red);
+  if (threadIdx.x == 0) finish_sweep(st, mode, red);
+}
+
+// ---------------------------------------------------------------------------
+// init: x0 = 1/n (pagerank.rs:133) or the normalised personalization
+// (ppr.rs:104-110); x = x0, y64[0] = d*x0*inv; partial dangling sums of x0.
+__global__ __launch_bounds__(kBlock) void init_kernel(SweepArgs a, int n) {
+  const RunState *st = a.st;
   const double d = st->damping;
   double red[1] = {0.0};
   for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
-    double x;
-    if (INIT) {
-      x = a.q64 ? a.q64[v] : st->inv_n;
-    } else {
-      x = a.xbar[v] + (double)a.e[v];
-    }
+    const double x = a.q64 ? a.q64[v] : st->inv_n;
     a.xbar[v] = x;
-    a.e[v] = 0.0f;
     const double inv = a.inv64[v];
     a.yout64[v] = (d * x) * inv;
     if (inv == 0.0) red[0] += x;
   }
-  double out[1];
-  if (grid_reduce<1>(red, a.partials, &st->ticket, out)) st->dsum_x = out[0];
+  block_sum<1>(red);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = red[0];
 }
 
-// out[o] = xbar[rank[o]] + e[rank[o]] in the caller's ids; the last workgroup
-// leaves the sum in st->dsum_x for the optional normalisation.
+// out[o] = x[rank[o]] in the caller's ids; partial sums of x.
 __global__ __launch_bounds__(kBlock) void finalize_kernel(SweepArgs a, const uint32_t *rank,
                                                           double *out, int n) {
   double red[1] = {0.0};
   for (int o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock) {
-    const uint32_t r = rank[o];
-    const double x = a.xbar[r] + (double)a.e[r];
+    const double x = a.xbar[rank[o]];
     out[o] = x;
     red[0] += x;
   }
-  double s[1];
-  if (grid_reduce<1>(red, a.partials, &a.st->ticket, s)) a.st->dsum_x = s[0];
+  block_sum<1>(red);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = red[0];
+}
+
+// One workgroup: *dst = sum of partials[0..count) in a fixed order.
+__global__ __launch_bounds__(kBlock) void sum_partials_kernel(const double *partials, int count,
+                                                              double *dst) {
+  double red[1] = {0.0};
+  for (int b = threadIdx.x; b < count; b += kBlock) red[0] += partials[b];
+  block_sum<1>(red);
+  if (threadIdx.x == 0) *dst = red[0];
 }
 
 // reference normalize() semantics (topk.rs:54-61): divide by the sum if > 0
@@ -421,96 +444,62 @@ static SweepArgs make_args(gops_graph *g, const SweepSetup &s) {
   DevGraph &dg = g->dev;
   RunBuffers &rb = g->run;
   a.desc = dg.desc;
+  a.num_desc = dg.num_desc;
   a.row_ptr = dg.row_ptr;
   a.col = dg.col;
   a.w = s.weighted ? dg.w : nullptr;
   a.inv32 = s.weighted ? dg.inv_ws32 : dg.inv_deg32;
   a.inv64 = s.weighted ? dg.inv_ws64 : dg.inv_deg64;
-  a.e = rb.e;
   a.xbar = rb.xbar;
   a.q32 = s.personalized ? rb.q32 : nullptr;
   a.q64 = s.personalized ? rb.q64 : nullptr;
   a.partials = rb.partials;
   a.hub_partial = rb.hub_partial;
-  a.hub_count = rb.hub_count;
   a.hub_row = dg.hub_row;
   a.hub_nchunks = dg.hub_nchunks;
   a.hub_first = dg.hub_first;
+  a.num_hubs = dg.num_hubs;
   a.st = rb.state;
   return a;
 }
 
 static int elementwise_grid(uint64_t n) {
-  uint64_t blocks = (n + kBlock * 4 - 1) / (kBlock * 4);
+  uint64_t blocks = (n + kBlock * 8 - 1) / (kBlock * 8);
   if (blocks < 1) blocks = 1;
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > 2048) blocks = 2048;
   return (int)blocks;
 }
+
+int sweep_grid(const gops_graph *g) { return std::min(g->dev.num_desc, kMaxSweepBlocks); }
 
 hipError_t launch_init(gops_context *ctx, gops_graph *g, const SweepSetup &s) {
   SweepArgs a = make_args(g, s);
   a.yout64 = g->run.y64[0];
   const int n = (int)g->dev.n;
-  fold_kernel<true><<<elementwise_grid(n), kBlock, 0, ctx->stream>>>(a, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_fold(gops_context *ctx, gops_graph *g, const SweepSetup &s) {
-  SweepArgs a = make_args(g, s);
-  a.yout64 = g->run.y64[0];
-  const int n = (int)g->dev.n;
-  fold_kernel<false><<<elementwise_grid(n), kBlock, 0, ctx->stream>>>(a, n);
+  const int grid = elementwise_grid(n);
+  init_kernel<<<grid, kBlock, 0, ctx->stream>>>(a, n);
+  sum_partials_kernel<<<1, kBlock, 0, ctx->stream>>>(a.partials, grid, &a.st->dsum_x);
   return hipGetLastError();
 }
 
 template <typename WT>
-static void launch_f32_wt(gops_context *ctx, const SweepArgs &a, int blocks) {
-  sweep_f32_kernel<WT><<<blocks, kBlock, 0, ctx->stream>>>(a);
+static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
+  RunBuffers &rb = g->run;
+  const int grid = sweep_grid(g);
+  sweep_kernel<WT><<<grid, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1]);
+  sweep_finish_kernel<<<1, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1],
+                                                     grid);
 }
 
-hipError_t launch_sweep_f32(gops_context *ctx, gops_graph *g, const SweepSetup &s, int parity) {
+hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s) {
   SweepArgs a = make_args(g, s);
-  a.yin32 = g->run.yd[parity];
-  a.yout32 = g->run.yd[parity ^ 1];
-  const int blocks = g->dev.num_desc;
   if (!s.weighted) {
-    launch_f32_wt<NoW>(ctx, a, blocks);
+    launch_wt<NoW>(ctx, g, a);
   } else {
     switch (g->dev.weight_storage) {
-      case GOPS_WEIGHT_F32: launch_f32_wt<float>(ctx, a, blocks); break;
-      case GOPS_WEIGHT_BF16: launch_f32_wt<Bf16W>(ctx, a, blocks); break;
-      default: launch_f32_wt<double>(ctx, a, blocks); break;
-    }
-  }
-  return hipGetLastError();
-}
-
-template <typename WT>
-static void launch_f64_wt(gops_context *ctx, const SweepArgs &a, int blocks, bool yd, bool y64) {
-  if (yd && y64)
-    sweep_f64_kernel<WT, true, true><<<blocks, kBlock, 0, ctx->stream>>>(a);
-  else if (yd)
-    sweep_f64_kernel<WT, true, false><<<blocks, kBlock, 0, ctx->stream>>>(a);
-  else
-    sweep_f64_kernel<WT, false, true><<<blocks, kBlock, 0, ctx->stream>>>(a);
-}
-
-// gathers y64[y64_in]; writes y64[y64_in ^ 1] (write_y64) and yd[yd_out]
-// (write_yd).
-hipError_t launch_sweep_f64(gops_context *ctx, gops_graph *g, const SweepSetup &s, int y64_in,
-                            int yd_out, bool write_y64, bool write_yd) {
-  SweepArgs a = make_args(g, s);
-  a.yin64 = g->run.y64[y64_in];
-  a.yout64 = g->run.y64[y64_in ^ 1];
-  a.yout32 = g->run.yd[yd_out];
-  const int blocks = g->dev.num_desc;
-  if (!s.weighted) {
-    launch_f64_wt<NoW>(ctx, a, blocks, write_yd, write_y64);
-  } else {
-    switch (g->dev.weight_storage) {
-      case GOPS_WEIGHT_F32: launch_f64_wt<float>(ctx, a, blocks, write_yd, write_y64); break;
-      case GOPS_WEIGHT_BF16: launch_f64_wt<Bf16W>(ctx, a, blocks, write_yd, write_y64); break;
-      default: launch_f64_wt<double>(ctx, a, blocks, write_yd, write_y64); break;
+      case GOPS_WEIGHT_F32: launch_wt<float>(ctx, g, a); break;
+      case GOPS_WEIGHT_BF16: launch_wt<Bf16W>(ctx, g, a); break;
+      default: launch_wt<double>(ctx, g, a); break;
     }
   }
   return hipGetLastError();
@@ -520,9 +509,10 @@ hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize) {
   SweepSetup s{0, 0};
   SweepArgs a = make_args(g, s);
   const int n = (int)g->dev.n;
-  finalize_kernel<<<elementwise_grid(n), kBlock, 0, ctx->stream>>>(a, g->dev.rank, g->run.out, n);
-  if (normalize)
-    scale_kernel<<<elementwise_grid(n), kBlock, 0, ctx->stream>>>(g->run.state, g->run.out, n);
+  const int grid = elementwise_grid(n);
+  finalize_kernel<<<grid, kBlock, 0, ctx->stream>>>(a, g->dev.rank, g->run.out, n);
+  sum_partials_kernel<<<1, kBlock, 0, ctx->stream>>>(a.partials, grid, &a.st->dsum_x);
+  if (normalize) scale_kernel<<<grid, kBlock, 0, ctx->stream>>>(g->run.state, g->run.out, n);
   return hipGetLastError();
 }
 

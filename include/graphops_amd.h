@@ -78,11 +78,14 @@ enum {
  * the reference is f64 throughout. */
 enum {
     GOPS_OPT_PRECISION = 1,     /* GOPS_PRECISION_MIXED (default) or _FP64 */
-    GOPS_OPT_ANCHOR_PERIOD = 2, /* sweeps between f64 anchor sweeps (mixed), default 32 */
+    GOPS_OPT_ANCHOR_PERIOD = 2, /* mixed: at most this many sweeps between f64 anchor
+                                   sweeps (default 32) */
     GOPS_OPT_NORMALIZE = 3,     /* 1: rescale the result to sum to 1 in f64 (default 0,
                                    the reference never renormalises, lib.rs:11-12) */
     GOPS_OPT_TIMING = 4,        /* 1: time every sweep kernel with HIP events */
-    GOPS_OPT_PPR_TILE = 5       /* batched PPR query tile width (0 = autotuned) */
+    GOPS_OPT_PPR_TILE = 5,      /* batched PPR query tile width (0 = autotuned) */
+    GOPS_OPT_ANCHOR_DROP = 6    /* mixed: re-anchor in f64 once the residual has dropped
+                                   by this factor since the last anchor (default 1e-2) */
 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
 
@@ -95,7 +98,7 @@ typedef struct gops_run_stats {
     double device_ms;       /* first sweep launch .. result ready, HIP events */
     double fp32_sweep_ms;   /* summed duration of f32 sweep kernels (GOPS_OPT_TIMING) */
     double fp64_sweep_ms;   /* summed duration of f64 sweep kernels (GOPS_OPT_TIMING) */
-    double other_ms;        /* init / fold / finalize kernels (GOPS_OPT_TIMING) */
+    double other_ms;        /* reserved */
 } gops_run_stats;
 
 /* ---- context ---------------------------------------------------------- */
