@@ -15,7 +15,9 @@ Adapters (src/graph.rs):
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -30,7 +32,7 @@ __all__ = [
     "pagerank_weighted", "pagerank_weighted_run", "pagerank_weighted_checked",
     "pagerank_weighted_checked_run", "personalized_pagerank", "personalized_pagerank_run",
     "personalized_pagerank_checked", "personalized_pagerank_checked_run", "ppr_batch_run",
-    "top_k", "normalize",
+    "top_k", "normalize", "Server",
 ]
 
 
@@ -80,6 +82,22 @@ def _check(rc, ctx=None, what=""):
     if ctx is not None:
         msg = (L.lib().gops_last_error(ctx) or b"").decode()
     raise _ERRORS.get(rc, GraphopsError)(msg or f"{what} failed with status {rc}")
+
+
+# Objects holding device resources.  At interpreter exit they are closed in
+# dependency order (servers, graphs, then contexts): finalizers alone may run
+# in any order then, and a graph or server must not outlive its context.
+_LIVE = {"server": weakref.WeakSet(), "graph": weakref.WeakSet(), "context": weakref.WeakSet()}
+
+
+@atexit.register
+def _close_all():
+    for kind in ("server", "graph", "context"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
 
 
 # ---------------------------------------------------------------- config/run
@@ -132,6 +150,7 @@ class Context:
                               "a HIP-capable GPU is required")
         self._h = h
         self.device = device
+        _LIVE["context"].add(self)
         self.set_precision(precision)
         self.set_option(L.OPT_ANCHOR_PERIOD, anchor_period)
         self.set_option(L.OPT_NORMALIZE, int(bool(normalize)))
@@ -263,6 +282,7 @@ class DeviceGraph:
         self._h = handle
         self.ctx = ctx
         self.weighted = weighted
+        _LIVE["graph"].add(self)
         n = ctypes.c_uint64()
         m = ctypes.c_uint64()
         L.lib().gops_graph_info(handle, ctypes.byref(n), ctypes.byref(m))
@@ -449,13 +469,15 @@ def personalized_pagerank_checked(graph, config: PageRankConfig, personalization
 
 
 def ppr_batch_run(graph, configs, seed_offsets, seeds, seed_weights=None, ctx: Context = None,
-                  with_scores=True):
+                  with_scores=True, tile=None):
     """Many personalized_pagerank_run calls (ppr.rs:90-156) in one device run.
     Query q's personalization is 1.0 (or seed_weights[j]) on
     seeds[seed_offsets[q]:seed_offsets[q+1]] and 0 elsewhere.  Returns
     (scores[num_queries, n] or None, list of PageRankRun without scores)."""
     dg = _device_graph(graph, False, ctx)
     c = dg.ctx.handle
+    if tile is not None:  # queries per tile; -1 = one query per device run
+        dg.ctx.set_option(L.OPT_PPR_TILE, int(tile))
     q = len(seed_offsets) - 1
     if isinstance(configs, PageRankConfig):
         configs = [configs] * q
@@ -468,6 +490,8 @@ def ppr_batch_run(graph, configs, seed_offsets, seeds, seed_weights=None, ctx: C
     rc = L.lib().gops_ppr_batch_run(c, dg.handle, q, cfgs, L.ptr(off, L._u64p),
                                     L.ptr(sd, L._u32p), L.ptr(sw, L._f64p),
                                     L.ptr(scores, L._f64p), infos)
+    if tile is not None:
+        dg.ctx.set_option(L.OPT_PPR_TILE, 0)
     _check(rc, c, "gops_ppr_batch_run")
     runs = [PageRankRun(None, int(infos[i].iterations), float(infos[i].diff_l1),
                         bool(infos[i].converged)) for i in range(q)]
@@ -492,3 +516,80 @@ def normalize(scores):
     if total > 0.0:
         scores /= total
     return scores
+
+
+# ---------------------------------------------------------------- serving
+class Server:
+    """Continuous-batching personalized PageRank server over a resident graph
+    (gops_server_*).  Queries wait in a host queue and take a device slot as
+    soon as one frees up; step() advances every tile with live queries."""
+
+    def __init__(self, graph: DeviceGraph, tile: int = 16, max_slots: int = 0,
+                 hbm_budget_bytes: float = 0.0, topk: int = 0):
+        self.graph = graph
+        self.ctx = graph.ctx
+        cfg = L.ServerConfigC(int(max_slots), float(hbm_budget_bytes), int(tile), int(topk))
+        h = ctypes.c_void_p()
+        rc = L.lib().gops_server_create(self.ctx.handle, graph.handle, ctypes.byref(cfg),
+                                        ctypes.byref(h))
+        _check(rc, self.ctx.handle, "gops_server_create")
+        self._h = h
+        self.topk = int(topk)
+        _LIVE["server"].add(self)
+
+    def submit(self, config: PageRankConfig, seeds, seed_weights=None) -> int:
+        sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+        sw = None if seed_weights is None else np.ascontiguousarray(seed_weights, dtype=np.float64)
+        t = ctypes.c_uint64()
+        cfg = config._c()
+        rc = L.lib().gops_server_submit(self._h, ctypes.byref(cfg), L.ptr(sd, L._u32p), sd.size,
+                                        L.ptr(sw, L._f64p), ctypes.byref(t))
+        _check(rc, self.ctx.handle, "gops_server_submit")
+        return int(t.value)
+
+    def step(self, sweeps: int = 4) -> int:
+        act = ctypes.c_uint32()
+        _check(L.lib().gops_server_step(self._h, int(sweeps), ctypes.byref(act)), self.ctx.handle,
+               "gops_server_step")
+        return int(act.value)
+
+    def poll(self, with_scores=True):
+        """One finished query or None: (ticket, PageRunRun, submit_ms,
+        finish_ms, topk) -- topk is a list of (node, score) if enabled."""
+        ready = ctypes.c_int()
+        res = L.ServerResultC()
+        scores = np.zeros(self.graph.n, dtype=np.float64) if with_scores else None
+        ids = np.zeros(max(self.topk, 1), dtype=np.uint32) if self.topk else None
+        tsc = np.zeros(max(self.topk, 1), dtype=np.float64) if self.topk else None
+        rc = L.lib().gops_server_poll(self._h, ctypes.byref(ready), ctypes.byref(res),
+                                      L.ptr(scores, L._f64p), L.ptr(ids, L._u32p),
+                                      L.ptr(tsc, L._f64p))
+        _check(rc, self.ctx.handle, "gops_server_poll")
+        if not ready.value:
+            return None
+        run = PageRankRun(scores, int(res.info.iterations), float(res.info.diff_l1),
+                          bool(res.info.converged))
+        top = None
+        if self.topk:
+            top = [(int(i), float(s)) for i, s in zip(ids, tsc) if i != 0xFFFFFFFF]
+        return int(res.ticket), run, float(res.submit_ms), float(res.finish_ms), top
+
+    def stats(self) -> dict:
+        q = ctypes.c_uint64()
+        a = ctypes.c_uint64()
+        f = ctypes.c_uint64()
+        s = ctypes.c_uint32()
+        _check(L.lib().gops_server_stats(self._h, ctypes.byref(q), ctypes.byref(a),
+                                         ctypes.byref(f), ctypes.byref(s)), self.ctx.handle, "stats")
+        return {"queued": q.value, "active": a.value, "finished": f.value, "slots": s.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().gops_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -27,6 +27,7 @@ GOPS_NOT_SUPPORTED = 6
 WEIGHT_NONE, WEIGHT_F64, WEIGHT_F32, WEIGHT_BF16 = 0, 1, 2, 3
 OPT_PRECISION, OPT_ANCHOR_PERIOD, OPT_NORMALIZE, OPT_TIMING, OPT_PPR_TILE = 1, 2, 3, 4, 5
 OPT_ANCHOR_DROP = 6
+OPT_F32_BELOW, OPT_PPR_F32_BELOW, OPT_PPR_ANCHOR_PERIOD = 7, 8, 9
 PRECISION_MIXED, PRECISION_FP64 = 0, 1
 GEN_UNIT, GEN_UNIFORM, GEN_LOGNORMAL = 0, 1, 2
 ROUND_F64, ROUND_F32, ROUND_BF16 = 0, 1, 2

@@ -262,8 +262,18 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   hs.mirror = ctx->mirror_dev;
   hs.all_f64 = ctx->precision == GOPS_PRECISION_FP64;
   hs.mode = kModeAnchor;  // the first sweep gathers x0 in f64
-  hs.anchor_period = (uint64_t)std::max(2, ctx->anchor_period);
-  hs.anchor_drop = ctx->anchor_drop;
+  // Mixed-precision schedule (see pagerank.hip).  Global PageRank: f32 delta
+  // sweeps from the second sweep, re-anchored in f64 every anchor_drop of
+  // residual.  Personalized PageRank: f64 sweeps while the residual is above
+  // f32_below * tolerance, then an f32 delta chain re-anchored only every
+  // anchor_period sweeps (its localized residual decays faster than the f32
+  // errors an anchor would feed back from the iterate).
+  const bool ppr = kind == kPersonalized;
+  hs.f32_below = ppr ? ctx->ppr_f32_below : ctx->f32_below;
+  hs.anchor_drop = ppr ? 0.0 : ctx->anchor_drop;
+  hs.anchor_period = (uint64_t)std::max(2, ppr ? ctx->ppr_anchor_period : ctx->anchor_period);
+  hs.anchor_min = ppr ? ctx->ppr_anchor_min : 0.0;
+  hs.f64_phase = hs.all_f64 || hs.f32_below > 0.0;
   hs.x_scale = 1.0;
   hs.sweep_log = rb.sweep_log;
   hs.sweep_log_cap = kSweepLogCap;
@@ -438,9 +448,18 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
       if (!(value > 0.0 && value < 1.0)) return fail(ctx, GOPS_INVALID_PARAMETER, "anchor drop");
       ctx->anchor_drop = value;
       return GOPS_OK;
+    case GOPS_OPT_F32_BELOW:
+    case GOPS_OPT_PPR_F32_BELOW:
+      if (!(value >= 0.0)) return fail(ctx, GOPS_INVALID_PARAMETER, "f32 threshold must be >= 0");
+      (option == GOPS_OPT_F32_BELOW ? ctx->f32_below : ctx->ppr_f32_below) = value;
+      return GOPS_OK;
+    case GOPS_OPT_PPR_ANCHOR_PERIOD:
+      if (!(value >= 2 && value <= 1e6)) return fail(ctx, GOPS_INVALID_PARAMETER, "anchor period");
+      ctx->ppr_anchor_period = (int)value;
+      return GOPS_OK;
     case GOPS_OPT_PPR_TILE:
-      if (!(value == 0 || value == 8 || value == 16 || value == 32 || value == 64))
-        return fail(ctx, GOPS_INVALID_PARAMETER, "ppr tile must be 0, 8, 16, 32 or 64");
+      if (!(value == -1 || value == 0 || value == 8 || value == 16 || value == 32 || value == 64))
+        return fail(ctx, GOPS_INVALID_PARAMETER, "ppr tile must be -1, 0, 8, 16, 32 or 64");
       ctx->ppr_tile = (int)value;
       return GOPS_OK;
     default:
@@ -532,6 +551,7 @@ void gops_graph_destroy(gops_graph *g) {
     hipSetDevice(g->ctx->device);
     hipStreamSynchronize(g->ctx->stream);
   }
+  free_ppr_cache(g);
   free_run(g->run);
   free_dev(g->dev);
   delete g;

@@ -182,7 +182,12 @@ void plan_blocks(const HostCSRT &g, Plan &p) {
   int32_t chunks = 0;
   while (i < n) {
     const int32_t deg = rp[i + 1] - rp[i];
-    if (deg > kHubThreshold) {
+    if (deg > kStageNnz && deg <= kHubChunk) {
+      p.desc.push_back({kRowFlag, (int32_t)i, 0, 0});
+      i++;
+      continue;
+    }
+    if (deg > kHubChunk) {
       const int32_t h = (int32_t)p.hub_row.size();
       const int32_t nch = (deg + kHubChunk - 1) / kHubChunk;
       p.hub_row.push_back((int32_t)i);
@@ -204,8 +209,8 @@ void plan_blocks(const HostCSRT &g, Plan &p) {
     int64_t nnz = 0;
     while (i < n && i - s < max_rows) {
       const int32_t d = rp[i + 1] - rp[i];
-      if (d > kHubThreshold || lanes_for(d) != G) break;
-      if (i - s >= groups && nnz + d > kRowsNnzTarget) break;
+      if (d > kStageNnz || lanes_for(d) != G) break;
+      if (nnz + d > kStageNnz) break;  // the run's edges must fit the LDS stage
       nnz += d;
       i++;
     }

@@ -12,15 +12,19 @@
 namespace gops {
 
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64s)
-constexpr int kHubThreshold = 4096;  // in-degree above which a row is split over workgroups
-constexpr int kHubChunk = 4096;      // edges per hub chunk (16 per thread)
-constexpr int kRowsNnzTarget = 8192; // edges per row block
-constexpr int kRowIters = 8;         // row iterations per group in one block
+constexpr int kStageNnz = 4096;      // edges of a run of rows staged in LDS (f32: 16 KB)
+constexpr int kHubChunk = 65536;     // rows longer than this are split into chunks of it
+constexpr int kRowIters = 8;         // row iterations per group in one run
+constexpr int kUnroll = 8;           // independent edge loads in flight per lane
 
-// Work descriptor of one workgroup in a sweep.
-//   ROWS: x = log2(lanes per row), y = first row, z = end row
-//   HUB : x = kHubFlag | chunk index, y = hub id, z = first edge, w = end edge
+// Work descriptor of one workgroup in a sweep (rows sorted by in-degree):
+//   ROWS : x = log2(lanes per row), y = first row, z = end row; the run has
+//          at most kStageNnz edges and rows of at most kStageNnz edges
+//   ROW  : x = kRowFlag, y = row; one row of (kStageNnz, kHubChunk] edges
+//   HUB  : x = kHubFlag | chunk index, y = hub id, z = first edge, w = end
+//          edge; one chunk of a row longer than kHubChunk
 constexpr int32_t kHubFlag = 1 << 28;
+constexpr int32_t kRowFlag = 1 << 27;
 struct BlockDesc {
   int32_t x, y, z, w;
 };
@@ -42,6 +46,10 @@ struct RunState {
   double diff_anchor;  // residual of the last anchor sweep
   double anchor_drop;  // re-anchor once diff <= anchor_drop * diff_anchor
   double x_scale;      // the next anchor gathers x = x_scale * xbar
+  double f32_below;    // f32 sweeps only once diff <= f32_below * tolerance (0: at once)
+  double anchor_min;   // periodic anchors only while diff > anchor_min * tolerance
+  int32_t f64_phase;   // still in the leading all-f64 phase
+  int32_t pad1;
   uint64_t iter;
   uint64_t max_iter;
   uint64_t since_anchor;
@@ -126,8 +134,13 @@ constexpr int kMaxSweepBlocks = 2048;  // grid of the sweep kernel (grid-stride 
 
 }  // namespace gops
 
+namespace gops {
+struct PprGraphCache;
+}
+
 struct gops_graph {
   gops_context *ctx = nullptr;
+  gops::PprGraphCache *ppr_cache = nullptr;  // batched PPR plans (server.cpp)
   gops::DevGraph dev;
   gops::RunBuffers run;
   bool weighted = false;   // created with edge weights
@@ -143,6 +156,10 @@ struct gops_context {
   int precision = GOPS_PRECISION_MIXED;
   int anchor_period = 32;
   double anchor_drop = 1e-2;
+  double f32_below = 0.0;          // global PageRank: f32 sweeps from the start
+  double ppr_f32_below = 1e4;      // personalized: f64 until diff <= 1e4 * tolerance
+  double ppr_anchor_min = 1e3;     // personalized: no anchor once diff <= 1e3 * tolerance
+  int ppr_anchor_period = 64;
   int normalize = 0;
   int timing = 0;
   int ppr_tile = 0;
@@ -178,6 +195,9 @@ gops_status ppr_batch_looped(gops_context *ctx, gops_graph *g, uint32_t nq,
                              const gops_pagerank_config *cfgs, const uint64_t *seed_off,
                              const uint32_t *seeds, const double *seed_w, double *scores,
                              gops_run_info *infos);
+
+// server.cpp
+void free_ppr_cache(gops_graph *g);
 
 // kernels (pagerank.hip)
 struct SweepSetup {

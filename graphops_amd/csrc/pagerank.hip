@@ -159,10 +159,11 @@ __device__ void finish_sweep(RunState *st, int mode, const double (&s)[kReduce])
     st->diff_anchor = diff;
     st->since_anchor = 0;
     st->x_scale = 1.0;
-    if (st->all_f64) {
-      st->y64_cur ^= 1;
+    if (st->f64_phase && (st->all_f64 || diff > st->f32_below * st->tolerance)) {
+      st->y64_cur ^= 1;  // this anchor wrote the next anchor's gathered iterate
       next = kModeAnchor;
     } else {
+      st->f64_phase = 0;
       next = kModeDelta;
     }
   } else if (mode == kModeDeltaFold) {
@@ -178,9 +179,9 @@ __device__ void finish_sweep(RunState *st, int mode, const double (&s)[kReduce])
     st->dsum_in = s[1];
     const uint64_t since = st->since_anchor + 1;
     st->since_anchor = since;
-    next = (diff <= st->anchor_drop * st->diff_anchor || since + 1 >= st->anchor_period)
-               ? kModeDeltaFold
-               : kModeDelta;
+    const bool anchor = diff <= st->anchor_drop * st->diff_anchor ||
+                        (since + 1 >= st->anchor_period && diff > st->anchor_min * st->tolerance);
+    next = anchor ? kModeDeltaFold : kModeDelta;
   }
   st->yd_cur ^= 1;
   st->mode = next;
@@ -246,7 +247,7 @@ struct AnchorEpilogue {
     const double dsum = st->dsum_x * sc;
     fill_u = (1.0 - d) / nf + d * dsum / nf;  // pagerank.rs:150-152
     ddsum = d * dsum;
-    write_y64 = st->all_f64 != 0;
+    write_y64 = st->f64_phase != 0;
   }
   __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kReduce]) const {
     double fill = fill_u;
@@ -270,9 +271,12 @@ struct AnchorEpilogue {
 };
 
 // f32 sweep: f32 operands; their product is exact in f64 and summed in f64
+// (runs of short rows stage it in LDS as f32)
 template <typename WT>
 struct EdgeF32 {
+  static constexpr bool kStaged = true;
   const SweepArgs &a;
+  __device__ __forceinline__ float weight(int k) const { return wload32<WT>(a.w, k); }
   __device__ __forceinline__ double operator()(int k) const {
     const int c = __builtin_nontemporal_load(a.col + k);
     return (double)wload32<WT>(a.w, k) * (double)a.yin32[c];
@@ -281,40 +285,106 @@ struct EdgeF32 {
 
 template <typename WT>
 struct EdgeF64 {
+  static constexpr bool kStaged = false;
   const SweepArgs &a;
+  __device__ __forceinline__ float weight(int k) const { return wload32<WT>(a.w, k); }
   __device__ __forceinline__ double operator()(int k) const {
     const int c = __builtin_nontemporal_load(a.col + k);
     return wload64<WT>(a.w, k) * a.yin64[c];
   }
 };
 
+// Sum over edges [e0, e1) by the whole workgroup, kUnroll independent
+// loads in flight per lane (out-of-range slots re-read edge e0 and count 0).
+template <typename LOAD>
+__device__ __forceinline__ double stream_sum(const LOAD &load, int e0, int e1) {
+  double acc = 0.0;
+  for (int k = e0 + (int)threadIdx.x; k < e1; k += kUnroll * kBlock) {
+    double v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int kk = k + u * kBlock;
+      const double x = load(kk < e1 ? kk : e0);
+      v[u] = kk < e1 ? x : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) acc += v[u];
+  }
+  return acc;
+}
+
 // One work descriptor: a run of rows (2^lg lanes per row), or one chunk of a
 // hub row whose sum goes to hub_partial for the finish kernel.
 template <typename LOAD, typename EPI>
 __device__ __forceinline__ void do_desc(const SweepArgs &a, const BlockDesc &bd, const LOAD &load,
                                         const EPI &epi, double (&red)[kReduce]) {
-  if ((bd.x & kHubFlag) == 0) {
+  if ((bd.x & (kHubFlag | kRowFlag)) == 0) {
     const int lg = bd.x;
     const int G = 1 << lg;
     const int lane = threadIdx.x & (G - 1);
     const int grp = threadIdx.x >> lg;
     const int ngrp = kBlock >> lg;
-    for (int base = bd.y; base < bd.z; base += ngrp) {
-      const int r = base + grp;
-      double acc = 0.0;
-      if (r < bd.z) {
-        const int b = __builtin_nontemporal_load(a.row_ptr + r);
-        const int e = __builtin_nontemporal_load(a.row_ptr + r + 1);
-        for (int k = b + lane; k < e; k += G) acc += load(k);
+    if constexpr (LOAD::kStaged) {
+      // CSR-stream: the run's edges (<= kStageNnz) are streamed coalesced,
+      // kUnroll independent loads in flight per lane, and their products
+      // staged in LDS; then each group of G lanes reduces one row from LDS.
+      __shared__ float prod[kStageNnz];
+      const int e0 = a.row_ptr[bd.y];
+      const int nnz = a.row_ptr[bd.z] - e0;
+      for (int i = threadIdx.x; i < nnz; i += kUnroll * kBlock) {
+        int c[kUnroll];
+        float wv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const int ii = i + u * kBlock;
+          const int k = e0 + (ii < nnz ? ii : 0);
+          c[u] = __builtin_nontemporal_load(a.col + k);
+          wv[u] = load.weight(k);
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const int ii = i + u * kBlock;
+          const float v = wv[u] * a.yin32[c[u]];
+          if (ii < nnz) prod[ii] = v;
+        }
       }
-      acc = group_sum(acc, lg);
-      if (r < bd.z && lane == 0) epi(r, acc, red);
+      __syncthreads();
+      for (int base = bd.y; base < bd.z; base += ngrp) {
+        const int r = base + grp;
+        double acc = 0.0;
+        if (r < bd.z) {
+          const int b = a.row_ptr[r] - e0;
+          const int e = a.row_ptr[r + 1] - e0;
+          for (int k = b + lane; k < e; k += G) acc += (double)prod[k];
+        }
+        acc = group_sum(acc, lg);
+        if (r < bd.z && lane == 0) epi(r, acc, red);
+      }
+      __syncthreads();
+    } else {
+      for (int base = bd.y; base < bd.z; base += ngrp) {
+        const int r = base + grp;
+        double acc = 0.0;
+        if (r < bd.z) {
+          const int b = __builtin_nontemporal_load(a.row_ptr + r);
+          const int e = __builtin_nontemporal_load(a.row_ptr + r + 1);
+          for (int k = b + lane; k < e; k += G) acc += load(k);
+        }
+        acc = group_sum(acc, lg);
+        if (r < bd.z && lane == 0) epi(r, acc, red);
+      }
     }
+  } else if (bd.x & kRowFlag) {
+    // one long row, the whole workgroup
+    const int r = bd.y;
+    double acc[1] = {stream_sum(load, a.row_ptr[r], a.row_ptr[r + 1])};
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) epi(r, acc[0], red);
   } else {
+    // one chunk of a split hub row; the finish kernel adds the chunks
     const int h = bd.y;
     const int chunk = bd.x & (kHubFlag - 1);
-    double acc[1] = {0.0};
-    for (int k = bd.z + (int)threadIdx.x; k < bd.w; k += kBlock) acc[0] += load(k);
+    double acc[1] = {stream_sum(load, bd.z, bd.w)};
     block_sum<1>(acc);
     if (threadIdx.x == 0) a.hub_partial[a.hub_first[h] + chunk] = acc[0];
   }

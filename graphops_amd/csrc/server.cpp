@@ -1,37 +1,779 @@
-// server.cpp -- batched personalized PageRank entry point and the serving loop.
+// server.cpp -- batched personalized PageRank (gops_ppr_batch_run) and the
+// serving loop (gops_server_*), both on resident tiles of T queries
+// (ppr_batch.hip).
+//
+// Serving: queries wait in a host queue; a tile slot is taken as soon as one
+// is free, between device steps.  Admission clears the slot's columns and
+// starts the new query with an f64 anchor sweep while the tile's running
+// queries continue, so new and running queries share every later sweep
+// (their work overlaps on the device).  Every query keeps its own sweep mode,
+// so its arithmetic does not depend on the other queries of its tile.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+#include <map>
+#include <new>
+
+
 #include "internal.h"
+#include "ppr_batch.h"
 
 using namespace gops;
 
+namespace {
+
+gops_status sfail(gops_context *ctx, gops_status st, const std::string &msg) {
+  if (ctx) ctx->last_error = msg;
+  return st;
+}
+
+#define STRY(ctx, expr)                                                                  \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return sfail(ctx,                                                                  \
+                   (_e == hipErrorOutOfMemory) ? GOPS_OUT_OF_MEMORY : GOPS_DEVICE_ERROR, \
+                   std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+  } while (0)
+
+inline int lanes_pow2(int want, int cap) {
+  int g = 1;
+  while (g < want && g < cap) g <<= 1;
+  return g;
+}
+
+// Work descriptors of the tile sweep for tile width T (rows already sorted
+// by in-degree): runs of rows with 2^s slices of T lanes per row, whole-block
+// rows, and the graph's hub chunks (rows longer than kHubChunk).
+void plan_tile(const std::vector<int32_t> &rp, int T, const DevGraph &dg,
+               const std::vector<int32_t> &hub_row, const std::vector<int32_t> &hub_nchunks,
+               std::vector<BlockDesc> &desc) {
+  desc.clear();
+  const int64_t n = (int64_t)rp.size() - 1;
+  const int smax = 64 / T;  // slices of a row within one wave
+  const int block_row_min = 16 * smax;
+  int h = 0;
+  int64_t i = 0;
+  while (i < n) {
+    const int32_t deg = rp[i + 1] - rp[i];
+    if (deg > kHubChunk) {
+      // graph plan: hub h is this row, same chunking
+      for (int32_t c = 0; c < hub_nchunks[h]; c++) {
+        const int32_t b = rp[i] + c * kHubChunk;
+        desc.push_back({kHubFlag | c, h, b, std::min(b + kHubChunk, rp[i + 1])});
+      }
+      (void)hub_row;
+      h++;
+      i++;
+      continue;
+    }
+    if (deg > block_row_min) {
+      desc.push_back({kRowFlag, (int32_t)i, 0, 0});
+      i++;
+      continue;
+    }
+    const int S = lanes_pow2((deg + 7) / 8, smax);
+    int ls = 0;
+    while ((1 << ls) < S) ls++;
+    const int L = T << ls;
+    const int64_t max_rows = (int64_t)(kBlock / L) * kRowIters;
+    const int64_t s = i;
+    int64_t nnz = 0;
+    while (i < n && i - s < max_rows) {
+      const int32_t d = rp[i + 1] - rp[i];
+      if (d > block_row_min || lanes_pow2((d + 7) / 8, smax) != S) break;
+      if (i > s && nnz + d > 8192) break;
+      nnz += d;
+      i++;
+    }
+    desc.push_back({ls, (int32_t)s, (int32_t)i, 0});
+  }
+  (void)dg;
+}
+
+struct Query {
+  uint64_t ticket = 0;
+  gops_pagerank_config cfg{};
+  std::vector<std::pair<uint32_t, double>> pers;  // (new id, normalised value)
+  double init_dsum = 0.0;
+  double submit_ms = 0.0;
+  bool dense_fallback = false;  // personalization sum <= 0 -> uniform (ppr.rs:105-109)
+};
+
+// One resident tile: device buffers for T queries.
+struct Tile {
+  int T = 16;
+  size_t n = 0;
+  double *x = nullptr, *y64a = nullptr, *y64b = nullptr, *partials = nullptr,
+         *hub_partial = nullptr;
+  float *yd0 = nullptr, *yd1 = nullptr;
+  int32_t *seed_idx = nullptr;
+  int32_t *seed_row = nullptr, *seed_q = nullptr;
+  double *seed_p = nullptr;
+  int seed_cap = 0;
+  int num_seed = 0;
+  std::vector<int32_t> marked_rows;  // rows currently marked in seed_idx
+  TileState *ts = nullptr;
+  TileState *hts = nullptr;  // pinned host copy
+  Query slot[kMaxTile];
+  bool used[kMaxTile] = {};
+  int grid = 0;
+};
+
+struct TilePlan {
+  int T = 0;
+  BlockDesc *desc = nullptr;
+  int num_desc = 0;
+};
+
+}  // namespace
+
+struct gops_server {
+  gops_context *ctx = nullptr;
+  gops_graph *g = nullptr;
+  int T = 16;
+  std::vector<Tile *> tiles;
+  std::deque<Query> queue;
+  struct Done {
+    Query q;
+    gops_run_info info;
+    double finish_ms;
+    double *scores;  // device copy of the result (n doubles), owned
+  };
+  std::deque<Done> finished;
+  uint64_t next_ticket = 1;
+  uint64_t n_finished = 0;
+  std::chrono::steady_clock::time_point t0;
+  uint32_t topk = 0;
+  double now_ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace gops {
+
+// Per-graph host data the batched path needs (built on first use).
+struct PprGraphCache {
+  std::vector<int32_t> row_ptr;
+  std::vector<uint8_t> dangling;  // new ids, out-degree 0
+  std::vector<int32_t> hub_row, hub_nchunks;
+  TilePlan plans[4];
+};
+
+}  // namespace gops
+
+namespace {
+
+gops_status graph_cache(gops_context *ctx, gops_graph *g, PprGraphCache **out) {
+  if (g->ppr_cache) {
+    *out = g->ppr_cache;
+    return GOPS_OK;
+  }
+  auto *c = new PprGraphCache;
+  const size_t n = g->dev.n;
+  c->row_ptr.resize(n + 1);
+  std::vector<double> inv(n);
+  hipError_t e = hipMemcpy(c->row_ptr.data(), g->dev.row_ptr, (n + 1) * sizeof(int32_t),
+                           hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(inv.data(), g->dev.inv_deg64, n * sizeof(double), hipMemcpyDeviceToHost);
+  c->dangling.resize(n);
+  for (size_t i = 0; i < n; i++) c->dangling[i] = inv[i] == 0.0;
+  c->hub_row.resize(g->dev.num_hubs);
+  c->hub_nchunks.resize(g->dev.num_hubs);
+  if (e == hipSuccess && g->dev.num_hubs)
+    e = hipMemcpy(c->hub_row.data(), g->dev.hub_row, g->dev.num_hubs * sizeof(int32_t),
+                  hipMemcpyDeviceToHost);
+  if (e == hipSuccess && g->dev.num_hubs)
+    e = hipMemcpy(c->hub_nchunks.data(), g->dev.hub_nchunks, g->dev.num_hubs * sizeof(int32_t),
+                  hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    delete c;
+    return sfail(ctx, GOPS_DEVICE_ERROR, std::string("graph cache: ") + hipGetErrorString(e));
+  }
+  g->ppr_cache = c;
+  *out = c;
+  return GOPS_OK;
+}
+
+int plan_slot(int T) { return T == 8 ? 0 : T == 16 ? 1 : T == 32 ? 2 : 3; }
+
+gops_status tile_plan(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, TilePlan **out) {
+  TilePlan &p = c->plans[plan_slot(T)];
+  if (!p.desc) {
+    std::vector<BlockDesc> desc;
+    plan_tile(c->row_ptr, T, g->dev, c->hub_row, c->hub_nchunks, desc);
+    STRY(ctx, hipMalloc((void **)&p.desc, std::max<size_t>(desc.size(), 1) * sizeof(BlockDesc)));
+    STRY(ctx, hipMemcpy(p.desc, desc.data(), desc.size() * sizeof(BlockDesc),
+                        hipMemcpyHostToDevice));
+    p.num_desc = (int)desc.size();
+    p.T = T;
+  }
+  *out = &p;
+  return GOPS_OK;
+}
+
+void free_tile(Tile *t) {
+  if (!t) return;
+  hipFree(t->x);
+  hipFree(t->y64a);
+  hipFree(t->y64b);
+  hipFree(t->partials);
+  hipFree(t->hub_partial);
+  hipFree(t->yd0);
+  hipFree(t->yd1);
+  hipFree(t->seed_idx);
+  hipFree(t->seed_row);
+  hipFree(t->seed_q);
+  hipFree(t->seed_p);
+  hipFree(t->ts);
+  hipHostFree(t->hts);
+  delete t;
+}
+
+size_t tile_bytes(size_t n, int T) { return n * (size_t)T * (8 + 8 + 8 + 4 + 4) + n * 4; }
+
+gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile **out) {
+  auto *t = new (std::nothrow) Tile;
+  if (!t) return sfail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
+  t->T = T;
+  t->n = g->dev.n;
+  t->grid = grid;
+  const size_t nt = t->n * (size_t)T;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t r) {
+    if (r != hipSuccess && e == hipSuccess) e = r;
+  };
+  chk(hipMalloc((void **)&t->x, std::max<size_t>(nt, 1) * 8));
+  chk(hipMalloc((void **)&t->y64a, std::max<size_t>(nt, 1) * 8));
+  chk(hipMalloc((void **)&t->y64b, std::max<size_t>(nt, 1) * 8));
+  chk(hipMalloc((void **)&t->yd0, std::max<size_t>(nt, 1) * 4));
+  chk(hipMalloc((void **)&t->yd1, std::max<size_t>(nt, 1) * 4));
+  chk(hipMalloc((void **)&t->partials, (size_t)std::max(grid, 2048) * T * 5 * 8));
+  chk(hipMalloc((void **)&t->hub_partial, (size_t)std::max(g->dev.num_hub_chunks, 1) * T * 8));
+  chk(hipMalloc((void **)&t->seed_idx, std::max<size_t>(t->n, 1) * 4));
+  chk(hipMalloc((void **)&t->ts, sizeof(TileState)));
+  chk(hipHostMalloc((void **)&t->hts, sizeof(TileState), hipHostMallocDefault));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->seed_idx, 0xff, std::max<size_t>(t->n, 1) * 4, ctx->stream));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->yd0, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->yd1, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
+  if (e == hipSuccess) chk(hipStreamSynchronize(ctx->stream));
+  if (e != hipSuccess) {
+    free_tile(t);
+    return sfail(ctx, e == hipErrorOutOfMemory ? GOPS_OUT_OF_MEMORY : GOPS_DEVICE_ERROR,
+                 std::string("tile allocation: ") + hipGetErrorString(e));
+  }
+  std::memset(t->hts, 0, sizeof(TileState));
+  t->hts->all_done = 1;
+  *out = t;
+  return GOPS_OK;
+}
+
+PprArgs tile_args(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, bool mirror) {
+  PprArgs a{};
+  a.ts = t->ts;
+  a.desc = p->desc;
+  a.num_desc = p->num_desc;
+  a.row_ptr = g->dev.row_ptr;
+  a.col = g->dev.col;
+  a.inv32 = g->dev.inv_deg32;
+  a.inv64 = g->dev.inv_deg64;
+  a.x = t->x;
+  a.yd0 = t->yd0;
+  a.yd1 = t->yd1;
+  a.y64a = t->y64a;
+  a.y64b = t->y64b;
+  a.partials = t->partials;
+  a.hub_partial = t->hub_partial;
+  a.hub_row = g->dev.hub_row;
+  a.hub_nchunks = g->dev.hub_nchunks;
+  a.hub_first = g->dev.hub_first;
+  a.num_hubs = g->dev.num_hubs;
+  a.seed_idx = t->seed_idx;
+  a.seed_row = t->seed_row;
+  a.seed_q = t->seed_q;
+  a.seed_p = t->seed_p;
+  a.num_seed_entries = t->num_seed;
+  a.mirror = mirror ? ctx->mirror_dev : nullptr;
+  return a;
+}
+
+// Normalised sparse personalization of one query (ppr.rs:104-109, summed in
+// node order like the reference's dense sum); duplicates merged.
+gops_status make_query(gops_context *ctx, gops_graph *g, PprGraphCache *c,
+                       const gops_pagerank_config *cfg, const uint32_t *seeds, uint64_t ns,
+                       const double *w, Query &q) {
+  const uint64_t n = g->dev.n;
+  std::map<uint32_t, double> acc;  // original id -> weight, node order
+  for (uint64_t j = 0; j < ns; j++) {
+    if (seeds[j] >= n)
+      return sfail(ctx, GOPS_INDEX_OUT_OF_BOUNDS, "index out of bounds: " + std::to_string(seeds[j]));
+    acc[seeds[j]] += w ? w[j] : 1.0;
+  }
+  double psum = 0.0;
+  for (auto &kv : acc) psum += kv.second;
+  q.cfg = *cfg;
+  q.pers.clear();
+  q.init_dsum = 0.0;
+  q.dense_fallback = !(psum > 0.0);
+  if (q.dense_fallback) return GOPS_OK;
+  for (auto &kv : acc) {
+    const uint32_t r = g->rank_host[kv.first];
+    const double v = kv.second / psum;
+    q.pers.push_back({r, v});
+    if (c->dangling[r]) q.init_dsum += v;
+  }
+  return GOPS_OK;
+}
+
+// Upload the seed entries of every used slot and re-mark seed_idx.
+gops_status upload_seeds(gops_context *ctx, Tile *t) {
+  struct E {
+    int32_t row, q;
+    double p;
+  };
+  std::vector<E> ent;
+  for (int s = 0; s < t->T; s++)
+    if (t->used[s])
+      for (auto &pv : t->slot[s].pers) ent.push_back({(int32_t)pv.first, s, pv.second});
+  std::sort(ent.begin(), ent.end(), [](const E &a, const E &b) {
+    return a.row != b.row ? a.row < b.row : a.q < b.q;
+  });
+  const int cnt = (int)ent.size();
+  if (cnt > t->seed_cap) {
+    hipFree(t->seed_row);
+    hipFree(t->seed_q);
+    hipFree(t->seed_p);
+    const int cap = std::max(cnt, 1024);
+    STRY(ctx, hipMalloc((void **)&t->seed_row, cap * 4));
+    STRY(ctx, hipMalloc((void **)&t->seed_q, cap * 4));
+    STRY(ctx, hipMalloc((void **)&t->seed_p, cap * 8));
+    t->seed_cap = cap;
+  }
+  // unmark the previous rows
+  if (!t->marked_rows.empty()) {
+    STRY(ctx, hipMemcpyAsync(t->seed_row, t->marked_rows.data(), t->marked_rows.size() * 4,
+                             hipMemcpyHostToDevice, ctx->stream));
+    STRY(ctx, ppr_launch_seed_mark(ctx->stream, t->seed_idx, t->seed_row,
+                                   (int)t->marked_rows.size(), 0));
+    STRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  std::vector<int32_t> rows(cnt), qs(cnt);
+  std::vector<double> ps(cnt);
+  for (int j = 0; j < cnt; j++) {
+    rows[j] = ent[j].row;
+    qs[j] = ent[j].q;
+    ps[j] = ent[j].p;
+  }
+  if (cnt) {
+    STRY(ctx, hipMemcpyAsync(t->seed_row, rows.data(), cnt * 4, hipMemcpyHostToDevice, ctx->stream));
+    STRY(ctx, hipMemcpyAsync(t->seed_q, qs.data(), cnt * 4, hipMemcpyHostToDevice, ctx->stream));
+    STRY(ctx, hipMemcpyAsync(t->seed_p, ps.data(), cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+    STRY(ctx, ppr_launch_seed_mark(ctx->stream, t->seed_idx, t->seed_row, cnt, 1));
+    STRY(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  t->num_seed = cnt;
+  t->marked_rows = rows;
+  return GOPS_OK;
+}
+
+void set_slot_params(Tile *t, int s, const Query &q, double f32_below, uint64_t period) {
+  TileState *h = t->hts;
+  h->damping[s] = q.cfg.damping;
+  h->tol[s] = q.cfg.tolerance;
+  h->max_iter[s] = q.cfg.max_iterations;
+  h->init_dsum[s] = q.init_dsum;
+  h->dsum_in[s] = 0.0;
+  h->mass_in[s] = 0.0;
+  h->diff_anchor[s] = 0.0;
+  h->anchor_drop = 0.0;  // personalized schedule (pagerank.hip): no residual-triggered anchors
+  h->f32_below = f32_below;
+  h->anchor_min = 1e3;
+  h->anchor_period = period;
+}
+
+// Admit the queries in `admit` (slot mask) into tile t: fold the live ones,
+// initialise the new ones; the next sweep is an anchor.
+gops_status admit(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, uint64_t mask) {
+  gops_status st = upload_seeds(ctx, t);
+  if (st != GOPS_OK) return st;
+  t->hts->all_done = 0;
+  STRY(ctx, hipMemcpyAsync(t->ts, t->hts, sizeof(TileState), hipMemcpyHostToDevice, ctx->stream));
+  PprArgs a = tile_args(ctx, g, t, p, false);
+  STRY(ctx, ppr_launch_admit(ctx->stream, t->T, a, (int)t->n, mask, mask));
+  return GOPS_OK;
+}
+
+gops_status sync_state(gops_context *ctx, Tile *t) {
+  STRY(ctx, hipMemcpyAsync(t->hts, t->ts, sizeof(TileState), hipMemcpyDeviceToHost, ctx->stream));
+  STRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GOPS_OK;
+}
+
+int pick_tile(gops_context *ctx, int nq) {
+  if (ctx->ppr_tile > 0) return ctx->ppr_tile;
+  return nq >= 32 ? 16 : (nq >= 8 ? 8 : 8);
+}
+
+int tile_grid(const gops_graph *g) {
+  (void)g;
+  return 1024;
+}
+
+}  // namespace
+
+namespace gops {
+void free_ppr_cache(gops_graph *g) {
+  PprGraphCache *c = g->ppr_cache;
+  if (!c) return;
+  for (TilePlan &p : c->plans) hipFree(p.desc);
+  delete c;
+  g->ppr_cache = nullptr;
+}
+}  // namespace gops
+
 extern "C" {
 
-gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t num_queries,
-                               const gops_pagerank_config *cfgs, const uint64_t *seed_offsets,
-                               const uint32_t *seeds, const double *seed_weights,
-                               double *scores, gops_run_info *infos) {
-  if (!ctx || !g || (num_queries && (!cfgs || !seed_offsets))) return GOPS_INVALID_PARAMETER;
-  return ppr_batch_looped(ctx, g, num_queries, cfgs, seed_offsets, seeds, seed_weights, scores,
-                          infos);
+gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
+                               const gops_pagerank_config *cfgs, const uint64_t *seed_off,
+                               const uint32_t *seeds, const double *seed_w, double *scores,
+                               gops_run_info *infos) {
+  if (!ctx || !g) return GOPS_INVALID_PARAMETER;
+  if (nq && (!cfgs || !seed_off)) return sfail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  if (g->ctx != ctx) return sfail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  ctx->last_error.clear();
+  if (ctx->ppr_tile < 0)  // plain path: one query per device run
+    return ppr_batch_looped(ctx, g, nq, cfgs, seed_off, seeds, seed_w, scores, infos);
+  const uint64_t n = g->dev.n;
+  if (nq == 0) return GOPS_OK;
+  if (n == 0) {
+    for (uint32_t q = 0; q < nq; q++)
+      if (infos) infos[q] = gops_run_info{0, 0.0, 1, 0};
+    return GOPS_OK;
+  }
+  PprGraphCache *c = nullptr;
+  gops_status st = graph_cache(ctx, g, &c);
+  if (st != GOPS_OK) return st;
+  const int T = pick_tile(ctx, (int)nq);
+  TilePlan *plan = nullptr;
+  st = tile_plan(ctx, g, c, T, &plan);
+  if (st != GOPS_OK) return st;
+  Tile *t = nullptr;
+  st = alloc_tile(ctx, g, T, tile_grid(g), &t);
+  if (st != GOPS_OK) return st;
+  double *dout = nullptr;  // one query's scores in the caller's ids
+  if (scores && hipMalloc((void **)&dout, n * sizeof(double)) != hipSuccess) {
+    free_tile(t);
+    return sfail(ctx, GOPS_OUT_OF_MEMORY, "batch output buffer: out of device memory");
+  }
+  std::vector<uint32_t> dense_q;  // queries that fall back to the dense uniform case
+  for (uint32_t q0 = 0; q0 < nq && st == GOPS_OK; q0 += T) {
+    const uint32_t q1 = std::min<uint32_t>(nq, q0 + T);
+    std::memset(t->hts, 0, sizeof(TileState));
+    uint64_t mask = 0, max_it = 0;
+    for (int s = 0; s < T; s++) t->used[s] = false;
+    for (uint32_t q = q0; q < q1; q++) {
+      const int s = (int)(q - q0);
+      Query qq;
+      st = make_query(ctx, g, c, &cfgs[q], seeds + seed_off[q], seed_off[q + 1] - seed_off[q],
+                      seed_w ? seed_w + seed_off[q] : nullptr, qq);
+      if (st != GOPS_OK) break;
+      if (qq.dense_fallback || cfgs[q].max_iterations == 0) {
+        dense_q.push_back(q);
+        continue;
+      }
+      t->slot[s] = qq;
+      t->used[s] = true;
+      set_slot_params(t, s, qq, ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
+      mask |= 1ull << s;
+      max_it = std::max<uint64_t>(max_it, cfgs[q].max_iterations);
+    }
+    if (st != GOPS_OK || !mask) continue;
+    st = admit(ctx, g, t, plan, mask);
+    if (st != GOPS_OK) break;
+    __atomic_store_n(ctx->mirror_host, (uint64_t)0, __ATOMIC_RELEASE);
+    PprArgs a = tile_args(ctx, g, t, plan, true);
+    // same host pacing as the single-vector chain: a few sweeps queued ahead
+    const uint64_t kAhead = 4;
+    uint64_t j = 0, seen = 0;
+    while (j < max_it && st == GOPS_OK) {
+      for (; j < max_it && j < seen + kAhead; j++) {
+        hipError_t e = ppr_launch_sweep(ctx->stream, T, a, t->grid);
+        if (e != hipSuccess) {
+          st = sfail(ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
+          break;
+        }
+      }
+      if (j >= max_it || st != GOPS_OK) break;
+      bool done = false;
+      for (uint64_t spin = 0;; spin++) {
+        const uint64_t w = __atomic_load_n(ctx->mirror_host, __ATOMIC_ACQUIRE);
+        if (w & 1) {
+          done = true;
+          break;
+        }
+        if ((w >> 1) > seen) {
+          seen = w >> 1;
+          break;
+        }
+        if ((spin & 1023) == 1023) {
+          const hipError_t qe = hipStreamQuery(ctx->stream);
+          if (qe == hipSuccess) {
+            const uint64_t w2 = __atomic_load_n(ctx->mirror_host, __ATOMIC_ACQUIRE);
+            done = (w2 & 1) != 0;
+            if (!done && (w2 >> 1) <= seen)
+              st = sfail(ctx, GOPS_DEVICE_ERROR, "tile sweep chain stalled without progress");
+            seen = w2 >> 1;
+            break;
+          }
+          if (qe != hipErrorNotReady) {
+            st = sfail(ctx, GOPS_DEVICE_ERROR, hipGetErrorString(qe));
+            break;
+          }
+        }
+        __builtin_ia32_pause();
+      }
+      if (done) break;
+    }
+    if (st != GOPS_OK) break;
+    st = sync_state(ctx, t);
+    if (st != GOPS_OK) break;
+    for (uint32_t q = q0; q < q1; q++) {
+      const int s = (int)(q - q0);
+      if (!t->used[s]) continue;
+      if (infos) {
+        infos[q].iterations = t->hts->iter[s];
+        infos[q].diff_l1 = t->hts->diff[s];
+        infos[q].converged = t->hts->converged[s];
+        infos[q].reserved = 0;
+      }
+      if (scores) {
+        hipError_t e = ppr_launch_gather(ctx->stream, T, a, g->dev.rank, (int)n, s, dout);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(scores + (size_t)q * n, dout, n * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = sfail(ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
+      }
+    }
+  }
+  free_tile(t);
+  hipFree(dout);
+  if (st != GOPS_OK) return st;
+  // queries whose personalization is not sparse-positive: the dense path
+  for (uint32_t q : dense_q) {
+    std::vector<double> p(n, 0.0);
+    for (uint64_t j = seed_off[q]; j < seed_off[q + 1]; j++) p[seeds[j]] += seed_w ? seed_w[j] : 1.0;
+    gops_run_info info{};
+    st = run_single(ctx, g, &cfgs[q], kPersonalized, p.data(), n,
+                    scores ? scores + (size_t)q * n : nullptr, &info);
+    if (st != GOPS_OK) return st;
+    if (infos) infos[q] = info;
+  }
+  return GOPS_OK;
 }
 
-gops_status gops_server_create(gops_context *, gops_graph *, const gops_server_config *,
+// ---------------------------------------------------------------------------
+gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_server_config *cfg,
                                gops_server **out) {
-  if (out) *out = nullptr;
-  return GOPS_NOT_SUPPORTED;
+  if (!ctx || !g || !out) return sfail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  *out = nullptr;
+  if (g->ctx != ctx) return sfail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  if (g->dev.n == 0) return sfail(ctx, GOPS_INVALID_PARAMETER, "empty graph");
+  PprGraphCache *c = nullptr;
+  gops_status st = graph_cache(ctx, g, &c);
+  if (st != GOPS_OK) return st;
+  const int T = (cfg && cfg->tile) ? (int)cfg->tile : 16;
+  if (!(T == 8 || T == 16 || T == 32 || T == 64))
+    return sfail(ctx, GOPS_INVALID_PARAMETER, "tile must be 8, 16, 32 or 64");
+  // admission by HBM: as many tiles as the budget holds
+  size_t free_b = 0, total_b = 0;
+  STRY(ctx, hipMemGetInfo(&free_b, &total_b));
+  double budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : 0.9 * (double)free_b;
+  const size_t per_tile = tile_bytes(g->dev.n, T);
+  uint32_t ntiles = (uint32_t)std::floor(budget / (double)per_tile);
+  if (cfg && cfg->max_slots) ntiles = std::min<uint32_t>(ntiles, (cfg->max_slots + T - 1) / T);
+  if (ntiles == 0) return sfail(ctx, GOPS_OUT_OF_MEMORY, "HBM budget holds no query tile");
+  TilePlan *plan = nullptr;
+  st = tile_plan(ctx, g, c, T, &plan);
+  if (st != GOPS_OK) return st;
+  auto *s = new (std::nothrow) gops_server;
+  if (!s) return sfail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
+  s->ctx = ctx;
+  s->g = g;
+  s->T = T;
+  s->topk = cfg ? cfg->topk : 0;
+  s->t0 = std::chrono::steady_clock::now();
+  for (uint32_t i = 0; i < ntiles; i++) {
+    Tile *t = nullptr;
+    st = alloc_tile(ctx, g, T, tile_grid(g), &t);
+    if (st != GOPS_OK) {
+      if (i == 0) {
+        delete s;
+        return st;
+      }
+      ctx->last_error.clear();
+      break;  // fewer tiles than the budget suggested
+    }
+    s->tiles.push_back(t);
+  }
+  *out = s;
+  return GOPS_OK;
 }
-void gops_server_destroy(gops_server *) {}
-gops_status gops_server_submit(gops_server *, const gops_pagerank_config *, const uint32_t *,
-                               uint64_t, const double *, uint64_t *) {
-  return GOPS_NOT_SUPPORTED;
+
+void gops_server_destroy(gops_server *s) {
+  if (!s) return;
+  hipStreamSynchronize(s->ctx->stream);
+  for (Tile *t : s->tiles) free_tile(t);
+  for (auto &d : s->finished) hipFree(d.scores);
+  delete s;
 }
-gops_status gops_server_step(gops_server *, uint32_t, uint32_t *) { return GOPS_NOT_SUPPORTED; }
-gops_status gops_server_poll(gops_server *, int *, gops_server_result *, double *, uint32_t *,
-                             double *) {
-  return GOPS_NOT_SUPPORTED;
+
+gops_status gops_server_submit(gops_server *s, const gops_pagerank_config *cfg,
+                               const uint32_t *seeds, uint64_t ns, const double *w,
+                               uint64_t *ticket) {
+  if (!s || !cfg || (ns && !seeds)) return GOPS_INVALID_PARAMETER;
+  PprGraphCache *c = nullptr;
+  gops_status st = graph_cache(s->ctx, s->g, &c);
+  if (st != GOPS_OK) return st;
+  Query q;
+  st = make_query(s->ctx, s->g, c, cfg, seeds, ns, w, q);
+  if (st != GOPS_OK) return st;
+  if (q.dense_fallback || cfg->max_iterations == 0)
+    return sfail(s->ctx, GOPS_NOT_SUPPORTED,
+                 "serving needs a personalization with positive sum and max_iterations > 0");
+  q.ticket = s->next_ticket++;
+  q.submit_ms = s->now_ms();
+  s->queue.push_back(std::move(q));
+  if (ticket) *ticket = s->queue.back().ticket;
+  return GOPS_OK;
 }
-gops_status gops_server_stats(const gops_server *, uint64_t *, uint64_t *, uint64_t *,
-                              uint32_t *) {
-  return GOPS_NOT_SUPPORTED;
+
+gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) {
+  if (!s) return GOPS_INVALID_PARAMETER;
+  gops_context *ctx = s->ctx;
+  gops_graph *g = s->g;
+  PprGraphCache *c = nullptr;
+  gops_status st = graph_cache(ctx, g, &c);
+  if (st != GOPS_OK) return st;
+  TilePlan *plan = nullptr;
+  st = tile_plan(ctx, g, c, s->T, &plan);
+  if (st != GOPS_OK) return st;
+  const int n = (int)g->dev.n;
+  // 1. admission into free slots (host state is current: no kernel is queued)
+  for (Tile *t : s->tiles) {
+    if (s->queue.empty()) break;
+    uint64_t mask = 0;
+    for (int q = 0; q < t->T && !s->queue.empty(); q++) {
+      if (t->used[q]) continue;
+      t->slot[q] = std::move(s->queue.front());
+      s->queue.pop_front();
+      t->used[q] = true;
+      set_slot_params(t, q, t->slot[q], ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
+      mask |= 1ull << q;
+    }
+    if (mask) {
+      st = admit(ctx, g, t, plan, mask);
+      if (st != GOPS_OK) return st;
+      st = sync_state(ctx, t);
+      if (st != GOPS_OK) return st;
+    }
+  }
+  // 2. sweeps of every tile with live queries, queued back to back
+  for (Tile *t : s->tiles) {
+    if (t->hts->all_done) continue;
+    PprArgs a = tile_args(ctx, g, t, plan, false);
+    for (uint32_t k = 0; k < sweeps; k++) STRY(ctx, ppr_launch_sweep(ctx->stream, t->T, a, t->grid));
+  }
+  STRY(ctx, hipStreamSynchronize(ctx->stream));
+  // 3. harvest finished queries
+  uint32_t live_total = 0;
+  for (Tile *t : s->tiles) {
+    st = sync_state(ctx, t);
+    if (st != GOPS_OK) return st;
+    PprArgs a = tile_args(ctx, g, t, plan, false);
+    for (int q = 0; q < t->T; q++) {
+      if (!t->used[q]) continue;
+      if (t->hts->live[q]) {
+        live_total++;
+        continue;
+      }
+      gops_server::Done d;
+      d.q = std::move(t->slot[q]);
+      d.info = gops_run_info{t->hts->iter[q], t->hts->diff[q], t->hts->converged[q], 0};
+      d.finish_ms = s->now_ms();
+      d.scores = nullptr;
+      STRY(ctx, hipMalloc((void **)&d.scores, (size_t)n * sizeof(double)));
+      STRY(ctx, ppr_launch_gather(ctx->stream, t->T, a, g->dev.rank, n, q, d.scores));
+      t->used[q] = false;
+      s->finished.push_back(std::move(d));
+      s->n_finished++;
+    }
+  }
+  STRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (active) *active = live_total;
+  return GOPS_OK;
+}
+
+gops_status gops_server_poll(gops_server *s, int *ready, gops_server_result *out, double *scores,
+                             uint32_t *topk_ids, double *topk_scores) {
+  if (!s || !ready) return GOPS_INVALID_PARAMETER;
+  *ready = 0;
+  if (s->finished.empty()) return GOPS_OK;
+  gops_server::Done d = std::move(s->finished.front());
+  s->finished.pop_front();
+  const size_t n = s->g->dev.n;
+  gops_status st = GOPS_OK;
+  std::vector<double> host;
+  if (scores || (s->topk && (topk_ids || topk_scores))) {
+    host.resize(n);
+    hipError_t e = hipMemcpy(host.data(), d.scores, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) st = sfail(s->ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
+  }
+  hipFree(d.scores);
+  if (st != GOPS_OK) return st;
+  if (scores) std::memcpy(scores, host.data(), n * sizeof(double));
+  if (s->topk && (topk_ids || topk_scores)) {
+    // top_k semantics of topk.rs:17-42: finite, > 0, descending
+    std::vector<uint32_t> idx;
+    idx.reserve(n);
+    for (size_t i = 0; i < n; i++)
+      if (std::isfinite(host[i]) && host[i] > 0.0) idx.push_back((uint32_t)i);
+    const size_t k = std::min<size_t>(s->topk, idx.size());
+    std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](uint32_t a, uint32_t b) {
+      return host[a] != host[b] ? host[a] > host[b] : a < b;
+    });
+    for (size_t i = 0; i < s->topk; i++) {
+      if (topk_ids) topk_ids[i] = i < k ? idx[i] : UINT32_MAX;
+      if (topk_scores) topk_scores[i] = i < k ? host[idx[i]] : 0.0;
+    }
+  }
+  if (out) {
+    out->ticket = d.q.ticket;
+    out->info = d.info;
+    out->submit_ms = d.q.submit_ms;
+    out->finish_ms = d.finish_ms;
+  }
+  *ready = 1;
+  return GOPS_OK;
+}
+
+gops_status gops_server_stats(const gops_server *s, uint64_t *queued, uint64_t *active,
+                              uint64_t *finished, uint32_t *slots) {
+  if (!s) return GOPS_INVALID_PARAMETER;
+  uint64_t act = 0;
+  for (Tile *t : s->tiles)
+    for (int q = 0; q < t->T; q++) act += t->used[q];
+  if (queued) *queued = s->queue.size();
+  if (active) *active = act;
+  if (finished) *finished = s->n_finished;
+  if (slots) *slots = (uint32_t)(s->tiles.size() * (size_t)s->T);
+  return GOPS_OK;
 }
 
 }  // extern "C"

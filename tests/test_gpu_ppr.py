@@ -1,0 +1,178 @@
+"""GPU parity of batched personalized PageRank and the serving loop.
+
+Every query is checked against the f64 oracle's personalized_pagerank_run
+(ppr.rs:90-156) on the same graph and personalization:
+  * iterations and converged equal the oracle's (a residual landing within
+    1e-4 of the tolerance, or within the reference's own f64 noise, may move
+    the stop by one sweep);
+  * L1(scores - oracle) <= 1e-2 * tolerance + 1e-12 for converged queries.
+"""
+import numpy as np
+import pytest
+
+import graphops_amd as ga
+from graphops_amd import generators
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return ga.Context(0)
+
+
+@pytest.fixture(scope="module")
+def graph13():
+    e = generators.rmat(13, 16, seed=13)
+    return e
+
+
+def oracle_ppr(oracle, e, p, d, tol, max_it):
+    in_off, in_src, _ = oracle.transpose(e.offsets, e.targets)
+    scale = np.diff(e.offsets).astype(np.float64)
+    return oracle.pull_run(2, in_off, in_src, scale, p=p, damping=d, max_iterations=max_it,
+                           tolerance=tol, threads=0)
+
+
+def dense_p(n, seeds, w=None):
+    p = np.zeros(n)
+    np.add.at(p, seeds, 1.0 if w is None else w)
+    return p
+
+
+def check(run_it, run_conv, run_diff, scores, ref, tol):
+    noise = 1e-15 + 4e-18 * ref.scores.size
+    assert run_conv == ref.converged
+    if run_it != ref.iterations:
+        assert abs(run_it - ref.iterations) == 1
+        assert abs(ref.diff_l1 - tol) <= 1e-4 * tol + noise or abs(run_diff - tol) <= 1e-4 * tol + noise
+    else:
+        assert abs(run_diff - ref.diff_l1) <= 1e-4 * ref.diff_l1 + noise
+    if scores is not None:
+        l1 = np.abs(scores - ref.scores).sum()
+        assert l1 <= (1e-2 * tol + 1e-12 if ref.converged else 1e-7), l1
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32, 64])
+def test_batch_matches_oracle(oracle, ctx, graph13, tile):
+    e = graph13
+    nq = 20
+    off, seeds = generators.seed_sets(e, nq, 1, 64, seed=tile)
+    rng = np.random.default_rng(tile)
+    w = rng.uniform(0.5, 2.0, size=seeds.size) if tile == 16 else None
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    cfgs = [ga.PageRankConfig([0.5, 0.85, 0.95][q % 3], 400, [1e-8, 1e-6, 1e-10][q % 3])
+            for q in range(nq)]
+    scores, runs = ga.ppr_batch_run(dg, cfgs, off, seeds, seed_weights=w, ctx=ctx, tile=tile)
+    for q in range(nq):
+        sl = slice(off[q], off[q + 1])
+        p = dense_p(e.n, seeds[sl], None if w is None else w[sl])
+        ref = oracle_ppr(oracle, e, p, cfgs[q].damping, cfgs[q].tolerance, 400)
+        check(runs[q].iterations, runs[q].converged, runs[q].diff_l1, scores[q], ref,
+              cfgs[q].tolerance)
+
+
+def test_plain_path_matches_tiles(ctx, graph13):
+    """tile=-1: one query per device run (the single-vector chain)."""
+    e = graph13
+    off, seeds = generators.seed_sets(e, 6, 1, 16, seed=3)
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    cfg = ga.PageRankConfig(0.85, 300, 1e-8)
+    s1, r1 = ga.ppr_batch_run(dg, cfg, off, seeds, ctx=ctx, tile=-1)
+    s2, r2 = ga.ppr_batch_run(dg, cfg, off, seeds, ctx=ctx, tile=16)
+    for q in range(6):
+        assert r1[q].iterations == r2[q].iterations
+        assert np.abs(s1[q] - s2[q]).sum() <= 2e-10
+
+
+def test_batch_edge_cases(oracle, ctx, graph13):
+    e = graph13
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    # duplicate seeds add up; zero-sum personalization falls back to uniform
+    off = np.array([0, 3, 4, 4], np.uint64)
+    seeds = np.array([5, 5, 9, 7], np.uint32)
+    w = np.array([1.0, 2.0, 1.0, 0.0])
+    cfg = ga.PageRankConfig(0.85, 200, 1e-9)
+    scores, runs = ga.ppr_batch_run(dg, cfg, off, seeds, seed_weights=w, ctx=ctx)
+    for q, p in enumerate([dense_p(e.n, [5, 5, 9], [1.0, 2.0, 1.0]), dense_p(e.n, [7], [0.0]),
+                           np.zeros(e.n)]):
+        ref = oracle_ppr(oracle, e, p, 0.85, 1e-9, 200)
+        check(runs[q].iterations, runs[q].converged, runs[q].diff_l1, scores[q], ref, 1e-9)
+    with pytest.raises(ga.IndexOutOfBounds):
+        ga.ppr_batch_run(dg, cfg, np.array([0, 1], np.uint64), np.array([e.n], np.uint32), ctx=ctx)
+
+
+def test_server_continuous_batching(oracle, ctx, graph13):
+    """Queries arrive while others run; each result equals the oracle and the
+    same query run alone, within the tolerance."""
+    e = graph13
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    srv = ga.Server(dg, tile=8, max_slots=16)
+    assert srv.stats()["slots"] == 16
+    rng = np.random.default_rng(11)
+    off, seeds = generators.seed_sets(e, 40, 1, 256, log_uniform=True, seed=17)
+    cfgs = [ga.PageRankConfig(float(rng.choice([0.5, 0.85, 0.95])), 500, 1e-6) for _ in range(40)]
+    tickets = {}
+    results = {}
+    nxt = 0
+    steps = 0
+    while len(results) < 40:
+        # a few arrivals between device steps
+        for _ in range(int(rng.integers(0, 4))):
+            if nxt < 40:
+                tickets[srv.submit(cfgs[nxt], seeds[off[nxt]:off[nxt + 1]])] = nxt
+                nxt += 1
+        if nxt < 40 and srv.stats()["queued"] == 0 and srv.stats()["active"] == 0:
+            tickets[srv.submit(cfgs[nxt], seeds[off[nxt]:off[nxt + 1]])] = nxt
+            nxt += 1
+        srv.step(4)
+        steps += 1
+        while True:
+            r = srv.poll()
+            if r is None:
+                break
+            ticket, run, t_sub, t_fin, _ = r
+            results[tickets[ticket]] = run
+            assert t_fin >= t_sub
+        assert steps < 5000
+    for q in range(40):
+        p = dense_p(e.n, seeds[off[q]:off[q + 1]])
+        ref = oracle_ppr(oracle, e, p, cfgs[q].damping, 1e-6, 500)
+        run = results[q]
+        check(run.iterations, run.converged, run.diff_l1, run.scores, ref, 1e-6)
+    # alone == in the mix (within the tolerance)
+    alone = ga.Server(dg, tile=8, max_slots=8)
+    for q in (0, 7, 23):
+        alone.submit(cfgs[q], seeds[off[q]:off[q + 1]])
+        got = None
+        while got is None:
+            alone.step(4)
+            got = alone.poll()
+        run = got[1]
+        # every query keeps its own sweep mode and lane arithmetic, so the
+        # result does not depend on the other queries or on the slot
+        assert run.iterations == results[q].iterations
+        assert run.diff_l1 == results[q].diff_l1
+        assert np.array_equal(run.scores, results[q].scores)
+    srv.close()
+    alone.close()
+
+
+def test_server_admission_refused_without_memory(ctx, graph13):
+    dg = ga.DeviceGraph.from_edges(graph13, ctx=ctx)
+    with pytest.raises(ga.OutOfMemory):
+        ga.Server(dg, tile=16, hbm_budget_bytes=1024.0)
+
+
+def test_server_topk(ctx, graph13):
+    e = graph13
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    srv = ga.Server(dg, tile=8, max_slots=8, topk=10)
+    srv.submit(ga.PageRankConfig(0.85, 300, 1e-8), [3, 17])
+    got = None
+    while got is None:
+        srv.step(8)
+        got = srv.poll()
+    _, run, _, _, top = got
+    assert top == ga.top_k(run.scores, 10)
+    srv.close()
