@@ -146,11 +146,28 @@ __device__ __forceinline__ double tile_query_sum(double v) {
 template <int T>
 __device__ __forceinline__ double tile_row_sum(const PprArgs &a, const double *y64, bool anchor,
                                                int b, int e, int step, int q) {
+  // four edges per iteration: their column loads, then their gathers, are in
+  // flight together
   double acc = 0.0;
+  int k = b;
   if (anchor) {
-    for (int k = b; k < e; k += step) acc += y64[(size_t)a.col[k] * T + q];
+    for (; k + 3 * step < e; k += 4 * step) {
+      const int c0 = a.col[k], c1 = a.col[k + step], c2 = a.col[k + 2 * step],
+                c3 = a.col[k + 3 * step];
+      const double v0 = y64[(size_t)c0 * T + q], v1 = y64[(size_t)c1 * T + q],
+                   v2 = y64[(size_t)c2 * T + q], v3 = y64[(size_t)c3 * T + q];
+      acc += (v0 + v1) + (v2 + v3);
+    }
+    for (; k < e; k += step) acc += y64[(size_t)a.col[k] * T + q];
   } else {
-    for (int k = b; k < e; k += step) acc += (double)a.yin32[(size_t)a.col[k] * T + q];
+    for (; k + 3 * step < e; k += 4 * step) {
+      const int c0 = a.col[k], c1 = a.col[k + step], c2 = a.col[k + 2 * step],
+                c3 = a.col[k + 3 * step];
+      const float v0 = a.yin32[(size_t)c0 * T + q], v1 = a.yin32[(size_t)c1 * T + q],
+                  v2 = a.yin32[(size_t)c2 * T + q], v3 = a.yin32[(size_t)c3 * T + q];
+      acc += ((double)v0 + (double)v1) + ((double)v2 + (double)v3);
+    }
+    for (; k < e; k += step) acc += (double)a.yin32[(size_t)a.col[k] * T + q];
   }
   return acc;
 }
