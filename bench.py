@@ -69,6 +69,145 @@ def oracle_reference(e, tol, max_it):
                            tolerance=tol, threads=16)
 
 
+PPR_GRAPH = dict(scale=22, seed=13)  # unit weights
+
+
+def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3):
+    """Batched personalized PageRank: nq queries, |S| ~ U{1..64} seeds drawn
+    proportional to out-degree (the source of a uniformly drawn edge),
+    p-hat uniform on S, d 0.85, tol 1e-8; the tile width from the library's
+    autotuner.  Compared with the plain one-query-per-run path on a sample
+    of the same queries, and with the oracle (f64) on a few of them."""
+    import oracle
+
+    cfg = ga.PageRankConfig(0.85, 200, 1e-8)
+    off, seeds = generators.seed_sets(e, nq, 1, 64, log_uniform=False, seed=5)
+    tile, costs = ga.ppr_tile_info(dg, nq)
+    log(f"ppr: autotuned tile {tile}, ms per query-sweep {costs}")
+    ga.ppr_batch_run(dg, cfg, off[:65], seeds[:off[64]], with_scores=False)  # warm-up
+    t0 = time.perf_counter()
+    _, runs = ga.ppr_batch_run(dg, cfg, off, seeds, with_scores=False)
+    dt = time.perf_counter() - t0
+    its = np.array([r.iterations for r in runs])
+    qps = nq / dt
+    log(f"ppr: {nq} queries in {dt:.3f} s -> {qps:.1f} q/s, iterations mean {its.mean():.2f}")
+    k = plain_sample
+    t0 = time.perf_counter()
+    _, pruns = ga.ppr_batch_run(dg, cfg, off[:k + 1], seeds[:off[k]], with_scores=False, tile=-1)
+    pdt = time.perf_counter() - t0
+    plain_qps = k / pdt
+    log(f"ppr plain: {k} queries in {pdt:.3f} s -> {plain_qps:.2f} q/s")
+    # accuracy on a sample: batched scores against the oracle's f64 run
+    sample = [0, 1, 2, 3]
+    sc, sruns = ga.ppr_batch_run(dg, cfg, off[:5], seeds[:off[4]], with_scores=True)
+    in_off, in_src, _ = oracle.transpose(e.offsets, e.targets)
+    deg = np.diff(e.offsets.astype(np.int64)).astype(np.float64)
+    l1 = []
+    mx = []
+    for j in sample:
+        p = np.zeros(e.n)
+        p[seeds[off[j]:off[j + 1]]] += 1.0
+        ref = oracle.pull_run(2, in_off, in_src, deg, p=p, damping=0.85, max_iterations=200,
+                              tolerance=1e-8, threads=16)
+        l1.append(float(np.abs(sc[j] - ref.scores).sum()))
+        mx.append(float(np.abs(sc[j] - ref.scores).max()))
+    del sc
+    # CPU: the oracle's push form of personalized_pagerank_run, one thread
+    t0 = time.perf_counter()
+    for j in range(cpu_queries):
+        p = np.zeros(e.n)
+        p[seeds[off[j]:off[j + 1]]] = 1.0
+        oracle.personalized_pagerank_run(e.offsets, e.targets, p, 0.85, 200, 1e-8)
+    cdt = time.perf_counter() - t0
+    n, m = e.n, e.m
+    # byte model per query-sweep: 8n + (4m + 8n)/K; roofline q/s at 8 TB/s
+    per_qsweep = 8 * n + (4 * m + 8 * n) / tile
+    roof_qps = HBM_PEAK_GBS * 1e9 / (its.mean() * per_qsweep)
+    return {
+        "workload": f"rmat{PPR_GRAPH['scale']}-unit ppr {nq} queries |S|~U{{1..64}} "
+                    "degree-proportional, d0.85 tol1e-8",
+        "n": n, "m": m, "queries": nq, "queries_per_s": round(qps, 2),
+        "ms_per_batch": round(dt * 1e3, 2), "tile": tile,
+        "tile_ms_per_query_sweep": {str(w): round(c, 5) for w, c in costs.items()},
+        "iterations_mean": round(float(its.mean()), 3), "iterations_max": int(its.max()),
+        "converged": int(sum(r.converged for r in runs)),
+        "plain_queries_per_s": round(plain_qps, 3), "plain_sample_queries": k,
+        "plain_iterations_mean": round(float(np.mean([r.iterations for r in pruns])), 3),
+        "ratio_vs_plain": round(qps / plain_qps, 2),
+        "accuracy": {"sample_queries": len(sample), "l1_vs_f64_oracle_max": max(l1),
+                     "max_abs": max(mx)},
+        "roofline": {"bound": "hbm", "queries_per_s_at_peak": round(roof_qps, 1),
+                     "frac": round(qps / roof_qps, 4),
+                     "bytes_per_query_sweep": round(per_qsweep)},
+        "cpu_baseline": {"value": round(cpu_queries / cdt, 4), "unit": "queries/s", "cores": 1,
+                         "kind": "port",
+                         "sample": f"first {cpu_queries} queries by the oracle's "
+                                   f"personalized_pagerank_run (f64, push), {cdt:.1f} s"},
+    }
+
+
+def serving_leg(ga, generators, e, dg, nq=1024, qps_offered=5000.0, max_slots=512, sweeps=2):
+    """Serving loop: Poisson arrivals at qps_offered, |S| log-uniform 1..256,
+    d uniform over {0.5, 0.85, 0.95}, tol 1e-6; continuous batching through
+    gops_server_* (admission into free slots between device steps).
+    Latency = completion (poll) time - scheduled arrival time."""
+    rng = np.random.default_rng(17)
+    off, seeds = generators.seed_sets(e, nq, 1, 256, log_uniform=True, seed=23)
+    damp = rng.choice([0.5, 0.85, 0.95], size=nq)
+    arrive = np.cumsum(rng.exponential(1.0 / qps_offered, size=nq))
+    srv = ga.Server(dg, tile=0, max_slots=max_slots)
+    st = srv.stats()
+    lat = np.zeros(nq)
+    its = np.zeros(nq)
+    conv = 0
+    by_ticket = {}
+    i = done = 0
+    t0 = time.perf_counter()
+    while done < nq:
+        now = time.perf_counter() - t0
+        while i < nq and arrive[i] <= now:
+            t = srv.submit(ga.PageRankConfig(float(damp[i]), 1000, 1e-6), seeds[off[i]:off[i + 1]])
+            by_ticket[t] = i
+            i += 1
+        s = srv.stats()
+        if s["queued"] or s["active"]:
+            srv.step(sweeps)
+        elif i < nq:
+            continue
+        while True:
+            r = srv.poll(with_scores=False)
+            if r is None:
+                break
+            j = by_ticket.pop(r[0])
+            lat[j] = time.perf_counter() - t0 - arrive[j]
+            its[j] = r[1].iterations
+            conv += r[1].converged
+            done += 1
+    span = time.perf_counter() - t0
+    srv.close()
+    log(f"serving: {nq} queries in {span:.3f} s, p50 {np.percentile(lat, 50) * 1e3:.1f} ms, "
+        f"p99 {np.percentile(lat, 99) * 1e3:.1f} ms")
+    return {"workload": f"rmat{PPR_GRAPH['scale']}-unit ppr serving, Poisson {qps_offered:.0f} QPS "
+                        "offered, |S| log-uniform 1..256, d in {0.5,0.85,0.95}, tol1e-6",
+            "queries": nq, "queries_per_s": round(nq / span, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "slots": st["slots"], "sweeps_per_step": sweeps,
+            "iterations_mean": round(float(its.mean()), 2), "converged": int(conv)}
+
+
+def ppr_legs(args, ga, generators):
+    t0 = time.perf_counter()
+    e = generators.rmat(PPR_GRAPH["scale"], 16, 0.57, 0.19, 0.19, PPR_GRAPH["seed"])
+    ctx = ga.Context(0, precision="mixed")
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    log(f"ppr graph n={e.n} m={e.m} resident in {time.perf_counter() - t0:.1f} s")
+    out = {"ppr_batch": ppr_batch_leg(ga, generators, e, dg, nq=args.ppr_queries)}
+    out["serving"] = serving_leg(ga, generators, e, dg, nq=args.serve_queries)
+    dg.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +216,9 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=3)
+    ap.add_argument("--no-ppr", action="store_true", help="skip the batched PPR and serving legs")
+    ap.add_argument("--ppr-queries", type=int, default=1024)
+    ap.add_argument("--serve-queries", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -153,6 +295,10 @@ def main():
                            "top100_agreement": len(top & top_ref) / 100.0,
                            "oracle_iterations": ref.iterations, "gpu_iterations": r.iterations}
     out["cpu_baseline"] = cpu_baseline(e, c, args.cpu_sweeps)
+    if not args.no_ppr:
+        dg.close()
+        del e
+        out.update(ppr_legs(args, ga, generators))
     print(json.dumps(out), flush=True)
 
 

@@ -498,6 +498,18 @@ def ppr_batch_run(graph, configs, seed_offsets, seeds, seed_weights=None, ctx: C
     return scores, runs
 
 
+def ppr_tile_info(graph, num_queries, ctx: Context = None):
+    """(tile width a batch of num_queries queries would use, {width: measured
+    ms per query-sweep}) -- gops_ppr_tile_info; runs the autotuner's trial on
+    first use."""
+    dg = _device_graph(graph, False, ctx)
+    t = ctypes.c_uint32()
+    cost = np.zeros(4, dtype=np.float64)
+    _check(L.lib().gops_ppr_tile_info(dg.ctx.handle, dg.handle, int(num_queries), ctypes.byref(t),
+                                      L.ptr(cost, L._f64p)), dg.ctx.handle, "gops_ppr_tile_info")
+    return int(t.value), {w: float(c) for w, c in zip((8, 16, 32, 64), cost)}
+
+
 # ---------------------------------------------------------------- ranking
 def top_k(scores, k):
     """top_k, topk.rs:17-42: finite, > 0 scores, descending (ties in

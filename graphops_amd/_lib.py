@@ -90,6 +90,7 @@ SIGNATURES = [
     ("gops_ppr_batch_run", ctypes.c_int,
      [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(PageRankConfigC), _u64p, _u32p, _f64p, _f64p,
       ctypes.POINTER(RunInfoC)]),
+    ("gops_ppr_tile_info", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _f64p]),
     ("gops_server_create", ctypes.c_int,
      [_vp, _vp, ctypes.POINTER(ServerConfigC), ctypes.POINTER(_vp)]),
     ("gops_server_destroy", None, [_vp]),

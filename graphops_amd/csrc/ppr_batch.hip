@@ -51,12 +51,21 @@ __device__ __forceinline__ double lane_bits_sum(double v, int lo_bit, int hi_bit
   return v;
 }
 
+// Per-lane partial sums of a sweep, per query: L1 residual, dangling part of
+// the delta, dangling part of the iterate, total delta, total iterate.  Named
+// fields, each updated once per epilogue: conditional adds in different
+// branches get merged into one add through a select between field addresses,
+// which keeps the partial sums in scratch memory instead of registers.
+struct Red {
+  double l1 = 0.0, dang = 0.0, dang_x = 0.0, mass = 0.0, sum_x = 0.0;
+};
+
 }  // namespace
 
 // Per (row, query) epilogue.  mode: kModeDelta / kModeDeltaFold / kModeAnchor.
 template <int T>
 __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *ts, int mode,
-                                             int r, int q, double acc, double (&red)[5]) {
+                                             int r, int q, double acc, Red &red) {
   if (!ts->live[q]) return;
   const double d = ts->damping[q];
   double seed_p = 0.0;
@@ -67,52 +76,60 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *
   }
   const size_t at = (size_t)r * T + q;
   const int c64 = ts->y64_cur[q];
+  // every path sets all five contributions; one update of red at the end
+  double delta, dang_x = 0.0, sum_x = 0.0;
+  bool dangling;
   if (mode == kModeAnchor) {
     const double sc = ts->x_scale[q];
     const double dsum = ts->dsum_x[q] * sc;
     const double t = sc * acc + ((1.0 - d) * seed_p + (d * dsum) * seed_p);  // ppr.rs:126-127
-    const double delta = t - sc * a.x[at];
+    delta = t - sc * a.x[at];
     a.x[at] = t;
     const double inv = a.inv64[r];
     a.yout32[at] = (float)((d * delta) * inv);
     if (ts->f64_phase[q]) (c64 ? a.y64a : a.y64b)[at] = (d * t) * inv;  // next anchor's input
-    red[0] += fabs(delta);
-    red[3] += delta;
-    if (inv == 0.0) {
-      red[1] += delta;
-      red[2] += t;
-    }
+    dangling = inv == 0.0;
+    dang_x = dangling ? t : 0.0;
   } else {
     // delta' = A delta: the dangling and (removed) total sums go to the seeds
-    const double delta = acc + d * (ts->dsum_in[q] - ts->mass_in[q]) * seed_p;
+    delta = acc + d * (ts->dsum_in[q] - ts->mass_in[q]) * seed_p;
     const double x = __builtin_nontemporal_load(a.x + at) + delta;
     __builtin_nontemporal_store(x, a.x + at);
     const float inv = a.inv32[r];
     a.yout32[at] = (float)(d * delta * (double)inv);
-    red[0] += fabs(delta);
-    red[3] += delta;
-    if (inv == 0.0f) red[1] += delta;
+    dangling = inv == 0.0f;
     if (mode == kModeDeltaFold) {
       const double inv64 = a.inv64[r];
       (c64 ? a.y64b : a.y64a)[at] = (d * x) * inv64;
-      if (inv64 == 0.0) red[2] += x;
-      red[4] += x;
+      dang_x = inv64 == 0.0 ? x : 0.0;
+      sum_x = x;
     }
   }
+  red.l1 += fabs(delta);
+  red.dang += dangling ? delta : 0.0;
+  red.dang_x += dang_x;
+  red.mass += delta;
+  red.sum_x += sum_x;
 }
 
 // Per-query partial sums of a workgroup (lane q = tid % T holds query q)
 // -> out[q * 5 + i] in thread q < T.
 template <int T>
-__device__ __forceinline__ void tile_block_sum(double (&red)[5], double *out) {
+__device__ __forceinline__ void tile_block_sum(Red &red, double *out) {
   __shared__ double sh[kBlock / 64][T][5];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int lb = 0;
   while ((1 << lb) < T) lb++;
-#pragma unroll
-  for (int i = 0; i < 5; i++) {
-    const double s = lane_bits_sum(red[i], lb, 6);
-    if (lane < T) sh[wave][lane][i] = s;
+  const double s0 = lane_bits_sum(red.l1, lb, 6), s1 = lane_bits_sum(red.dang, lb, 6),
+               s2 = lane_bits_sum(red.dang_x, lb, 6), s3 = lane_bits_sum(red.mass, lb, 6),
+               s4 = lane_bits_sum(red.sum_x, lb, 6);
+  if (lane < T) {
+    double *o = sh[wave][lane];
+    o[0] = s0;
+    o[1] = s1;
+    o[2] = s2;
+    o[3] = s3;
+    o[4] = s4;
   }
   __syncthreads();
   if (threadIdx.x < T) {
@@ -183,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
   const int mode = ts->mode[q];  // every query has its own sweep mode
   const bool anchor = mode == kModeAnchor;
   const double *y64 = ts->y64_cur[q] ? a.y64b : a.y64a;
-  double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  Red red;
   for (int di = blockIdx.x; di < a.num_desc; di += gridDim.x) {
     const BlockDesc bd = a.desc[di];
     if ((bd.x & (kHubFlag | kRowFlag)) == 0) {
@@ -200,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
         double acc = 0.0;
         if (r < bd.z)
           acc = tile_row_sum<T>(a, y64, anchor, a.row_ptr[r] + slice, a.row_ptr[r + 1], 1 << ls, q);
-        acc = lane_bits_sum(acc, lb, lb + ls);
+        acc = lane_bits_sum(acc, lb, lb + ls < 6 ? lb + ls : 6);  // T << ls <= 64
         if (r < bd.z && slice == 0) ppr_epilogue<T>(a, ts, mode, r, q, acc, red);
       }
     } else {
@@ -242,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
   a.yout32 = ycur ? a.yd0 : a.yd1;
   const int q = threadIdx.x % T;
   const int mode = ts->mode[q];
-  double red[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  Red red;
   // hub rows: (hub, query) pairs; thread t always gets query t % T
   for (int p = threadIdx.x; p < a.num_hubs * T; p += kBlock) {
     const int h = p / T;
@@ -253,8 +270,14 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
   }
   for (int p = threadIdx.x; p < sweep_blocks * T; p += kBlock) {
     const int b = p / T;
-#pragma unroll
-    for (int i = 0; i < 5; i++) red[i] += a.partials[((size_t)b * T + q) * 5 + i];
+    {
+      const double *pp = a.partials + ((size_t)b * T + q) * 5;
+      red.l1 += pp[0];
+      red.dang += pp[1];
+      red.dang_x += pp[2];
+      red.mass += pp[3];
+      red.sum_x += pp[4];
+    }
   }
   __shared__ double tot[T * 5];
   tile_block_sum<T>(red, tot);

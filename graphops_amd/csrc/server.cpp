@@ -163,6 +163,8 @@ struct PprGraphCache {
   std::vector<uint8_t> dangling;  // new ids, out-degree 0
   std::vector<int32_t> hub_row, hub_nchunks;
   TilePlan plans[4];
+  int tuned_T = 0;          // autotuned tile width (0: not measured yet)
+  double tile_cost[4] = {}; // measured ms per query-sweep of widths 8/16/32/64
 };
 
 }  // namespace gops
@@ -415,14 +417,100 @@ gops_status sync_state(gops_context *ctx, Tile *t) {
   return GOPS_OK;
 }
 
-int pick_tile(gops_context *ctx, int nq) {
-  if (ctx->ppr_tile > 0) return ctx->ppr_tile;
-  return nq >= 32 ? 16 : (nq >= 8 ? 8 : 8);
-}
-
+// Workgroups of a tile sweep (grid-stride over the descriptors): 8 waves of
+// 64 lanes on every SIMD of 256 CUs.
 int tile_grid(const gops_graph *g) {
   (void)g;
-  return 1024;
+  return kMaxSweepBlocks;
+}
+
+// Time per query-sweep of tile width T on this graph: T single-seed trial
+// queries (tolerance 0, four sweeps: one f64 anchor, then three f32 delta
+// sweeps), timed with events.  The estimate weights the two sweep kinds
+// equally, as the personalized schedule spends about half of its sweeps in
+// each.  Infinity if the tile does not fit in device memory.
+gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, double *cost) {
+  *cost = std::numeric_limits<double>::infinity();
+  TilePlan *plan = nullptr;
+  gops_status st = tile_plan(ctx, g, c, T, &plan);
+  if (st != GOPS_OK) return st;
+  Tile *t = nullptr;
+  st = alloc_tile(ctx, g, T, tile_grid(g), &t);
+  if (st == GOPS_OUT_OF_MEMORY) {
+    ctx->last_error.clear();
+    return GOPS_OK;
+  }
+  if (st != GOPS_OK) return st;
+  const uint64_t n = g->dev.n;
+  std::memset(t->hts, 0, sizeof(TileState));
+  uint64_t mask = 0;
+  for (int s = 0; s < T; s++) {
+    Query q;
+    q.cfg = gops_pagerank_config{0.85, 4, 0.0};
+    const uint32_t r = (uint32_t)(((uint64_t)s * 2654435761ull + 12345) % n);
+    q.pers.push_back({r, 1.0});
+    q.init_dsum = c->dangling[r] ? 1.0 : 0.0;
+    t->slot[s] = q;
+    t->used[s] = true;
+    set_slot_params(t, s, q, 0.0, 64);
+    mask |= 1ull << s;
+  }
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < 3 && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
+  if (e == hipSuccess) {
+    st = admit(ctx, g, t, plan, mask);
+    if (st == GOPS_OK) {
+      PprArgs a = tile_args(ctx, g, t, plan, false);
+      e = hipEventRecord(ev[0], ctx->stream);
+      if (e == hipSuccess) e = ppr_launch_sweep(ctx->stream, T, a, t->grid);  // f64 anchor
+      if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
+      for (int k = 0; k < 3 && e == hipSuccess; k++) e = ppr_launch_sweep(ctx->stream, T, a, t->grid);
+      if (e == hipSuccess) e = hipEventRecord(ev[2], ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+      float ms_anchor = 0.f, ms_delta = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms_anchor, ev[0], ev[1]);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms_delta, ev[1], ev[2]);
+      if (e == hipSuccess) *cost = 0.5 * ((double)ms_anchor + (double)ms_delta / 3.0) / T;
+    }
+  }
+  for (hipEvent_t x : ev)
+    if (x) hipEventDestroy(x);
+  free_tile(t);
+  if (st != GOPS_OK) return st;
+  if (e != hipSuccess) return sfail(ctx, GOPS_DEVICE_ERROR, std::string("tile trial: ") + hipGetErrorString(e));
+  return GOPS_OK;
+}
+
+// Tile width of a batch of nq queries.  A fixed width (GOPS_OPT_PPR_TILE)
+// wins; otherwise the autotuned width of this graph (the least time per
+// query-sweep over 8/16/32/64, measured on first use and kept with the
+// graph), narrowed to the smallest width that holds nq queries.
+gops_status pick_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int nq, int *T) {
+  if (ctx->ppr_tile > 0) {
+    *T = ctx->ppr_tile;
+    return GOPS_OK;
+  }
+  if (!c->tuned_T) {
+    double best = std::numeric_limits<double>::infinity();
+    int best_T = 0;
+    for (int w : {8, 16, 32, 64}) {
+      double cost;
+      gops_status st = trial_tile(ctx, g, c, w, &cost);
+      if (st != GOPS_OK) return st;
+      c->tile_cost[plan_slot(w)] = cost;
+      if (cost < best) {
+        best = cost;
+        best_T = w;
+      }
+    }
+    if (!best_T) return sfail(ctx, GOPS_OUT_OF_MEMORY, "no tile width fits in device memory");
+    c->tuned_T = best_T;
+  }
+  int w = 8;
+  while (w < nq && w < c->tuned_T) w <<= 1;
+  *T = w;
+  return GOPS_OK;
 }
 
 }  // namespace
@@ -459,7 +547,9 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
   PprGraphCache *c = nullptr;
   gops_status st = graph_cache(ctx, g, &c);
   if (st != GOPS_OK) return st;
-  const int T = pick_tile(ctx, (int)nq);
+  int T = 0;
+  st = pick_tile(ctx, g, c, (int)std::min<uint32_t>(nq, 1u << 20), &T);
+  if (st != GOPS_OK) return st;
   TilePlan *plan = nullptr;
   st = tile_plan(ctx, g, c, T, &plan);
   if (st != GOPS_OK) return st;
@@ -578,6 +668,30 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
   return GOPS_OK;
 }
 
+gops_status gops_ppr_tile_info(gops_context *ctx, gops_graph *g, uint32_t nq, uint32_t *tile,
+                               double *ms_per_query_sweep) {
+  if (!ctx || !g) return GOPS_INVALID_PARAMETER;
+  if (g->ctx != ctx) return sfail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  if (g->dev.n == 0) return sfail(ctx, GOPS_INVALID_PARAMETER, "empty graph");
+  ctx->last_error.clear();
+  PprGraphCache *c = nullptr;
+  gops_status st = graph_cache(ctx, g, &c);
+  if (st != GOPS_OK) return st;
+  // the trial runs whatever the options say, so the costs are always there
+  const int saved = ctx->ppr_tile;
+  ctx->ppr_tile = 0;
+  int T = 0;
+  st = pick_tile(ctx, g, c, (int)std::min<uint32_t>(nq, 1u << 20), &T);
+  ctx->ppr_tile = saved;
+  if (st != GOPS_OK) return st;
+  if (saved > 0) T = saved;
+  if (saved < 0) T = 0;
+  if (tile) *tile = (uint32_t)T;
+  if (ms_per_query_sweep)
+    for (int i = 0; i < 4; i++) ms_per_query_sweep[i] = c->tile_cost[i];
+  return GOPS_OK;
+}
+
 // ---------------------------------------------------------------------------
 gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_server_config *cfg,
                                gops_server **out) {
@@ -588,7 +702,14 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
   PprGraphCache *c = nullptr;
   gops_status st = graph_cache(ctx, g, &c);
   if (st != GOPS_OK) return st;
-  const int T = (cfg && cfg->tile) ? (int)cfg->tile : 16;
+  int T = cfg ? (int)cfg->tile : 0;
+  if (T == 0) {
+    const int saved = ctx->ppr_tile;
+    ctx->ppr_tile = 0;
+    st = pick_tile(ctx, g, c, 64, &T);
+    ctx->ppr_tile = saved;
+    if (st != GOPS_OK) return st;
+  }
   if (!(T == 8 || T == 16 || T == 32 || T == 64))
     return sfail(ctx, GOPS_INVALID_PARAMETER, "tile must be 8, 16, 32 or 64");
   // admission by HBM: as many tiles as the budget holds

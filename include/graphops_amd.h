@@ -189,6 +189,15 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t num_qu
                                const double *seed_weights, double *scores,
                                gops_run_info *infos);
 
+/* Tile width a batch of num_queries queries runs with on g under ctx's
+ * options (0: the one-query-per-run path), and the autotuner's measured cost
+ * of widths 8, 16, 32 and 64 in ms per query-sweep (nullable, 4 doubles;
+ * infinity where a width does not fit in device memory).  Runs the
+ * autotuner's trial on g if it has not run yet.  No reference counterpart
+ * (diagnostics of the batched path). */
+gops_status gops_ppr_tile_info(gops_context *ctx, gops_graph *g, uint32_t num_queries,
+                               uint32_t *tile, double *ms_per_query_sweep);
+
 /* ---- serving loop (continuous batching of PPR queries) ---------------- */
 typedef struct gops_server_config {
     uint32_t max_slots;      /* query slots resident on the device (0 = from HBM budget) */
