@@ -8,10 +8,16 @@ f64 anchor sweeps, f64 residual / dangling sums.  One step = one complete
 pagerank_weighted_run with the graph already resident in HBM (results left
 on the device).  value = edges * sweeps / seconds (GTEPS per iteration).
 
-Also reported: the roofline of the dominant kernel (f32 delta sweep,
-algorithmic bytes (4 + 2) * m + 16 * n + 4 per sweep against 8 TB/s), the
-error against the f64 oracle, and the oracle's single-core CPU baseline on a
-bounded sample of sweeps.
+Also reported: the roofline of the dominant kernel (the sweep kernel over
+all its sweeps, algorithmic bytes (4 + 2) * m + 16 * n + 4 per sweep against
+8 TB/s, durations from HIP events on the library's stream), the error against
+the f64 oracle, the oracle's single-core CPU baseline on a bounded sample of
+sweeps, and two more legs on an R-MAT scale-22 graph: batched personalized
+PageRank (1024 queries) against the plain one-query-per-run path, and a
+serving run with Poisson arrivals.
+
+--copy-probe BYTES runs a plain device copy of BYTES instead (the
+calibration of the FETCH_SIZE / WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -25,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 CONFIGS = {
     # scale, seed, weights, storage, tol, max_it, w_bytes
@@ -208,6 +215,31 @@ def ppr_legs(args, ga, generators):
     return out
 
 
+def copy_probe(nbytes, reps=5):
+    """reps device copies of nbytes (torch's copy of a contiguous f32
+    tensor): the known-size launch that scales the byte counters."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes // 4 * 4, "reps": reps}), flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per sweep launch of this workload from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE scaled by a copy of known
+    size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("bytes_per_launch") if t.get("workload") == workload else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,15 +251,23 @@ def main():
     ap.add_argument("--no-ppr", action="store_true", help="skip the batched PPR and serving legs")
     ap.add_argument("--ppr-queries", type=int, default=1024)
     ap.add_argument("--serve-queries", type=int, default=1024)
+    ap.add_argument("--ppr-only", action="store_true", help="only the PPR and serving legs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is designed, not built)"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import graphops_amd as ga
     from graphops_amd import generators
 
+    if args.ppr_only:
+        print(json.dumps(ppr_legs(args, ga, generators)), flush=True)
+        return
     c = CONFIGS[args.config]
     t0 = time.perf_counter()
     e = generators.rmat(c["scale"], 16, 0.57, 0.19, 0.19, c["seed"], c["weights"], c["p0"],
@@ -262,25 +302,31 @@ def main():
     alg_bytes = (4 + c["w_bytes"]) * m + 16 * n + 4
     avg_f32 = f32_ms / max(n32, 1) / 1e3
     avg_f64 = f64_ms / max(n64, 1) / 1e3
-    achieved = alg_bytes / avg_f32 / 1e9 if n32 else None
+    # every sweep (f32 delta and f64 anchor) is a launch of the same kernel
+    avg_sweep = (f32_ms + f64_ms) / max(n32 + n64, 1) / 1e3
+    achieved = alg_bytes / avg_sweep / 1e9 if n32 + n64 else None
+    wname = "Bf16W" if c["w_bytes"] == 2 else "float"
     out = {
         "metric": "PageRank GTEPS/iter (% HBM roofline) + batched PPR queries/s @ L1<=1e-6",
         "value": round(value, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (bf16 weights; f64 anchors, residual, dangling sums)"
-        if c["w_bytes"] == 2 else "f32 (f32 weights; f64 anchors, residual, dangling sums)",
+        "dtype": "f32",
         "data": "synthetic R-MAT (own counter-RNG generator)",
         "config": {"workload": c["workload"], "n": n, "m": m, "sweeps_per_run": r.iterations,
                    "f32_sweeps_per_run": st["sweeps_fp32"], "f64_sweeps_per_run": st["sweeps_fp64"],
                    "converged": r.converged, "diff_l1": r.diff_l1,
+                   "precision": f"{'bf16' if c['w_bytes'] == 2 else 'f32'} weights, f32 delta "
+                                "sweeps with f64 sums, f64 anchor sweeps, f64 iterate, residual "
+                                "and dangling sums",
+                   "sweep_us": round(avg_sweep * 1e6, 2),
                    "f32_sweep_us": round(avg_f32 * 1e6, 2), "f64_sweep_us": round(avg_f64 * 1e6, 2),
                    "gteps_f32_sweep": round(m / avg_f32 / 1e9, 2) if n32 else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": "sweep_f32_kernel<Bf16W>"
-                     if c["w_bytes"] == 2 else "sweep_f32_kernel<float>",
+                     "traffic": measured_traffic(c["workload"]),
+                     "kernel": f"gops::sweep_kernel<gops::{wname}>",
                      "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_accuracy:
@@ -294,7 +340,7 @@ def main():
         out["accuracy"] = {"l1_vs_f64_oracle": float(diff.sum()), "max_abs": float(diff.max()),
                            "top100_agreement": len(top & top_ref) / 100.0,
                            "oracle_iterations": ref.iterations, "gpu_iterations": r.iterations}
-    out["cpu_baseline"] = cpu_baseline(e, c, args.cpu_sweeps)
+    out["cpu_baseline"] = cpu_baseline(e, c, args.cpu_sweeps) if args.cpu_sweeps > 0 else None
     if not args.no_ppr:
         dg.close()
         del e

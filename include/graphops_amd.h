@@ -83,16 +83,18 @@ enum {
     GOPS_OPT_NORMALIZE = 3,     /* 1: rescale the result to sum to 1 in f64 (default 0,
                                    the reference never renormalises, lib.rs:11-12) */
     GOPS_OPT_TIMING = 4,        /* 1: time every sweep kernel with HIP events */
-    GOPS_OPT_PPR_TILE = 5,      /* batched PPR: queries per tile, 8/16/32/64 (0 = chosen
-                                   by query count; -1 = one query per device run) */
+    GOPS_OPT_PPR_TILE = 5,      /* batched PPR: queries per tile, 8/16/32/64 (0 = the
+                                   autotuned width, see gops_ppr_tile_info; -1 = one
+                                   query per device run) */
     GOPS_OPT_ANCHOR_DROP = 6,   /* mixed, global PageRank: re-anchor in f64 once the
                                    residual has dropped by this factor since the last
                                    anchor (default 1e-2) */
     GOPS_OPT_F32_BELOW = 7,     /* mixed, global PageRank: f64 sweeps while the residual
                                    is above this times the tolerance (default 0: none) */
-    GOPS_OPT_PPR_F32_BELOW = 8, /* mixed, personalized: same (default 1e6) */
+    GOPS_OPT_PPR_F32_BELOW = 8, /* mixed, personalized: same (default 1e4) */
     GOPS_OPT_PPR_ANCHOR_PERIOD = 9 /* mixed, personalized: f32 sweeps between f64
-                                   anchors (default 64) */
+                                   anchors while the residual is above 1e3 times
+                                   the tolerance (default 64) */
 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
 
