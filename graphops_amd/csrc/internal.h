@@ -15,7 +15,7 @@ constexpr int kBlock = 256;          // threads per workgroup (4 wave64s)
 constexpr int kStageNnz = 4096;      // edges of a run of rows staged in LDS (f32: 16 KB)
 constexpr int kHubChunk = 65536;     // rows longer than this are split into chunks of it
 constexpr int kRowIters = 8;         // row iterations per group in one run
-constexpr int kUnroll = 8;           // independent edge loads in flight per lane
+constexpr int kUnroll = 16;          // independent edge loads in flight per lane
 
 // Work descriptor of one workgroup in a sweep (rows sorted by in-degree):
 //   ROWS : x = log2(lanes per row), y = first row, z = end row; the run has

@@ -7,12 +7,16 @@
 namespace gops {
 
 constexpr int kMaxTile = 64;  // queries per tile, at most
+constexpr int kLaneQ = 4;     // queries per lane in the tile sweep (one 16-byte gather)
 
 // Device state of one tile (T <= kMaxTile queries).  The host writes it while
 // no kernel of the tile is queued; the finish kernel updates it every sweep.
 struct TileState {
   int32_t yd_cur;        // yd buffer the next sweep gathers
   int32_t all_done;      // no live query left
+  int32_t y64_par;       // y64 buffer (0: y64a, 1: y64b) the next sweep's anchors gather;
+                         // flips every sweep, so each sweep writes the other one
+  int32_t pad0;
   uint64_t sweep;        // sweeps executed by the tile
   uint64_t anchor_period;
   double anchor_drop;    // 0: no residual-triggered anchors
@@ -33,7 +37,6 @@ struct TileState {
   uint64_t since_anchor[kMaxTile];
   int32_t mode[kMaxTile];  // mode of the query's next sweep (kModeDelta / DeltaFold / Anchor)
   int32_t f64_phase[kMaxTile];  // still in the leading all-f64 phase
-  int32_t y64_cur[kMaxTile];    // y64 buffer the query's next anchor gathers
   int32_t live[kMaxTile];
   int32_t converged[kMaxTile];
 };
@@ -50,7 +53,7 @@ struct PprArgs {
   float *yd0, *yd1;     // [n][T] prescaled deltas (f32), ping-pong
   const float *yin32;   // picked per sweep
   float *yout32;
-  double *y64a, *y64b;  // [n][T] prescaled iterate for anchor sweeps, ping-pong per query
+  double *y64a, *y64b;  // [n][T] prescaled iterate for anchor sweeps, ping-pong
   double *partials;     // [grid][T][5]
   double *hub_partial;  // [chunks][T]
   const int32_t *hub_row, *hub_nchunks, *hub_first;

@@ -75,7 +75,7 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *
       if (a.seed_q[j] == q) seed_p += a.seed_p[j];
   }
   const size_t at = (size_t)r * T + q;
-  const int c64 = ts->y64_cur[q];
+  const int c64 = ts->y64_par;  // this sweep's anchors gather y64[c64]; the next gathers the other
   // every path sets all five contributions; one update of red at the end
   double delta, dang_x = 0.0, sum_x = 0.0;
   bool dangling;
@@ -100,7 +100,7 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *
     dangling = inv == 0.0f;
     if (mode == kModeDeltaFold) {
       const double inv64 = a.inv64[r];
-      (c64 ? a.y64b : a.y64a)[at] = (d * x) * inv64;
+      (c64 ? a.y64a : a.y64b)[at] = (d * x) * inv64;  // the anchor that follows
       dang_x = inv64 == 0.0 ? x : 0.0;
       sum_x = x;
     }
@@ -141,88 +141,188 @@ __device__ __forceinline__ void tile_block_sum(Red &red, double *out) {
   __syncthreads();
 }
 
-// Sum of v over the block's lanes with the same query (lane % T), for
-// block-wide rows; valid in threads < T.
+// ---------------------------------------------------------------------------
+// The tile sweep.  Lane layout: kLaneQ (4) consecutive queries per lane, so
+// Q = T / 4 lanes cover one edge's T queries with one 16-byte gather each
+// (a float4 of deltas, or two double2 of the f64 iterate); a wave gathers
+// 64 / Q edges per load instruction.  A row gets 2^s slices of Q lanes (slice
+// i sums edges i, i + 2^s, ...), and the slices are summed by lane shuffles.
+
+// The four queries of a lane: sums, modes.
+struct Q4 {
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+};
+
+struct LaneModes {
+  bool a0, a1, a2, a3;  // query j of the lane is in an f64 anchor sweep
+  bool need32, need64;  // some query of the lane gathers f32 deltas / the f64 iterate
+};
+
 template <int T>
-__device__ __forceinline__ double tile_query_sum(double v) {
-  __shared__ double sh[kBlock / 64][T];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int lb = 0;
-  while ((1 << lb) < T) lb++;
-  const double s = lane_bits_sum(v, lb, 6);
-  if (lane < T) sh[wave][lane] = s;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x < T) r = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
-  __syncthreads();
-  return r;
+__device__ __forceinline__ void add_f32(const PprArgs &a, int c, int q0, Q4 &acc) {
+  const float4 f = *reinterpret_cast<const float4 *>(a.yin32 + (size_t)c * T + q0);
+  acc.v0 += (double)f.x;
+  acc.v1 += (double)f.y;
+  acc.v2 += (double)f.z;
+  acc.v3 += (double)f.w;
 }
 
-// Gather-sum of one query's slice of a row: f64 iterate for a query in an
-// anchor sweep, f32 delta otherwise.
 template <int T>
-__device__ __forceinline__ double tile_row_sum(const PprArgs &a, const double *y64, bool anchor,
-                                               int b, int e, int step, int q) {
-  // four edges per iteration: their column loads, then their gathers, are in
-  // flight together
-  double acc = 0.0;
+__device__ __forceinline__ void add_f64(const double *y64, int c, int q0, Q4 &acc) {
+  const double2 lo = *reinterpret_cast<const double2 *>(y64 + (size_t)c * T + q0);
+  const double2 hi = *reinterpret_cast<const double2 *>(y64 + (size_t)c * T + q0 + 2);
+  acc.v0 += lo.x;
+  acc.v1 += lo.y;
+  acc.v2 += hi.x;
+  acc.v3 += hi.y;
+}
+
+// queries of the lane in different modes: each takes its own vector
+template <int T>
+__device__ __forceinline__ void add_mixed(const PprArgs &a, const double *y64, const LaneModes &m,
+                                          int c, int q0, Q4 &acc) {
+  const size_t off = (size_t)c * T + q0;
+  const float4 f = *reinterpret_cast<const float4 *>(a.yin32 + off);
+  const double2 lo = *reinterpret_cast<const double2 *>(y64 + off);
+  const double2 hi = *reinterpret_cast<const double2 *>(y64 + off + 2);
+  acc.v0 += m.a0 ? lo.x : (double)f.x;
+  acc.v1 += m.a1 ? lo.y : (double)f.y;
+  acc.v2 += m.a2 ? hi.x : (double)f.z;
+  acc.v3 += m.a3 ? hi.y : (double)f.w;
+}
+
+// Sums of one lane's slice of a row: edges b, b + step, ... < e, added in
+// edge order.  f32 lanes keep four edges' loads in flight, f64 and mixed
+// lanes two.
+template <int T>
+__device__ __forceinline__ Q4 row_slice_sum(const PprArgs &a, const double *y64,
+                                            const LaneModes &m, int b, int e, int step, int q0) {
+  Q4 acc;
   int k = b;
-  if (anchor) {
+  if (!m.need64) {
     for (; k + 3 * step < e; k += 4 * step) {
       const int c0 = a.col[k], c1 = a.col[k + step], c2 = a.col[k + 2 * step],
                 c3 = a.col[k + 3 * step];
-      const double v0 = y64[(size_t)c0 * T + q], v1 = y64[(size_t)c1 * T + q],
-                   v2 = y64[(size_t)c2 * T + q], v3 = y64[(size_t)c3 * T + q];
-      acc += (v0 + v1) + (v2 + v3);
+      add_f32<T>(a, c0, q0, acc);
+      add_f32<T>(a, c1, q0, acc);
+      add_f32<T>(a, c2, q0, acc);
+      add_f32<T>(a, c3, q0, acc);
     }
-    for (; k < e; k += step) acc += y64[(size_t)a.col[k] * T + q];
+    for (; k < e; k += step) add_f32<T>(a, a.col[k], q0, acc);
+  } else if (!m.need32) {
+    for (; k + step < e; k += 2 * step) {
+      const int c0 = a.col[k], c1 = a.col[k + step];
+      add_f64<T>(y64, c0, q0, acc);
+      add_f64<T>(y64, c1, q0, acc);
+    }
+    for (; k < e; k += step) add_f64<T>(y64, a.col[k], q0, acc);
   } else {
-    for (; k + 3 * step < e; k += 4 * step) {
-      const int c0 = a.col[k], c1 = a.col[k + step], c2 = a.col[k + 2 * step],
-                c3 = a.col[k + 3 * step];
-      const float v0 = a.yin32[(size_t)c0 * T + q], v1 = a.yin32[(size_t)c1 * T + q],
-                  v2 = a.yin32[(size_t)c2 * T + q], v3 = a.yin32[(size_t)c3 * T + q];
-      acc += ((double)v0 + (double)v1) + ((double)v2 + (double)v3);
+    for (; k + step < e; k += 2 * step) {
+      const int c0 = a.col[k], c1 = a.col[k + step];
+      add_mixed<T>(a, y64, m, c0, q0, acc);
+      add_mixed<T>(a, y64, m, c1, q0, acc);
     }
-    for (; k < e; k += step) acc += (double)a.yin32[(size_t)a.col[k] * T + q];
+    for (; k < e; k += step) add_mixed<T>(a, y64, m, a.col[k], q0, acc);
   }
   return acc;
 }
 
+__device__ __forceinline__ void q4_lane_sum(Q4 &v, int lo_bit, int hi_bit) {
+  v.v0 = lane_bits_sum(v.v0, lo_bit, hi_bit);
+  v.v1 = lane_bits_sum(v.v1, lo_bit, hi_bit);
+  v.v2 = lane_bits_sum(v.v2, lo_bit, hi_bit);
+  v.v3 = lane_bits_sum(v.v3, lo_bit, hi_bit);
+}
+
+// Sum over the block's slices of Q lanes (threads with the same tid % Q);
+// valid in threads < Q.
+template <int T>
+__device__ __forceinline__ Q4 block_slices_sum(Q4 v) {
+  constexpr int Q = T / kLaneQ;
+  __shared__ double sh[kBlock / 64][Q][kLaneQ];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int lq = 0;
+  while ((1 << lq) < Q) lq++;
+  q4_lane_sum(v, lq, 6);
+  if (lane < Q) {
+    sh[wave][lane][0] = v.v0;
+    sh[wave][lane][1] = v.v1;
+    sh[wave][lane][2] = v.v2;
+    sh[wave][lane][3] = v.v3;
+  }
+  __syncthreads();
+  Q4 r;
+  if (threadIdx.x < Q) {
+    const int t = threadIdx.x;
+    r.v0 = (sh[0][t][0] + sh[1][t][0]) + (sh[2][t][0] + sh[3][t][0]);
+    r.v1 = (sh[0][t][1] + sh[1][t][1]) + (sh[2][t][1] + sh[3][t][1]);
+    r.v2 = (sh[0][t][2] + sh[1][t][2]) + (sh[2][t][2] + sh[3][t][2]);
+    r.v3 = (sh[0][t][3] + sh[1][t][3]) + (sh[2][t][3] + sh[3][t][3]);
+  }
+  __syncthreads();
+  return r;
+}
+
+// Row sums leave the gather phase in the slice-0 lanes, four queries each;
+// they go through LDS so that the epilogue runs one query per lane (lane
+// q = tid % T always handles query q), with one query's state in registers.
 template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
+  constexpr int Q = T / kLaneQ;  // lanes per edge
+  __shared__ double rs[kBlock * kLaneQ];  // row sums of a pass: rows (<= kBlock / Q) x T
   const TileState *ts = a.ts;
   if (ts->all_done) return;
   const int ycur = ts->yd_cur;
   a.yin32 = ycur ? a.yd1 : a.yd0;
   a.yout32 = ycur ? a.yd0 : a.yd1;
+  const double *y64 = ts->y64_par ? a.y64b : a.y64a;
+  // gather phase: queries q0 .. q0 + 3 of this lane
+  const int qg = threadIdx.x % Q;
+  const int q0 = qg * kLaneQ;
+  LaneModes m;
+  m.a0 = ts->mode[q0] == kModeAnchor;
+  m.a1 = ts->mode[q0 + 1] == kModeAnchor;
+  m.a2 = ts->mode[q0 + 2] == kModeAnchor;
+  m.a3 = ts->mode[q0 + 3] == kModeAnchor;
+  m.need64 = m.a0 || m.a1 || m.a2 || m.a3;
+  m.need32 = !(m.a0 && m.a1 && m.a2 && m.a3);
+  // epilogue phase: query q of this lane
   const int q = threadIdx.x % T;
-  const int mode = ts->mode[q];  // every query has its own sweep mode
-  const bool anchor = mode == kModeAnchor;
-  const double *y64 = ts->y64_cur[q] ? a.y64b : a.y64a;
+  const int mode = ts->mode[q];
+  int lq = 0;
+  while ((1 << lq) < Q) lq++;
   Red red;
   for (int di = blockIdx.x; di < a.num_desc; di += gridDim.x) {
     const BlockDesc bd = a.desc[di];
     if ((bd.x & (kHubFlag | kRowFlag)) == 0) {
-      // a run of rows, L = T * 2^s lanes per row (s slices of the edges)
+      // a run of rows, L = Q * 2^s lanes per row; ngrp rows per pass
       const int ls = bd.x;
-      const int L = T << ls;
+      const int L = Q << ls;
       const int g = threadIdx.x / L;
-      const int slice = (threadIdx.x % L) / T;
+      const int slice = (threadIdx.x % L) / Q;
       const int ngrp = kBlock / L;
-      int lb = 0;
-      while ((1 << lb) < T) lb++;
       for (int base = bd.y; base < bd.z; base += ngrp) {
         const int r = base + g;
-        double acc = 0.0;
+        Q4 acc;
         if (r < bd.z)
-          acc = tile_row_sum<T>(a, y64, anchor, a.row_ptr[r] + slice, a.row_ptr[r + 1], 1 << ls, q);
-        acc = lane_bits_sum(acc, lb, lb + ls < 6 ? lb + ls : 6);  // T << ls <= 64
-        if (r < bd.z && slice == 0) ppr_epilogue<T>(a, ts, mode, r, q, acc, red);
+          acc = row_slice_sum<T>(a, y64, m, a.row_ptr[r] + slice, a.row_ptr[r + 1], 1 << ls, q0);
+        q4_lane_sum(acc, lq, lq + ls < 6 ? lq + ls : 6);  // Q << ls <= 64
+        if (slice == 0) {
+          double *o = rs + g * T + q0;
+          o[0] = acc.v0;
+          o[1] = acc.v1;
+          o[2] = acc.v2;
+          o[3] = acc.v3;
+        }
+        __syncthreads();
+        const int rows = bd.z - base < ngrp ? bd.z - base : ngrp;
+        for (int p = threadIdx.x; p < rows * T; p += kBlock)
+          ppr_epilogue<T>(a, ts, mode, base + p / T, q, rs[p], red);
+        __syncthreads();
       }
     } else {
       // one long row (or one chunk of a hub row) over the whole block
-      const int slice = threadIdx.x / T;
+      const int slice = threadIdx.x / Q;
       int b, e, r = -1;
       if (bd.x & kRowFlag) {
         r = bd.y;
@@ -232,16 +332,24 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
         b = bd.z;
         e = bd.w;
       }
-      double acc = tile_row_sum<T>(a, y64, anchor, b + slice, e, kBlock / T, q);
-      acc = tile_query_sum<T>(acc);
+      const Q4 acc = block_slices_sum<T>(row_slice_sum<T>(a, y64, m, b + slice, e, kBlock / Q, q0));
+      if (threadIdx.x < Q) {
+        double *o = rs + q0;
+        o[0] = acc.v0;
+        o[1] = acc.v1;
+        o[2] = acc.v2;
+        o[3] = acc.v3;
+      }
+      __syncthreads();
       if (threadIdx.x < T) {
         if (r >= 0) {
-          ppr_epilogue<T>(a, ts, mode, r, q, acc, red);
+          ppr_epilogue<T>(a, ts, mode, r, q, rs[q], red);
         } else {
           const int h = bd.y, chunk = bd.x & (kHubFlag - 1);
-          a.hub_partial[(size_t)(a.hub_first[h] + chunk) * T + q] = acc;
+          a.hub_partial[(size_t)(a.hub_first[h] + chunk) * T + q] = rs[q];
         }
       }
+      __syncthreads();
     }
   }
   tile_block_sum<T>(red, a.partials + (size_t)blockIdx.x * T * 5);
@@ -294,8 +402,7 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
       ts->x_scale[qq] = 1.0;
       ts->since_anchor[qq] = 0;
       if (ts->f64_phase[qq] && diff > ts->f32_below * ts->tol[qq]) {
-        ts->y64_cur[qq] ^= 1;  // this anchor wrote the next one's input
-        next = kModeAnchor;
+        next = kModeAnchor;  // this anchor wrote the next one's input
       } else {
         ts->f64_phase[qq] = 0;
         next = kModeDelta;
@@ -327,6 +434,7 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
     int live = 0;
     for (int i = 0; i < T; i++) live += ts->live[i];
     ts->yd_cur = ycur ^ 1;
+    ts->y64_par ^= 1;
     ts->sweep += 1;
     ts->all_done = live == 0;
     if (a.mirror)
@@ -341,12 +449,13 @@ template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uint64_t clear_mask) {
   const int q = threadIdx.x % T;
   if (!((clear_mask >> q) & 1ull)) return;
+  double *y64 = a.ts->y64_par ? a.y64b : a.y64a;  // the admitted query's first anchor gathers it
   const int rows_per_block = kBlock / T;
   for (int r = blockIdx.x * rows_per_block + threadIdx.x / T; r < n;
        r += gridDim.x * rows_per_block) {
     const size_t at = (size_t)r * T + q;
     a.x[at] = 0.0;
-    a.y64a[at] = 0.0;  // the admitted query's first anchor gathers y64a
+    y64[at] = 0.0;
   }
 }
 
@@ -355,6 +464,7 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
 template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t admit_mask) {
   TileState *ts = a.ts;
+  double *y64 = ts->y64_par ? a.y64b : a.y64a;
   for (int j = threadIdx.x; j < a.num_seed_entries; j += kBlock) {
     const int qq = a.seed_q[j];
     if (!((admit_mask >> qq) & 1ull)) continue;
@@ -362,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
     const size_t at = (size_t)r * T + qq;
     const double p = a.seed_p[j];
     a.x[at] = p;  // one entry per (row, query): the host merges duplicate seeds
-    a.y64a[at] = (ts->damping[qq] * p) * a.inv64[r];
+    y64[at] = (ts->damping[qq] * p) * a.inv64[r];
   }
   __syncthreads();
   if (threadIdx.x < T) {
@@ -376,7 +486,6 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
       ts->diff[qq] = 0.0;
       ts->mode[qq] = kModeAnchor;
       ts->since_anchor[qq] = 0;
-      ts->y64_cur[qq] = 0;
       ts->f64_phase[qq] = ts->f32_below > 0.0;
     }
   }

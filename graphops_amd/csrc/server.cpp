@@ -48,15 +48,19 @@ inline int lanes_pow2(int want, int cap) {
 }
 
 // Work descriptors of the tile sweep for tile width T (rows already sorted
-// by in-degree): runs of rows with 2^s slices of T lanes per row, whole-block
-// rows, and the graph's hub chunks (rows longer than kHubChunk).
+// by in-degree).  One edge's T queries take Q = T / kLaneQ lanes; a row gets
+// 2^s slices of Q lanes, about four edges per slice.  Runs of rows share a
+// slice count; rows long enough to give every slice of a workgroup four
+// edges get the whole workgroup; rows longer than kHubChunk are split into
+// the graph's hub chunks.
 void plan_tile(const std::vector<int32_t> &rp, int T, const DevGraph &dg,
                const std::vector<int32_t> &hub_row, const std::vector<int32_t> &hub_nchunks,
                std::vector<BlockDesc> &desc) {
   desc.clear();
   const int64_t n = (int64_t)rp.size() - 1;
-  const int smax = 64 / T;  // slices of a row within one wave
-  const int block_row_min = 16 * smax;
+  const int Q = T / kLaneQ;
+  const int smax = 64 / Q;  // slices of a row within one wave
+  const int block_row_min = 4 * (kBlock / Q);
   int h = 0;
   int64_t i = 0;
   while (i < n) {
@@ -77,16 +81,16 @@ void plan_tile(const std::vector<int32_t> &rp, int T, const DevGraph &dg,
       i++;
       continue;
     }
-    const int S = lanes_pow2((deg + 7) / 8, smax);
+    const int S = lanes_pow2((deg + 3) / 4, smax);
     int ls = 0;
     while ((1 << ls) < S) ls++;
-    const int L = T << ls;
+    const int L = Q << ls;
     const int64_t max_rows = (int64_t)(kBlock / L) * kRowIters;
     const int64_t s = i;
     int64_t nnz = 0;
     while (i < n && i - s < max_rows) {
       const int32_t d = rp[i + 1] - rp[i];
-      if (d > block_row_min || lanes_pow2((d + 7) / 8, smax) != S) break;
+      if (d > block_row_min || lanes_pow2((d + 3) / 4, smax) != S) break;
       if (i > s && nnz + d > 8192) break;
       nnz += d;
       i++;
