@@ -8,6 +8,7 @@ namespace gops {
 
 constexpr int kMaxTile = 64;  // queries per tile, at most
 constexpr int kLaneQ = 4;     // queries per lane in the tile sweep (one 16-byte gather)
+constexpr int kTileReduce = 64;  // workgroups of the first stage of the per-query sums
 
 // Device state of one tile (T <= kMaxTile queries).  The host writes it while
 // no kernel of the tile is queued; the finish kernel updates it every sweep.
@@ -54,7 +55,8 @@ struct PprArgs {
   const float *yin32;   // picked per sweep
   float *yout32;
   double *y64a, *y64b;  // [n][T] prescaled iterate for anchor sweeps, ping-pong
-  double *partials;     // [grid][T][5]
+  double *partials;     // [grid][T][5] per sweep workgroup, then [kTileReduce][T][5]
+                        // after the first reduction stage (at kMaxSweepBlocks * T * 5)
   double *hub_partial;  // [chunks][T]
   const int32_t *hub_row, *hub_nchunks, *hub_first;
   int num_hubs;
@@ -63,7 +65,7 @@ struct PprArgs {
   const int32_t *seed_q;
   const double *seed_p;     // normalised personalization value (ppr.rs:104-109)
   int num_seed_entries;
-  uint64_t *mirror;         // mapped host word: (sweeps << 1) | all_done
+  uint64_t *mirror;         // mapped host word: (sweeps << 8) | (live queries << 1) | all_done
 };
 
 hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid);

@@ -355,6 +355,22 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
   tile_block_sum<T>(red, a.partials + (size_t)blockIdx.x * T * 5);
 }
 
+// First stage of the per-query sums: workgroup b adds the partials of sweep
+// workgroups [b * per, (b + 1) * per) in order.
+template <int T>
+__global__ __launch_bounds__(kBlock) void ppr_reduce_kernel(PprArgs a, int sweep_blocks) {
+  if (a.ts->all_done) return;
+  const int per = (sweep_blocks + kTileReduce - 1) / kTileReduce;
+  const int b0 = blockIdx.x * per;
+  const int b1 = b0 + per < sweep_blocks ? b0 + per : sweep_blocks;
+  double *stage = a.partials + (size_t)kMaxSweepBlocks * T * 5;
+  for (int v = threadIdx.x; v < T * 5; v += kBlock) {
+    double s = 0.0;
+    for (int b = b0; b < b1; b++) s += a.partials[(size_t)b * T * 5 + v];
+    stage[(size_t)blockIdx.x * T * 5 + v] = s;
+  }
+}
+
 // Per-query bookkeeping (pagerank.rs:163-179 for every query) and each
 // query's next mode (the single-vector schedule of pagerank.hip).  One
 // workgroup.
@@ -376,10 +392,11 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
     for (int c = 0; c < nch; c++) s += a.hub_partial[(size_t)(first + c) * T + q];
     ppr_epilogue<T>(a, ts, mode, a.hub_row[h], q, s, red);
   }
-  for (int p = threadIdx.x; p < sweep_blocks * T; p += kBlock) {
+  const double *stage = a.partials + (size_t)kMaxSweepBlocks * T * 5;
+  for (int p = threadIdx.x; p < kTileReduce * T; p += kBlock) {
     const int b = p / T;
     {
-      const double *pp = a.partials + ((size_t)b * T + q) * 5;
+      const double *pp = stage + ((size_t)b * T + q) * 5;
       red.l1 += pp[0];
       red.dang += pp[1];
       red.dang_x += pp[2];
@@ -438,24 +455,41 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
     ts->sweep += 1;
     ts->all_done = live == 0;
     if (a.mirror)
-      __hip_atomic_store(a.mirror, (ts->sweep << 1) | (uint64_t)(live == 0), __ATOMIC_RELAXED,
+      __hip_atomic_store(a.mirror, (ts->sweep << 8) | ((uint64_t)live << 1) | (uint64_t)(live == 0),
+                         __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
-// Admission, step 1: clear the admitted queries' columns (x = 0, y64 = 0).
-// The tile's other queries are not touched.
+// Admission, step 1: clear the admitted queries' columns (x = 0, and 0 in
+// the y64 buffer the next sweep's anchors gather).  One thread per row and
+// group of four queries; the tile's other queries are not touched.
 template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uint64_t clear_mask) {
-  const int q = threadIdx.x % T;
-  if (!((clear_mask >> q) & 1ull)) return;
-  double *y64 = a.ts->y64_par ? a.y64b : a.y64a;  // the admitted query's first anchor gathers it
-  const int rows_per_block = kBlock / T;
-  for (int r = blockIdx.x * rows_per_block + threadIdx.x / T; r < n;
+  constexpr int Q = T / kLaneQ;
+  const int qg = threadIdx.x % Q;
+  const unsigned m4 = (unsigned)(clear_mask >> (qg * kLaneQ)) & 0xFu;
+  if (!m4) return;
+  double *y64 = a.ts->y64_par ? a.y64b : a.y64a;
+  const int rows_per_block = kBlock / Q;
+  for (int r = blockIdx.x * rows_per_block + threadIdx.x / Q; r < n;
        r += gridDim.x * rows_per_block) {
-    const size_t at = (size_t)r * T + q;
-    a.x[at] = 0.0;
-    y64[at] = 0.0;
+    const size_t at = (size_t)r * T + qg * kLaneQ;
+    if (m4 == 0xFu) {
+      const double2 z = {0.0, 0.0};
+      *reinterpret_cast<double2 *>(a.x + at) = z;
+      *reinterpret_cast<double2 *>(a.x + at + 2) = z;
+      *reinterpret_cast<double2 *>(y64 + at) = z;
+      *reinterpret_cast<double2 *>(y64 + at + 2) = z;
+    } else {
+#pragma unroll
+      for (int j = 0; j < kLaneQ; j++) {
+        if ((m4 >> j) & 1u) {
+          a.x[at + j] = 0.0;
+          y64[at + j] = 0.0;
+        }
+      }
+    }
   }
 }
 
@@ -520,6 +554,7 @@ __global__ void seed_mark_kernel(int32_t *seed_idx, const int32_t *rows, int cou
 template <int T>
 static hipError_t launch_sweep_t(hipStream_t s, const PprArgs &a, int grid) {
   ppr_sweep_kernel<T><<<grid, kBlock, 0, s>>>(a);
+  ppr_reduce_kernel<T><<<kTileReduce, kBlock, 0, s>>>(a, grid);
   ppr_finish_kernel<T><<<1, kBlock, 0, s>>>(a, grid);
   return hipGetLastError();
 }
@@ -536,7 +571,7 @@ hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid) {
 template <int T>
 static hipError_t launch_admit_t(hipStream_t s, const PprArgs &a, int n, uint64_t clear_mask,
                                  uint64_t admit_mask) {
-  const int rows_per_block = kBlock / T;
+  const int rows_per_block = kBlock / (T / kLaneQ);
   int grid = (n + rows_per_block - 1) / rows_per_block;
   grid = std::max(1, std::min(grid, 4096));
   ppr_clear_kernel<T><<<grid, kBlock, 0, s>>>(a, n, clear_mask);

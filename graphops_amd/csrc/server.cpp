@@ -260,7 +260,7 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile *
   chk(hipMalloc((void **)&t->y64b, std::max<size_t>(nt, 1) * 8));
   chk(hipMalloc((void **)&t->yd0, std::max<size_t>(nt, 1) * 4));
   chk(hipMalloc((void **)&t->yd1, std::max<size_t>(nt, 1) * 4));
-  chk(hipMalloc((void **)&t->partials, (size_t)std::max(grid, 2048) * T * 5 * 8));
+  chk(hipMalloc((void **)&t->partials, (size_t)(kMaxSweepBlocks + kTileReduce) * T * 5 * 8));
   chk(hipMalloc((void **)&t->hub_partial, (size_t)std::max(g->dev.num_hub_chunks, 1) * T * 8));
   chk(hipMalloc((void **)&t->seed_idx, std::max<size_t>(t->n, 1) * 4));
   chk(hipMalloc((void **)&t->ts, sizeof(TileState)));
@@ -565,64 +565,87 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
     free_tile(t);
     return sfail(ctx, GOPS_OUT_OF_MEMORY, "batch output buffer: out of device memory");
   }
+  // Continuous refill: a slot takes the next query as soon as its query has
+  // stopped.  The sweeps run until the tile is done or, while queries wait,
+  // until refill_at slots are free; then the finished queries are collected
+  // and the free slots refilled, while the others keep their state.
   std::vector<uint32_t> dense_q;  // queries that fall back to the dense uniform case
-  for (uint32_t q0 = 0; q0 < nq && st == GOPS_OK; q0 += T) {
-    const uint32_t q1 = std::min<uint32_t>(nq, q0 + T);
-    std::memset(t->hts, 0, sizeof(TileState));
-    uint64_t mask = 0, max_it = 0;
-    for (int s = 0; s < T; s++) t->used[s] = false;
-    for (uint32_t q = q0; q < q1; q++) {
-      const int s = (int)(q - q0);
-      Query qq;
-      st = make_query(ctx, g, c, &cfgs[q], seeds + seed_off[q], seed_off[q + 1] - seed_off[q],
-                      seed_w ? seed_w + seed_off[q] : nullptr, qq);
-      if (st != GOPS_OK) break;
-      if (qq.dense_fallback || cfgs[q].max_iterations == 0) {
-        dense_q.push_back(q);
-        continue;
+  std::vector<uint32_t> slot_q(T, 0);
+  std::memset(t->hts, 0, sizeof(TileState));
+  t->hts->all_done = 1;
+  for (int s = 0; s < T; s++) t->used[s] = false;
+  uint32_t next = 0;
+  const int refill_at = std::max(1, T / 4);
+  const uint64_t kAhead = 4;  // sweeps queued beyond the last one seen to finish
+  while (st == GOPS_OK) {
+    // 1. admission into free slots (the host copy of the tile state is current)
+    uint64_t mask = 0;
+    for (int s = 0; s < T && next < nq && st == GOPS_OK; s++) {
+      if (t->used[s]) continue;
+      while (next < nq) {
+        const uint32_t q = next++;
+        Query qq;
+        st = make_query(ctx, g, c, &cfgs[q], seeds + seed_off[q], seed_off[q + 1] - seed_off[q],
+                        seed_w ? seed_w + seed_off[q] : nullptr, qq);
+        if (st != GOPS_OK) break;
+        if (qq.dense_fallback || cfgs[q].max_iterations == 0) {
+          dense_q.push_back(q);
+          continue;
+        }
+        t->slot[s] = qq;
+        t->used[s] = true;
+        slot_q[s] = q;
+        set_slot_params(t, s, qq, ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
+        mask |= 1ull << s;
+        break;
       }
-      t->slot[s] = qq;
-      t->used[s] = true;
-      set_slot_params(t, s, qq, ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
-      mask |= 1ull << s;
-      max_it = std::max<uint64_t>(max_it, cfgs[q].max_iterations);
     }
-    if (st != GOPS_OK || !mask) continue;
-    st = admit(ctx, g, t, plan, mask);
     if (st != GOPS_OK) break;
-    __atomic_store_n(ctx->mirror_host, (uint64_t)0, __ATOMIC_RELEASE);
-    PprArgs a = tile_args(ctx, g, t, plan, true);
-    // same host pacing as the single-vector chain: a few sweeps queued ahead
-    const uint64_t kAhead = 4;
+    uint64_t cap = 0;  // no query runs more sweeps than its cap from now on
+    for (int s = 0; s < T; s++)
+      if (t->used[s]) cap = std::max<uint64_t>(cap, t->slot[s].cfg.max_iterations);
+    if (cap == 0) break;  // no query left
+    if (mask) {
+      st = admit(ctx, g, t, plan, mask);
+      if (st != GOPS_OK) break;
+    }
+    // (after admission: it may move the seed arrays)
+    const PprArgs a = tile_args(ctx, g, t, plan, true);
+    // 2. sweeps, a few queued ahead of the last one seen to finish
+    const uint64_t base = t->hts->sweep;
+    const bool waiting = next < nq;
+    __atomic_store_n(ctx->mirror_host, base << 8, __ATOMIC_RELEASE);
     uint64_t j = 0, seen = 0;
-    while (j < max_it && st == GOPS_OK) {
-      for (; j < max_it && j < seen + kAhead; j++) {
+    while (j < cap && st == GOPS_OK) {
+      for (; j < cap && j < seen + kAhead; j++) {
         hipError_t e = ppr_launch_sweep(ctx->stream, T, a, t->grid);
         if (e != hipSuccess) {
           st = sfail(ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
           break;
         }
       }
-      if (j >= max_it || st != GOPS_OK) break;
-      bool done = false;
+      if (j >= cap || st != GOPS_OK) break;
+      bool stop = false;
       for (uint64_t spin = 0;; spin++) {
         const uint64_t w = __atomic_load_n(ctx->mirror_host, __ATOMIC_ACQUIRE);
         if (w & 1) {
-          done = true;
+          stop = true;
           break;
         }
-        if ((w >> 1) > seen) {
-          seen = w >> 1;
+        if ((w >> 8) - base > seen) {
+          seen = (w >> 8) - base;
+          const int live = (int)((w >> 1) & 127);
+          stop = waiting && T - live >= refill_at;
           break;
         }
         if ((spin & 1023) == 1023) {
           const hipError_t qe = hipStreamQuery(ctx->stream);
           if (qe == hipSuccess) {
             const uint64_t w2 = __atomic_load_n(ctx->mirror_host, __ATOMIC_ACQUIRE);
-            done = (w2 & 1) != 0;
-            if (!done && (w2 >> 1) <= seen)
+            stop = (w2 & 1) != 0;
+            if (!stop && (w2 >> 8) - base <= seen)
               st = sfail(ctx, GOPS_DEVICE_ERROR, "tile sweep chain stalled without progress");
-            seen = w2 >> 1;
+            seen = (w2 >> 8) - base;
             break;
           }
           if (qe != hipErrorNotReady) {
@@ -632,14 +655,15 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
         }
         __builtin_ia32_pause();
       }
-      if (done) break;
+      if (stop) break;
     }
     if (st != GOPS_OK) break;
-    st = sync_state(ctx, t);
+    st = sync_state(ctx, t);  // waits for the queued sweeps
     if (st != GOPS_OK) break;
-    for (uint32_t q = q0; q < q1; q++) {
-      const int s = (int)(q - q0);
-      if (!t->used[s]) continue;
+    // 3. collect the queries that have stopped
+    for (int s = 0; s < T && st == GOPS_OK; s++) {
+      if (!t->used[s] || t->hts->live[s]) continue;
+      const uint32_t q = slot_q[s];
       if (infos) {
         infos[q].iterations = t->hts->iter[s];
         infos[q].diff_l1 = t->hts->diff[s];
@@ -654,6 +678,7 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) st = sfail(ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
       }
+      t->used[s] = false;
     }
   }
   free_tile(t);

@@ -72,6 +72,25 @@ def test_batch_matches_oracle(oracle, ctx, graph13, tile):
               cfgs[q].tolerance)
 
 
+def test_batch_refill_same_as_alone(ctx, graph13):
+    """More queries than slots: finished slots are refilled while the others
+    run on; every query's result equals the same query run alone, bit for
+    bit (a query's arithmetic does not depend on its tile mates)."""
+    e = graph13
+    nq = 24
+    off, seeds = generators.seed_sets(e, nq, 1, 32, seed=11)
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    cfgs = [ga.PageRankConfig([0.5, 0.95, 0.85][q % 3], 300, [1e-6, 1e-9][q % 2])
+            for q in range(nq)]
+    scores, runs = ga.ppr_batch_run(dg, cfgs, off, seeds, ctx=ctx, tile=8)
+    for q in [0, 1, 2, 7, 13, 23]:
+        s1, r1 = ga.ppr_batch_run(dg, [cfgs[q]], off[q:q + 2] - off[q], seeds[off[q]:off[q + 1]],
+                                  ctx=ctx, tile=8)
+        assert r1[0].iterations == runs[q].iterations
+        assert r1[0].diff_l1 == runs[q].diff_l1
+        assert np.array_equal(s1[0], scores[q])
+
+
 def test_plain_path_matches_tiles(ctx, graph13):
     """tile=-1: one query per device run (the single-vector chain)."""
     e = graph13
