@@ -65,7 +65,8 @@ struct PprArgs {
   const int32_t *seed_q;
   const double *seed_p;     // normalised personalization value (ppr.rs:104-109)
   int num_seed_entries;
-  uint64_t *mirror;         // mapped host word: (sweeps << 8) | (live queries << 1) | all_done
+  uint64_t *mirror;         // mapped host word: (sweeps << 8) | (free groups << 1) | all_done,
+                            // a group being kLaneQ consecutive slots with no live query
 };
 
 hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid);

@@ -448,16 +448,21 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    int live = 0;
-    for (int i = 0; i < T; i++) live += ts->live[i];
+    int live = 0, free_groups = 0;  // groups of kLaneQ slots (one lane's queries) with no live query
+    for (int g = 0; g < T; g += kLaneQ) {
+      int gl = 0;
+      for (int i = g; i < g + kLaneQ; i++) gl += ts->live[i];
+      live += gl;
+      free_groups += gl == 0;
+    }
     ts->yd_cur = ycur ^ 1;
     ts->y64_par ^= 1;
     ts->sweep += 1;
     ts->all_done = live == 0;
     if (a.mirror)
-      __hip_atomic_store(a.mirror, (ts->sweep << 8) | ((uint64_t)live << 1) | (uint64_t)(live == 0),
-                         __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(a.mirror,
+                         (ts->sweep << 8) | ((uint64_t)free_groups << 1) | (uint64_t)(live == 0),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 

@@ -565,23 +565,29 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
     free_tile(t);
     return sfail(ctx, GOPS_OUT_OF_MEMORY, "batch output buffer: out of device memory");
   }
-  // Continuous refill: a slot takes the next query as soon as its query has
-  // stopped.  The sweeps run until the tile is done or, while queries wait,
-  // until refill_at slots are free; then the finished queries are collected
-  // and the free slots refilled, while the others keep their state.
+  // Continuous refill by groups of kLaneQ slots (the queries one lane
+  // gathers for): a group takes the next queries once all of its queries
+  // have stopped, so a lane's queries start together and mostly sweep in the
+  // same precision.  The sweeps run until the tile is done or, while queries
+  // wait, until refill_groups groups are free; then the finished queries are
+  // collected and the free groups refilled, while the others keep their state.
   std::vector<uint32_t> dense_q;  // queries that fall back to the dense uniform case
   std::vector<uint32_t> slot_q(T, 0);
   std::memset(t->hts, 0, sizeof(TileState));
   t->hts->all_done = 1;
   for (int s = 0; s < T; s++) t->used[s] = false;
   uint32_t next = 0;
-  const int refill_at = std::max(1, T / 4);
+  const int refill_groups = std::max(1, T / kLaneQ / 4);
   const uint64_t kAhead = 4;  // sweeps queued beyond the last one seen to finish
   while (st == GOPS_OK) {
     // 1. admission into free slots (the host copy of the tile state is current)
     uint64_t mask = 0;
     for (int s = 0; s < T && next < nq && st == GOPS_OK; s++) {
       if (t->used[s]) continue;
+      const int g0 = s - s % kLaneQ;  // only groups with no query left take new ones
+      bool group_free = true;
+      for (int i = g0; i < g0 + kLaneQ; i++) group_free = group_free && (!t->used[i] || ((mask >> i) & 1));
+      if (!group_free) continue;
       while (next < nq) {
         const uint32_t q = next++;
         Query qq;
@@ -634,8 +640,8 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
         }
         if ((w >> 8) - base > seen) {
           seen = (w >> 8) - base;
-          const int live = (int)((w >> 1) & 127);
-          stop = waiting && T - live >= refill_at;
+          const int free_groups = (int)((w >> 1) & 127);
+          stop = waiting && free_groups >= refill_groups;
           break;
         }
         if ((spin & 1023) == 1023) {
