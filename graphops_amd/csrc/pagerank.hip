@@ -270,24 +270,29 @@ struct AnchorEpilogue {
   }
 };
 
-// f32 sweep: f32 operands; their product is exact in f64 and summed in f64
-// (runs of short rows stage it in LDS as f32)
+// f32 sweep: f32 operands, summed in f64 (long rows: the product in f64;
+// runs of short rows stage it in LDS as f32)
 template <typename WT>
 struct EdgeF32 {
-  static constexpr bool kStaged = true;
+  using P = float;                               // staged product
+  static constexpr int kStageUnroll = kUnroll;   // loads in flight per lane
   const SweepArgs &a;
-  __device__ __forceinline__ float weight(int k) const { return wload32<WT>(a.w, k); }
+  __device__ __forceinline__ P stage_weight(int k) const { return wload32<WT>(a.w, k); }
+  __device__ __forceinline__ P stage_gather(int c) const { return a.yin32[c]; }
   __device__ __forceinline__ double operator()(int k) const {
     const int c = __builtin_nontemporal_load(a.col + k);
     return (double)wload32<WT>(a.w, k) * (double)a.yin32[c];
   }
 };
 
+// f64 sweep: f64 products and sums (staged in LDS as f64)
 template <typename WT>
 struct EdgeF64 {
-  static constexpr bool kStaged = false;
+  using P = double;
+  static constexpr int kStageUnroll = kUnroll / 2;
   const SweepArgs &a;
-  __device__ __forceinline__ float weight(int k) const { return wload32<WT>(a.w, k); }
+  __device__ __forceinline__ P stage_weight(int k) const { return wload64<WT>(a.w, k); }
+  __device__ __forceinline__ P stage_gather(int c) const { return a.yin64[c]; }
   __device__ __forceinline__ double operator()(int k) const {
     const int c = __builtin_nontemporal_load(a.col + k);
     return wload64<WT>(a.w, k) * a.yin64[c];
@@ -317,63 +322,53 @@ __device__ __forceinline__ double stream_sum(const LOAD &load, int e0, int e1) {
 // hub row whose sum goes to hub_partial for the finish kernel.
 template <typename LOAD, typename EPI>
 __device__ __forceinline__ void do_desc(const SweepArgs &a, const BlockDesc &bd, const LOAD &load,
-                                        const EPI &epi, double (&red)[kReduce]) {
+                                        const EPI &epi, double (&red)[kReduce], double *stage) {
   if ((bd.x & (kHubFlag | kRowFlag)) == 0) {
     const int lg = bd.x;
     const int G = 1 << lg;
     const int lane = threadIdx.x & (G - 1);
     const int grp = threadIdx.x >> lg;
     const int ngrp = kBlock >> lg;
-    if constexpr (LOAD::kStaged) {
-      // CSR-stream: the run's edges (<= kStageNnz) are streamed coalesced,
-      // kUnroll independent loads in flight per lane, and their products
-      // staged in LDS; then each group of G lanes reduces one row from LDS.
-      __shared__ float prod[kStageNnz];
-      const int e0 = a.row_ptr[bd.y];
-      const int nnz = a.row_ptr[bd.z] - e0;
-      for (int i = threadIdx.x; i < nnz; i += kUnroll * kBlock) {
-        int c[kUnroll];
-        float wv[kUnroll];
+    // CSR-stream: the run's edges (<= kStageNnz) are streamed coalesced,
+    // several independent loads in flight per lane, and their products
+    // staged in LDS (f32 or f64, one buffer); then each group of G lanes
+    // reduces one row from LDS, in the order of a direct pull (lane, lane +
+    // G, ...; then across the group).  stage: kStageNnz doubles of LDS.
+    using P = typename LOAD::P;
+    constexpr int U = LOAD::kStageUnroll;
+    P *prod = reinterpret_cast<P *>(stage);
+    const int e0 = a.row_ptr[bd.y];
+    const int nnz = a.row_ptr[bd.z] - e0;
+    for (int i = threadIdx.x; i < nnz; i += U * kBlock) {
+      int c[U];
+      P wv[U];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const int ii = i + u * kBlock;
-          const int k = e0 + (ii < nnz ? ii : 0);
-          c[u] = __builtin_nontemporal_load(a.col + k);
-          wv[u] = load.weight(k);
-        }
+      for (int u = 0; u < U; u++) {
+        const int ii = i + u * kBlock;
+        const int k = e0 + (ii < nnz ? ii : 0);
+        c[u] = __builtin_nontemporal_load(a.col + k);
+        wv[u] = load.stage_weight(k);
+      }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const int ii = i + u * kBlock;
-          const float v = wv[u] * a.yin32[c[u]];
-          if (ii < nnz) prod[ii] = v;
-        }
-      }
-      __syncthreads();
-      for (int base = bd.y; base < bd.z; base += ngrp) {
-        const int r = base + grp;
-        double acc = 0.0;
-        if (r < bd.z) {
-          const int b = a.row_ptr[r] - e0;
-          const int e = a.row_ptr[r + 1] - e0;
-          for (int k = b + lane; k < e; k += G) acc += (double)prod[k];
-        }
-        acc = group_sum(acc, lg);
-        if (r < bd.z && lane == 0) epi(r, acc, red);
-      }
-      __syncthreads();
-    } else {
-      for (int base = bd.y; base < bd.z; base += ngrp) {
-        const int r = base + grp;
-        double acc = 0.0;
-        if (r < bd.z) {
-          const int b = __builtin_nontemporal_load(a.row_ptr + r);
-          const int e = __builtin_nontemporal_load(a.row_ptr + r + 1);
-          for (int k = b + lane; k < e; k += G) acc += load(k);
-        }
-        acc = group_sum(acc, lg);
-        if (r < bd.z && lane == 0) epi(r, acc, red);
+      for (int u = 0; u < U; u++) {
+        const int ii = i + u * kBlock;
+        const P v = wv[u] * load.stage_gather(c[u]);
+        if (ii < nnz) prod[ii] = v;
       }
     }
+    __syncthreads();
+    for (int base = bd.y; base < bd.z; base += ngrp) {
+      const int r = base + grp;
+      double acc = 0.0;
+      if (r < bd.z) {
+        const int b = a.row_ptr[r] - e0;
+        const int e = a.row_ptr[r + 1] - e0;
+        for (int k = b + lane; k < e; k += G) acc += (double)prod[k];
+      }
+      acc = group_sum(acc, lg);
+      if (r < bd.z && lane == 0) epi(r, acc, red);
+    }
+    __syncthreads();
   } else if (bd.x & kRowFlag) {
     // one long row, the whole workgroup
     const int r = bd.y;
@@ -414,14 +409,17 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, 
   if (st->done) return;
   pick_buffers(a, st, yd0, yd1, y640, y641);
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  __shared__ double stage[kStageNnz];  // a run's products (f32 or f64)
   if (st->mode == kModeAnchor) {
     const AnchorEpilogue epi(a, st);
     const EdgeF64<WT> load{a};
-    for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x) do_desc(a, a.desc[i], load, epi, red);
+    for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x)
+      do_desc(a, a.desc[i], load, epi, red, stage);
   } else {
     const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
     const EdgeF32<WT> load{a};
-    for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x) do_desc(a, a.desc[i], load, epi, red);
+    for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x)
+      do_desc(a, a.desc[i], load, epi, red, stage);
   }
   block_sum<kReduce>(red);
   if (threadIdx.x == 0) {
