@@ -142,7 +142,7 @@ class Context:
     """One HIP device + stream + the memory it allocates (gops_context)."""
 
     def __init__(self, device: int = 0, precision: str = "mixed", anchor_period: int = 32,
-                 normalize: bool = False, timing: bool = False):
+                 normalize: bool = False, timing: bool = False, graph_build: str = "device"):
         h = ctypes.c_void_p()
         rc = L.lib().gops_context_create(device, ctypes.byref(h))
         if rc != 0:
@@ -155,6 +155,7 @@ class Context:
         self.set_option(L.OPT_ANCHOR_PERIOD, anchor_period)
         self.set_option(L.OPT_NORMALIZE, int(bool(normalize)))
         self.set_option(L.OPT_TIMING, int(bool(timing)))
+        self.set_graph_build(graph_build)
 
     @property
     def handle(self):
@@ -166,6 +167,11 @@ class Context:
     def set_precision(self, precision: str):
         self.set_option(L.OPT_PRECISION,
                         {"mixed": L.PRECISION_MIXED, "fp64": L.PRECISION_FP64}[precision])
+
+    def set_graph_build(self, where: str):
+        """Where graphs are preprocessed: "device" (default) or "host"."""
+        self.set_option(L.OPT_GRAPH_BUILD,
+                        {"device": L.BUILD_DEVICE, "host": L.BUILD_HOST}[where])
 
     def last_stats(self) -> dict:
         s = L.RunStatsC()
@@ -278,10 +284,12 @@ class DeviceGraph:
     """A graph resident on the GPU (gops_graph): the CSR of in-edges built
     once from the adapter's neighbour lists (pagerank.rs:136, 226-234)."""
 
-    def __init__(self, handle, ctx: Context, weighted: bool):
+    def __init__(self, handle, ctx: Context, weighted: bool, weight_storage: int = None):
         self._h = handle
         self.ctx = ctx
         self.weighted = weighted
+        self.weight_storage = (weight_storage if weight_storage is not None
+                               else (L.WEIGHT_F64 if weighted else L.WEIGHT_NONE))
         _LIVE["graph"].add(self)
         n = ctypes.c_uint64()
         m = ctypes.c_uint64()
@@ -295,6 +303,27 @@ class DeviceGraph:
 
     def node_count(self) -> int:
         return self.n
+
+    def export(self) -> dict:
+        """Host copies of the preprocessed graph (gops_graph_export): row_ptr,
+        col, weights (storage dtype, bf16 as uint16), order, inv_deg, inv_ws."""
+        n, m = self.n, self.nnz
+        out = {"row_ptr": np.zeros(n + 1, np.int32), "col": np.zeros(m, np.int32),
+               "order": np.zeros(n, np.uint32), "inv_deg": np.zeros(n)}
+        w = ws = None
+        if self.weighted:
+            storage = self.weight_storage
+            w = np.zeros(m, {L.WEIGHT_F64: np.float64, L.WEIGHT_F32: np.float32,
+                             L.WEIGHT_BF16: np.uint16}[storage])
+            ws = np.zeros(n)
+        _check(L.lib().gops_graph_export(self._h, L.ptr(out["row_ptr"], L._i32p),
+                                         L.ptr(out["col"], L._i32p),
+                                         None if w is None else w.ctypes.data_as(L._vp),
+                                         L.ptr(out["order"], L._u32p),
+                                         L.ptr(out["inv_deg"], L._f64p), L.ptr(ws, L._f64p)),
+               self.ctx.handle, "gops_graph_export")
+        out["weights"], out["inv_ws"] = w, ws
+        return out
 
     @classmethod
     def from_lists(cls, n, offsets, targets, edge_weights=None, weight_storage="f64",
@@ -317,7 +346,7 @@ class DeviceGraph:
         rc = L.lib().gops_graph_create(ctx.handle, n, L.ptr(off, L._u64p), L.ptr(tg, L._u32p),
                                        L.ptr(ew, L._f64p), storage, ctypes.byref(h))
         _check(rc, ctx.handle, "gops_graph_create")
-        return cls(h, ctx, storage != L.WEIGHT_NONE)
+        return cls(h, ctx, storage != L.WEIGHT_NONE, storage)
 
     @classmethod
     def from_dense(cls, matrix, weighted=True, weight_storage="f64", ctx: Context = None):
@@ -330,7 +359,7 @@ class DeviceGraph:
         rc = L.lib().gops_graph_from_dense(ctx.handle, n, L.ptr(mat, L._f64p), storage,
                                            ctypes.byref(h))
         _check(rc, ctx.handle, "gops_graph_from_dense")
-        return cls(h, ctx, storage != L.WEIGHT_NONE)
+        return cls(h, ctx, storage != L.WEIGHT_NONE, storage)
 
     @classmethod
     def from_edges(cls, edges, weight_storage="f64", ctx: Context = None):

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libgraphops_amd.so")
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
 _vp = ctypes.c_void_p
 
 GOPS_OK = 0
@@ -28,6 +29,8 @@ WEIGHT_NONE, WEIGHT_F64, WEIGHT_F32, WEIGHT_BF16 = 0, 1, 2, 3
 OPT_PRECISION, OPT_ANCHOR_PERIOD, OPT_NORMALIZE, OPT_TIMING, OPT_PPR_TILE = 1, 2, 3, 4, 5
 OPT_ANCHOR_DROP = 6
 OPT_F32_BELOW, OPT_PPR_F32_BELOW, OPT_PPR_ANCHOR_PERIOD = 7, 8, 9
+OPT_GRAPH_BUILD = 10
+BUILD_DEVICE, BUILD_HOST = 0, 1
 PRECISION_MIXED, PRECISION_FP64 = 0, 1
 GEN_UNIT, GEN_UNIFORM, GEN_LOGNORMAL = 0, 1, 2
 ROUND_F64, ROUND_F32, ROUND_BF16 = 0, 1, 2
@@ -73,6 +76,7 @@ SIGNATURES = [
      [_vp, ctypes.c_uint64, _f64p, ctypes.c_int, ctypes.POINTER(_vp)]),
     ("gops_graph_destroy", None, [_vp]),
     ("gops_graph_info", ctypes.c_int, [_vp, _u64p, _u64p]),
+    ("gops_graph_export", ctypes.c_int, [_vp, _i32p, _i32p, _vp, _u32p, _f64p, _f64p]),
     ("gops_pagerank_run", ctypes.c_int,
      [_vp, _vp, ctypes.POINTER(PageRankConfigC), _f64p, ctypes.POINTER(RunInfoC)]),
     ("gops_pagerank_checked_run", ctypes.c_int,

@@ -116,16 +116,9 @@ gops_status alloc_run(gops_context *ctx, gops_graph *g) {
   return GOPS_OK;
 }
 
-gops_status upload_graph(gops_context *ctx, gops_graph *g, const HostCSRT &h, const Plan &p,
-                         int weight_storage) {
+// the block plan of the sweep kernels
+gops_status upload_plan(gops_context *ctx, gops_graph *g, const Plan &p) {
   DevGraph &dg = g->dev;
-  dg.n = h.n;
-  dg.m = h.m;
-  dg.weight_storage = h.has_weights ? weight_storage : GOPS_WEIGHT_NONE;
-  HIP_TRY(ctx, upload(ctx, &dg.row_ptr, h.row_ptr.data(), h.row_ptr.size()));
-  HIP_TRY(ctx, upload(ctx, &dg.col, h.col.data(), h.col.size()));
-  HIP_TRY(ctx, upload(ctx, &dg.rank, h.rank.data(), h.rank.size()));
-  HIP_TRY(ctx, upload(ctx, &dg.order, h.order.data(), h.order.size()));
   HIP_TRY(ctx, upload(ctx, &dg.desc, p.desc.data(), p.desc.size()));
   dg.num_desc = (int)p.desc.size();
   HIP_TRY(ctx, upload(ctx, &dg.hub_row, p.hub_row.data(), p.hub_row.size()));
@@ -134,6 +127,20 @@ gops_status upload_graph(gops_context *ctx, gops_graph *g, const HostCSRT &h, co
   dg.num_hubs = p.num_hubs;
   dg.num_hub_chunks = 0;
   for (int32_t c : p.hub_nchunks) dg.num_hub_chunks += c;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return GOPS_OK;
+}
+
+// the arrays build_csrt made on the host
+gops_status upload_graph(gops_context *ctx, gops_graph *g, const HostCSRT &h, int weight_storage) {
+  DevGraph &dg = g->dev;
+  dg.n = h.n;
+  dg.m = h.m;
+  dg.weight_storage = h.has_weights ? weight_storage : GOPS_WEIGHT_NONE;
+  HIP_TRY(ctx, upload(ctx, &dg.row_ptr, h.row_ptr.data(), h.row_ptr.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.col, h.col.data(), h.col.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.rank, h.rank.data(), h.rank.size()));
+  HIP_TRY(ctx, upload(ctx, &dg.order, h.order.data(), h.order.size()));
 
   // per-node 1/out-degree (and 1/out_wsum): f64 for anchor sweeps, f32 for
   // delta sweeps.  0 marks a dangling node; a non-dangling value that
@@ -462,6 +469,11 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
         return fail(ctx, GOPS_INVALID_PARAMETER, "ppr tile must be -1, 0, 8, 16, 32 or 64");
       ctx->ppr_tile = (int)value;
       return GOPS_OK;
+    case GOPS_OPT_GRAPH_BUILD:
+      if (value != GOPS_BUILD_DEVICE && value != GOPS_BUILD_HOST)
+        return fail(ctx, GOPS_INVALID_PARAMETER, "graph build must be GOPS_BUILD_DEVICE or _HOST");
+      ctx->graph_build = (int)value;
+      return GOPS_OK;
     default:
       return fail(ctx, GOPS_INVALID_PARAMETER, "unknown option");
   }
@@ -489,20 +501,33 @@ gops_status gops_graph_create(gops_context *ctx, uint64_t n, const uint64_t *off
   const double *ew = nullptr;
   if (weight_storage != GOPS_WEIGHT_NONE) ew = edge_weights ? edge_weights : kNoWeights;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  HostCSRT h;
-  std::string err;
-  gops_status st = build_csrt(n, offsets, targets, ew, h, err);
-  if (st != GOPS_OK) return fail(ctx, st, err);
-  Plan p;
-  plan_blocks(h, p);
   auto *g = new (std::nothrow) gops_graph;
   if (!g) return fail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
   g->ctx = ctx;
   g->weighted = weight_storage != GOPS_WEIGHT_NONE;
-  g->first_bad_weight = h.first_bad_weight;
-  g->order_host = h.order;
-  g->rank_host = h.rank;
-  st = upload_graph(ctx, g, h, p, weight_storage);
+  std::string err;
+  Plan p;
+  gops_status st;
+  if (ctx->graph_build == GOPS_BUILD_DEVICE) {
+    st = build_on_device(ctx, n, offsets, targets, ew, weight_storage, g, p, err);
+    if (st != GOPS_OK) {
+      delete g;
+      return fail(ctx, st, err);
+    }
+  } else {
+    HostCSRT h;
+    st = build_csrt(n, offsets, targets, ew, h, err);
+    if (st != GOPS_OK) {
+      delete g;
+      return fail(ctx, st, err);
+    }
+    plan_blocks(h, p);
+    g->first_bad_weight = h.first_bad_weight;
+    g->order_host = h.order;
+    g->rank_host = h.rank;
+    st = upload_graph(ctx, g, h, weight_storage);
+  }
+  if (st == GOPS_OK) st = upload_plan(ctx, g, p);
   if (st != GOPS_OK) {
     free_dev(g->dev);
     delete g;
@@ -561,6 +586,31 @@ gops_status gops_graph_info(const gops_graph *g, uint64_t *n, uint64_t *nnz) {
   if (!g) return GOPS_INVALID_PARAMETER;
   if (n) *n = g->dev.n;
   if (nnz) *nnz = g->dev.m;
+  return GOPS_OK;
+}
+
+gops_status gops_graph_export(const gops_graph *g, int32_t *row_ptr, int32_t *col, void *weights,
+                              uint32_t *order, double *inv_deg, double *inv_ws) {
+  if (!g || !g->ctx) return GOPS_INVALID_PARAMETER;
+  gops_context *ctx = g->ctx;
+  const DevGraph &dg = g->dev;
+  const size_t n = dg.n, m = dg.m;
+  if (weights && dg.weight_storage == GOPS_WEIGHT_NONE)
+    return fail(ctx, GOPS_INVALID_PARAMETER, "graph has no edge weights");
+  if (inv_ws && !dg.inv_ws64) return fail(ctx, GOPS_INVALID_PARAMETER, "graph has no edge weights");
+  const size_t wb = dg.weight_storage == GOPS_WEIGHT_F64 ? 8 : dg.weight_storage == GOPS_WEIGHT_F32 ? 4 : 2;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  auto down = [&](void *dst, const void *src, size_t bytes) {
+    return (dst && bytes) ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s) : hipSuccess;
+  };
+  HIP_TRY(ctx, down(row_ptr, dg.row_ptr, (n + 1) * sizeof(int32_t)));
+  HIP_TRY(ctx, down(col, dg.col, m * sizeof(int32_t)));
+  HIP_TRY(ctx, down(weights, dg.w, m * wb));
+  HIP_TRY(ctx, down(order, dg.order, n * sizeof(uint32_t)));
+  HIP_TRY(ctx, down(inv_deg, dg.inv_deg64, n * sizeof(double)));
+  HIP_TRY(ctx, down(inv_ws, dg.inv_ws64, n * sizeof(double)));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
   return GOPS_OK;
 }
 

@@ -1,0 +1,364 @@
+// graph_build.hip -- the preprocessed graph (CSR of in-edges, rows renumbered
+// by in-degree) built on the device.  The arrays are those of build_csrt on
+// the host, bit for bit, from the adapter's neighbour lists uploaded once.
+//
+// Steps, each a grid-stride kernel or a hipCUB primitive over the graph:
+//   1. the largest id >= n, if any (the host's bounds message);
+//   2. per-edge source ids: the first entry of each non-empty neighbour list
+//      marked with its node, then an inclusive max-scan;
+//   3. per node, 1/out-degree and 1/out_wsum with out_wsum summed in
+//      neighbour order (pagerank.rs:137, 227-234), and the first invalid
+//      weight in adapter order (pagerank.rs:300-314);
+//   4. the in-degree histogram (atomics);
+//   5. the order: a stable radix sort of ids by descending in-degree, so
+//      that ties keep ascending ids;
+//   6. row pointers: an exclusive scan of the in-degrees in that order;
+//   7. the edges: a stable radix sort of (row, new source id) keys whose
+//      values are the adapter positions, so duplicate edges keep adapter
+//      order, the order of the reference's push loop (pagerank.rs:154-162);
+//   8. col and the weights (f64 / f32 / bf16) from the sorted keys.
+// Only the row pointers come back to the host, for the block plan, with the
+// order and rank that map query seeds and results.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "internal.h"
+
+namespace gops {
+
+namespace {
+
+constexpr int kB = 256;
+
+inline int grid_for(uint64_t count) {
+  const uint64_t g = (count + kB - 1) / kB;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(g, 1u << 16));
+}
+
+#define GRID_STRIDE(i, count) \
+  for (uint64_t i = blockIdx.x * (uint64_t)kB + threadIdx.x; i < (count); i += (uint64_t)gridDim.x * kB)
+
+// *bad = largest id >= n, plus one (0 when every id is in range)
+__global__ void bad_ids_kernel(const uint32_t *nb, uint64_t m, uint64_t n, unsigned long long *bad) {
+  GRID_STRIDE(k, m) {
+    if (nb[k] >= n) atomicMax(bad, (unsigned long long)nb[k] + 1ull);
+  }
+}
+
+// marks[off[u]] = u for each u with neighbours (their starts are distinct)
+__global__ void mark_starts_kernel(const uint64_t *off, uint64_t n, uint32_t *marks) {
+  GRID_STRIDE(u, n) {
+    if (off[u + 1] > off[u]) marks[off[u]] = (uint32_t)u;
+  }
+}
+
+// per node (original ids): 1/out-degree, 1/out_wsum (0 when 0), and the
+// smallest adapter position of a weight that is negative or not finite
+__global__ void node_kernel(const uint64_t *off, const double *ew, uint64_t n, double *inv_deg_o,
+                            double *inv_ws_o, unsigned long long *first_bad) {
+  GRID_STRIDE(u, n) {
+    const uint64_t b = off[u], e = off[u + 1];
+    inv_deg_o[u] = e > b ? 1.0 / (double)(e - b) : 0.0;
+    if (ew) {
+      double s = 0.0;
+      for (uint64_t j = b; j < e; j++) {
+        const double w = ew[j];
+        if (!isfinite(w) || w < 0.0) atomicMin(first_bad, (unsigned long long)j);
+        s += fmax(w, 0.0);
+      }
+      inv_ws_o[u] = s != 0.0 ? 1.0 / s : 0.0;
+    }
+  }
+}
+
+__global__ void indeg_kernel(const uint32_t *nb, uint64_t m, uint32_t *indeg) {
+  GRID_STRIDE(k, m) { atomicAdd(&indeg[nb[k]], 1u); }
+}
+
+// ascending key = descending in-degree; value = the id
+__global__ void relabel_keys_kernel(const uint32_t *indeg, uint64_t n, uint32_t *key, uint32_t *val) {
+  GRID_STRIDE(v, n) {
+    key[v] = 0xFFFFFFFFu - indeg[v];
+    val[v] = (uint32_t)v;
+  }
+}
+
+__global__ void rank_kernel(const uint32_t *order, const uint32_t *indeg, uint64_t n, uint32_t *rank,
+                            int32_t *cnt) {
+  GRID_STRIDE(r, n) {
+    const uint32_t v = order[r];
+    rank[v] = (uint32_t)r;
+    cnt[r] = (int32_t)indeg[v];
+  }
+}
+
+// key = rank[target] << cb | rank[source]; value = adapter position
+__global__ void edge_keys_kernel(const uint32_t *nb, const uint32_t *src, const uint32_t *rank,
+                                 uint64_t m, int cb, uint64_t *key, uint32_t *val) {
+  GRID_STRIDE(k, m) {
+    key[k] = ((uint64_t)rank[nb[k]] << cb) | (uint64_t)rank[src[k]];
+    if (val) val[k] = (uint32_t)k;
+  }
+}
+
+// f32 -> bf16, round to nearest even, NaN kept quiet (as f32_to_bf16)
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) {
+    uint16_t h = (uint16_t)(u >> 16);
+    if (u & 0x007fffffu) h |= 0x40;
+    return h;
+  }
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__global__ void emit_kernel(const uint64_t *key, const uint32_t *val, const double *ew, uint64_t m,
+                            uint64_t col_mask, int storage, int32_t *col, void *w) {
+  GRID_STRIDE(i, m) {
+    col[i] = (int32_t)(key[i] & col_mask);
+    if (ew) {
+      const double x = fmax(ew[val[i]], 0.0);
+      if (storage == GOPS_WEIGHT_F64) static_cast<double *>(w)[i] = x;
+      else if (storage == GOPS_WEIGHT_F32) static_cast<float *>(w)[i] = (float)x;
+      else static_cast<uint16_t *>(w)[i] = to_bf16((float)x);
+    }
+  }
+}
+
+// per row (new ids): the f64 value and its f32 copy; a non-zero value that
+// underflows in f32 keeps the smallest subnormal so it stays non-dangling
+__global__ void inv_kernel(const uint32_t *order, const double *src, uint64_t n, double *d64,
+                           float *d32) {
+  GRID_STRIDE(r, n) {
+    const double x = src[order[r]];
+    d64[r] = x;
+    float f = (float)x;
+    if (x != 0.0 && f == 0.0f) f = std::numeric_limits<float>::denorm_min();
+    d32[r] = f;
+  }
+}
+
+#undef GRID_STRIDE
+
+// device allocations of one build; whatever is not released is freed
+struct Pool {
+  std::vector<void *> ptrs;
+  hipError_t err = hipSuccess;
+  template <typename T>
+  T *get(size_t count) {
+    if (err != hipSuccess) return nullptr;
+    void *p = nullptr;
+    err = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (err != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return static_cast<T *>(p);
+  }
+  void release(void *p) { ptrs.erase(std::remove(ptrs.begin(), ptrs.end(), p), ptrs.end()); }
+  ~Pool() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+// bits that hold every value in [0, x]
+int bits_for(uint64_t x) {
+  int b = 1;
+  while (b < 63 && (1ull << b) <= x) b++;
+  return b;
+}
+
+}  // namespace
+
+gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, const uint32_t *nb,
+                            const double *ew, int weight_storage, gops_graph *g, Plan &plan,
+                            std::string &err) {
+  // offset checks, as build_csrt
+  if (n >= (1ull << 31)) {
+    err = "node count must be < 2^31";
+    return GOPS_NOT_SUPPORTED;
+  }
+  const uint64_t m = off[n];
+  if (m >= (1ull << 31)) {
+    err = "edge count must be < 2^31";
+    return GOPS_NOT_SUPPORTED;
+  }
+  if (off[0] != 0) {
+    err = "offsets[0] must be 0";
+    return GOPS_INVALID_PARAMETER;
+  }
+  for (uint64_t u = 0; u < n; u++) {
+    if (off[u + 1] < off[u]) {
+      err = "offsets must be non-decreasing";
+      return GOPS_INVALID_PARAMETER;
+    }
+  }
+  hipStream_t s = ctx->stream;
+  Pool pool;
+  hipError_t herr = hipSuccess;
+  const char *what = "";
+#define DTRY(expr)            \
+  do {                        \
+    herr = (expr);            \
+    if (herr != hipSuccess) { \
+      what = #expr;           \
+      goto hip_error;         \
+    }                         \
+  } while (0)
+  {
+    uint64_t *d_off = pool.get<uint64_t>(n + 1);
+    uint32_t *d_nb = pool.get<uint32_t>(m);
+    double *d_ew = ew ? pool.get<double>(m) : nullptr;
+    unsigned long long *d_flags = pool.get<unsigned long long>(2);  // bad id + 1, first bad weight
+    unsigned long long flags[2] = {0ull, ~0ull};
+    DTRY(pool.err);
+    DTRY(hipMemcpyAsync(d_off, off, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    if (m) DTRY(hipMemcpyAsync(d_nb, nb, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (ew && m) DTRY(hipMemcpyAsync(d_ew, ew, m * sizeof(double), hipMemcpyHostToDevice, s));
+    DTRY(hipMemcpyAsync(d_flags, flags, sizeof(flags), hipMemcpyHostToDevice, s));
+    bad_ids_kernel<<<grid_for(m), kB, 0, s>>>(d_nb, m, n, d_flags);
+    DTRY(hipGetLastError());
+    DTRY(hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    DTRY(hipStreamSynchronize(s));
+    if (flags[0]) {
+      err = "index out of bounds: " + std::to_string(flags[0] - 1);
+      return GOPS_INDEX_OUT_OF_BOUNDS;
+    }
+
+    const int cb = bits_for(n ? n - 1 : 0);  // bits of a new id
+    uint32_t *d_marks = pool.get<uint32_t>(m), *d_src = pool.get<uint32_t>(m);
+    double *d_inv_deg_o = pool.get<double>(n);
+    double *d_inv_ws_o = ew ? pool.get<double>(n) : nullptr;
+    uint32_t *d_indeg = pool.get<uint32_t>(n);
+    uint32_t *d_key32 = pool.get<uint32_t>(n), *d_key32b = pool.get<uint32_t>(n);
+    uint32_t *d_val32 = pool.get<uint32_t>(n);
+    uint32_t *d_order = pool.get<uint32_t>(n), *d_rank = pool.get<uint32_t>(n);
+    int32_t *d_cnt = pool.get<int32_t>(n + 1), *d_row_ptr = pool.get<int32_t>(n + 1);
+    uint64_t *d_key64 = pool.get<uint64_t>(m), *d_key64b = pool.get<uint64_t>(m);
+    uint32_t *d_val64 = ew ? pool.get<uint32_t>(m) : nullptr;
+    uint32_t *d_val64b = ew ? pool.get<uint32_t>(m) : nullptr;
+    DTRY(pool.err);
+
+    // scratch of the hipCUB calls: the largest of their needs
+    size_t tmp_bytes = 1, b = 0;
+    DTRY(hipcub::DeviceScan::InclusiveScan(nullptr, b, d_marks, d_src, hipcub::Max(), (int)m, s));
+    tmp_bytes = std::max(tmp_bytes, b);
+    DTRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, d_key32, d_key32b, d_val32, d_order, (int)n,
+                                            0, 32, s));
+    tmp_bytes = std::max(tmp_bytes, b);
+    DTRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b, d_cnt, d_row_ptr, (int)n + 1, s));
+    tmp_bytes = std::max(tmp_bytes, b);
+    if (ew)
+      DTRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, d_key64, d_key64b, d_val64, d_val64b,
+                                              (int)m, 0, 2 * cb, s));
+    else
+      DTRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b, d_key64, d_key64b, (int)m, 0, 2 * cb, s));
+    tmp_bytes = std::max(tmp_bytes, b);
+    void *d_tmp = pool.get<uint8_t>(tmp_bytes);
+    DTRY(pool.err);
+
+    // 2-4: sources, per-node sums, in-degrees
+    if (m) {
+      DTRY(hipMemsetAsync(d_marks, 0, m * sizeof(uint32_t), s));
+      mark_starts_kernel<<<grid_for(n), kB, 0, s>>>(d_off, n, d_marks);
+      b = tmp_bytes;
+      DTRY(hipcub::DeviceScan::InclusiveScan(d_tmp, b, d_marks, d_src, hipcub::Max(), (int)m, s));
+    }
+    if (n) DTRY(hipMemsetAsync(d_indeg, 0, n * sizeof(uint32_t), s));
+    node_kernel<<<grid_for(n), kB, 0, s>>>(d_off, d_ew, n, d_inv_deg_o, d_inv_ws_o, d_flags + 1);
+    indeg_kernel<<<grid_for(m), kB, 0, s>>>(d_nb, m, d_indeg);
+    DTRY(hipGetLastError());
+
+    // 5-6: order, rank, row pointers
+    relabel_keys_kernel<<<grid_for(n), kB, 0, s>>>(d_indeg, n, d_key32, d_val32);
+    if (n) {
+      b = tmp_bytes;
+      DTRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, b, d_key32, d_key32b, d_val32, d_order, (int)n,
+                                              0, 32, s));
+    }
+    DTRY(hipMemsetAsync(d_cnt + n, 0, sizeof(int32_t), s));
+    rank_kernel<<<grid_for(n), kB, 0, s>>>(d_order, d_indeg, n, d_rank, d_cnt);
+    DTRY(hipGetLastError());
+    b = tmp_bytes;
+    DTRY(hipcub::DeviceScan::ExclusiveSum(d_tmp, b, d_cnt, d_row_ptr, (int)n + 1, s));
+
+    // 7-8: edges by (row, new source id), ties in adapter order
+    edge_keys_kernel<<<grid_for(m), kB, 0, s>>>(d_nb, d_src, d_rank, m, cb, d_key64, d_val64);
+    DTRY(hipGetLastError());
+    if (m) {
+      b = tmp_bytes;
+      if (ew)
+        DTRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, b, d_key64, d_key64b, d_val64, d_val64b,
+                                                (int)m, 0, 2 * cb, s));
+      else
+        DTRY(hipcub::DeviceRadixSort::SortKeys(d_tmp, b, d_key64, d_key64b, (int)m, 0, 2 * cb, s));
+    }
+    const int storage = ew ? weight_storage : GOPS_WEIGHT_NONE;
+    const size_t wbytes = storage == GOPS_WEIGHT_F64 ? 8 : storage == GOPS_WEIGHT_F32 ? 4 : 2;
+    int32_t *d_col = pool.get<int32_t>(m);
+    void *d_w = ew ? (void *)pool.get<uint8_t>(m * wbytes) : nullptr;
+    double *d_inv_deg64 = pool.get<double>(n);
+    float *d_inv_deg32 = pool.get<float>(n);
+    double *d_inv_ws64 = ew ? pool.get<double>(n) : nullptr;
+    float *d_inv_ws32 = ew ? pool.get<float>(n) : nullptr;
+    DTRY(pool.err);
+    emit_kernel<<<grid_for(m), kB, 0, s>>>(d_key64b, d_val64b, d_ew, m, (1ull << cb) - 1, storage,
+                                           d_col, d_w);
+    inv_kernel<<<grid_for(n), kB, 0, s>>>(d_order, d_inv_deg_o, n, d_inv_deg64, d_inv_deg32);
+    if (ew) inv_kernel<<<grid_for(n), kB, 0, s>>>(d_order, d_inv_ws_o, n, d_inv_ws64, d_inv_ws32);
+    DTRY(hipGetLastError());
+
+    // host copies: row pointers for the plan, ids for query seeds and results
+    HostCSRT h;
+    h.n = n;
+    h.m = m;
+    h.row_ptr.resize(n + 1);
+    std::vector<uint32_t> order(n), rank(n);
+    DTRY(hipMemcpyAsync(h.row_ptr.data(), d_row_ptr, (n + 1) * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, s));
+    if (n) {
+      DTRY(hipMemcpyAsync(order.data(), d_order, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      DTRY(hipMemcpyAsync(rank.data(), d_rank, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    DTRY(hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));
+    DTRY(hipStreamSynchronize(s));
+    plan_blocks(h, plan);
+
+    // success: the graph takes the arrays it keeps
+    DevGraph &dg = g->dev;
+    dg.n = n;
+    dg.m = m;
+    dg.weight_storage = storage;
+    dg.row_ptr = d_row_ptr;
+    dg.col = d_col;
+    dg.w = d_w;
+    dg.rank = d_rank;
+    dg.order = d_order;
+    dg.inv_deg64 = d_inv_deg64;
+    dg.inv_deg32 = d_inv_deg32;
+    dg.inv_ws64 = d_inv_ws64;
+    dg.inv_ws32 = d_inv_ws32;
+    for (void *p : {(void *)d_row_ptr, (void *)d_col, d_w, (void *)d_rank, (void *)d_order,
+                    (void *)d_inv_deg64, (void *)d_inv_deg32, (void *)d_inv_ws64,
+                    (void *)d_inv_ws32})
+      if (p) pool.release(p);
+    g->order_host = std::move(order);
+    g->rank_host = std::move(rank);
+    g->first_bad_weight = 0;
+    if (ew && flags[1] != ~0ull) g->first_bad_weight = std::isfinite(ew[flags[1]]) ? 2 : 1;
+    return GOPS_OK;
+  }
+hip_error:
+#undef DTRY
+  (void)hipStreamSynchronize(s);
+  err = std::string("device graph build: ") + what + ": " + hipGetErrorString(herr);
+  return (herr == hipErrorOutOfMemory || herr == hipErrorMemoryAllocation) ? GOPS_OUT_OF_MEMORY
+                                                                           : GOPS_DEVICE_ERROR;
+}
+
+}  // namespace gops

@@ -117,9 +117,11 @@ gops_status build_csrt(uint64_t n, const uint64_t *off, const uint32_t *nb, cons
   g.row_ptr.assign(n + 1, 0);
   for (uint64_t r = 0; r < n; r++) g.row_ptr[r + 1] = g.row_ptr[r] + (int32_t)indeg[g.order[r]];
 
-  // scatter in-edges, then sort every row by new source id
+  // scatter in-edges, then sort every row by (new source id, adapter
+  // position): duplicate edges u -> v keep the order in which the
+  // reference's push loop adds them (pagerank.rs:154-162)
   g.col.assign(m, 0);
-  if (ew) g.w.assign(m, 0.0);
+  std::vector<int32_t> src_pos(ew ? m : 0);
   std::vector<int32_t> cursor(g.row_ptr.begin(), g.row_ptr.end() - 1);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t u = 0; u < (int64_t)n; u++) {
@@ -128,28 +130,28 @@ gops_status build_csrt(uint64_t n, const uint64_t *off, const uint32_t *nb, cons
       const uint32_t r = g.rank[nb[j]];
       const int32_t pos = __atomic_fetch_add(&cursor[r], 1, __ATOMIC_RELAXED);
       g.col[pos] = su;
-      if (ew) g.w[pos] = std::fmax(ew[j], 0.0);
+      if (ew) src_pos[pos] = (int32_t)j;
     }
   }
   cursor.clear();
   cursor.shrink_to_fit();
+  if (ew) g.w.assign(m, 0.0);
 #pragma omp parallel
   {
-    std::vector<std::pair<int32_t, double>> tmp;
+    std::vector<std::pair<int32_t, int32_t>> tmp;
 #pragma omp for schedule(dynamic, 256)
     for (int64_t r = 0; r < (int64_t)n; r++) {
       const int32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
-      if (e - b < 2) continue;
       if (!ew) {
-        std::sort(g.col.begin() + b, g.col.begin() + e);
-      } else {
-        tmp.resize(e - b);
-        for (int32_t k = b; k < e; k++) tmp[k - b] = {g.col[k], g.w[k]};
-        std::sort(tmp.begin(), tmp.end());
-        for (int32_t k = b; k < e; k++) {
-          g.col[k] = tmp[k - b].first;
-          g.w[k] = tmp[k - b].second;
-        }
+        if (e - b > 1) std::sort(g.col.begin() + b, g.col.begin() + e);
+        continue;
+      }
+      tmp.resize(e - b);
+      for (int32_t k = b; k < e; k++) tmp[k - b] = {g.col[k], src_pos[k]};
+      std::sort(tmp.begin(), tmp.end());
+      for (int32_t k = b; k < e; k++) {
+        g.col[k] = tmp[k - b].first;
+        g.w[k] = std::fmax(ew[tmp[k - b].second], 0.0);
       }
     }
   }

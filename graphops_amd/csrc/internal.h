@@ -163,6 +163,7 @@ struct gops_context {
   int normalize = 0;
   int timing = 0;
   int ppr_tile = 0;
+  int graph_build = GOPS_BUILD_DEVICE;
   gops_run_stats stats{};
   gops::RunState *host_state = nullptr;  // pinned, 2 slots
   uint64_t *mirror_host = nullptr;        // mapped pinned progress word
@@ -185,6 +186,13 @@ gops_status build_csrt(uint64_t n, const uint64_t *off, const uint32_t *nb, cons
                        HostCSRT &out, std::string &err);
 void plan_blocks(const HostCSRT &g, Plan &plan);
 uint16_t f32_to_bf16(float f);
+
+// graph_build.hip: build_csrt's arrays made on the device and kept there
+// (g->dev apart from the block plan, g->order_host, g->rank_host,
+// g->first_bad_weight); plan receives the block plan
+gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, const uint32_t *nb,
+                            const double *ew, int weight_storage, gops_graph *g, Plan &plan,
+                            std::string &err);
 
 // api.cpp
 enum RunKind { kPlain = 0, kWeighted = 1, kPersonalized = 2 };

@@ -92,11 +92,15 @@ enum {
     GOPS_OPT_F32_BELOW = 7,     /* mixed, global PageRank: f64 sweeps while the residual
                                    is above this times the tolerance (default 0: none) */
     GOPS_OPT_PPR_F32_BELOW = 8, /* mixed, personalized: same (default 1e4) */
-    GOPS_OPT_PPR_ANCHOR_PERIOD = 9 /* mixed, personalized: f32 sweeps between f64
+    GOPS_OPT_PPR_ANCHOR_PERIOD = 9, /* mixed, personalized: f32 sweeps between f64
                                    anchors while the residual is above 1e3 times
                                    the tolerance (default 64) */
+    GOPS_OPT_GRAPH_BUILD = 10   /* where gops_graph_create builds the CSR of in-edges:
+                                   GOPS_BUILD_DEVICE (default) or GOPS_BUILD_HOST; the
+                                   arrays, and so every result, are the same */
 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
+enum { GOPS_BUILD_DEVICE = 0, GOPS_BUILD_HOST = 1 };
 
 /* Statistics of the most recent run on a context (no reference counterpart). */
 typedef struct gops_run_stats {
@@ -126,9 +130,10 @@ gops_status gops_last_run_stats(const gops_context *ctx, gops_run_stats *out);
  * (they count toward out_degree, pagerank.rs:137); edge_weights (nullable) are
  * WeightedGraph::edge_weight(u, v) per entry (graph.rs:67-72).  Replaces the
  * reference's per-call neighbour materialisation (pagerank.rs:136, 226-234;
- * ppr.rs:112, 133).  The CSR transpose is built on the host and uploaded
- * once; the caller's arrays may be freed on return.  Ids must be < n
- * (GOPS_INDEX_OUT_OF_BOUNDS otherwise); n and offsets[n] must be < 2^31. */
+ * ppr.rs:112, 133).  The lists are uploaded once and the CSR transpose is
+ * built on the device (or on the host, GOPS_OPT_GRAPH_BUILD); the caller's
+ * arrays may be freed on return.  Ids must be < n (GOPS_INDEX_OUT_OF_BOUNDS
+ * otherwise); n and offsets[n] must be < 2^31. */
 gops_status gops_graph_create(gops_context *ctx, uint64_t n, const uint64_t *offsets,
                               const uint32_t *targets, const double *edge_weights,
                               int weight_storage, gops_graph **out);
@@ -139,6 +144,17 @@ gops_status gops_graph_from_dense(gops_context *ctx, uint64_t n, const double *m
                                   int weight_storage, gops_graph **out);
 void gops_graph_destroy(gops_graph *g);
 gops_status gops_graph_info(const gops_graph *g, uint64_t *n, uint64_t *nnz);
+/* Host copies of the preprocessed graph (no reference counterpart: the
+ * reference keeps no preprocessed form).  Row r is the node order[r]; nodes
+ * are ordered by in-degree, descending, ties by id.  Every pointer may be
+ * null.  row_ptr: n+1 entries; col: m entries, the new ids of the row's
+ * sources, ascending, duplicate edges in adapter order; weights: m entries
+ * of the graph's weight storage (f64, f32 or bf16 as uint16; null for an
+ * unweighted graph); order: n; inv_deg / inv_ws: n f64 values per row,
+ * 1/out-degree and 1/out_wsum of the row's node (0 when zero). */
+gops_status gops_graph_export(const gops_graph *g, int32_t *row_ptr, int32_t *col,
+                              void *weights, uint32_t *order, double *inv_deg,
+                              double *inv_ws);
 
 /* ---- PageRank family -------------------------------------------------- */
 /* pagerank_run, pagerank.rs:122-187 (pagerank, pagerank.rs:114-116, is this
