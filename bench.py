@@ -203,6 +203,28 @@ def serving_leg(ga, generators, e, dg, nq=1024, qps_offered=5000.0, max_slots=51
             "iterations_mean": round(float(its.mean()), 2), "converged": int(conv)}
 
 
+def build_leg(ga, e, storage):
+    """Time gops_graph_create on the device build and on the host build (the
+    same arrays): adapter lists in host memory -> graph resident on the GPU."""
+    out = {}
+    for where, reps in (("device", 3), ("host", 1)):
+        cx = ga.Context(0, graph_build=where)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            g = ga.DeviceGraph.from_edges(e, weight_storage=storage, ctx=cx)
+            ts.append(time.perf_counter() - t0)
+            g.close()
+        out[f"{where}_s"] = round(min(ts), 4)
+        cx.close()
+        log(f"graph build on the {where}: {min(ts):.3f} s (best of {reps})")
+    out["speedup"] = round(out["host_s"] / out["device_s"], 2)
+    out["device_gedges_per_s"] = round(e.m / out["device_s"] / 1e9, 3)
+    out["host_threads"] = os.cpu_count() if not os.environ.get("OMP_NUM_THREADS") \
+        else int(os.environ["OMP_NUM_THREADS"])
+    return out
+
+
 def ppr_legs(args, ga, generators):
     t0 = time.perf_counter()
     e = generators.rmat(PPR_GRAPH["scale"], 16, 0.57, 0.19, 0.19, PPR_GRAPH["seed"])
@@ -252,6 +274,8 @@ def main():
     ap.add_argument("--ppr-queries", type=int, default=1024)
     ap.add_argument("--serve-queries", type=int, default=1024)
     ap.add_argument("--ppr-only", action="store_true", help="only the PPR and serving legs")
+    ap.add_argument("--no-build", action="store_true", help="skip the graph build timing leg")
+    ap.add_argument("--build-only", action="store_true", help="only the graph build timing leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -273,6 +297,10 @@ def main():
     e = generators.rmat(c["scale"], 16, 0.57, 0.19, 0.19, c["seed"], c["weights"], c["p0"],
                         c["p1"], c["rounding"])
     log(f"generated n={e.n} m={e.m} in {time.perf_counter() - t0:.1f} s")
+    if args.build_only:
+        print(json.dumps({"workload": c["workload"], "n": e.n, "m": e.m,
+                          "graph_build": build_leg(ga, e, c["storage"])}), flush=True)
+        return
     ctx = ga.Context(0, precision="mixed", timing=True)
     t0 = time.perf_counter()
     dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"], ctx=ctx)
@@ -341,6 +369,8 @@ def main():
                            "top100_agreement": len(top & top_ref) / 100.0,
                            "oracle_iterations": ref.iterations, "gpu_iterations": r.iterations}
     out["cpu_baseline"] = cpu_baseline(e, c, args.cpu_sweeps) if args.cpu_sweeps > 0 else None
+    if not args.no_build:
+        out["graph_build"] = build_leg(ga, e, c["storage"])
     if not args.no_ppr:
         dg.close()
         del e

@@ -8,15 +8,18 @@
 //      marked with its node, then an inclusive max-scan;
 //   3. per node, 1/out-degree and 1/out_wsum with out_wsum summed in
 //      neighbour order (pagerank.rs:137, 227-234), and the first invalid
-//      weight in adapter order (pagerank.rs:300-314);
-//   4. the in-degree histogram (atomics);
+//      weight in adapter order (pagerank.rs:300-314): a lane per short list,
+//      a wave per long one;
+//   4. in-degrees: the targets radix-sorted, each id's run measured (no
+//      device-wide atomics: on a chip of several L2s they go to memory);
 //   5. the order: a stable radix sort of ids by descending in-degree, so
 //      that ties keep ascending ids;
 //   6. row pointers: an exclusive scan of the in-degrees in that order;
-//   7. the edges: a stable radix sort of (row, new source id) keys whose
-//      values are the adapter positions, so duplicate edges keep adapter
-//      order, the order of the reference's push loop (pagerank.rs:154-162);
-//   8. col and the weights (f64 / f32 / bf16) from the sorted keys.
+//   7. the edges: a stable radix sort of (row, new source id) keys carrying
+//      max(w, 0) in the storage type, so duplicate edges keep adapter order,
+//      the order of the reference's push loop (pagerank.rs:154-162); the
+//      sorted values are the graph's weight array;
+//   8. col from the sorted keys.
 // Only the row pointers come back to the host, for the block plan, with the
 // order and rank that map query seeds and results.
 #include <hip/hip_runtime.h>
@@ -58,33 +61,114 @@ __global__ void mark_starts_kernel(const uint64_t *off, uint64_t n, uint32_t *ma
   }
 }
 
-// per node (original ids): 1/out-degree, 1/out_wsum (0 when 0), and the
-// smallest adapter position of a weight that is negative or not finite
+// per node (original ids): 1/out-degree, and for a list of at most
+// kShortList entries 1/out_wsum (0 when 0) and the smallest adapter position
+// of a weight that is negative or not finite; longer lists are flagged for
+// long_list_kernel.
+constexpr uint64_t kShortList = 64;
 __global__ void node_kernel(const uint64_t *off, const double *ew, uint64_t n, double *inv_deg_o,
-                            double *inv_ws_o, unsigned long long *first_bad) {
+                            double *inv_ws_o, unsigned long long *first_bad, uint8_t *is_long) {
   GRID_STRIDE(u, n) {
     const uint64_t b = off[u], e = off[u + 1];
     inv_deg_o[u] = e > b ? 1.0 / (double)(e - b) : 0.0;
-    if (ew) {
-      double s = 0.0;
-      for (uint64_t j = b; j < e; j++) {
-        const double w = ew[j];
-        if (!isfinite(w) || w < 0.0) atomicMin(first_bad, (unsigned long long)j);
-        s += fmax(w, 0.0);
+    if (!ew) continue;
+    const bool longl = e - b > kShortList;
+    is_long[u] = longl;
+    if (longl) continue;
+    double s = 0.0;
+    uint64_t bad = ~0ull;
+    for (uint64_t j = b; j < e; j++) {
+      const double w = ew[j];
+      if ((!isfinite(w) || w < 0.0) && bad == ~0ull) bad = j;
+      s += fmax(w, 0.0);
+    }
+    inv_ws_o[u] = s != 0.0 ? 1.0 / s : 0.0;
+    if (bad != ~0ull) atomicMin(first_bad, (unsigned long long)bad);
+  }
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// the flagged lists, one wave each (hubs have ids close together -- R-MAT
+// puts them at the low ids -- so one wave per list keeps them apart).  The
+// wave loads 256 entries per step, the next step's while it adds the
+// current one's in list order (one broadcast each), so the sum is the
+// sequential one and the chain of additions is the only wait.
+constexpr int kLongChunks = 4;
+__global__ void long_list_kernel(const uint64_t *off, const double *ew, const uint32_t *nodes,
+                                 const uint32_t *count, double *inv_ws_o,
+                                 unsigned long long *first_bad) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t num = *count;
+  const uint32_t waves = gridDim.x * (kB / 64);
+  const uint32_t first = __builtin_amdgcn_readfirstlane((blockIdx.x * kB + threadIdx.x) / 64);
+  for (uint32_t i = first; i < num; i += waves) {
+    const uint32_t u = __builtin_amdgcn_readfirstlane(nodes[i]);
+    const uint64_t b = uniform_u64(off[u]), e = uniform_u64(off[u + 1]);
+    double acc = 0.0;
+    uint64_t bad = ~0ull;
+    double cur[kLongChunks], nxt[kLongChunks];
+    // unconditional loads (indices clamped to the list) keep the waits exact:
+    // the adds wait for the current step's loads only
+#pragma unroll
+    for (int q = 0; q < kLongChunks; q++) cur[q] = ew[min(b + 64 * q + lane, e - 1)];
+    for (uint64_t j0 = b; j0 < e; j0 += 64 * kLongChunks) {
+      const uint64_t jn = j0 + 64 * kLongChunks;
+#pragma unroll
+      for (int q = 0; q < kLongChunks; q++) nxt[q] = ew[min(jn + 64 * q + lane, e - 1)];
+#pragma unroll
+      for (int q = 0; q < kLongChunks; q++) {
+        const uint64_t c0 = j0 + 64 * q;
+        if (c0 >= e) break;
+        const double w = cur[q];
+        const uint64_t bm = __ballot(c0 + lane < e && (!isfinite(w) || w < 0.0));
+        if (bm && bad == ~0ull) bad = c0 + __ffsll((unsigned long long)bm) - 1;
+        const double wc = fmax(w, 0.0);
+        if (e - c0 >= 64) {
+#pragma unroll
+          for (int k = 0; k < 64; k++) acc += readlane_f64(wc, k);
+        } else {
+          const int cnt = (int)(e - c0);
+          for (int k = 0; k < cnt; k++) acc += readlane_f64(wc, k);
+        }
       }
-      inv_ws_o[u] = s != 0.0 ? 1.0 / s : 0.0;
+#pragma unroll
+      for (int q = 0; q < kLongChunks; q++) cur[q] = nxt[q];
+    }
+    if (lane == 0) {
+      inv_ws_o[u] = acc != 0.0 ? 1.0 / acc : 0.0;
+      if (bad != ~0ull) atomicMin(first_bad, (unsigned long long)bad);
     }
   }
 }
 
-__global__ void indeg_kernel(const uint32_t *nb, uint64_t m, uint32_t *indeg) {
-  GRID_STRIDE(k, m) { atomicAdd(&indeg[nb[k]], 1u); }
+// run [beg, end) of each id present in the sorted targets (absent ids keep
+// beg = end = 0)
+__global__ void runs_kernel(const uint32_t *t, uint64_t m, uint32_t *beg, uint32_t *end) {
+  GRID_STRIDE(i, m) {
+    const uint32_t v = t[i];
+    if (i == 0 || t[i - 1] != v) beg[v] = (uint32_t)i;
+    if (i + 1 == m || t[i + 1] != v) end[v] = (uint32_t)(i + 1);
+  }
 }
 
-// ascending key = descending in-degree; value = the id
-__global__ void relabel_keys_kernel(const uint32_t *indeg, uint64_t n, uint32_t *key, uint32_t *val) {
+// indeg = end - beg; ascending key = descending in-degree; value = the id
+__global__ void relabel_keys_kernel(const uint32_t *beg, const uint32_t *end, uint64_t n,
+                                    uint32_t *indeg, uint32_t *key, uint32_t *val) {
   GRID_STRIDE(v, n) {
-    key[v] = 0xFFFFFFFFu - indeg[v];
+    const uint32_t d = end[v] - beg[v];
+    indeg[v] = d;
+    key[v] = 0xFFFFFFFFu - d;
     val[v] = (uint32_t)v;
   }
 }
@@ -95,15 +179,6 @@ __global__ void rank_kernel(const uint32_t *order, const uint32_t *indeg, uint64
     const uint32_t v = order[r];
     rank[v] = (uint32_t)r;
     cnt[r] = (int32_t)indeg[v];
-  }
-}
-
-// key = rank[target] << cb | rank[source]; value = adapter position
-__global__ void edge_keys_kernel(const uint32_t *nb, const uint32_t *src, const uint32_t *rank,
-                                 uint64_t m, int cb, uint64_t *key, uint32_t *val) {
-  GRID_STRIDE(k, m) {
-    key[k] = ((uint64_t)rank[nb[k]] << cb) | (uint64_t)rank[src[k]];
-    if (val) val[k] = (uint32_t)k;
   }
 }
 
@@ -119,17 +194,27 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
   return (uint16_t)(u >> 16);
 }
 
-__global__ void emit_kernel(const uint64_t *key, const uint32_t *val, const double *ew, uint64_t m,
-                            uint64_t col_mask, int storage, int32_t *col, void *w) {
-  GRID_STRIDE(i, m) {
-    col[i] = (int32_t)(key[i] & col_mask);
-    if (ew) {
-      const double x = fmax(ew[val[i]], 0.0);
-      if (storage == GOPS_WEIGHT_F64) static_cast<double *>(w)[i] = x;
-      else if (storage == GOPS_WEIGHT_F32) static_cast<float *>(w)[i] = (float)x;
-      else static_cast<uint16_t *>(w)[i] = to_bf16((float)x);
-    }
+template <typename V>
+__device__ __forceinline__ V stored(double x);
+template <>
+__device__ __forceinline__ double stored<double>(double x) { return x; }
+template <>
+__device__ __forceinline__ float stored<float>(double x) { return (float)x; }
+template <>
+__device__ __forceinline__ uint16_t stored<uint16_t>(double x) { return to_bf16((float)x); }
+
+// key = rank[target] << cb | rank[source]; value = max(w, 0) as stored
+template <typename V>
+__global__ void edge_keys_kernel(const uint32_t *nb, const uint32_t *src, const uint32_t *rank,
+                                 const double *ew, uint64_t m, int cb, uint64_t *key, V *val) {
+  GRID_STRIDE(k, m) {
+    key[k] = ((uint64_t)rank[nb[k]] << cb) | (uint64_t)rank[src[k]];
+    if (ew) val[k] = stored<V>(fmax(ew[k], 0.0));
   }
+}
+
+__global__ void col_kernel(const uint64_t *key, uint64_t m, uint64_t col_mask, int32_t *col) {
+  GRID_STRIDE(i, m) { col[i] = (int32_t)(key[i] & col_mask); }
 }
 
 // per row (new ids): the f64 value and its f32 copy; a non-zero value that
@@ -173,6 +258,40 @@ int bits_for(uint64_t x) {
   return b;
 }
 
+// the edge sort and the kernel that feeds it, for one weight type (V unused
+// when unweighted: keys only)
+template <typename V>
+hipError_t sort_edges(void *tmp, size_t &bytes, const uint32_t *nb, const uint32_t *src,
+                      const uint32_t *rank, const double *ew, uint64_t m, int cb, uint64_t *k_in,
+                      uint64_t *k_out, V *v_in, V *v_out, hipStream_t s) {
+  if (tmp) {
+    edge_keys_kernel<V><<<grid_for(m), kB, 0, s>>>(nb, src, rank, ew, m, cb, k_in, v_in);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || m == 0) return e;
+  }
+  if (ew)
+    return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k_in, k_out, v_in, v_out, (int)m, 0,
+                                              2 * cb, s);
+  return hipcub::DeviceRadixSort::SortKeys(tmp, bytes, k_in, k_out, (int)m, 0, 2 * cb, s);
+}
+
+hipError_t sort_edges_for(int storage, void *tmp, size_t &bytes, const uint32_t *nb,
+                          const uint32_t *src, const uint32_t *rank, const double *ew, uint64_t m,
+                          int cb, uint64_t *k_in, uint64_t *k_out, void *v_in, void *v_out,
+                          hipStream_t s) {
+  switch (storage) {
+    case GOPS_WEIGHT_F64:
+      return sort_edges(tmp, bytes, nb, src, rank, ew, m, cb, k_in, k_out,
+                        static_cast<double *>(v_in), static_cast<double *>(v_out), s);
+    case GOPS_WEIGHT_F32:
+      return sort_edges(tmp, bytes, nb, src, rank, ew, m, cb, k_in, k_out,
+                        static_cast<float *>(v_in), static_cast<float *>(v_out), s);
+    default:  // bf16, or unweighted (keys only)
+      return sort_edges(tmp, bytes, nb, src, rank, ew, m, cb, k_in, k_out,
+                        static_cast<uint16_t *>(v_in), static_cast<uint16_t *>(v_out), s);
+  }
+}
+
 }  // namespace
 
 gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, const uint32_t *nb,
@@ -211,6 +330,8 @@ gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, 
     }                         \
   } while (0)
   {
+    const int storage = ew ? weight_storage : GOPS_WEIGHT_NONE;
+    const size_t wbytes = storage == GOPS_WEIGHT_F64 ? 8 : storage == GOPS_WEIGHT_F32 ? 4 : 2;
     uint64_t *d_off = pool.get<uint64_t>(n + 1);
     uint32_t *d_nb = pool.get<uint32_t>(m);
     double *d_ew = ew ? pool.get<double>(m) : nullptr;
@@ -230,7 +351,7 @@ gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, 
       return GOPS_INDEX_OUT_OF_BOUNDS;
     }
 
-    const int cb = bits_for(n ? n - 1 : 0);  // bits of a new id
+    const int cb = bits_for(n ? n - 1 : 0);  // bits of an id
     uint32_t *d_marks = pool.get<uint32_t>(m), *d_src = pool.get<uint32_t>(m);
     double *d_inv_deg_o = pool.get<double>(n);
     double *d_inv_ws_o = ew ? pool.get<double>(n) : nullptr;
@@ -240,42 +361,70 @@ gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, 
     uint32_t *d_order = pool.get<uint32_t>(n), *d_rank = pool.get<uint32_t>(n);
     int32_t *d_cnt = pool.get<int32_t>(n + 1), *d_row_ptr = pool.get<int32_t>(n + 1);
     uint64_t *d_key64 = pool.get<uint64_t>(m), *d_key64b = pool.get<uint64_t>(m);
-    uint32_t *d_val64 = ew ? pool.get<uint32_t>(m) : nullptr;
-    uint32_t *d_val64b = ew ? pool.get<uint32_t>(m) : nullptr;
+    void *d_wv = ew ? (void *)pool.get<uint8_t>(m * wbytes) : nullptr;
+    void *d_w = ew ? (void *)pool.get<uint8_t>(m * wbytes) : nullptr;  // sorted: the graph's
     DTRY(pool.err);
 
     // scratch of the hipCUB calls: the largest of their needs
     size_t tmp_bytes = 1, b = 0;
     DTRY(hipcub::DeviceScan::InclusiveScan(nullptr, b, d_marks, d_src, hipcub::Max(), (int)m, s));
     tmp_bytes = std::max(tmp_bytes, b);
+    DTRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b, d_nb, d_marks, (int)m, 0, cb, s));
+    tmp_bytes = std::max(tmp_bytes, b);
     DTRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, d_key32, d_key32b, d_val32, d_order, (int)n,
                                             0, 32, s));
     tmp_bytes = std::max(tmp_bytes, b);
     DTRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b, d_cnt, d_row_ptr, (int)n + 1, s));
     tmp_bytes = std::max(tmp_bytes, b);
-    if (ew)
-      DTRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, d_key64, d_key64b, d_val64, d_val64b,
-                                              (int)m, 0, 2 * cb, s));
-    else
-      DTRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b, d_key64, d_key64b, (int)m, 0, 2 * cb, s));
+    DTRY(sort_edges_for(storage, nullptr, b, d_nb, d_src, d_rank, d_ew, m, cb, d_key64, d_key64b,
+                        d_wv, d_w, s));
     tmp_bytes = std::max(tmp_bytes, b);
+    uint8_t *d_is_long = ew ? pool.get<uint8_t>(n) : nullptr;
+    uint32_t *d_long = ew ? pool.get<uint32_t>(n) : nullptr;
+    uint32_t *d_num_long = pool.get<uint32_t>(1);
+    DTRY(pool.err);
+    if (ew) {
+      DTRY(hipcub::DeviceSelect::Flagged(nullptr, b, hipcub::CountingInputIterator<uint32_t>(0),
+                                         d_is_long, d_long, d_num_long, (int64_t)n, s));
+      tmp_bytes = std::max(tmp_bytes, b);
+    }
     void *d_tmp = pool.get<uint8_t>(tmp_bytes);
     DTRY(pool.err);
 
-    // 2-4: sources, per-node sums, in-degrees
+    // 2-3: sources, per-node sums
     if (m) {
       DTRY(hipMemsetAsync(d_marks, 0, m * sizeof(uint32_t), s));
       mark_starts_kernel<<<grid_for(n), kB, 0, s>>>(d_off, n, d_marks);
       b = tmp_bytes;
       DTRY(hipcub::DeviceScan::InclusiveScan(d_tmp, b, d_marks, d_src, hipcub::Max(), (int)m, s));
     }
-    if (n) DTRY(hipMemsetAsync(d_indeg, 0, n * sizeof(uint32_t), s));
-    node_kernel<<<grid_for(n), kB, 0, s>>>(d_off, d_ew, n, d_inv_deg_o, d_inv_ws_o, d_flags + 1);
-    indeg_kernel<<<grid_for(m), kB, 0, s>>>(d_nb, m, d_indeg);
+    node_kernel<<<grid_for(n), kB, 0, s>>>(d_off, d_ew, n, d_inv_deg_o, d_inv_ws_o, d_flags + 1,
+                                           d_is_long);
+    DTRY(hipGetLastError());
+    if (ew && n) {
+      b = tmp_bytes;
+      DTRY(hipcub::DeviceSelect::Flagged(d_tmp, b, hipcub::CountingInputIterator<uint32_t>(0),
+                                         d_is_long, d_long, d_num_long, (int64_t)n, s));
+      long_list_kernel<<<8192, kB, 0, s>>>(d_off, d_ew, d_long, d_num_long, d_inv_ws_o,
+                                          d_flags + 1);
+      DTRY(hipGetLastError());
+    }
+
+    // 4: in-degrees (the marks buffer holds the sorted targets, the key
+    // buffers the runs)
+    if (n) {
+      DTRY(hipMemsetAsync(d_key32, 0, n * sizeof(uint32_t), s));
+      DTRY(hipMemsetAsync(d_key32b, 0, n * sizeof(uint32_t), s));
+    }
+    if (m) {
+      b = tmp_bytes;
+      DTRY(hipcub::DeviceRadixSort::SortKeys(d_tmp, b, d_nb, d_marks, (int)m, 0, cb, s));
+      runs_kernel<<<grid_for(m), kB, 0, s>>>(d_marks, m, d_key32, d_key32b);
+    }
+    relabel_keys_kernel<<<grid_for(n), kB, 0, s>>>(d_key32, d_key32b, n, d_indeg, d_key32, d_val32);
     DTRY(hipGetLastError());
 
     // 5-6: order, rank, row pointers
-    relabel_keys_kernel<<<grid_for(n), kB, 0, s>>>(d_indeg, n, d_key32, d_val32);
     if (n) {
       b = tmp_bytes;
       DTRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, b, d_key32, d_key32b, d_val32, d_order, (int)n,
@@ -287,28 +436,17 @@ gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, 
     b = tmp_bytes;
     DTRY(hipcub::DeviceScan::ExclusiveSum(d_tmp, b, d_cnt, d_row_ptr, (int)n + 1, s));
 
-    // 7-8: edges by (row, new source id), ties in adapter order
-    edge_keys_kernel<<<grid_for(m), kB, 0, s>>>(d_nb, d_src, d_rank, m, cb, d_key64, d_val64);
-    DTRY(hipGetLastError());
-    if (m) {
-      b = tmp_bytes;
-      if (ew)
-        DTRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, b, d_key64, d_key64b, d_val64, d_val64b,
-                                                (int)m, 0, 2 * cb, s));
-      else
-        DTRY(hipcub::DeviceRadixSort::SortKeys(d_tmp, b, d_key64, d_key64b, (int)m, 0, 2 * cb, s));
-    }
-    const int storage = ew ? weight_storage : GOPS_WEIGHT_NONE;
-    const size_t wbytes = storage == GOPS_WEIGHT_F64 ? 8 : storage == GOPS_WEIGHT_F32 ? 4 : 2;
+    // 7-8: edges by (row, new source id), ties in adapter order; col
+    b = tmp_bytes;
+    DTRY(sort_edges_for(storage, d_tmp, b, d_nb, d_src, d_rank, d_ew, m, cb, d_key64, d_key64b,
+                        d_wv, d_w, s));
     int32_t *d_col = pool.get<int32_t>(m);
-    void *d_w = ew ? (void *)pool.get<uint8_t>(m * wbytes) : nullptr;
     double *d_inv_deg64 = pool.get<double>(n);
     float *d_inv_deg32 = pool.get<float>(n);
     double *d_inv_ws64 = ew ? pool.get<double>(n) : nullptr;
     float *d_inv_ws32 = ew ? pool.get<float>(n) : nullptr;
     DTRY(pool.err);
-    emit_kernel<<<grid_for(m), kB, 0, s>>>(d_key64b, d_val64b, d_ew, m, (1ull << cb) - 1, storage,
-                                           d_col, d_w);
+    col_kernel<<<grid_for(m), kB, 0, s>>>(d_key64b, m, (1ull << cb) - 1, d_col);
     inv_kernel<<<grid_for(n), kB, 0, s>>>(d_order, d_inv_deg_o, n, d_inv_deg64, d_inv_deg32);
     if (ew) inv_kernel<<<grid_for(n), kB, 0, s>>>(d_order, d_inv_ws_o, n, d_inv_ws64, d_inv_ws32);
     DTRY(hipGetLastError());

@@ -127,6 +127,34 @@ def test_device_build_weighted_multigraph(ctx_dev, ctx_host, storage):
     check_build(ctx_dev, ctx_host, 3000, off, nb, ew, storage)
 
 
+@pytest.mark.parametrize("first_bad", ["none", "nan", "negative"])
+def test_device_build_long_out_lists(ctx_dev, ctx_host, first_bad):
+    """Out-lists longer than one wave (64 entries) are summed by a wave each:
+    lengths around the 64-entry chunks, hubs next to each other, and the
+    first invalid weight deep inside a long list."""
+    rng = np.random.default_rng(11)
+    n = 4000
+    deg = rng.integers(0, 8, n)
+    deg[:8] = [65, 64, 128, 129, 1000, 5000, 70000, 63]
+    deg[2000] = 300
+    off = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint64)
+    m = int(off[-1])
+    nb = rng.integers(0, n, m).astype(np.uint32)
+    ew = rng.lognormal(0.0, 2.0, m)  # a wide range: the summation order shows
+    ew[rng.random(m) < 0.05] = 0.0
+    if first_bad != "none":
+        j = int(off[5]) + 4099  # list 5, chunk 64, lane 3
+        ew[j] = np.nan if first_bad == "nan" else -0.5
+        ew[int(off[6]) + 70] = -1.0 if first_bad == "nan" else np.inf
+    check_build(ctx_dev, ctx_host, n, off, nb, ew, "f64")
+    if first_bad != "none":
+        msg = "finite" if first_bad == "nan" else "non-negative"
+        for cx in (ctx_dev, ctx_host):
+            dg = ga.DeviceGraph.from_lists(n, off, nb, ew, ctx=cx)
+            with pytest.raises(ga.InvalidParameter, match=msg):
+                ga.pagerank_weighted_checked(dg, ga.PageRankConfig(), ctx=cx)
+
+
 def test_device_build_unweighted_multigraph(ctx_dev, ctx_host):
     off, nb, _ = random_multigraph(5000, 12, seed=6, weighted=False)
     check_build(ctx_dev, ctx_host, 5000, off, nb)
