@@ -1,7 +1,8 @@
 """graphops_amd -- graphops' PageRank family on AMD MI355X (gfx950).
 
 Python mirror of the reference's operator API (crate graphops v0.4.1,
-src/pagerank.rs and src/ppr.rs): same function names, the same
+src/pagerank.rs and src/ppr.rs; Katz, eigenvector and HITS centrality from
+src/katz.rs, src/eigenvector.rs and src/centrality.rs): same function names, the same
 PageRankConfig / PageRankRun, the same checked/unchecked split and error
 messages.  Every call goes through the C ABI of libgraphops_amd.so
 (include/graphops_amd.h) to hand-written HIP kernels; there is no CPU
@@ -32,7 +33,9 @@ __all__ = [
     "pagerank_weighted", "pagerank_weighted_run", "pagerank_weighted_checked",
     "pagerank_weighted_checked_run", "personalized_pagerank", "personalized_pagerank_run",
     "personalized_pagerank_checked", "personalized_pagerank_checked_run", "ppr_batch_run",
-    "top_k", "normalize", "Server",
+    "top_k", "normalize", "Server", "KatzConfig", "KatzRun", "katz_centrality",
+    "katz_centrality_run", "katz_centrality_checked", "EigenvectorRun",
+    "eigenvector_centrality", "eigenvector_centrality_run", "hits", "hits_run",
 ]
 
 
@@ -540,6 +543,131 @@ def ppr_tile_info(graph, num_queries, ctx: Context = None):
 
 
 # ---------------------------------------------------------------- ranking
+# ---------------------------------------------------------------- centrality
+@dataclass
+class KatzConfig:
+    """KatzConfig, katz.rs:25-53 (defaults 0.1 / 1.0 / 100 / 1e-6)."""
+
+    alpha: float = 0.1
+    beta: float = 1.0
+    max_iterations: int = 100
+    tolerance: float = 1e-6
+
+    def validate(self) -> None:
+        """KatzConfig::validate, katz.rs:57-84."""
+        a, b, t = float(self.alpha), float(self.beta), float(self.tolerance)
+        if not np.isfinite(a) or a <= 0.0:
+            raise InvalidParameter("invalid parameter: alpha must be finite and > 0")
+        if a >= 1.0:
+            raise InvalidParameter(
+                "invalid parameter: alpha must be < 1 (required for convergence; tighter bound "
+                "alpha < 1/spectral_radius(A) may be needed for large graphs)")
+        if not np.isfinite(b):
+            raise InvalidParameter("invalid parameter: beta must be finite")
+        if int(self.max_iterations) == 0:
+            raise InvalidParameter("invalid parameter: max_iterations must be > 0")
+        if not np.isfinite(t) or t <= 0.0:
+            raise InvalidParameter("invalid parameter: tolerance must be finite and > 0")
+
+    def _c(self):
+        return L.KatzConfigC(float(self.alpha), float(self.beta), int(self.max_iterations),
+                             float(self.tolerance))
+
+
+@dataclass
+class KatzRun:
+    """KatzRun, katz.rs:87-104."""
+
+    scores: np.ndarray
+    iterations: int
+    diff_l1: float
+    converged: bool
+    stats: dict = field(default_factory=dict, repr=False)
+
+
+@dataclass
+class EigenvectorRun:
+    """EigenvectorRun, eigenvector.rs:12-23."""
+
+    scores: np.ndarray
+    iterations: int
+    diff_l1: float
+    converged: bool
+    stats: dict = field(default_factory=dict, repr=False)
+
+
+def _katz(graph, config, checked, ctx):
+    config = config if config is not None else KatzConfig()
+    dg = _device_graph(graph, False, ctx)
+    c = dg.ctx.handle
+    scores = np.zeros(max(dg.n, 1))
+    info = L.RunInfoC()
+    cfg = config._c()
+    name = "gops_katz_centrality_checked_run" if checked else "gops_katz_centrality_run"
+    rc = getattr(L.lib(), name)(c, dg.handle, ctypes.byref(cfg), L.ptr(scores, L._f64p),
+                                ctypes.byref(info))
+    _check(rc, c, name)
+    return KatzRun(scores[:dg.n], int(info.iterations), float(info.diff_l1),
+                   bool(info.converged), dg.ctx.last_stats())
+
+
+def katz_centrality_run(graph, config: KatzConfig = None, ctx: Context = None) -> KatzRun:
+    """katz_centrality_run, katz.rs:136-201."""
+    return _katz(graph, config, False, ctx)
+
+
+def katz_centrality(graph, config: KatzConfig = None, ctx: Context = None) -> np.ndarray:
+    """katz_centrality, katz.rs:131-133."""
+    return _katz(graph, config, False, ctx).scores
+
+
+def katz_centrality_checked(graph, config: KatzConfig, ctx: Context = None) -> np.ndarray:
+    """katz_centrality_checked, katz.rs:207-210."""
+    return _katz(graph, config, True, ctx).scores
+
+
+def eigenvector_centrality_run(graph, max_iterations: int = 100, tolerance: float = 1e-6,
+                               ctx: Context = None) -> EigenvectorRun:
+    """eigenvector_centrality_run, eigenvector.rs:49-125."""
+    dg = _device_graph(graph, False, ctx)
+    c = dg.ctx.handle
+    scores = np.zeros(max(dg.n, 1))
+    info = L.RunInfoC()
+    rc = L.lib().gops_eigenvector_centrality_run(c, dg.handle, int(max_iterations),
+                                                 float(tolerance), L.ptr(scores, L._f64p),
+                                                 ctypes.byref(info))
+    _check(rc, c, "gops_eigenvector_centrality_run")
+    return EigenvectorRun(scores[:dg.n], int(info.iterations), float(info.diff_l1),
+                          bool(info.converged), dg.ctx.last_stats())
+
+
+def eigenvector_centrality(graph, ctx: Context = None) -> np.ndarray:
+    """eigenvector_centrality, eigenvector.rs:26-28 (100 iterations, 1e-6)."""
+    return eigenvector_centrality_run(graph, 100, 1e-6, ctx).scores
+
+
+def hits_run(graph, max_iter: int = 100, tol: float = 1e-6, ctx: Context = None):
+    """hits (centrality.rs:114-166) with the iteration count, the last hub
+    change and whether both changes ended below tol: (hub, authority, info)."""
+    dg = _device_graph(graph, False, ctx)
+    c = dg.ctx.handle
+    hub = np.zeros(max(dg.n, 1))
+    auth = np.zeros(max(dg.n, 1))
+    info = L.RunInfoC()
+    rc = L.lib().gops_hits(c, dg.handle, int(max_iter), float(tol), L.ptr(hub, L._f64p),
+                           L.ptr(auth, L._f64p), ctypes.byref(info))
+    _check(rc, c, "gops_hits")
+    return hub[:dg.n], auth[:dg.n], {"iterations": int(info.iterations),
+                                     "hub_diff": float(info.diff_l1),
+                                     "converged": bool(info.converged)}
+
+
+def hits(graph, max_iter: int, tol: float, ctx: Context = None):
+    """hits, centrality.rs:114-166: (hub_scores, authority_scores)."""
+    hub, auth, _ = hits_run(graph, max_iter, tol, ctx)
+    return hub, auth
+
+
 def top_k(scores, k):
     """top_k, topk.rs:17-42: finite, > 0 scores, descending (ties in
     unspecified order, as the reference's unstable sort)."""

@@ -41,6 +41,11 @@ class PageRankConfigC(ctypes.Structure):
                 ("tolerance", ctypes.c_double)]
 
 
+class KatzConfigC(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("max_iterations", ctypes.c_uint64), ("tolerance", ctypes.c_double)]
+
+
 class RunInfoC(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_uint64), ("diff_l1", ctypes.c_double),
                 ("converged", ctypes.c_int32), ("reserved", ctypes.c_int32)]
@@ -91,6 +96,14 @@ SIGNATURES = [
     ("gops_personalized_pagerank_checked_run", ctypes.c_int,
      [_vp, _vp, ctypes.POINTER(PageRankConfigC), _f64p, ctypes.c_uint64, _f64p,
       ctypes.POINTER(RunInfoC)]),
+    ("gops_katz_centrality_run", ctypes.c_int,
+     [_vp, _vp, ctypes.POINTER(KatzConfigC), _f64p, ctypes.POINTER(RunInfoC)]),
+    ("gops_katz_centrality_checked_run", ctypes.c_int,
+     [_vp, _vp, ctypes.POINTER(KatzConfigC), _f64p, ctypes.POINTER(RunInfoC)]),
+    ("gops_eigenvector_centrality_run", ctypes.c_int,
+     [_vp, _vp, ctypes.c_uint64, ctypes.c_double, _f64p, ctypes.POINTER(RunInfoC)]),
+    ("gops_hits", ctypes.c_int,
+     [_vp, _vp, ctypes.c_uint64, ctypes.c_double, _f64p, _f64p, ctypes.POINTER(RunInfoC)]),
     ("gops_ppr_batch_run", ctypes.c_int,
      [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(PageRankConfigC), _u64p, _u32p, _f64p, _f64p,
       ctypes.POINTER(RunInfoC)]),

@@ -77,6 +77,14 @@ void free_run(RunBuffers &rb) {
   rb = RunBuffers{};
 }
 
+void free_power(PowerBuffers &pb) {
+  for (double *v : pb.v) hipFree(v);
+  hipFree(pb.partials);
+  hipFree(pb.hub_partial);
+  hipFree(pb.state);
+  pb = PowerBuffers{};
+}
+
 void free_dev(DevGraph &dg) {
   hipFree(dg.row_ptr);
   hipFree(dg.col);
@@ -118,16 +126,7 @@ gops_status alloc_run(gops_context *ctx, gops_graph *g) {
 
 // the block plan of the sweep kernels
 gops_status upload_plan(gops_context *ctx, gops_graph *g, const Plan &p) {
-  DevGraph &dg = g->dev;
-  HIP_TRY(ctx, upload(ctx, &dg.desc, p.desc.data(), p.desc.size()));
-  dg.num_desc = (int)p.desc.size();
-  HIP_TRY(ctx, upload(ctx, &dg.hub_row, p.hub_row.data(), p.hub_row.size()));
-  HIP_TRY(ctx, upload(ctx, &dg.hub_nchunks, p.hub_nchunks.data(), p.hub_nchunks.size()));
-  HIP_TRY(ctx, upload(ctx, &dg.hub_first, p.hub_first.data(), p.hub_first.size()));
-  dg.num_hubs = p.num_hubs;
-  dg.num_hub_chunks = 0;
-  for (int32_t c : p.hub_nchunks) dg.num_hub_chunks += c;
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, upload_plan_to(ctx->stream, g->dev, p));
   return GOPS_OK;
 }
 
@@ -578,7 +577,9 @@ void gops_graph_destroy(gops_graph *g) {
   }
   free_ppr_cache(g);
   free_run(g->run);
+  free_power(g->pow);
   free_dev(g->dev);
+  free_dev(g->fwd);
   delete g;
 }
 

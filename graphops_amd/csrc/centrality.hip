@@ -1,0 +1,310 @@
+// centrality.hip -- gfx950 kernels of Katz, eigenvector and HITS centrality
+// (katz.rs:136-201, eigenvector.rs:49-125, centrality.rs:114-166): power
+// iterations in f64 over the same pull machinery as PageRank (work
+// descriptors, LDS-staged runs, split hub rows, no inter-workgroup waits).
+//
+//   Katz         x <- beta + A^T (alpha x): one pull over the in-edge CSR
+//                gathering the prescaled alpha * x, residual sum |x - x'|.
+//   eigenvector  t = x + A x: a pull over the out-edge CSR gathering x;
+//                then x' = t * sign / |t|_2 (sign of sum t), residual
+//                sum |x - x'|.
+//   HITS         a = A^T h (in-edge CSR), a' = a / |a|_2; h = A a' (out-edge
+//                CSR), h' = h / |h|_2; residuals |a - a'|_2 and |h - h'|_2.
+//
+// Every kernel of an iteration first reads PowerState::done and returns at
+// once when the chain has finished, so the host can queue iterations ahead.
+// Reductions are per workgroup and then summed in a fixed order by one
+// workgroup, so results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "internal.h"
+#include "sweep_common.h"
+
+namespace gops {
+
+constexpr int kPowR = 2;  // reductions per workgroup
+
+struct PowerArgs {
+  const BlockDesc *desc;
+  int num_desc;
+  const int32_t *row_ptr;
+  const int32_t *col;
+  double *hub_partial;
+  const int32_t *hub_row;
+  const int32_t *hub_nchunks;
+  const int32_t *hub_first;
+  int num_hubs;
+  const double *yin;  // the gathered vector (Katz: picked by PowerState::par)
+  double *x;          // Katz: the iterate; eigenvector: the iterate (its own term)
+  double *t;          // eigenvector, HITS: the row sums
+  double *partials;
+  PowerState *st;
+};
+
+// f64 gathers of an unweighted graph (the staged product is 1.0 * y, exact)
+struct GatherF64 {
+  using P = double;
+  static constexpr int kStageUnroll = kUnroll / 2;
+  const double *y;
+  const int32_t *col;
+  __device__ __forceinline__ P stage_weight(int) const { return 1.0; }
+  __device__ __forceinline__ P stage_gather(int c) const { return y[c]; }
+  __device__ __forceinline__ double operator()(int k) const {
+    return y[__builtin_nontemporal_load(col + k)];
+  }
+};
+
+// katz.rs:166-174: x'[v] = beta + sum over in-edges of alpha * x[u]
+struct KatzEpi {
+  const PowerArgs &a;
+  double alpha, beta;
+  double *yout;
+  __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kPowR]) const {
+    const double t = beta + acc;
+    red[0] += fabs(a.x[r] - t);
+    a.x[r] = t;
+    yout[r] = alpha * t;
+  }
+};
+
+// eigenvector.rs:76-84: t[v] = x[v] + sum over v's neighbours of x[u]
+struct EigenEpi {
+  const PowerArgs &a;
+  __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kPowR]) const {
+    const double t = a.x[r] + acc;
+    a.t[r] = t;
+    red[0] += t * t;
+    red[1] += t;
+  }
+};
+
+// centrality.rs:136-152: t[v] = sum of the gathered vector over the row
+struct SumEpi {
+  const PowerArgs &a;
+  __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kPowR]) const {
+    a.t[r] = acc;
+    red[0] += acc * acc;
+  }
+};
+
+__device__ __forceinline__ void publish(PowerState *st, double diff, bool conv) {
+  const uint64_t it = st->iter + 1;
+  const bool done = conv || it >= st->max_iter;
+  st->iter = it;
+  st->diff = diff;
+  st->converged = conv;
+  st->done = done;
+  if (st->mirror) {
+    __hip_atomic_store(st->mirror, (it << 2) | (uint64_t)(conv ? 2 : 0) | (uint64_t)done,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// body(epilogue, gathered vector) with the algorithm's epilogue
+template <int ALGO, typename F>
+__device__ __forceinline__ void with_epilogue(const PowerArgs &a, double *y0, double *y1,
+                                              const F &body) {
+  const PowerState *st = a.st;
+  if constexpr (ALGO == kKatz) {
+    const KatzEpi epi{a, st->alpha, st->beta, st->par ? y0 : y1};
+    body(epi, st->par ? y1 : y0);
+  } else if constexpr (ALGO == kEigen) {
+    body(EigenEpi{a}, a.yin);
+  } else {
+    body(SumEpi{a}, a.yin);
+  }
+}
+
+// One pull over a CSR (in-edges or out-edges) with the algorithm's row epilogue.
+template <int ALGO>
+__global__ __launch_bounds__(kBlock) void power_sweep_kernel(PowerArgs a, double *y0, double *y1) {
+  if (a.st->done) return;
+  __shared__ double stage[kStageNnz];
+  double red[kPowR] = {0.0, 0.0};
+  with_epilogue<ALGO>(a, y0, y1, [&](const auto &epi, const double *yin) {
+    const GatherF64 load{yin, a.col};
+    for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x)
+      do_desc(a, a.desc[i], load, epi, red, stage);
+  });
+  block_sum<kPowR>(red);
+  if (threadIdx.x == 0) {
+    a.partials[(size_t)blockIdx.x * kPowR] = red[0];
+    a.partials[(size_t)blockIdx.x * kPowR + 1] = red[1];
+  }
+}
+
+// One workgroup: hub-row epilogues (chunks summed in chunk order), the
+// fixed-order sum of the sweep's partials, and the step's bookkeeping.
+template <int ALGO>
+__global__ __launch_bounds__(kBlock) void power_finish_kernel(PowerArgs a, double *y0, double *y1,
+                                                              int sweep_blocks) {
+  PowerState *st = a.st;
+  if (st->done) return;
+  double red[kPowR] = {0.0, 0.0};
+  with_epilogue<ALGO>(a, y0, y1, [&](const auto &epi, const double *) {
+    for (int h = threadIdx.x; h < a.num_hubs; h += kBlock) {
+      const int first = a.hub_first[h], nch = a.hub_nchunks[h];
+      double s = 0.0;
+      for (int c = 0; c < nch; c++) s += a.hub_partial[first + c];
+      epi(a.hub_row[h], s, red);
+    }
+  });
+  for (int b = threadIdx.x; b < sweep_blocks; b += kBlock) {
+    red[0] += a.partials[(size_t)b * kPowR];
+    red[1] += a.partials[(size_t)b * kPowR + 1];
+  }
+  block_sum<kPowR>(red);
+  if (threadIdx.x == 0) {
+    if constexpr (ALGO == kKatz) {  // katz.rs:187-192
+      st->par ^= 1;
+      publish(st, red[0], red[0] < st->tolerance);
+    } else {  // eigenvector.rs:87-94, centrality.rs:168-175
+      st->norm = sqrt(red[0]);
+      st->sign = (ALGO == kEigen && !(red[1] >= 0.0)) ? -1.0 : 1.0;
+    }
+  }
+}
+
+// x' = t * sign / norm (t itself when norm is not > 0); partial residuals:
+// sum |x - x'| (L1) or sum (x - x')^2 (L2).  x is overwritten with x'.
+template <bool L2>
+__global__ __launch_bounds__(kBlock) void power_scale_kernel(const double *t, double *x, int n,
+                                                             const PowerState *st,
+                                                             double *partials) {
+  if (st->done) return;
+  const double norm = st->norm, sign = st->sign;
+  const bool scale = norm > 0.0;
+  double red[1] = {0.0};
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
+    const double nv = scale ? t[v] * sign / norm : t[v];
+    const double dv = x[v] - nv;
+    red[0] += L2 ? dv * dv : fabs(dv);
+    x[v] = nv;
+  }
+  block_sum<1>(red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// One workgroup: the residual of a scale step and the iteration's end.
+//   ALGO kEigen: diff = sum |x - x'| (eigenvector.rs:101-116)
+//   ALGO kHits, stage 0: the authority change, kept for stage 1
+//   ALGO kHits, stage 1: the hub change; stop when both are below tol
+//   (centrality.rs:156-162)
+template <int ALGO>
+__global__ __launch_bounds__(kBlock) void power_step_kernel(PowerState *st, const double *partials,
+                                                            int count, int stage) {
+  if (st->done) return;
+  double red[1] = {0.0};
+  for (int b = threadIdx.x; b < count; b += kBlock) red[0] += partials[b];
+  block_sum<1>(red);
+  if (threadIdx.x == 0) {
+    if constexpr (ALGO == kEigen) {
+      publish(st, red[0], red[0] < st->tolerance);
+    } else if (stage == 0) {
+      st->diff2 = sqrt(red[0]);
+    } else {
+      const double hd = sqrt(red[0]);
+      publish(st, hd, st->diff2 < st->tolerance && hd < st->tolerance);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(double *v, int n, double value) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v[i] = value;
+}
+
+__global__ __launch_bounds__(kBlock) void permute_kernel(const double *src, const uint32_t *rank,
+                                                         double *out, int n) {
+  for (int o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock)
+    out[o] = src[rank[o]];
+}
+
+// ---------------------------------------------------------------------------
+static int elementwise_blocks(uint64_t n) {
+  uint64_t blocks = (n + kBlock * 8 - 1) / (kBlock * 8);
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 2048));
+}
+
+static PowerArgs power_args(gops_graph *g, const DevGraph &dg) {
+  PowerArgs a{};
+  a.desc = dg.desc;
+  a.num_desc = dg.num_desc;
+  a.row_ptr = dg.row_ptr;
+  a.col = dg.col;
+  a.hub_partial = g->pow.hub_partial;
+  a.hub_row = dg.hub_row;
+  a.hub_nchunks = dg.hub_nchunks;
+  a.hub_first = dg.hub_first;
+  a.num_hubs = dg.num_hubs;
+  a.partials = g->pow.partials;
+  a.st = g->pow.state;
+  return a;
+}
+
+template <int ALGO>
+static void sweep(gops_context *ctx, const PowerArgs &a, double *y0, double *y1) {
+  const int grid = std::min(a.num_desc, kMaxSweepBlocks);
+  power_sweep_kernel<ALGO><<<grid, kBlock, 0, ctx->stream>>>(a, y0, y1);
+  power_finish_kernel<ALGO><<<1, kBlock, 0, ctx->stream>>>(a, y0, y1, grid);
+}
+
+template <int ALGO, bool L2>
+static void scale_step(gops_context *ctx, gops_graph *g, const double *t, double *x, int stage) {
+  const int n = (int)g->dev.n;
+  const int grid = elementwise_blocks(n);
+  PowerBuffers &pb = g->pow;
+  power_scale_kernel<L2><<<grid, kBlock, 0, ctx->stream>>>(t, x, n, pb.state, pb.partials);
+  power_step_kernel<ALGO><<<1, kBlock, 0, ctx->stream>>>(pb.state, pb.partials, grid, stage);
+}
+
+// Buffer roles: Katz v0 = x, v1/v2 = alpha*x (ping-pong); eigenvector
+// v0 = x, v1 = t; HITS v0 = hub, v2 = authority, v1 = t.  value: the start
+// value of every node; prescaled: Katz's first gathered value, alpha * beta.
+hipError_t launch_power_init(gops_context *ctx, gops_graph *g, PowerAlgo algo, double value,
+                             double prescaled) {
+  const int n = (int)g->dev.n;
+  const int grid = elementwise_blocks(n);
+  PowerBuffers &pb = g->pow;
+  fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[0], n, value);
+  if (algo == kKatz) fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[1], n, prescaled);
+  if (algo == kHits) fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[2], n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_power_iteration(gops_context *ctx, gops_graph *g, PowerAlgo algo) {
+  PowerBuffers &pb = g->pow;
+  if (algo == kKatz) {
+    PowerArgs a = power_args(g, g->dev);
+    a.x = pb.v[0];
+    sweep<kKatz>(ctx, a, pb.v[1], pb.v[2]);
+  } else if (algo == kEigen) {
+    PowerArgs a = power_args(g, g->fwd);
+    a.yin = pb.v[0];
+    a.x = pb.v[0];
+    a.t = pb.v[1];
+    sweep<kEigen>(ctx, a, nullptr, nullptr);
+    scale_step<kEigen, false>(ctx, g, pb.v[1], pb.v[0], 0);
+  } else {
+    PowerArgs a = power_args(g, g->dev);  // authority: in-edges, gathers hub
+    a.yin = pb.v[0];
+    a.t = pb.v[1];
+    sweep<kHits>(ctx, a, nullptr, nullptr);
+    scale_step<kHits, true>(ctx, g, pb.v[1], pb.v[2], 0);
+    PowerArgs h = power_args(g, g->fwd);  // hub: out-edges, gathers the new authority
+    h.yin = pb.v[2];
+    h.t = pb.v[1];
+    sweep<kHits>(ctx, h, nullptr, nullptr);
+    scale_step<kHits, true>(ctx, g, pb.v[1], pb.v[0], 1);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_permute(gops_context *ctx, gops_graph *g, const double *src, double *out) {
+  const int n = (int)g->dev.n;
+  permute_kernel<<<elementwise_blocks(n), kBlock, 0, ctx->stream>>>(src, g->dev.rank, out, n);
+  return hipGetLastError();
+}
+
+}  // namespace gops

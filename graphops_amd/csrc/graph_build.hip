@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -228,6 +229,34 @@ __global__ void inv_kernel(const uint32_t *order, const double *src, uint64_t n,
     if (x != 0.0 && f == 0.0f) f = std::numeric_limits<float>::denorm_min();
     d32[r] = f;
   }
+}
+
+// marks[row_ptr[r]] = r for each non-empty row (their starts are distinct)
+__global__ void mark_rows_kernel(const int32_t *row_ptr, uint64_t n, uint32_t *marks) {
+  GRID_STRIDE(r, n) {
+    if (row_ptr[r + 1] > row_ptr[r]) marks[row_ptr[r]] = (uint32_t)r;
+  }
+}
+
+// key = source << cb | target, from the in-edge CSR (k: edge, row[k]: target)
+__global__ void fwd_keys_kernel(const int32_t *col, const uint32_t *row, uint64_t m, int cb,
+                                uint64_t *key) {
+  GRID_STRIDE(k, m) { key[k] = ((uint64_t)(uint32_t)col[k] << cb) | row[k]; }
+}
+
+// sorted keys -> targets (col) and sources (for the runs)
+__global__ void fwd_split_kernel(const uint64_t *key, uint64_t m, int cb, int32_t *col,
+                                 uint32_t *src) {
+  GRID_STRIDE(k, m) {
+    const uint64_t x = key[k];
+    col[k] = (int32_t)(x & ((1ull << cb) - 1));
+    src[k] = (uint32_t)(x >> cb);
+  }
+}
+
+__global__ void run_counts_kernel(const uint32_t *beg, const uint32_t *end, uint64_t n,
+                                  int32_t *cnt) {
+  GRID_STRIDE(v, n) { cnt[v] = (int32_t)(end[v] - beg[v]); }
 }
 
 #undef GRID_STRIDE
@@ -495,6 +524,117 @@ hip_error:
 #undef DTRY
   (void)hipStreamSynchronize(s);
   err = std::string("device graph build: ") + what + ": " + hipGetErrorString(herr);
+  return (herr == hipErrorOutOfMemory || herr == hipErrorMemoryAllocation) ? GOPS_OUT_OF_MEMORY
+                                                                           : GOPS_DEVICE_ERROR;
+}
+
+hipError_t upload_plan_to(hipStream_t stream, DevGraph &dg, const Plan &p) {
+  auto up = [&](auto **dst, const auto &vec) -> hipError_t {
+    using T = typename std::decay_t<decltype(vec)>::value_type;
+    *dst = nullptr;
+    hipError_t e = hipMalloc((void **)dst, std::max<size_t>(vec.size(), 1) * sizeof(T));
+    if (e != hipSuccess || vec.empty()) return e;
+    return hipMemcpyAsync(*dst, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice, stream);
+  };
+  hipError_t e;
+  if ((e = up(&dg.desc, p.desc)) != hipSuccess) return e;
+  dg.num_desc = (int)p.desc.size();
+  if ((e = up(&dg.hub_row, p.hub_row)) != hipSuccess) return e;
+  if ((e = up(&dg.hub_nchunks, p.hub_nchunks)) != hipSuccess) return e;
+  if ((e = up(&dg.hub_first, p.hub_first)) != hipSuccess) return e;
+  dg.num_hubs = p.num_hubs;
+  dg.num_hub_chunks = 0;
+  for (int32_t c : p.hub_nchunks) dg.num_hub_chunks += c;
+  return hipStreamSynchronize(stream);
+}
+
+// The out-edge CSR: edges (source = col[k], target = row of k) of the in-edge
+// CSR, radix-sorted by (source, target); rows are the sources' new ids.
+gops_status build_forward(gops_context *ctx, gops_graph *g, std::string &err) {
+  if (g->has_fwd) return GOPS_OK;
+  const DevGraph &dg = g->dev;
+  const uint64_t n = dg.n, m = dg.m;
+  hipStream_t s = ctx->stream;
+  Pool pool;
+  hipError_t herr = hipSuccess;
+  const char *what = "";
+#define DTRY(expr)            \
+  do {                        \
+    herr = (expr);            \
+    if (herr != hipSuccess) { \
+      what = #expr;           \
+      goto hip_error;         \
+    }                         \
+  } while (0)
+  {
+    const int cb = bits_for(n ? n - 1 : 0);
+    uint32_t *d_marks = pool.get<uint32_t>(m), *d_row = pool.get<uint32_t>(m);
+    uint64_t *d_key = pool.get<uint64_t>(m), *d_keyb = pool.get<uint64_t>(m);
+    uint32_t *d_beg = pool.get<uint32_t>(n), *d_end = pool.get<uint32_t>(n);
+    int32_t *d_cnt = pool.get<int32_t>(n + 1);
+    int32_t *d_row_ptr = pool.get<int32_t>(n + 1), *d_col = pool.get<int32_t>(m);
+    DTRY(pool.err);
+    size_t tmp_bytes = 1, b = 0;
+    DTRY(hipcub::DeviceScan::InclusiveScan(nullptr, b, d_marks, d_row, hipcub::Max(), (int)m, s));
+    tmp_bytes = std::max(tmp_bytes, b);
+    DTRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b, d_key, d_keyb, (int)m, 0, 2 * cb, s));
+    tmp_bytes = std::max(tmp_bytes, b);
+    DTRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b, d_cnt, d_row_ptr, (int)n + 1, s));
+    tmp_bytes = std::max(tmp_bytes, b);
+    void *d_tmp = pool.get<uint8_t>(tmp_bytes);
+    DTRY(pool.err);
+    if (n) {
+      DTRY(hipMemsetAsync(d_beg, 0, n * sizeof(uint32_t), s));
+      DTRY(hipMemsetAsync(d_end, 0, n * sizeof(uint32_t), s));
+    }
+    if (m) {
+      DTRY(hipMemsetAsync(d_marks, 0, m * sizeof(uint32_t), s));
+      mark_rows_kernel<<<grid_for(n), kB, 0, s>>>(dg.row_ptr, n, d_marks);
+      b = tmp_bytes;
+      DTRY(hipcub::DeviceScan::InclusiveScan(d_tmp, b, d_marks, d_row, hipcub::Max(), (int)m, s));
+      fwd_keys_kernel<<<grid_for(m), kB, 0, s>>>(dg.col, d_row, m, cb, d_key);
+      b = tmp_bytes;
+      DTRY(hipcub::DeviceRadixSort::SortKeys(d_tmp, b, d_key, d_keyb, (int)m, 0, 2 * cb, s));
+      fwd_split_kernel<<<grid_for(m), kB, 0, s>>>(d_keyb, m, cb, d_col, d_marks);
+      runs_kernel<<<grid_for(m), kB, 0, s>>>(d_marks, m, d_beg, d_end);
+    }
+    run_counts_kernel<<<grid_for(n), kB, 0, s>>>(d_beg, d_end, n, d_cnt);
+    DTRY(hipGetLastError());
+    DTRY(hipMemsetAsync(d_cnt + n, 0, sizeof(int32_t), s));
+    b = tmp_bytes;
+    DTRY(hipcub::DeviceScan::ExclusiveSum(d_tmp, b, d_cnt, d_row_ptr, (int)n + 1, s));
+    HostCSRT h;
+    h.n = n;
+    h.m = m;
+    h.row_ptr.resize(n + 1);
+    DTRY(hipMemcpyAsync(h.row_ptr.data(), d_row_ptr, (n + 1) * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, s));
+    DTRY(hipStreamSynchronize(s));
+    Plan plan;
+    plan_blocks(h, plan);
+    DevGraph f;
+    f.n = n;
+    f.m = m;
+    f.row_ptr = d_row_ptr;
+    f.col = d_col;
+    pool.release(d_row_ptr);
+    pool.release(d_col);
+    herr = upload_plan_to(s, f, plan);
+    if (herr != hipSuccess) {
+      for (void *p : {(void *)f.row_ptr, (void *)f.col, (void *)f.desc, (void *)f.hub_row,
+                      (void *)f.hub_nchunks, (void *)f.hub_first})
+        (void)hipFree(p);
+      what = "upload of the out-edge plan";
+      goto hip_error;
+    }
+    g->fwd = f;
+    g->has_fwd = true;
+    return GOPS_OK;
+  }
+hip_error:
+#undef DTRY
+  (void)hipStreamSynchronize(s);
+  err = std::string("out-edge CSR build: ") + what + ": " + hipGetErrorString(herr);
   return (herr == hipErrorOutOfMemory || herr == hipErrorMemoryAllocation) ? GOPS_OUT_OF_MEMORY
                                                                            : GOPS_DEVICE_ERROR;
 }

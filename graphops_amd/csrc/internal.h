@@ -128,6 +128,32 @@ struct RunBuffers {
   size_t partial_slots = 0;
 };
 
+// State of a Katz / eigenvector / HITS power iteration in device memory.
+struct PowerState {
+  double alpha, beta;   // Katz
+  double tolerance;
+  double norm, sign;    // normalisation of the last sweep's vector (eigenvector, HITS)
+  double diff;          // residual of the last iteration (HITS: the hub change)
+  double diff2;         // HITS: the authority change of the last iteration
+  uint64_t iter, max_iter;
+  int32_t done, converged;
+  int32_t par;          // Katz: which prescaled buffer the next sweep gathers
+  int32_t pad;
+  // host-mapped progress word: (iterations << 2) | (converged << 1) | done
+  uint64_t *mirror;
+};
+
+// Per-graph buffers of the power iterations (allocated on first use).
+struct PowerBuffers {
+  uint64_t n = 0;
+  double *v[4] = {nullptr, nullptr, nullptr, nullptr};  // iterates, prescaled copies, row sums;
+                                                        // v[3]: results in caller ids
+  double *partials = nullptr;
+  double *hub_partial = nullptr;
+  size_t hub_cap = 0;
+  PowerState *state = nullptr;
+};
+
 constexpr uint64_t kSweepLogCap = 1 << 16;
 constexpr size_t kMaxReduce = 8;  // partial sums per workgroup, at most
 constexpr int kMaxSweepBlocks = 2048;  // grid of the sweep kernel (grid-stride over descriptors)
@@ -142,7 +168,10 @@ struct gops_graph {
   gops_context *ctx = nullptr;
   gops::PprGraphCache *ppr_cache = nullptr;  // batched PPR plans (server.cpp)
   gops::DevGraph dev;
+  gops::DevGraph fwd;      // out-edge CSR in new ids (row_ptr, col, plan only), built on demand
+  bool has_fwd = false;
   gops::RunBuffers run;
+  gops::PowerBuffers pow;
   bool weighted = false;   // created with edge weights
   int first_bad_weight = 0;
   std::vector<uint32_t> order_host;    // new -> original id
@@ -193,6 +222,21 @@ uint16_t f32_to_bf16(float f);
 gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, const uint32_t *nb,
                             const double *ew, int weight_storage, gops_graph *g, Plan &plan,
                             std::string &err);
+
+// graph_build.hip: the out-edge CSR (rows = sources, new ids, targets
+// ascending) from the in-edge CSR, with its block plan, into g->fwd
+gops_status build_forward(gops_context *ctx, gops_graph *g, std::string &err);
+// the block plan's arrays into dg (desc, hubs), synchronised on stream
+hipError_t upload_plan_to(hipStream_t stream, DevGraph &dg, const Plan &p);
+
+// centrality.hip: launches of one Katz / eigenvector / HITS iteration (all
+// kernels return at once once the state says done), initial vectors, and
+// the permutation back to caller ids
+enum PowerAlgo { kKatz = 0, kEigen = 1, kHits = 2 };
+hipError_t launch_power_init(gops_context *ctx, gops_graph *g, PowerAlgo algo, double value,
+                             double prescaled);
+hipError_t launch_power_iteration(gops_context *ctx, gops_graph *g, PowerAlgo algo);
+hipError_t launch_permute(gops_context *ctx, gops_graph *g, const double *src, double *out);
 
 // api.cpp
 enum RunKind { kPlain = 0, kWeighted = 1, kPersonalized = 2 };

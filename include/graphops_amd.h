@@ -194,6 +194,42 @@ gops_status gops_personalized_pagerank_checked_run(gops_context *ctx, gops_graph
                                                    uint64_t len, double *scores,
                                                    gops_run_info *info);
 
+/* ---- Katz, eigenvector and HITS centrality ----------------------------- */
+/* KatzConfig, katz.rs:25-53 (defaults alpha 0.1, beta 1.0, 100, 1e-6). */
+typedef struct gops_katz_config {
+    double alpha;
+    double beta;
+    uint64_t max_iterations;
+    double tolerance;
+} gops_katz_config;
+/* katz_centrality_run, katz.rs:136-201 (katz_centrality, katz.rs:131-133, is
+ * this call with the info ignored): x <- alpha A^T x + beta from x = beta,
+ * stop on sum |x - x'| < tolerance.  KatzRun's fields are gops_run_info's.
+ * The graph's edge weights, if any, are not used. */
+gops_status gops_katz_centrality_run(gops_context *ctx, gops_graph *g,
+                                     const gops_katz_config *cfg, double *scores,
+                                     gops_run_info *info);
+/* katz_centrality_checked, katz.rs:207-210: KatzConfig::validate (katz.rs:57-84,
+ * messages verbatim) first. */
+gops_status gops_katz_centrality_checked_run(gops_context *ctx, gops_graph *g,
+                                             const gops_katz_config *cfg, double *scores,
+                                             gops_run_info *info);
+/* eigenvector_centrality_run, eigenvector.rs:49-125 (eigenvector_centrality,
+ * eigenvector.rs:26-28, is this call with 100, 1e-6): x <- (A + I) x,
+ * L2-normalised with a non-negative sum, from x = 1/sqrt(n).  (A x)[v] sums
+ * over v's own neighbour list; the out-edge CSR it needs is built on the
+ * device at the first call and kept with the graph. */
+gops_status gops_eigenvector_centrality_run(gops_context *ctx, gops_graph *g,
+                                            uint64_t max_iterations, double tolerance,
+                                            double *scores, gops_run_info *info);
+/* hits, centrality.rs:114-166: hub and authority scores (each nullable),
+ * L2-normalised after every half step, from 1/sqrt(n); stops when both L2
+ * changes are below tol.  The reference returns only the two vectors; info
+ * (nullable) adds the iteration count, the last hub change (diff_l1) and
+ * whether both changes were below tol. */
+gops_status gops_hits(gops_context *ctx, gops_graph *g, uint64_t max_iter, double tol,
+                      double *hub, double *authority, gops_run_info *info);
+
 /* Batched personalized PageRank: num_queries independent calls of
  * personalized_pagerank_run (ppr.rs:90-156) in one device run over the shared
  * graph.  Query q's personalization is sparse: seeds[seed_offsets[q] ..

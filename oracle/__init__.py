@@ -58,6 +58,13 @@ def lib():
                                      ctypes.c_double, _f64p, _f64p]
         L.orc_push_sweep.restype = ctypes.c_double
         L.orc_transpose.argtypes = [ctypes.c_uint64, _u64p, _u32p, _f64p, _u64p, _u32p, _f64p]
+        L.orc_validate_katz.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                        ctypes.c_double, ctypes.c_char_p, ctypes.c_size_t]
+        L.orc_katz_run.argtypes = [ctypes.c_uint64, _u64p, _u32p, ctypes.c_double] + run_tail
+        L.orc_eigenvector_run.argtypes = [ctypes.c_uint64, _u64p, _u32p, ctypes.c_uint64,
+                                          ctypes.c_double, _f64p, _u64p, _f64p, _i32p]
+        L.orc_hits.argtypes = [ctypes.c_uint64, _u64p, _u32p, ctypes.c_uint64, ctypes.c_double,
+                               _f64p, _f64p, _u64p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -225,3 +232,60 @@ def push_sweep(off, nb, cur, damping=0.85, ew=None, ws=None):
     r = lib().orc_push_sweep(int(weighted), n, _p(off, _u64p), _p(nb, _u32p), _p(ewc, _f64p),
                              _p(wsc, _f64p), damping, _p(cur, _f64p), _p(out, _f64p))
     return out, float(r)
+
+
+def validate_katz(alpha: float, beta: float, max_iterations: int, tolerance: float) -> None:
+    """KatzConfig::validate, katz.rs:57-84."""
+    _msg_call(lib().orc_validate_katz, float(alpha), float(beta), int(max_iterations),
+              float(tolerance))
+
+
+def katz_run(off, nb, alpha=0.1, beta=1.0, max_iterations=100, tolerance=1e-6) -> Run:
+    """katz_centrality_run, katz.rs:136-201 (KatzRun has PageRankRun's fields)."""
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    nb = np.ascontiguousarray(nb, dtype=np.uint32)
+    n = off.size - 1
+    s, it, d, c = _tail(n)
+    lib().orc_katz_run(n, _p(off, _u64p), _p(nb, _u32p), float(alpha), float(beta),
+                       int(max_iterations), float(tolerance), _p(s, _f64p), _p(it, _u64p),
+                       _p(d, _f64p), _p(c, _i32p))
+    return _run(s, it, d, c, n)
+
+
+def eigenvector_run(off, nb, max_iterations=100, tolerance=1e-6) -> Run:
+    """eigenvector_centrality_run, eigenvector.rs:49-125."""
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    nb = np.ascontiguousarray(nb, dtype=np.uint32)
+    n = off.size - 1
+    s, it, d, c = _tail(n)
+    lib().orc_eigenvector_run(n, _p(off, _u64p), _p(nb, _u32p), int(max_iterations),
+                              float(tolerance), _p(s, _f64p), _p(it, _u64p), _p(d, _f64p),
+                              _p(c, _i32p))
+    return _run(s, it, d, c, n)
+
+
+@dataclass
+class HitsRun:
+    """hits, centrality.rs:114-166: (hub, authority), plus the loop count and
+    the last L2 changes, which the reference does not return."""
+
+    hub: np.ndarray
+    auth: np.ndarray
+    iterations: int
+    auth_diff: float
+    hub_diff: float
+
+
+def hits(off, nb, max_iter=100, tol=1e-6) -> HitsRun:
+    """hits, centrality.rs:114-166."""
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    nb = np.ascontiguousarray(nb, dtype=np.uint32)
+    n = off.size - 1
+    hub = np.zeros(max(n, 1))
+    auth = np.zeros(max(n, 1))
+    it = np.zeros(1, dtype=np.uint64)
+    ad = np.zeros(1)
+    hd = np.zeros(1)
+    lib().orc_hits(n, _p(off, _u64p), _p(nb, _u32p), int(max_iter), float(tol), _p(hub, _f64p),
+                   _p(auth, _f64p), _p(it, _u64p), _p(ad, _f64p), _p(hd, _f64p))
+    return HitsRun(hub[:n].copy(), auth[:n].copy(), int(it[0]), float(ad[0]), float(hd[0]))

@@ -18,11 +18,21 @@
  *   stop      : diff < tol (strict), checked after the swap  pagerank.rs:174-179
  * Compile with -O2 -ffp-contract=off (no FMA contraction, no fast-math).
  *
+ * Katz, eigenvector and HITS (katz.rs, eigenvector.rs, centrality.rs) are
+ * restated in the same way: the reference's loops and operation order,
+ * sequential sums in index order.
+ *
  * Parity status (pins live in tests/test_oracle.py):
  *   orc_pagerank_run, orc_pagerank_weighted_run, orc_ppr_run: pinned by exact
  *     rational fixed points, exact-rational iteration trajectories on small
  *     graphs, dense linear solves and networkx fixed points.
  *   orc_*_pull_run: pinned bit-for-bit against the push forms above.
+ *   orc_katz_run: pinned by exact-rational trajectories, the linear solve
+ *     beta (I - alpha A^T)^-1 1 and the reference's unit tests (katz.rs:231-329).
+ *   orc_eigenvector_run, orc_hits: pinned by 60-digit decimal trajectories of
+ *     the same iterations, the dominant eigenvector of A + I (numpy) and the
+ *     top singular vectors of A (numpy SVD), and the reference's unit tests
+ *     (eigenvector.rs:145-197, centrality.rs:244-273).
  *   The reference's optional `simd` L1 residual (third-party innr 0.4,
  *     pagerank.rs:166-167) is NOT restated: parity unpinned for that feature;
  *     the default scalar residual is what is restated here.
@@ -475,4 +485,196 @@ void orc_transpose(uint64_t n, const uint64_t *off, const uint32_t *nb, const do
         }
     }
     free(cur);
+}
+
+/* KatzConfig::validate, katz.rs:57-84 (messages verbatim). */
+int orc_validate_katz(double alpha, double beta, uint64_t max_iterations, double tolerance,
+                      char *msg, size_t cap) {
+    if (!isfinite(alpha) || alpha <= 0.0) {
+        put_msg(msg, cap, "alpha must be finite and > 0");
+        return ORC_INVALID;
+    }
+    if (alpha >= 1.0) {
+        put_msg(msg, cap,
+                "alpha must be < 1 (required for convergence; tighter bound "
+                "alpha < 1/spectral_radius(A) may be needed for large graphs)");
+        return ORC_INVALID;
+    }
+    if (!isfinite(beta)) {
+        put_msg(msg, cap, "beta must be finite");
+        return ORC_INVALID;
+    }
+    if (max_iterations == 0) {
+        put_msg(msg, cap, "max_iterations must be > 0");
+        return ORC_INVALID;
+    }
+    if (!isfinite(tolerance) || tolerance <= 0.0) {
+        put_msg(msg, cap, "tolerance must be finite and > 0");
+        return ORC_INVALID;
+    }
+    return ORC_OK;
+}
+
+/* katz_centrality_run, katz.rs:136-201 (push form, neighbour lists in
+ * adapter order; ids >= n are skipped, katz.rs:170). */
+void orc_katz_run(uint64_t n, const uint64_t *off, const uint32_t *nb, double alpha,
+                  double beta, uint64_t max_iterations, double tolerance, double *scores,
+                  uint64_t *iters_out, double *diff_out, int *conv_out) {
+    if (n == 0) { /* :138-145 */
+        run_empty(diff_out, iters_out, conv_out);
+        return;
+    }
+    double *cur = scores;
+    double *nxt = (double *)malloc(n * sizeof(double));
+    for (uint64_t i = 0; i < n; i++) cur[i] = beta; /* :149 */
+    uint64_t iters = 0;
+    double last_diff = INFINITY;
+    int converged = 0;
+    for (uint64_t it = 0; it < max_iterations; it++) { /* :160 */
+        iters++;
+        for (uint64_t i = 0; i < n; i++) nxt[i] = beta; /* :166 */
+        for (uint64_t u = 0; u < n; u++) {              /* :167-174 */
+            const double contrib = alpha * cur[u];
+            for (uint64_t j = off[u]; j < off[u + 1]; j++)
+                if (nb[j] < n) nxt[nb[j]] += contrib;
+        }
+        const double diff = l1_seq(n, cur, nxt); /* :181-185 */
+        last_diff = diff;
+        double *t = cur; cur = nxt; nxt = t;      /* :188 */
+        if (diff < tolerance) {                   /* :189-192 */
+            converged = 1;
+            break;
+        }
+    }
+    if (cur != scores) {
+        memcpy(scores, cur, n * sizeof(double));
+        free(cur);
+    } else {
+        free(nxt);
+    }
+    *iters_out = iters;
+    *diff_out = last_diff;
+    *conv_out = converged;
+}
+
+/* eigenvector_centrality_run, eigenvector.rs:49-125: x <- (A + I) x with
+ * (A x)[v] summed over v's own neighbour list, L2-normalised with the sign
+ * that makes the sum non-negative. */
+void orc_eigenvector_run(uint64_t n, const uint64_t *off, const uint32_t *nb,
+                         uint64_t max_iterations, double tolerance, double *scores,
+                         uint64_t *iters_out, double *diff_out, int *conv_out) {
+    if (n == 0) { /* :55-62 */
+        run_empty(diff_out, iters_out, conv_out);
+        return;
+    }
+    double *cur = scores;
+    double *nxt = (double *)malloc(n * sizeof(double));
+    const double init = 1.0 / sqrt((double)n); /* :65 */
+    for (uint64_t i = 0; i < n; i++) cur[i] = init;
+    double diff = 0.0; /* :68 */
+    uint64_t iters = 0;
+    int converged = 0;
+    for (uint64_t it = 0; it < max_iterations; it++) { /* :71 */
+        iters = it + 1;
+        for (uint64_t v = 0; v < n; v++) { /* :76-84 */
+            double sum = cur[v];
+            for (uint64_t j = off[v]; j < off[v + 1]; j++)
+                if (nb[j] < n) sum += cur[nb[j]];
+            nxt[v] = sum;
+        }
+        double sq = 0.0; /* :87 */
+        for (uint64_t i = 0; i < n; i++) sq += nxt[i] * nxt[i];
+        const double norm = sqrt(sq);
+        if (norm > 0.0) { /* :88-98 */
+            double tot = 0.0;
+            for (uint64_t i = 0; i < n; i++) tot += nxt[i];
+            const double sign = tot >= 0.0 ? 1.0 : -1.0;
+            for (uint64_t i = 0; i < n; i++) nxt[i] = nxt[i] * sign / norm;
+        }
+        diff = l1_seq(n, cur, nxt); /* :101-105 */
+        double *t = cur; cur = nxt; nxt = t; /* :107 */
+        if (diff < tolerance) {           /* :109-116 */
+            converged = 1;
+            break;
+        }
+    }
+    if (cur != scores) {
+        memcpy(scores, cur, n * sizeof(double));
+        free(cur);
+    } else {
+        free(nxt);
+    }
+    *iters_out = iters;
+    *diff_out = diff;
+    *conv_out = converged;
+}
+
+static void l2_normalize_seq(uint64_t n, double *v) { /* centrality.rs:168-175 */
+    double sq = 0.0;
+    for (uint64_t i = 0; i < n; i++) sq += v[i] * v[i];
+    const double norm = sqrt(sq);
+    if (norm > 0.0)
+        for (uint64_t i = 0; i < n; i++) v[i] /= norm;
+}
+
+static double l2_diff_seq(uint64_t n, const double *a, const double *b) { /* :177-183 */
+    double sq = 0.0;
+    for (uint64_t i = 0; i < n; i++) sq += (a[i] - b[i]) * (a[i] - b[i]);
+    return sqrt(sq);
+}
+
+/* hits, centrality.rs:114-166.  The reference returns (hub, authority);
+ * iters_out, auth_diff and hub_diff report the loop count and the last L2
+ * changes (0 iterations: diffs 0). */
+void orc_hits(uint64_t n, const uint64_t *off, const uint32_t *nb, uint64_t max_iter,
+              double tol, double *hub, double *auth, uint64_t *iters_out,
+              double *auth_diff_out, double *hub_diff_out) {
+    *iters_out = 0;
+    *auth_diff_out = 0.0;
+    *hub_diff_out = 0.0;
+    if (n == 0) return; /* :116-118 */
+    /* in-neighbour lists, u ascending (:121-128) */
+    uint64_t *in_off = (uint64_t *)calloc(n + 1, sizeof(uint64_t));
+    const uint64_t m = off[n];
+    for (uint64_t j = 0; j < m; j++)
+        if (nb[j] < n) in_off[nb[j] + 1]++;
+    for (uint64_t v = 0; v < n; v++) in_off[v + 1] += in_off[v];
+    uint32_t *in_src = (uint32_t *)malloc((in_off[n] ? in_off[n] : 1) * sizeof(uint32_t));
+    uint64_t *cursor = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    memcpy(cursor, in_off, n * sizeof(uint64_t));
+    for (uint64_t u = 0; u < n; u++)
+        for (uint64_t j = off[u]; j < off[u + 1]; j++)
+            if (nb[j] < n) in_src[cursor[nb[j]]++] = (uint32_t)u;
+    free(cursor);
+    const double init = 1.0 / sqrt((double)n); /* :130-132 */
+    for (uint64_t i = 0; i < n; i++) hub[i] = auth[i] = init;
+    double *new_auth = (double *)malloc(n * sizeof(double));
+    double *new_hub = (double *)malloc(n * sizeof(double));
+    for (uint64_t it = 0; it < max_iter; it++) { /* :134 */
+        *iters_out = it + 1;
+        for (uint64_t v = 0; v < n; v++) { /* :136-141 */
+            double s = 0.0;
+            for (uint64_t k = in_off[v]; k < in_off[v + 1]; k++) s += hub[in_src[k]];
+            new_auth[v] = s;
+        }
+        l2_normalize_seq(n, new_auth); /* :142 */
+        for (uint64_t u = 0; u < n; u++) { /* :145-152 */
+            double s = 0.0;
+            for (uint64_t j = off[u]; j < off[u + 1]; j++)
+                if (nb[j] < n) s += new_auth[nb[j]];
+            new_hub[u] = s;
+        }
+        l2_normalize_seq(n, new_hub); /* :153 */
+        const double ad = l2_diff_seq(n, auth, new_auth); /* :156-157 */
+        const double hd = l2_diff_seq(n, hub, new_hub);
+        memcpy(auth, new_auth, n * sizeof(double)); /* :158-159 */
+        memcpy(hub, new_hub, n * sizeof(double));
+        *auth_diff_out = ad;
+        *hub_diff_out = hd;
+        if (ad < tol && hd < tol) break; /* :160-162 */
+    }
+    free(new_auth);
+    free(new_hub);
+    free(in_off);
+    free(in_src);
 }
