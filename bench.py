@@ -16,6 +16,9 @@ sweeps, and two more legs on an R-MAT scale-22 graph: batched personalized
 PageRank (1024 queries) against the plain one-query-per-run path, and a
 serving run with Poisson arrivals.
 
+Further legs on the same resident graph: its device and host builds, and
+Katz / eigenvector / HITS centrality (f64 power iterations).
+
 --copy-probe BYTES runs a plain device copy of BYTES instead (the
 calibration of the FETCH_SIZE / WRITE_SIZE counters).
 """
@@ -225,6 +228,64 @@ def build_leg(ga, e, storage):
     return out
 
 
+def centrality_leg(ga, e, dg, iters=20):
+    """Katz, eigenvector and HITS on the resident graph (edge weights unused):
+    iters iterations each at tolerance 0 (no early stop), results left on the
+    device; the oracle's single-core Katz push sweep as the CPU baseline."""
+    import ctypes
+
+    import oracle
+    from graphops_amd import _lib as L
+
+    n, m = e.n, e.m
+    lib, c, h = L.lib(), dg.ctx.handle, dg.handle
+    info = L.RunInfoC()
+    alpha = 0.85 / int(np.bincount(e.targets, minlength=n).max())
+
+    def katz(k):
+        cfg = L.KatzConfigC(alpha, 1.0, k, 0.0)
+        rc = lib.gops_katz_centrality_run(c, h, ctypes.byref(cfg), None, ctypes.byref(info))
+        assert rc == 0 and info.iterations == k
+
+    def eig(k):
+        rc = lib.gops_eigenvector_centrality_run(c, h, k, 0.0, None, ctypes.byref(info))
+        assert rc == 0 and info.iterations == k
+
+    def hits(k):
+        rc = lib.gops_hits(c, h, k, 0.0, None, None, ctypes.byref(info))
+        assert rc == 0 and info.iterations == k
+
+    def timed(fn):
+        fn(2)
+        t0 = time.perf_counter()
+        fn(iters)
+        return (time.perf_counter() - t0) / iters
+
+    out = {"iterations": iters, "katz_alpha": alpha}
+    t0 = time.perf_counter()
+    eig(1)  # builds the out-edge CSR once
+    out["out_edge_csr_build_s"] = round(time.perf_counter() - t0, 4)
+    # algorithmic bytes per iteration: 4 B of col per edge; per node the row
+    # pointer, the gathered vector once and the vectors the epilogues and the
+    # normalisations read and write
+    for name, fn, sweeps, alg in (("katz", katz, 1, 4 * m + 36 * n),
+                                  ("eigenvector", eig, 1, 4 * m + 52 * n),
+                                  ("hits", hits, 2, 8 * m + 88 * n)):
+        dt = timed(fn)
+        out[name] = {"ms_per_iteration": round(dt * 1e3, 3),
+                     "gteps": round(sweeps * m / dt / 1e9, 2),
+                     "alg_bytes_per_iteration": alg,
+                     "hbm_frac": round(alg / dt / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"{name}: {dt * 1e3:.3f} ms per iteration, {sweeps * m / dt / 1e9:.1f} GTEPS")
+    t0 = time.perf_counter()
+    oracle.katz_run(e.offsets, e.targets, alpha, 1.0, 1, 0.0)
+    cdt = time.perf_counter() - t0
+    out["cpu_baseline"] = {"value": round(m / cdt / 1e9, 4), "unit": "GTEPS", "cores": 1,
+                           "kind": "port", "sample": "one Katz push sweep of the oracle"}
+    log(f"oracle Katz sweep on one core: {cdt:.2f} s")
+    return out
+
+
 def ppr_legs(args, ga, generators):
     t0 = time.perf_counter()
     e = generators.rmat(PPR_GRAPH["scale"], 16, 0.57, 0.19, 0.19, PPR_GRAPH["seed"])
@@ -276,6 +337,9 @@ def main():
     ap.add_argument("--ppr-only", action="store_true", help="only the PPR and serving legs")
     ap.add_argument("--no-build", action="store_true", help="skip the graph build timing leg")
     ap.add_argument("--build-only", action="store_true", help="only the graph build timing leg")
+    ap.add_argument("--no-centrality", action="store_true", help="skip the Katz/eigenvector/HITS leg")
+    ap.add_argument("--centrality-only", action="store_true",
+                    help="only the Katz/eigenvector/HITS leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -300,6 +364,12 @@ def main():
     if args.build_only:
         print(json.dumps({"workload": c["workload"], "n": e.n, "m": e.m,
                           "graph_build": build_leg(ga, e, c["storage"])}), flush=True)
+        return
+    if args.centrality_only:
+        dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"],
+                                       ctx=ga.Context(0, timing=False))
+        print(json.dumps({"workload": f"rmat{c['scale']} unweighted", "n": e.n, "m": e.m,
+                          "centrality": centrality_leg(ga, e, dg)}), flush=True)
         return
     ctx = ga.Context(0, precision="mixed", timing=True)
     t0 = time.perf_counter()
@@ -371,6 +441,8 @@ def main():
     out["cpu_baseline"] = cpu_baseline(e, c, args.cpu_sweeps) if args.cpu_sweeps > 0 else None
     if not args.no_build:
         out["graph_build"] = build_leg(ga, e, c["storage"])
+    if not args.no_centrality:
+        out["centrality"] = centrality_leg(ga, e, dg)
     if not args.no_ppr:
         dg.close()
         del e

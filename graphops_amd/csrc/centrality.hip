@@ -5,7 +5,8 @@
 //
 //   Katz         x <- beta + A^T (alpha x): one pull over the in-edge CSR
 //                gathering the prescaled alpha * x, residual sum |x - x'|.
-//   eigenvector  t = x + A x: a pull over the out-edge CSR gathering x;
+//   eigenvector  t = x + A x: a pull over the out-edge CSR gathering x
+//                (its rows ordered by out-degree, mapped to nodes);
 //                then x' = t * sign / |t|_2 (sign of sum t), residual
 //                sum |x - x'|.
 //   HITS         a = A^T h (in-edge CSR), a' = a / |a|_2; h = A a' (out-edge
@@ -36,6 +37,7 @@ struct PowerArgs {
   const int32_t *hub_nchunks;
   const int32_t *hub_first;
   int num_hubs;
+  const uint32_t *rows;  // out-edge CSR: the node of each row (null: row = node)
   const double *yin;  // the gathered vector (Katz: picked by PowerState::par)
   double *x;          // Katz: the iterate; eigenvector: the iterate (its own term)
   double *t;          // eigenvector, HITS: the row sums
@@ -73,8 +75,9 @@ struct KatzEpi {
 struct EigenEpi {
   const PowerArgs &a;
   __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kPowR]) const {
-    const double t = a.x[r] + acc;
-    a.t[r] = t;
+    const int v = a.rows ? (int)a.rows[r] : r;
+    const double t = a.x[v] + acc;
+    a.t[v] = t;
     red[0] += t * t;
     red[1] += t;
   }
@@ -84,7 +87,7 @@ struct EigenEpi {
 struct SumEpi {
   const PowerArgs &a;
   __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kPowR]) const {
-    a.t[r] = acc;
+    a.t[a.rows ? (int)a.rows[r] : r] = acc;
     red[0] += acc * acc;
   }
 };
@@ -227,7 +230,8 @@ static int elementwise_blocks(uint64_t n) {
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 2048));
 }
 
-static PowerArgs power_args(gops_graph *g, const DevGraph &dg) {
+// rows: the node of each row of dg (the out-edge CSR's order), or null
+static PowerArgs power_args(gops_graph *g, const DevGraph &dg, const uint32_t *rows) {
   PowerArgs a{};
   a.desc = dg.desc;
   a.num_desc = dg.num_desc;
@@ -238,6 +242,7 @@ static PowerArgs power_args(gops_graph *g, const DevGraph &dg) {
   a.hub_nchunks = dg.hub_nchunks;
   a.hub_first = dg.hub_first;
   a.num_hubs = dg.num_hubs;
+  a.rows = rows;
   a.partials = g->pow.partials;
   a.st = g->pow.state;
   return a;
@@ -276,23 +281,23 @@ hipError_t launch_power_init(gops_context *ctx, gops_graph *g, PowerAlgo algo, d
 hipError_t launch_power_iteration(gops_context *ctx, gops_graph *g, PowerAlgo algo) {
   PowerBuffers &pb = g->pow;
   if (algo == kKatz) {
-    PowerArgs a = power_args(g, g->dev);
+    PowerArgs a = power_args(g, g->dev, nullptr);
     a.x = pb.v[0];
     sweep<kKatz>(ctx, a, pb.v[1], pb.v[2]);
   } else if (algo == kEigen) {
-    PowerArgs a = power_args(g, g->fwd);
+    PowerArgs a = power_args(g, g->fwd, g->fwd.order);
     a.yin = pb.v[0];
     a.x = pb.v[0];
     a.t = pb.v[1];
     sweep<kEigen>(ctx, a, nullptr, nullptr);
     scale_step<kEigen, false>(ctx, g, pb.v[1], pb.v[0], 0);
   } else {
-    PowerArgs a = power_args(g, g->dev);  // authority: in-edges, gathers hub
+    PowerArgs a = power_args(g, g->dev, nullptr);  // authority: in-edges, gathers hub
     a.yin = pb.v[0];
     a.t = pb.v[1];
     sweep<kHits>(ctx, a, nullptr, nullptr);
     scale_step<kHits, true>(ctx, g, pb.v[1], pb.v[2], 0);
-    PowerArgs h = power_args(g, g->fwd);  // hub: out-edges, gathers the new authority
+    PowerArgs h = power_args(g, g->fwd, g->fwd.order);  // hub: out-edges, gathers the new authority
     h.yin = pb.v[2];
     h.t = pb.v[1];
     sweep<kHits>(ctx, h, nullptr, nullptr);

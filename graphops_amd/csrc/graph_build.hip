@@ -238,25 +238,23 @@ __global__ void mark_rows_kernel(const int32_t *row_ptr, uint64_t n, uint32_t *m
   }
 }
 
-// key = source << cb | target, from the in-edge CSR (k: edge, row[k]: target)
-__global__ void fwd_keys_kernel(const int32_t *col, const uint32_t *row, uint64_t m, int cb,
-                                uint64_t *key) {
-  GRID_STRIDE(k, m) { key[k] = ((uint64_t)(uint32_t)col[k] << cb) | row[k]; }
-}
-
-// sorted keys -> targets (col) and sources (for the runs)
-__global__ void fwd_split_kernel(const uint64_t *key, uint64_t m, int cb, int32_t *col,
-                                 uint32_t *src) {
-  GRID_STRIDE(k, m) {
-    const uint64_t x = key[k];
-    col[k] = (int32_t)(x & ((1ull << cb) - 1));
-    src[k] = (uint32_t)(x >> cb);
+// out-degree of each row's node from its 1/out-degree (RN(1/RN(1/d)) = d
+// for every d < 2^51); ascending key = descending out-degree; value = row
+__global__ void outdeg_keys_kernel(const double *inv_deg, uint64_t n, uint32_t *deg,
+                                   uint32_t *key, uint32_t *val) {
+  GRID_STRIDE(r, n) {
+    const double inv = inv_deg[r];
+    const uint32_t d = inv != 0.0 ? (uint32_t)rint(1.0 / inv) : 0u;
+    deg[r] = d;
+    key[r] = 0xFFFFFFFFu - d;
+    val[r] = (uint32_t)r;
   }
 }
 
-__global__ void run_counts_kernel(const uint32_t *beg, const uint32_t *end, uint64_t n,
-                                  int32_t *cnt) {
-  GRID_STRIDE(v, n) { cnt[v] = (int32_t)(end[v] - beg[v]); }
+// key = out-row of the source << cb | target (k: edge of the in-edge CSR)
+__global__ void fwd_keys_kernel(const int32_t *col, const uint32_t *row, const uint32_t *frank,
+                                uint64_t m, int cb, uint64_t *key) {
+  GRID_STRIDE(k, m) { key[k] = ((uint64_t)frank[col[k]] << cb) | row[k]; }
 }
 
 #undef GRID_STRIDE
@@ -548,8 +546,11 @@ hipError_t upload_plan_to(hipStream_t stream, DevGraph &dg, const Plan &p) {
   return hipStreamSynchronize(stream);
 }
 
-// The out-edge CSR: edges (source = col[k], target = row of k) of the in-edge
-// CSR, radix-sorted by (source, target); rows are the sources' new ids.
+// The out-edge CSR: its rows are the nodes (new ids) ordered by out-degree,
+// descending, ties by id (so that runs of rows have similar lengths, as in
+// the in-edge CSR), fwd.order maps a row to its node; each row lists the
+// node's targets (new ids) ascending.  The edges (source = col[k], target =
+// row of k) of the in-edge CSR are radix-sorted by (out-row, target).
 gops_status build_forward(gops_context *ctx, gops_graph *g, std::string &err) {
   if (g->has_fwd) return GOPS_OK;
   const DevGraph &dg = g->dev;
@@ -570,7 +571,9 @@ gops_status build_forward(gops_context *ctx, gops_graph *g, std::string &err) {
     const int cb = bits_for(n ? n - 1 : 0);
     uint32_t *d_marks = pool.get<uint32_t>(m), *d_row = pool.get<uint32_t>(m);
     uint64_t *d_key = pool.get<uint64_t>(m), *d_keyb = pool.get<uint64_t>(m);
-    uint32_t *d_beg = pool.get<uint32_t>(n), *d_end = pool.get<uint32_t>(n);
+    uint32_t *d_deg = pool.get<uint32_t>(n), *d_k32 = pool.get<uint32_t>(n);
+    uint32_t *d_k32b = pool.get<uint32_t>(n), *d_v32 = pool.get<uint32_t>(n);
+    uint32_t *d_forder = pool.get<uint32_t>(n), *d_frank = pool.get<uint32_t>(n);
     int32_t *d_cnt = pool.get<int32_t>(n + 1);
     int32_t *d_row_ptr = pool.get<int32_t>(n + 1), *d_col = pool.get<int32_t>(m);
     DTRY(pool.err);
@@ -579,30 +582,38 @@ gops_status build_forward(gops_context *ctx, gops_graph *g, std::string &err) {
     tmp_bytes = std::max(tmp_bytes, b);
     DTRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b, d_key, d_keyb, (int)m, 0, 2 * cb, s));
     tmp_bytes = std::max(tmp_bytes, b);
+    DTRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, d_k32, d_k32b, d_v32, d_forder, (int)n, 0,
+                                            32, s));
+    tmp_bytes = std::max(tmp_bytes, b);
     DTRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b, d_cnt, d_row_ptr, (int)n + 1, s));
     tmp_bytes = std::max(tmp_bytes, b);
     void *d_tmp = pool.get<uint8_t>(tmp_bytes);
     DTRY(pool.err);
+    // rows: nodes by descending out-degree
+    outdeg_keys_kernel<<<grid_for(n), kB, 0, s>>>(dg.inv_deg64, n, d_deg, d_k32, d_v32);
+    DTRY(hipGetLastError());
     if (n) {
-      DTRY(hipMemsetAsync(d_beg, 0, n * sizeof(uint32_t), s));
-      DTRY(hipMemsetAsync(d_end, 0, n * sizeof(uint32_t), s));
+      b = tmp_bytes;
+      DTRY(hipcub::DeviceRadixSort::SortPairs(d_tmp, b, d_k32, d_k32b, d_v32, d_forder, (int)n, 0,
+                                              32, s));
     }
+    DTRY(hipMemsetAsync(d_cnt + n, 0, sizeof(int32_t), s));
+    rank_kernel<<<grid_for(n), kB, 0, s>>>(d_forder, d_deg, n, d_frank, d_cnt);
+    DTRY(hipGetLastError());
+    b = tmp_bytes;
+    DTRY(hipcub::DeviceScan::ExclusiveSum(d_tmp, b, d_cnt, d_row_ptr, (int)n + 1, s));
+    // edges by (out-row, target)
     if (m) {
       DTRY(hipMemsetAsync(d_marks, 0, m * sizeof(uint32_t), s));
       mark_rows_kernel<<<grid_for(n), kB, 0, s>>>(dg.row_ptr, n, d_marks);
       b = tmp_bytes;
       DTRY(hipcub::DeviceScan::InclusiveScan(d_tmp, b, d_marks, d_row, hipcub::Max(), (int)m, s));
-      fwd_keys_kernel<<<grid_for(m), kB, 0, s>>>(dg.col, d_row, m, cb, d_key);
+      fwd_keys_kernel<<<grid_for(m), kB, 0, s>>>(dg.col, d_row, d_frank, m, cb, d_key);
       b = tmp_bytes;
       DTRY(hipcub::DeviceRadixSort::SortKeys(d_tmp, b, d_key, d_keyb, (int)m, 0, 2 * cb, s));
-      fwd_split_kernel<<<grid_for(m), kB, 0, s>>>(d_keyb, m, cb, d_col, d_marks);
-      runs_kernel<<<grid_for(m), kB, 0, s>>>(d_marks, m, d_beg, d_end);
+      col_kernel<<<grid_for(m), kB, 0, s>>>(d_keyb, m, (1ull << cb) - 1, d_col);
+      DTRY(hipGetLastError());
     }
-    run_counts_kernel<<<grid_for(n), kB, 0, s>>>(d_beg, d_end, n, d_cnt);
-    DTRY(hipGetLastError());
-    DTRY(hipMemsetAsync(d_cnt + n, 0, sizeof(int32_t), s));
-    b = tmp_bytes;
-    DTRY(hipcub::DeviceScan::ExclusiveSum(d_tmp, b, d_cnt, d_row_ptr, (int)n + 1, s));
     HostCSRT h;
     h.n = n;
     h.m = m;
@@ -617,12 +628,14 @@ gops_status build_forward(gops_context *ctx, gops_graph *g, std::string &err) {
     f.m = m;
     f.row_ptr = d_row_ptr;
     f.col = d_col;
+    f.order = d_forder;
     pool.release(d_row_ptr);
     pool.release(d_col);
+    pool.release(d_forder);
     herr = upload_plan_to(s, f, plan);
     if (herr != hipSuccess) {
-      for (void *p : {(void *)f.row_ptr, (void *)f.col, (void *)f.desc, (void *)f.hub_row,
-                      (void *)f.hub_nchunks, (void *)f.hub_first})
+      for (void *p : {(void *)f.row_ptr, (void *)f.col, (void *)f.order, (void *)f.desc,
+                      (void *)f.hub_row, (void *)f.hub_nchunks, (void *)f.hub_first})
         (void)hipFree(p);
       what = "upload of the out-edge plan";
       goto hip_error;
