@@ -33,7 +33,7 @@ __all__ = [
     "pagerank_weighted", "pagerank_weighted_run", "pagerank_weighted_checked",
     "pagerank_weighted_checked_run", "personalized_pagerank", "personalized_pagerank_run",
     "personalized_pagerank_checked", "personalized_pagerank_checked_run", "ppr_batch_run",
-    "top_k", "normalize", "Server", "KatzConfig", "KatzRun", "katz_centrality",
+    "top_k", "result_top_k", "normalize", "Server", "KatzConfig", "KatzRun", "katz_centrality",
     "katz_centrality_run", "katz_centrality_checked", "EigenvectorRun",
     "eigenvector_centrality", "eigenvector_centrality_run", "hits", "hits_run",
 ]
@@ -668,15 +668,35 @@ def hits(graph, max_iter: int, tol: float, ctx: Context = None):
     return hub, auth
 
 
-def top_k(scores, k):
-    """top_k, topk.rs:17-42: finite, > 0 scores, descending (ties in
-    unspecified order, as the reference's unstable sort)."""
-    s = np.asarray(scores, dtype=np.float64)
+def top_k(scores, k, ctx: Context = None):
+    """top_k, topk.rs:17-42, on the device (gops_top_k): the k largest
+    finite, positive scores as (node, score), descending; at a tie on the
+    k-th score the entries the reference's heap keeps, equal scores by
+    ascending node."""
+    ctx = ctx or default_context()
+    s = np.ascontiguousarray(scores, dtype=np.float64)
+    k = int(k)
     if k == 0 or s.size == 0:
         return []
-    idx = np.nonzero(np.isfinite(s) & (s > 0.0))[0]
-    order = idx[np.argsort(-s[idx], kind="stable")][:k]
-    return [(int(i), float(s[i])) for i in order]
+    ids = np.zeros(k, dtype=np.uint32)
+    vals = np.zeros(k, dtype=np.float64)
+    cnt = ctypes.c_uint32()
+    _check(L.lib().gops_top_k(ctx.handle, L.ptr(s, L._f64p), s.size, k, L.ptr(ids, L._u32p),
+                              L.ptr(vals, L._f64p), ctypes.byref(cnt)), ctx.handle, "gops_top_k")
+    return [(int(i), float(v)) for i, v in zip(ids[:cnt.value], vals[:cnt.value])]
+
+
+def result_top_k(graph: "DeviceGraph", k):
+    """top_k of the last PageRank-family run on graph, selected where the
+    scores stay (gops_result_top_k): only the k pairs cross PCIe."""
+    k = int(k)
+    ids = np.zeros(max(k, 1), dtype=np.uint32)
+    vals = np.zeros(max(k, 1), dtype=np.float64)
+    cnt = ctypes.c_uint32()
+    c = graph.ctx.handle
+    _check(L.lib().gops_result_top_k(c, graph.handle, k, L.ptr(ids, L._u32p), L.ptr(vals, L._f64p),
+                                     ctypes.byref(cnt)), c, "gops_result_top_k")
+    return [(int(i), float(v)) for i, v in zip(ids[:cnt.value], vals[:cnt.value])]
 
 
 def normalize(scores):
@@ -722,9 +742,12 @@ class Server:
                "gops_server_step")
         return int(act.value)
 
-    def poll(self, with_scores=True):
-        """One finished query or None: (ticket, PageRunRun, submit_ms,
-        finish_ms, topk) -- topk is a list of (node, score) if enabled."""
+    def poll(self, with_scores=None):
+        """One finished query or None: (ticket, PageRankRun, submit_ms,
+        finish_ms, topk) -- topk is a list of (node, score) if enabled; a
+        top-k server returns no score vector (PageRankRun.scores is None)."""
+        if with_scores is None:
+            with_scores = not self.topk
         ready = ctypes.c_int()
         res = L.ServerResultC()
         scores = np.zeros(self.graph.n, dtype=np.float64) if with_scores else None

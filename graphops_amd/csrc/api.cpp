@@ -342,6 +342,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
     if (done) break;
   }
   HIP_TRY(ctx, launch_finalize(ctx, g, ctx->normalize));
+  rb.has_result = true;
   HIP_TRY(ctx, hipEventRecord(ev_end, ctx->stream));
   HIP_TRY(ctx, hipMemcpyAsync(&ctx->host_state[0], rb.state, sizeof(RunState),
                               hipMemcpyDeviceToHost, ctx->stream));
@@ -424,6 +425,8 @@ void gops_context_destroy(gops_context *ctx) {
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+  free_topk(ctx->topk);
+  hipFree(ctx->staging);
   hipHostFree(ctx->host_state);
   hipHostFree(ctx->mirror_host);
   hipStreamDestroy(ctx->stream);
@@ -674,6 +677,40 @@ gops_status gops_personalized_pagerank_checked_run(gops_context *ctx, gops_graph
   for (uint64_t i = 0; i < len; i++) sum += p[i];
   if (sum <= 0.0) return invalid(ctx, "personalization sum must be > 0");  // ppr.rs:37-42
   return run_single(ctx, g, cfg, kPersonalized, p, len, scores, info);
+}
+
+gops_status gops_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
+                       uint32_t *ids, double *values, uint32_t *count) {
+  if (!ctx || !count) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  *count = 0;
+  if (k == 0 || n == 0) return GOPS_OK;  // topk.rs:18-20
+  if (!scores) return fail(ctx, GOPS_INVALID_PARAMETER, "scores must not be null");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (ctx->staging_cap < n) {
+    hipFree(ctx->staging);
+    ctx->staging = nullptr;
+    ctx->staging_cap = 0;
+    HIP_TRY(ctx, dalloc(&ctx->staging, n));
+    ctx->staging_cap = n;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->staging, scores, n * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+  std::string err;
+  const gops_status st = device_top_k(ctx, ctx->staging, n, k, ids, values, count, err);
+  return st == GOPS_OK ? st : fail(ctx, st, err);
+}
+
+gops_status gops_result_top_k(gops_context *ctx, gops_graph *g, uint32_t k, uint32_t *ids,
+                              double *values, uint32_t *count) {
+  if (!ctx || !g || !count) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  if (g->ctx != ctx) return fail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  *count = 0;
+  if (!g->run.has_result)
+    return fail(ctx, GOPS_INVALID_PARAMETER, "no PageRank result on this graph yet");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::string err;
+  const gops_status st = device_top_k(ctx, g->run.out, g->dev.n, k, ids, values, count, err);
+  return st == GOPS_OK ? st : fail(ctx, st, err);
 }
 
 }  // extern "C"

@@ -126,6 +126,7 @@ struct RunBuffers {
   RunState *state = nullptr;
   uint8_t *sweep_log = nullptr;
   size_t partial_slots = 0;
+  bool has_result = false;       // out holds the last run's scores
 };
 
 // State of a Katz / eigenvector / HITS power iteration in device memory.
@@ -162,6 +163,7 @@ constexpr int kMaxSweepBlocks = 2048;  // grid of the sweep kernel (grid-stride 
 
 namespace gops {
 struct PprGraphCache;
+struct TopkScratch;
 }
 
 struct gops_graph {
@@ -194,6 +196,9 @@ struct gops_context {
   int ppr_tile = 0;
   int graph_build = GOPS_BUILD_DEVICE;
   gops_run_stats stats{};
+  gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
+  double *staging = nullptr;              // device copy of host scores for gops_top_k
+  size_t staging_cap = 0;
   gops::RunState *host_state = nullptr;  // pinned, 2 slots
   uint64_t *mirror_host = nullptr;        // mapped pinned progress word
   uint64_t *mirror_dev = nullptr;         // its device address
@@ -237,6 +242,11 @@ hipError_t launch_power_init(gops_context *ctx, gops_graph *g, PowerAlgo algo, d
                              double prescaled);
 hipError_t launch_power_iteration(gops_context *ctx, gops_graph *g, PowerAlgo algo);
 hipError_t launch_permute(gops_context *ctx, gops_graph *g, const double *src, double *out);
+
+// topk.hip: top_k (topk.rs:17-42) of n device scores into host ids / values
+gops_status device_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
+                         uint32_t *ids, double *vals, uint32_t *count, std::string &err);
+void free_topk(TopkScratch *t);
 
 // api.cpp
 enum RunKind { kPlain = 0, kWeighted = 1, kPersonalized = 2 };

@@ -147,8 +147,11 @@ struct gops_server {
     Query q;
     gops_run_info info;
     double finish_ms;
-    double *scores;  // device copy of the result (n doubles), owned
+    double *scores;  // device copy of the result (n doubles), owned; null with topk
+    std::vector<uint32_t> top_ids;  // topk > 0: the result's top_k, made on the device
+    std::vector<double> top_vals;
   };
+  double *gather = nullptr;  // topk > 0: one query's scores, reused by every harvest
   std::deque<Done> finished;
   uint64_t next_ticket = 1;
   uint64_t n_finished = 0;
@@ -787,6 +790,7 @@ void gops_server_destroy(gops_server *s) {
   hipStreamSynchronize(s->ctx->stream);
   for (Tile *t : s->tiles) free_tile(t);
   for (auto &d : s->finished) hipFree(d.scores);
+  hipFree(s->gather);
   delete s;
 }
 
@@ -864,8 +868,23 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
       d.info = gops_run_info{t->hts->iter[q], t->hts->diff[q], t->hts->converged[q], 0};
       d.finish_ms = s->now_ms();
       d.scores = nullptr;
-      STRY(ctx, hipMalloc((void **)&d.scores, (size_t)n * sizeof(double)));
-      STRY(ctx, ppr_launch_gather(ctx->stream, t->T, a, g->dev.rank, n, q, d.scores));
+      if (s->topk) {
+        // top_k on the device: only k pairs per query come back
+        if (!s->gather) STRY(ctx, hipMalloc((void **)&s->gather, (size_t)n * sizeof(double)));
+        STRY(ctx, ppr_launch_gather(ctx->stream, t->T, a, g->dev.rank, n, q, s->gather));
+        d.top_ids.resize(s->topk);
+        d.top_vals.resize(s->topk);
+        uint32_t cnt = 0;
+        std::string err;
+        st = device_top_k(ctx, s->gather, n, s->topk, d.top_ids.data(), d.top_vals.data(), &cnt,
+                          err);
+        if (st != GOPS_OK) return sfail(ctx, st, err.c_str());
+        d.top_ids.resize(cnt);
+        d.top_vals.resize(cnt);
+      } else {
+        STRY(ctx, hipMalloc((void **)&d.scores, (size_t)n * sizeof(double)));
+        STRY(ctx, ppr_launch_gather(ctx->stream, t->T, a, g->dev.rank, n, q, d.scores));
+      }
       t->used[q] = false;
       s->finished.push_back(std::move(d));
       s->n_finished++;
@@ -880,33 +899,25 @@ gops_status gops_server_poll(gops_server *s, int *ready, gops_server_result *out
                              uint32_t *topk_ids, double *topk_scores) {
   if (!s || !ready) return GOPS_INVALID_PARAMETER;
   *ready = 0;
+  if (s->topk && scores)
+    return sfail(s->ctx, GOPS_INVALID_PARAMETER, "a top-k server returns no score vectors");
   if (s->finished.empty()) return GOPS_OK;
   gops_server::Done d = std::move(s->finished.front());
   s->finished.pop_front();
   const size_t n = s->g->dev.n;
   gops_status st = GOPS_OK;
-  std::vector<double> host;
-  if (scores || (s->topk && (topk_ids || topk_scores))) {
-    host.resize(n);
-    hipError_t e = hipMemcpy(host.data(), d.scores, n * sizeof(double), hipMemcpyDeviceToHost);
+  if (scores && d.scores) {
+    hipError_t e = hipMemcpy(scores, d.scores, n * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) st = sfail(s->ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
   }
   hipFree(d.scores);
   if (st != GOPS_OK) return st;
-  if (scores) std::memcpy(scores, host.data(), n * sizeof(double));
-  if (s->topk && (topk_ids || topk_scores)) {
-    // top_k semantics of topk.rs:17-42: finite, > 0, descending
-    std::vector<uint32_t> idx;
-    idx.reserve(n);
-    for (size_t i = 0; i < n; i++)
-      if (std::isfinite(host[i]) && host[i] > 0.0) idx.push_back((uint32_t)i);
-    const size_t k = std::min<size_t>(s->topk, idx.size());
-    std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](uint32_t a, uint32_t b) {
-      return host[a] != host[b] ? host[a] > host[b] : a < b;
-    });
+  if (s->topk) {
+    // top_k semantics of topk.rs:17-42, from the device selection
     for (size_t i = 0; i < s->topk; i++) {
-      if (topk_ids) topk_ids[i] = i < k ? idx[i] : UINT32_MAX;
-      if (topk_scores) topk_scores[i] = i < k ? host[idx[i]] : 0.0;
+      const bool have = i < d.top_ids.size();
+      if (topk_ids) topk_ids[i] = have ? d.top_ids[i] : UINT32_MAX;
+      if (topk_scores) topk_scores[i] = have ? d.top_vals[i] : 0.0;
     }
   }
   if (out) {

@@ -1,0 +1,451 @@
+// topk.hip -- top_k (topk.rs:17-42) on the device: the k largest finite,
+// positive scores, descending, with the entry set the reference's heap
+// keeps when scores tie at the boundary.
+//
+// The reference scans i = 0..n, keeps a min-heap of (score, i) of at most k
+// entries and replaces its minimum only by a strictly larger score.  Its
+// result is therefore: every score above T, the k-th largest, and among the
+// entries equal to T: those that entered the heap (an entry t enters while
+// fewer than k entries >= T precede it) minus the ones a larger score later
+// evicted (the smallest indices first) -- the last c of them in index order,
+// c = k - (number above T).  The final sort of the reference is unstable;
+// equal scores come out here by ascending index.
+//
+// On the device: a radix selection of T over the 64-bit patterns of the
+// scores (monotone for positive doubles; invalid scores count as 0), eight
+// 8-bit passes of workgroup histograms; then ordered passes over contiguous
+// chunks (per-chunk counts, a scan, per-element ranks from wave ballots)
+// that write the selected entries in index order; then a sort of the k
+// entries (one workgroup's LDS for k <= 4096, hipCUB beyond).
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace gops {
+
+namespace {
+
+constexpr int kTB = 256;        // threads per workgroup of the passes
+constexpr int kTGrid = 1024;    // workgroups (contiguous chunks) of the passes
+constexpr int kSortMax = 4096;  // entries sorted in one workgroup's LDS
+constexpr int kSortThreads = 1024;
+
+struct TkState {
+  uint64_t prefix;  // radix passes: the bits of T found so far; then T
+  uint64_t k_rem;   // keys still to take within the prefix range; then c
+  uint64_t above;   // keys above the prefix range; then keys above T
+  uint64_t valid;   // finite, positive scores
+  uint64_t ties_in; // entries equal to T that entered the heap (j)
+  uint32_t all;     // valid <= k: every valid score is selected
+  uint32_t pad;
+};
+
+__device__ __forceinline__ uint64_t skey(double s) {
+  return (isfinite(s) && s > 0.0) ? (uint64_t)__double_as_longlong(s) : 0ull;
+}
+
+// digit histogram of the valid keys whose bits above `shift + 8` match the prefix
+__global__ __launch_bounds__(kTB) void tk_hist_kernel(const double *s, uint64_t n, int shift,
+                                                       const TkState *st, uint32_t *part) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  if (!st->all) {
+    const int hi = shift + 8;
+    const uint64_t pre = hi < 64 ? (st->prefix >> hi) : 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kTB + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kTB) {
+      const uint64_t key = skey(s[i]);
+      if (key && (hi >= 64 || (key >> hi) == pre)) atomicAdd(&h[(key >> shift) & 255], 1u);
+    }
+  }
+  __syncthreads();
+  part[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+// one workgroup: sum the histograms, take the digit of the k_rem-th largest
+__global__ __launch_bounds__(kTB) void tk_pick_kernel(TkState *st, const uint32_t *part, int nblocks,
+                                                       int shift, uint64_t k) {
+  __shared__ uint64_t tot[256];
+  uint64_t t = 0;
+  for (int w = 0; w < nblocks; w++) t += part[(size_t)w * 256 + threadIdx.x];
+  tot[threadIdx.x] = t;
+  __syncthreads();
+  if (threadIdx.x != 0 || st->all) return;
+  if (shift == 56) {  // the first pass sees every valid key
+    uint64_t v = 0;
+    for (int d = 0; d < 256; d++) v += tot[d];
+    st->valid = v;
+    if (v <= k) {  // all of them; T = 0 selects every valid key as "above"
+      st->all = 1;
+      st->prefix = 0;
+      st->above = v;
+      st->k_rem = 0;
+      return;
+    }
+    st->prefix = 0;
+    st->above = 0;
+    st->k_rem = k;
+  }
+  const uint64_t need = st->k_rem;
+  uint64_t acc = 0;
+  int d = 255;
+  for (; d > 0; d--) {
+    if (acc + tot[d] >= need) break;
+    acc += tot[d];
+  }
+  st->above += acc;
+  st->k_rem = need - acc;
+  st->prefix |= (uint64_t)d << shift;
+}
+
+// exclusive ranks of two flags over a workgroup of kTB threads, and totals
+__device__ __forceinline__ void block_rank2(bool fa, bool fb, uint32_t &ra, uint32_t &rb,
+                                            uint32_t &ta, uint32_t &tb) {
+  __shared__ uint32_t wa[kTB / 64], wb[kTB / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t ba = __ballot(fa), bb = __ballot(fb);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  if (lane == 0) {
+    wa[wave] = (uint32_t)__popcll(ba);
+    wb[wave] = (uint32_t)__popcll(bb);
+  }
+  __syncthreads();
+  uint32_t oa = 0, ob = 0;
+  ta = tb = 0;
+#pragma unroll
+  for (int w = 0; w < kTB / 64; w++) {
+    if (w < wave) {
+      oa += wa[w];
+      ob += wb[w];
+    }
+    ta += wa[w];
+    tb += wb[w];
+  }
+  __syncthreads();
+  ra = oa + (uint32_t)__popcll(ba & below);
+  rb = ob + (uint32_t)__popcll(bb & below);
+}
+
+__device__ __forceinline__ void chunk_of(uint64_t n, uint64_t &b, uint64_t &e) {
+  const uint64_t c = (n + gridDim.x - 1) / gridDim.x;
+  b = std::min<uint64_t>(n, blockIdx.x * c);
+  e = std::min<uint64_t>(n, b + c);
+}
+
+// per chunk: entries >= T and == T (valid ones)
+__global__ __launch_bounds__(kTB) void tk_count_kernel(const double *s, uint64_t n,
+                                                        const TkState *st, uint64_t *cnt) {
+  const uint64_t T = st->prefix;
+  uint64_t b, e;
+  chunk_of(n, b, e);
+  uint32_t ge = 0, eq = 0;
+  for (uint64_t i = b + threadIdx.x; i < e; i += kTB) {
+    const uint64_t key = skey(s[i]);
+    ge += key != 0 && key >= T;
+    eq += key != 0 && key == T;
+  }
+  __shared__ uint32_t sg[kTB], se[kTB];
+  sg[threadIdx.x] = ge;
+  se[threadIdx.x] = eq;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t tg = 0, te = 0;
+    for (int t = 0; t < kTB; t++) {
+      tg += sg[t];
+      te += se[t];
+    }
+    cnt[2 * blockIdx.x] = tg;
+    cnt[2 * blockIdx.x + 1] = te;
+  }
+}
+
+// one workgroup: exclusive scans of the chunk counts
+__global__ void tk_scan_kernel(const uint64_t *cnt, int nblocks, uint64_t *off) {
+  if (threadIdx.x != 0) return;
+  uint64_t g = 0, q = 0;
+  for (int w = 0; w < nblocks; w++) {
+    off[2 * w] = g;
+    off[2 * w + 1] = q;
+    g += cnt[2 * w];
+    q += cnt[2 * w + 1];
+  }
+}
+
+// per chunk: entries equal to T that entered the heap (fewer than k entries
+// >= T before them)
+__global__ __launch_bounds__(kTB) void tk_entered_kernel(const double *s, uint64_t n, uint64_t k,
+                                                          const TkState *st, const uint64_t *off,
+                                                          uint64_t *part) {
+  const uint64_t T = st->prefix;
+  uint64_t b, e;
+  chunk_of(n, b, e);
+  uint64_t run_ge = off[2 * blockIdx.x];
+  uint32_t entered = 0;
+  for (uint64_t i0 = b; i0 < e; i0 += kTB) {
+    const uint64_t i = i0 + threadIdx.x;
+    const uint64_t key = i < e ? skey(s[i]) : 0;
+    const bool ge = key != 0 && key >= T, eq = key != 0 && key == T;
+    uint32_t rg, rq, tg, tq;
+    block_rank2(ge, eq, rg, rq, tg, tq);
+    if (eq && run_ge + rg < k) entered++;
+    run_ge += tg;
+  }
+  __shared__ uint32_t sh[kTB];
+  sh[threadIdx.x] = entered;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int x = 0; x < kTB; x++) t += sh[x];
+    part[blockIdx.x] = t;
+  }
+}
+
+// the selected entries in index order: those above T at [0, above), the kept
+// ties (the last c of the j that entered) at [above, above + c)
+__global__ __launch_bounds__(kTB) void tk_select_kernel(const double *s, uint64_t n, TkState *st,
+                                                         const uint64_t *off,
+                                                         const uint64_t *entered, int nblocks,
+                                                         uint64_t *out_key, uint32_t *out_id) {
+  const uint64_t T = st->prefix, above = st->above, c = st->k_rem;
+  __shared__ uint64_t sj;
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    if (!st->all)
+      for (int w = 0; w < nblocks; w++) t += entered[w];
+    sj = t;
+  }
+  __syncthreads();
+  const uint64_t j = sj;
+  const uint64_t first_kept = j - c;  // tie rank of the first kept tie
+  uint64_t b, e;
+  chunk_of(n, b, e);
+  uint64_t run_ge = off[2 * blockIdx.x], run_eq = off[2 * blockIdx.x + 1];
+  for (uint64_t i0 = b; i0 < e; i0 += kTB) {
+    const uint64_t i = i0 + threadIdx.x;
+    const uint64_t key = i < e ? skey(s[i]) : 0;
+    const bool ge = key != 0 && key >= T, eq = key != 0 && key == T;
+    uint32_t rg, rq, tg, tq;
+    block_rank2(ge, eq, rg, rq, tg, tq);
+    if (ge && !eq) {
+      const uint64_t pos = (run_ge + rg) - (run_eq + rq);  // entries above T before i
+      out_key[pos] = key;
+      out_id[pos] = (uint32_t)i;
+    } else if (eq) {
+      const uint64_t tr = run_eq + rq;
+      if (tr >= first_kept && tr < j) {
+        out_key[above + (tr - first_kept)] = key;
+        out_id[above + (tr - first_kept)] = (uint32_t)i;
+      }
+    }
+    run_ge += tg;
+    run_eq += tq;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->ties_in = j;
+}
+
+// one workgroup: sort count (<= kSortMax) entries by descending key, then
+// ascending index; write ids and scores
+__global__ __launch_bounds__(kSortThreads) void tk_sort_kernel(const uint64_t *key_in,
+                                                                const uint32_t *id_in,
+                                                                const TkState *st, uint64_t k,
+                                                                uint32_t *ids, double *vals,
+                                                                uint32_t *count) {
+  __shared__ uint64_t sk[kSortMax];
+  __shared__ uint32_t si[kSortMax];
+  const uint64_t K = st->all ? st->valid : k;
+  uint32_t P = 1;
+  while (P < K) P <<= 1;
+  for (uint32_t i = threadIdx.x; i < P; i += kSortThreads) {
+    sk[i] = i < K ? key_in[i] : 0ull;
+    si[i] = i < K ? id_in[i] : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  for (uint32_t size = 2; size <= P; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t i = threadIdx.x; i < P; i += kSortThreads) {
+        const uint32_t l = i ^ stride;
+        if (l > i) {
+          const bool before = sk[i] > sk[l] || (sk[i] == sk[l] && si[i] < si[l]);
+          const bool up = (i & size) == 0;  // this run sorted "first before"
+          if (before != up) {
+            const uint64_t tk = sk[i];
+            sk[i] = sk[l];
+            sk[l] = tk;
+            const uint32_t ti = si[i];
+            si[i] = si[l];
+            si[l] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < K; i += kSortThreads) {
+    ids[i] = si[i];
+    vals[i] = __longlong_as_double((long long)sk[i]);
+  }
+  if (threadIdx.x == 0) *count = (uint32_t)K;
+}
+
+// after a hipCUB sort of count entries: ids and scores, and the count
+__global__ __launch_bounds__(kTB) void tk_emit_kernel(const uint64_t *key, const uint32_t *id,
+                                                       const TkState *st, uint64_t k,
+                                                       uint32_t *ids, double *vals,
+                                                       uint32_t *count) {
+  const uint64_t K = st->all ? st->valid : k;
+  for (uint64_t i = blockIdx.x * (uint64_t)kTB + threadIdx.x; i < K;
+       i += (uint64_t)gridDim.x * kTB) {
+    ids[i] = id[i];
+    vals[i] = __longlong_as_double((long long)key[i]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *count = (uint32_t)K;
+}
+
+}  // namespace
+
+struct TopkScratch {
+  uint32_t *part = nullptr;     // kTGrid * 256 histogram bins
+  uint64_t *cnt = nullptr;      // 2 * kTGrid chunk counts
+  uint64_t *off = nullptr;      // 2 * kTGrid chunk offsets
+  uint64_t *entered = nullptr;  // kTGrid
+  TkState *st = nullptr;
+  size_t cap = 0;               // entries of the arrays below
+  uint64_t *key[2] = {nullptr, nullptr};
+  uint32_t *id[2] = {nullptr, nullptr};
+  uint32_t *ids = nullptr;
+  double *vals = nullptr;
+  uint32_t *count = nullptr;
+  void *tmp = nullptr;
+  size_t tmp_bytes = 0;
+};
+
+void free_topk(TopkScratch *t) {
+  if (!t) return;
+  for (void *p : {(void *)t->part, (void *)t->cnt, (void *)t->off, (void *)t->entered,
+                  (void *)t->st, (void *)t->key[0], (void *)t->key[1], (void *)t->id[0],
+                  (void *)t->id[1], (void *)t->ids, (void *)t->vals, (void *)t->count, t->tmp})
+    (void)hipFree(p);
+  delete t;
+}
+
+static hipError_t topk_scratch(gops_context *ctx, uint32_t k) {
+  if (!ctx->topk) ctx->topk = new TopkScratch;
+  TopkScratch &t = *ctx->topk;
+  hipError_t e = hipSuccess;
+  auto get = [&](auto **p, size_t count) {
+    if (e == hipSuccess && !*p) e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(**p));
+  };
+  get(&t.part, (size_t)kTGrid * 256);
+  get(&t.cnt, 2 * kTGrid);
+  get(&t.off, 2 * kTGrid);
+  get(&t.entered, kTGrid);
+  get(&t.st, 1);
+  get(&t.count, 1);
+  if (e != hipSuccess) return e;
+  if (t.cap < k) {
+    for (void *p : {(void *)t.key[0], (void *)t.key[1], (void *)t.id[0], (void *)t.id[1],
+                    (void *)t.ids, (void *)t.vals, t.tmp})
+      (void)hipFree(p);
+    t.key[0] = t.key[1] = nullptr;
+    t.id[0] = t.id[1] = nullptr;
+    t.ids = nullptr;
+    t.vals = nullptr;
+    t.tmp = nullptr;
+    t.cap = 0;
+    t.tmp_bytes = 0;
+    get(&t.key[0], k);
+    get(&t.key[1], k);
+    get(&t.id[0], k);
+    get(&t.id[1], k);
+    get(&t.ids, k);
+    get(&t.vals, k);
+    if (e != hipSuccess) return e;
+    if (k > (uint32_t)kSortMax) {
+      e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, t.tmp_bytes, t.key[0], t.key[1],
+                                                       t.id[0], t.id[1], (int)k, 0, 64,
+                                                       ctx->stream);
+      if (e != hipSuccess) return e;
+      e = hipMalloc(&t.tmp, std::max<size_t>(t.tmp_bytes, 1));
+      if (e != hipSuccess) return e;
+    }
+    t.cap = k;
+  }
+  return hipSuccess;
+}
+
+// top_k of n device scores into host ids / values (count entries, <= k)
+gops_status device_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
+                         uint32_t *ids, double *vals, uint32_t *count, std::string &err) {
+  *count = 0;
+  if (k == 0 || n == 0) return GOPS_OK;  // topk.rs:18-20
+  if (n >= (1ull << 32)) {
+    err = "top_k: at most 2^32 - 1 scores";
+    return GOPS_NOT_SUPPORTED;
+  }
+  const uint64_t kk = std::min<uint64_t>(k, n);
+  hipStream_t s = ctx->stream;
+  hipError_t e = topk_scratch(ctx, (uint32_t)kk);
+  TopkScratch &t = *ctx->topk;
+  if (e == hipSuccess) e = hipMemsetAsync(t.st, 0, sizeof(TkState), s);
+  if (e == hipSuccess) {
+    for (int shift = 56; shift >= 0; shift -= 8) {
+      tk_hist_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, shift, t.st, t.part);
+      tk_pick_kernel<<<1, kTB, 0, s>>>(t.st, t.part, kTGrid, shift, kk);
+    }
+    tk_count_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.cnt);
+    tk_scan_kernel<<<1, 64, 0, s>>>(t.cnt, kTGrid, t.off);
+    tk_entered_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, kk, t.st, t.off, t.entered);
+    tk_select_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.off, t.entered, kTGrid,
+                                           t.key[0], t.id[0]);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    if (kk <= (uint64_t)kSortMax) {
+      tk_sort_kernel<<<1, kSortThreads, 0, s>>>(t.key[0], t.id[0], t.st, kk, t.ids, t.vals,
+                                                t.count);
+      e = hipGetLastError();
+    } else {
+      // entries past the selected count sort last as key 0; the sort is
+      // stable and the entries are in index order within equal keys
+      TkState hs{};
+      e = hipMemcpyAsync(&hs, t.st, sizeof(hs), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      const uint64_t K = hs.all ? hs.valid : kk;
+      if (e == hipSuccess && K < kk)
+        e = hipMemsetAsync(t.key[0] + K, 0, (kk - K) * sizeof(uint64_t), s);
+      size_t b = t.tmp_bytes;
+      if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairsDescending(t.tmp, b, t.key[0], t.key[1], t.id[0],
+                                                         t.id[1], (int)kk, 0, 64, s);
+      if (e == hipSuccess) {
+        tk_emit_kernel<<<(int)std::min<uint64_t>((kk + kTB - 1) / kTB, 4096), kTB, 0, s>>>(
+            t.key[1], t.id[1], t.st, kk, t.ids, t.vals, t.count);
+        e = hipGetLastError();
+      }
+    }
+  }
+  uint32_t cnt = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&cnt, t.count, sizeof(cnt), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && cnt) {
+    if (ids) e = hipMemcpyAsync(ids, t.ids, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && vals)
+      e = hipMemcpyAsync(vals, t.vals, cnt * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) {
+    err = std::string("top_k: ") + hipGetErrorString(e);
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GOPS_OUT_OF_MEMORY
+                                                                       : GOPS_DEVICE_ERROR;
+  }
+  *count = cnt;
+  return GOPS_OK;
+}
+
+}  // namespace gops

@@ -194,6 +194,22 @@ gops_status gops_personalized_pagerank_checked_run(gops_context *ctx, gops_graph
                                                    uint64_t len, double *scores,
                                                    gops_run_info *info);
 
+/* ---- top_k ----------------------------------------------------------- */
+/* top_k, topk.rs:17-42: the (id, score) pairs of the k largest finite,
+ * positive scores, descending, computed on the device; *count (<= k)
+ * entries are written to ids / values (each nullable, k entries).  When
+ * scores tie at the k-th value, the entries kept are the ones the
+ * reference's heap keeps; equal scores come out by ascending id (the
+ * reference's unstable sort leaves their order open). */
+gops_status gops_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
+                       uint32_t *ids, double *values, uint32_t *count);
+/* top_k of the last pagerank / weighted / personalized run on g, whose
+ * scores stay on the device (call the run with scores = NULL): only the k
+ * pairs cross PCIe.  No reference counterpart (it composes the reference's
+ * run and top_k). */
+gops_status gops_result_top_k(gops_context *ctx, gops_graph *g, uint32_t k, uint32_t *ids,
+                              double *values, uint32_t *count);
+
 /* ---- Katz, eigenvector and HITS centrality ----------------------------- */
 /* KatzConfig, katz.rs:25-53 (defaults alpha 0.1, beta 1.0, 100, 1e-6). */
 typedef struct gops_katz_config {
@@ -278,9 +294,12 @@ gops_status gops_server_submit(gops_server *s, const gops_pagerank_config *cfg,
 /* Advance the device by up to `sweeps` sweeps, admitting queued queries into
  * free slots first; returns the number of active queries via *active. */
 gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active);
-/* Collect a finished query.  Returns GOPS_OK and fills out / scores (n
- * doubles, nullable) or topk arrays (nullable), or GOPS_NOT_SUPPORTED-free
- * status GOPS_OK with *ready = 0 when nothing has finished. */
+/* Collect a finished query: *ready = 1 and out / scores (n doubles,
+ * nullable) are filled, or *ready = 0 when nothing has finished.  A server
+ * created with topk > 0 selects each result's top_k (topk.rs:17-42) on the
+ * device when the query finishes and keeps only those pairs: scores must be
+ * NULL, and topk_ids / topk_scores (topk entries each, nullable) receive
+ * them, padded with UINT32_MAX / 0.0 when fewer scores are positive. */
 gops_status gops_server_poll(gops_server *s, int *ready, gops_server_result *out,
                              double *scores, uint32_t *topk_ids, double *topk_scores);
 gops_status gops_server_stats(const gops_server *s, uint64_t *queued, uint64_t *active,

@@ -58,6 +58,8 @@ def lib():
                                      ctypes.c_double, _f64p, _f64p]
         L.orc_push_sweep.restype = ctypes.c_double
         L.orc_transpose.argtypes = [ctypes.c_uint64, _u64p, _u32p, _f64p, _u64p, _u32p, _f64p]
+        L.orc_top_k.argtypes = [ctypes.c_uint64, _f64p, ctypes.c_uint64, _u64p, _f64p]
+        L.orc_top_k.restype = ctypes.c_uint64
         L.orc_validate_katz.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                                         ctypes.c_double, ctypes.c_char_p, ctypes.c_size_t]
         L.orc_katz_run.argtypes = [ctypes.c_uint64, _u64p, _u32p, ctypes.c_double] + run_tail
@@ -289,3 +291,14 @@ def hits(off, nb, max_iter=100, tol=1e-6) -> HitsRun:
     lib().orc_hits(n, _p(off, _u64p), _p(nb, _u32p), int(max_iter), float(tol), _p(hub, _f64p),
                    _p(auth, _f64p), _p(it, _u64p), _p(ad, _f64p), _p(hd, _f64p))
     return HitsRun(hub[:n].copy(), auth[:n].copy(), int(it[0]), float(ad[0]), float(hd[0]))
+
+
+def top_k(scores, k):
+    """top_k, topk.rs:17-42: [(index, score)], descending; equal scores by
+    ascending index (the reference's unstable sort leaves that open)."""
+    s = np.ascontiguousarray(scores, dtype=np.float64)
+    k = int(k)
+    ids = np.zeros(max(min(k, s.size), 1), dtype=np.uint64)
+    vals = np.zeros(max(min(k, s.size), 1), dtype=np.float64)
+    cnt = lib().orc_top_k(s.size, _p(s, _f64p), min(k, s.size), _p(ids, _u64p), _p(vals, _f64p))
+    return [(int(i), float(v)) for i, v in zip(ids[:cnt], vals[:cnt])]

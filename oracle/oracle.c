@@ -29,6 +29,9 @@
  *   orc_*_pull_run: pinned bit-for-bit against the push forms above.
  *   orc_katz_run: pinned by exact-rational trajectories, the linear solve
  *     beta (I - alpha A^T)^-1 1 and the reference's unit tests (katz.rs:231-329).
+ *   orc_top_k: pinned by the reference's doc example (topk.rs:9-16), by brute
+ *     force on tie-free inputs and, at ties, by a closed description of the
+ *     entries the heap keeps (tests/test_oracle_topk.py).
  *   orc_eigenvector_run, orc_hits: pinned by 60-digit decimal trajectories of
  *     the same iterations, the dominant eigenvector of A + I (numpy) and the
  *     top singular vectors of A (numpy SVD), and the reference's unit tests
@@ -677,4 +680,73 @@ void orc_hits(uint64_t n, const uint64_t *off, const uint32_t *nb, uint64_t max_
     free(new_hub);
     free(in_off);
     free(in_src);
+}
+
+/* top_k, topk.rs:17-42: a min-heap of (score, index) of at most k entries
+ * (Reverse((NotNan, usize)) in a max-heap), scores that are not finite or
+ * <= 0 skipped, the minimum replaced only by a strictly larger score; then
+ * a sort by descending score.  The reference's sort is unstable; equal
+ * scores are ordered here by ascending index.  Returns the entry count. */
+static int tk_less(double sa, uint64_t ia, double sb, uint64_t ib) {
+    return sa < sb || (sa == sb && ia < ib);
+}
+
+static void tk_sift_down(double *hs, uint64_t *hi, uint64_t len, uint64_t p) {
+    for (;;) {
+        uint64_t c = 2 * p + 1;
+        if (c >= len) return;
+        if (c + 1 < len && tk_less(hs[c + 1], hi[c + 1], hs[c], hi[c])) c++;
+        if (!tk_less(hs[c], hi[c], hs[p], hi[p])) return;
+        double ts = hs[p]; hs[p] = hs[c]; hs[c] = ts;
+        uint64_t ti = hi[p]; hi[p] = hi[c]; hi[c] = ti;
+        p = c;
+    }
+}
+
+static void tk_sift_up(double *hs, uint64_t *hi, uint64_t c) {
+    while (c > 0) {
+        uint64_t p = (c - 1) / 2;
+        if (!tk_less(hs[c], hi[c], hs[p], hi[p])) return;
+        double ts = hs[p]; hs[p] = hs[c]; hs[c] = ts;
+        uint64_t ti = hi[p]; hi[p] = hi[c]; hi[c] = ti;
+        c = p;
+    }
+}
+
+static double *tk_sort_scores;
+static int tk_desc(const void *a, const void *b) {
+    const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    const double sx = tk_sort_scores[x], sy = tk_sort_scores[y];
+    if (sx != sy) return sx > sy ? -1 : 1;
+    return x < y ? -1 : (x > y);
+}
+
+uint64_t orc_top_k(uint64_t n, const double *scores, uint64_t k, uint64_t *ids, double *vals) {
+    if (k == 0 || n == 0) return 0; /* :18-20 */
+    double *hs = (double *)malloc(k * sizeof(double));
+    uint64_t *hi = (uint64_t *)malloc(k * sizeof(uint64_t));
+    uint64_t len = 0;
+    for (uint64_t i = 0; i < n; i++) { /* :22-35 */
+        const double s = scores[i];
+        if (!isfinite(s) || s <= 0.0) continue;
+        if (len < k) {
+            hs[len] = s;
+            hi[len] = i;
+            tk_sift_up(hs, hi, len);
+            len++;
+        } else if (s > hs[0]) {
+            hs[0] = s;
+            hi[0] = i;
+            tk_sift_down(hs, hi, len, 0);
+        }
+    }
+    tk_sort_scores = (double *)scores; /* :36-40 */
+    qsort(hi, len, sizeof(uint64_t), tk_desc);
+    for (uint64_t r = 0; r < len; r++) {
+        ids[r] = hi[r];
+        vals[r] = scores[hi[r]];
+    }
+    free(hs);
+    free(hi);
+    return len;
 }

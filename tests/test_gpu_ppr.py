@@ -183,15 +183,27 @@ def test_server_admission_refused_without_memory(ctx, graph13):
         ga.Server(dg, tile=16, hbm_budget_bytes=1024.0)
 
 
-def test_server_topk(ctx, graph13):
+def test_server_topk(oracle, ctx, graph13):
+    """A top-k server selects each result's top_k on the device and returns
+    only those pairs; they equal the oracle's top_k (topk.rs:17-42) of the
+    same query's full scores from a server without top-k (the same tile
+    width and slot, so bitwise the same scores)."""
     e = graph13
     dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
-    srv = ga.Server(dg, tile=8, max_slots=8, topk=10)
-    srv.submit(ga.PageRankConfig(0.85, 300, 1e-8), [3, 17])
-    got = None
-    while got is None:
-        srv.step(8)
-        got = srv.poll()
-    _, run, _, _, top = got
-    assert top == ga.top_k(run.scores, 10)
-    srv.close()
+    cfg = ga.PageRankConfig(0.85, 300, 1e-8)
+    res = []
+    for topk in (0, 10):
+        srv = ga.Server(dg, tile=8, max_slots=8, topk=topk)
+        srv.submit(cfg, [3, 17])
+        got = None
+        while got is None:
+            srv.step(8)
+            got = srv.poll()
+        res.append(got)
+        if topk:
+            with pytest.raises(ga.InvalidParameter, match="top-k server"):
+                srv.poll(with_scores=True)
+        srv.close()
+    full, top = res[0][1].scores, res[1][4]
+    assert res[1][1].scores is None and res[1][1].iterations == res[0][1].iterations
+    assert top == oracle.top_k(full, 10)
