@@ -156,16 +156,18 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
     }
 
 
-def serving_leg(ga, generators, e, dg, nq=1024, qps_offered=5000.0, max_slots=512, sweeps=2):
+def serving_leg(ga, generators, e, dg, nq=1024, qps_offered=5000.0, max_slots=512, sweeps=2,
+                topk=100):
     """Serving loop: Poisson arrivals at qps_offered, |S| log-uniform 1..256,
     d uniform over {0.5, 0.85, 0.95}, tol 1e-6; continuous batching through
-    gops_server_* (admission into free slots between device steps).
+    gops_server_* (admission into free slots between device steps); each
+    response is the query's top-100 (node, score), selected on the device.
     Latency = completion (poll) time - scheduled arrival time."""
     rng = np.random.default_rng(17)
     off, seeds = generators.seed_sets(e, nq, 1, 256, log_uniform=True, seed=23)
     damp = rng.choice([0.5, 0.85, 0.95], size=nq)
     arrive = np.cumsum(rng.exponential(1.0 / qps_offered, size=nq))
-    srv = ga.Server(dg, tile=0, max_slots=max_slots)
+    srv = ga.Server(dg, tile=0, max_slots=max_slots, topk=topk)
     st = srv.stats()
     lat = np.zeros(nq)
     its = np.zeros(nq)
@@ -202,8 +204,33 @@ def serving_leg(ga, generators, e, dg, nq=1024, qps_offered=5000.0, max_slots=51
             "queries": nq, "queries_per_s": round(nq / span, 2),
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-            "slots": st["slots"], "sweeps_per_step": sweeps,
+            "slots": st["slots"], "sweeps_per_step": sweeps, "response": f"top-{topk}",
             "iterations_mean": round(float(its.mean()), 2), "converged": int(conv)}
+
+
+def topk_leg(ga, dg, scores, ks=(100, 1000), reps=20):
+    """top_k (topk.rs:17-42) of the result of the last run, which stays on the
+    device: gops_result_top_k selects it there and only k pairs cross PCIe
+    (checked against the oracle's top_k of the same scores); the oracle's
+    top_k on one core, over scores already on the host, is the CPU baseline."""
+    import oracle
+
+    out = {"n": dg.n}
+    for k in ks:
+        got = ga.result_top_k(dg, k)
+        assert got == oracle.top_k(scores, k), f"top_k mismatch at k={k}"
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ga.result_top_k(dg, k)
+        out[f"device_ms_k{k}"] = round((time.perf_counter() - t0) / reps * 1e3, 3)
+        log(f"top_k k={k}: {out[f'device_ms_k{k}']} ms on the device")
+    t0 = time.perf_counter()
+    oracle.top_k(scores, ks[0])
+    cdt = time.perf_counter() - t0
+    out["pcie_bytes"] = {"device_select": 12 * ks[0], "download_then_select": 8 * dg.n}
+    out["cpu_baseline"] = {"value": round(cdt * 1e3, 2), "unit": "ms", "cores": 1, "kind": "port",
+                           "sample": f"oracle top_k, k={ks[0]}, of the same {dg.n} scores"}
+    return out
 
 
 def build_leg(ga, e, storage):
@@ -443,6 +470,10 @@ def main():
         out["graph_build"] = build_leg(ga, e, c["storage"])
     if not args.no_centrality:
         out["centrality"] = centrality_leg(ga, e, dg)
+    # top_k of a weighted run's result, which stays on the device
+    full = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
+    out["top_k"] = topk_leg(ga, dg, full)
+    del full
     if not args.no_ppr:
         dg.close()
         del e

@@ -247,6 +247,10 @@ hipError_t launch_permute(gops_context *ctx, gops_graph *g, const double *src, d
 gops_status device_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
                          uint32_t *ids, double *vals, uint32_t *count, std::string &err);
 void free_topk(TopkScratch *t);
+// the same, queued on ctx's stream with the results left on the device
+// (ids / vals: k entries, *count); n < 2^32
+hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
+                        uint32_t *ids, double *vals, uint32_t *count);
 
 // api.cpp
 enum RunKind { kPlain = 0, kWeighted = 1, kPersonalized = 2 };

@@ -152,6 +152,10 @@ struct gops_server {
     std::vector<double> top_vals;
   };
   double *gather = nullptr;  // topk > 0: one query's scores, reused by every harvest
+  uint32_t *top_ids = nullptr;  // topk > 0: a harvest's selections on the device,
+  double *top_vals = nullptr;   // top_cap results of topk pairs, and their counts
+  uint32_t *top_cnt = nullptr;
+  uint32_t top_cap = 0;
   std::deque<Done> finished;
   uint64_t next_ticket = 1;
   uint64_t n_finished = 0;
@@ -791,6 +795,9 @@ void gops_server_destroy(gops_server *s) {
   for (Tile *t : s->tiles) free_tile(t);
   for (auto &d : s->finished) hipFree(d.scores);
   hipFree(s->gather);
+  hipFree(s->top_ids);
+  hipFree(s->top_vals);
+  hipFree(s->top_cnt);
   delete s;
 }
 
@@ -851,8 +858,11 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
     for (uint32_t k = 0; k < sweeps; k++) STRY(ctx, ppr_launch_sweep(ctx->stream, t->T, a, t->grid));
   }
   STRY(ctx, hipStreamSynchronize(ctx->stream));
-  // 3. harvest finished queries
+  // 3. harvest finished queries (with topk: each result's top_k selected on
+  // the device, all of this step's pairs copied back once at the end)
   uint32_t live_total = 0;
+  const size_t first_new = s->finished.size();
+  uint32_t n_top = 0;  // results selected in this harvest
   for (Tile *t : s->tiles) {
     st = sync_state(ctx, t);
     if (st != GOPS_OK) return st;
@@ -869,18 +879,39 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
       d.finish_ms = s->now_ms();
       d.scores = nullptr;
       if (s->topk) {
-        // top_k on the device: only k pairs per query come back
+        // the gather buffer is reused: stream order puts each query's
+        // selection before the next query's gather
         if (!s->gather) STRY(ctx, hipMalloc((void **)&s->gather, (size_t)n * sizeof(double)));
+        if (n_top == s->top_cap) {
+          const uint32_t cap = std::max<uint32_t>(64, 2 * s->top_cap);
+          const size_t k = s->topk;
+          uint32_t *ids = nullptr, *cnt = nullptr;
+          double *vals = nullptr;
+          STRY(ctx, hipMalloc((void **)&ids, (size_t)cap * k * sizeof(uint32_t)));
+          STRY(ctx, hipMalloc((void **)&vals, (size_t)cap * k * sizeof(double)));
+          STRY(ctx, hipMalloc((void **)&cnt, (size_t)cap * sizeof(uint32_t)));
+          if (n_top) {  // this harvest's earlier selections
+            STRY(ctx, hipMemcpyAsync(ids, s->top_ids, (size_t)n_top * k * sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+            STRY(ctx, hipMemcpyAsync(vals, s->top_vals, (size_t)n_top * k * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+            STRY(ctx, hipMemcpyAsync(cnt, s->top_cnt, (size_t)n_top * sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+            STRY(ctx, hipStreamSynchronize(ctx->stream));
+          }
+          hipFree(s->top_ids);
+          hipFree(s->top_vals);
+          hipFree(s->top_cnt);
+          s->top_ids = ids;
+          s->top_vals = vals;
+          s->top_cnt = cnt;
+          s->top_cap = cap;
+        }
         STRY(ctx, ppr_launch_gather(ctx->stream, t->T, a, g->dev.rank, n, q, s->gather));
-        d.top_ids.resize(s->topk);
-        d.top_vals.resize(s->topk);
-        uint32_t cnt = 0;
-        std::string err;
-        st = device_top_k(ctx, s->gather, n, s->topk, d.top_ids.data(), d.top_vals.data(), &cnt,
-                          err);
-        if (st != GOPS_OK) return sfail(ctx, st, err.c_str());
-        d.top_ids.resize(cnt);
-        d.top_vals.resize(cnt);
+        const size_t off = (size_t)n_top * s->topk;
+        STRY(ctx, launch_top_k(ctx, s->gather, n, s->topk, s->top_ids + off, s->top_vals + off,
+                               s->top_cnt + n_top));
+        n_top++;
       } else {
         STRY(ctx, hipMalloc((void **)&d.scores, (size_t)n * sizeof(double)));
         STRY(ctx, ppr_launch_gather(ctx->stream, t->T, a, g->dev.rank, n, q, d.scores));
@@ -888,6 +919,23 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
       t->used[q] = false;
       s->finished.push_back(std::move(d));
       s->n_finished++;
+    }
+  }
+  if (n_top) {
+    const size_t k = s->topk;
+    std::vector<uint32_t> ids((size_t)n_top * k), cnt(n_top);
+    std::vector<double> vals((size_t)n_top * k);
+    STRY(ctx, hipMemcpyAsync(ids.data(), s->top_ids, ids.size() * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, ctx->stream));
+    STRY(ctx, hipMemcpyAsync(vals.data(), s->top_vals, vals.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx->stream));
+    STRY(ctx, hipMemcpyAsync(cnt.data(), s->top_cnt, cnt.size() * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, ctx->stream));
+    STRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (uint32_t i = 0; i < n_top; i++) {
+      gops_server::Done &d = s->finished[first_new + i];
+      d.top_ids.assign(ids.begin() + (size_t)i * k, ids.begin() + (size_t)i * k + cnt[i]);
+      d.top_vals.assign(vals.begin() + (size_t)i * k, vals.begin() + (size_t)i * k + cnt[i]);
     }
   }
   STRY(ctx, hipStreamSynchronize(ctx->stream));

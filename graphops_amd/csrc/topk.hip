@@ -32,8 +32,11 @@ namespace {
 
 constexpr int kTB = 256;        // threads per workgroup of the passes
 constexpr int kTGrid = 1024;    // workgroups (contiguous chunks) of the passes
+constexpr int kHB = 1024;       // threads per workgroup of the histogram passes
+constexpr int kHGrid = 256;     // workgroups (histogram rows) of the histogram passes
 constexpr int kSortMax = 4096;  // entries sorted in one workgroup's LDS
 constexpr int kSortThreads = 1024;
+static_assert(kTGrid <= 1024, "tk_scan_kernel scans one count per thread");
 
 struct TkState {
   uint64_t prefix;  // radix passes: the bits of T found so far; then T
@@ -49,37 +52,66 @@ __device__ __forceinline__ uint64_t skey(double s) {
   return (isfinite(s) && s > 0.0) ? (uint64_t)__double_as_longlong(s) : 0ull;
 }
 
-// digit histogram of the valid keys whose bits above `shift + 8` match the prefix
-__global__ __launch_bounds__(kTB) void tk_hist_kernel(const double *s, uint64_t n, int shift,
+// digit histogram of the valid keys whose bits above `shift + 8` match the
+// prefix: one row of 256 bins per workgroup
+__global__ __launch_bounds__(kHB) void tk_hist_kernel(const double *s, uint64_t n, int shift,
                                                        const TkState *st, uint32_t *part) {
   __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  if (threadIdx.x < 256) h[threadIdx.x] = 0;
   __syncthreads();
   if (!st->all) {
     const int hi = shift + 8;
     const uint64_t pre = hi < 64 ? (st->prefix >> hi) : 0;
-    for (uint64_t i = blockIdx.x * (uint64_t)kTB + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * kTB) {
+    for (uint64_t i = blockIdx.x * (uint64_t)kHB + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kHB) {
       const uint64_t key = skey(s[i]);
       if (key && (hi >= 64 || (key >> hi) == pre)) atomicAdd(&h[(key >> shift) & 255], 1u);
     }
   }
   __syncthreads();
-  part[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+  if (threadIdx.x < 256) part[(size_t)blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
 }
 
-// one workgroup: sum the histograms, take the digit of the k_rem-th largest
-__global__ __launch_bounds__(kTB) void tk_pick_kernel(TkState *st, const uint32_t *part, int nblocks,
-                                                       int shift, uint64_t k) {
-  __shared__ uint64_t tot[256];
-  uint64_t t = 0;
-  for (int w = 0; w < nblocks; w++) t += part[(size_t)w * 256 + threadIdx.x];
-  tot[threadIdx.x] = t;
+// one workgroup: sum the kHGrid histogram rows (a thread per four bins and
+// sixteenth of the rows, its loads all in flight at once), take the digit of
+// the k_rem-th largest
+constexpr int kPickThreads = 1024;
+constexpr int kPickSlices = kPickThreads / 64;  // row slices (64 threads x 4 bins = 256)
+static_assert(kHGrid % kPickSlices == 0, "rows split evenly over the slices");
+__global__ __launch_bounds__(kPickThreads) void tk_pick_kernel(TkState *st, const uint32_t *part,
+                                                                int shift, uint64_t k) {
+  __shared__ uint64_t tot[kPickSlices][256];
+  const int g = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  constexpr int kRows = kHGrid / kPickSlices;
+  uint4 v[kRows];
+#pragma unroll
+  for (int r = 0; r < kRows; r++)
+    v[r] = reinterpret_cast<const uint4 *>(part + (size_t)(sl * kRows + r) * 256)[g];
+  uint64_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+  for (int r = 0; r < kRows; r++) {
+    a0 += v[r].x;
+    a1 += v[r].y;
+    a2 += v[r].z;
+    a3 += v[r].w;
+  }
+  tot[sl][4 * g] = a0;
+  tot[sl][4 * g + 1] = a1;
+  tot[sl][4 * g + 2] = a2;
+  tot[sl][4 * g + 3] = a3;
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    uint64_t sum = 0;
+#pragma unroll
+    for (int r = 0; r < kPickSlices; r++) sum += tot[r][threadIdx.x];
+    tot[0][threadIdx.x] = sum;
+  }
   __syncthreads();
   if (threadIdx.x != 0 || st->all) return;
+  const uint64_t *tb = tot[0];
   if (shift == 56) {  // the first pass sees every valid key
     uint64_t v = 0;
-    for (int d = 0; d < 256; d++) v += tot[d];
+    for (int d = 0; d < 256; d++) v += tb[d];
     st->valid = v;
     if (v <= k) {  // all of them; T = 0 selects every valid key as "above"
       st->all = 1;
@@ -96,8 +128,8 @@ __global__ __launch_bounds__(kTB) void tk_pick_kernel(TkState *st, const uint32_
   uint64_t acc = 0;
   int d = 255;
   for (; d > 0; d--) {
-    if (acc + tot[d] >= need) break;
-    acc += tot[d];
+    if (acc + tb[d] >= need) break;
+    acc += tb[d];
   }
   st->above += acc;
   st->k_rem = need - acc;
@@ -165,15 +197,25 @@ __global__ __launch_bounds__(kTB) void tk_count_kernel(const double *s, uint64_t
   }
 }
 
-// one workgroup: exclusive scans of the chunk counts
-__global__ void tk_scan_kernel(const uint64_t *cnt, int nblocks, uint64_t *off) {
-  if (threadIdx.x != 0) return;
-  uint64_t g = 0, q = 0;
-  for (int w = 0; w < nblocks; w++) {
-    off[2 * w] = g;
-    off[2 * w + 1] = q;
-    g += cnt[2 * w];
-    q += cnt[2 * w + 1];
+// one workgroup of nblocks (<= 1024) threads: exclusive scans of the chunk counts
+__global__ __launch_bounds__(1024) void tk_scan_kernel(const uint64_t *cnt, int nblocks,
+                                                        uint64_t *off) {
+  __shared__ uint64_t sg[1024], sq[1024];
+  const int t = threadIdx.x;
+  const uint64_t g = t < nblocks ? cnt[2 * t] : 0, q = t < nblocks ? cnt[2 * t + 1] : 0;
+  sg[t] = g;
+  sq[t] = q;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {  // inclusive Hillis-Steele scan
+    const uint64_t ag = t >= d ? sg[t - d] : 0, aq = t >= d ? sq[t - d] : 0;
+    __syncthreads();
+    sg[t] += ag;
+    sq[t] += aq;
+    __syncthreads();
+  }
+  if (t < nblocks) {
+    off[2 * t] = sg[t] - g;
+    off[2 * t + 1] = sq[t] - q;
   }
 }
 
@@ -213,15 +255,16 @@ __global__ __launch_bounds__(kTB) void tk_select_kernel(const double *s, uint64_
                                                          const uint64_t *entered, int nblocks,
                                                          uint64_t *out_key, uint32_t *out_id) {
   const uint64_t T = st->prefix, above = st->above, c = st->k_rem;
-  __shared__ uint64_t sj;
-  if (threadIdx.x == 0) {
-    uint64_t t = 0;
-    if (!st->all)
-      for (int w = 0; w < nblocks; w++) t += entered[w];
-    sj = t;
-  }
+  __shared__ uint64_t sj[kTB];
+  uint64_t tj = 0;
+  for (int w = threadIdx.x; w < nblocks; w += kTB) tj += entered[w];
+  sj[threadIdx.x] = tj;
   __syncthreads();
-  const uint64_t j = sj;
+  for (int d = kTB / 2; d > 0; d >>= 1) {
+    if ((int)threadIdx.x < d) sj[threadIdx.x] += sj[threadIdx.x + d];
+    __syncthreads();
+  }
+  const uint64_t j = st->all ? 0 : sj[0];
   const uint64_t first_kept = j - c;  // tie rank of the first kept tie
   uint64_t b, e;
   chunk_of(n, b, e);
@@ -293,6 +336,14 @@ __global__ __launch_bounds__(kSortThreads) void tk_sort_kernel(const uint64_t *k
   if (threadIdx.x == 0) *count = (uint32_t)K;
 }
 
+// entries [K, k) (fewer valid scores than k) sort last as key 0
+__global__ __launch_bounds__(kTB) void tk_pad_kernel(uint64_t *key, const TkState *st, uint64_t k) {
+  const uint64_t K = st->all ? st->valid : k;
+  for (uint64_t i = K + blockIdx.x * (uint64_t)kTB + threadIdx.x; i < k;
+       i += (uint64_t)gridDim.x * kTB)
+    key[i] = 0;
+}
+
 // after a hipCUB sort of count entries: ids and scores, and the count
 __global__ __launch_bounds__(kTB) void tk_emit_kernel(const uint64_t *key, const uint32_t *id,
                                                        const TkState *st, uint64_t k,
@@ -310,7 +361,7 @@ __global__ __launch_bounds__(kTB) void tk_emit_kernel(const uint64_t *key, const
 }  // namespace
 
 struct TopkScratch {
-  uint32_t *part = nullptr;     // kTGrid * 256 histogram bins
+  uint32_t *part = nullptr;     // kHGrid * 256 histogram bins
   uint64_t *cnt = nullptr;      // 2 * kTGrid chunk counts
   uint64_t *off = nullptr;      // 2 * kTGrid chunk offsets
   uint64_t *entered = nullptr;  // kTGrid
@@ -341,7 +392,7 @@ static hipError_t topk_scratch(gops_context *ctx, uint32_t k) {
   auto get = [&](auto **p, size_t count) {
     if (e == hipSuccess && !*p) e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(**p));
   };
-  get(&t.part, (size_t)kTGrid * 256);
+  get(&t.part, (size_t)kHGrid * 256);
   get(&t.cnt, 2 * kTGrid);
   get(&t.off, 2 * kTGrid);
   get(&t.entered, kTGrid);
@@ -379,6 +430,44 @@ static hipError_t topk_scratch(gops_context *ctx, uint32_t k) {
   return hipSuccess;
 }
 
+// Queues the selection of top_k of n device scores on ctx's stream: ids /
+// vals (k entries) and *count on the device.  Calls on one context share its
+// scratch and run in stream order.
+hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
+                        uint32_t *ids, double *vals, uint32_t *count) {
+  hipStream_t s = ctx->stream;
+  if (k == 0 || n == 0) return hipMemsetAsync(count, 0, sizeof(uint32_t), s);
+  const uint64_t kk = std::min<uint64_t>(k, n);
+  hipError_t e = topk_scratch(ctx, (uint32_t)kk);
+  if (e != hipSuccess) return e;
+  TopkScratch &t = *ctx->topk;
+  e = hipMemsetAsync(t.st, 0, sizeof(TkState), s);
+  if (e != hipSuccess) return e;
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    tk_hist_kernel<<<kHGrid, kHB, 0, s>>>(scores, n, shift, t.st, t.part);
+    tk_pick_kernel<<<1, kPickThreads, 0, s>>>(t.st, t.part, shift, kk);
+  }
+  tk_count_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.cnt);
+  tk_scan_kernel<<<1, 1024, 0, s>>>(t.cnt, kTGrid, t.off);
+  tk_entered_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, kk, t.st, t.off, t.entered);
+  tk_select_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.off, t.entered, kTGrid, t.key[0],
+                                         t.id[0]);
+  if (kk <= (uint64_t)kSortMax) {
+    tk_sort_kernel<<<1, kSortThreads, 0, s>>>(t.key[0], t.id[0], t.st, kk, ids, vals, count);
+    return hipGetLastError();
+  }
+  // the sort is stable and the entries are in index order within equal keys
+  const int g = (int)std::min<uint64_t>((kk + kTB - 1) / kTB, 4096);
+  tk_pad_kernel<<<g, kTB, 0, s>>>(t.key[0], t.st, kk);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  size_t b = t.tmp_bytes;
+  e = hipcub::DeviceRadixSort::SortPairsDescending(t.tmp, b, t.key[0], t.key[1], t.id[0], t.id[1],
+                                                   (int)kk, 0, 64, s);
+  if (e != hipSuccess) return e;
+  tk_emit_kernel<<<g, kTB, 0, s>>>(t.key[1], t.id[1], t.st, kk, ids, vals, count);
+  return hipGetLastError();
+}
+
 // top_k of n device scores into host ids / values (count entries, <= k)
 gops_status device_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
                          uint32_t *ids, double *vals, uint32_t *count, std::string &err) {
@@ -388,56 +477,20 @@ gops_status device_top_k(gops_context *ctx, const double *scores, uint64_t n, ui
     err = "top_k: at most 2^32 - 1 scores";
     return GOPS_NOT_SUPPORTED;
   }
-  const uint64_t kk = std::min<uint64_t>(k, n);
   hipStream_t s = ctx->stream;
-  hipError_t e = topk_scratch(ctx, (uint32_t)kk);
-  TopkScratch &t = *ctx->topk;
-  if (e == hipSuccess) e = hipMemsetAsync(t.st, 0, sizeof(TkState), s);
-  if (e == hipSuccess) {
-    for (int shift = 56; shift >= 0; shift -= 8) {
-      tk_hist_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, shift, t.st, t.part);
-      tk_pick_kernel<<<1, kTB, 0, s>>>(t.st, t.part, kTGrid, shift, kk);
-    }
-    tk_count_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.cnt);
-    tk_scan_kernel<<<1, 64, 0, s>>>(t.cnt, kTGrid, t.off);
-    tk_entered_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, kk, t.st, t.off, t.entered);
-    tk_select_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.off, t.entered, kTGrid,
-                                           t.key[0], t.id[0]);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) {
-    if (kk <= (uint64_t)kSortMax) {
-      tk_sort_kernel<<<1, kSortThreads, 0, s>>>(t.key[0], t.id[0], t.st, kk, t.ids, t.vals,
-                                                t.count);
-      e = hipGetLastError();
-    } else {
-      // entries past the selected count sort last as key 0; the sort is
-      // stable and the entries are in index order within equal keys
-      TkState hs{};
-      e = hipMemcpyAsync(&hs, t.st, sizeof(hs), hipMemcpyDeviceToHost, s);
-      if (e == hipSuccess) e = hipStreamSynchronize(s);
-      const uint64_t K = hs.all ? hs.valid : kk;
-      if (e == hipSuccess && K < kk)
-        e = hipMemsetAsync(t.key[0] + K, 0, (kk - K) * sizeof(uint64_t), s);
-      size_t b = t.tmp_bytes;
-      if (e == hipSuccess)
-        e = hipcub::DeviceRadixSort::SortPairsDescending(t.tmp, b, t.key[0], t.key[1], t.id[0],
-                                                         t.id[1], (int)kk, 0, 64, s);
-      if (e == hipSuccess) {
-        tk_emit_kernel<<<(int)std::min<uint64_t>((kk + kTB - 1) / kTB, 4096), kTB, 0, s>>>(
-            t.key[1], t.id[1], t.st, kk, t.ids, t.vals, t.count);
-        e = hipGetLastError();
-      }
-    }
-  }
+  hipError_t e = topk_scratch(ctx, (uint32_t)std::min<uint64_t>(k, n));
   uint32_t cnt = 0;
-  if (e == hipSuccess) e = hipMemcpyAsync(&cnt, t.count, sizeof(cnt), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e == hipSuccess && cnt) {
-    if (ids) e = hipMemcpyAsync(ids, t.ids, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && vals)
-      e = hipMemcpyAsync(vals, t.vals, cnt * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) {
+    TopkScratch &t = *ctx->topk;
+    e = launch_top_k(ctx, scores, n, k, t.ids, t.vals, t.count);
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, t.count, sizeof(cnt), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess && cnt) {
+      if (ids) e = hipMemcpyAsync(ids, t.ids, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess && vals)
+        e = hipMemcpyAsync(vals, t.vals, cnt * sizeof(double), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
   }
   if (e != hipSuccess) {
     err = std::string("top_k: ") + hipGetErrorString(e);
