@@ -367,6 +367,7 @@ def main():
     ap.add_argument("--no-centrality", action="store_true", help="skip the Katz/eigenvector/HITS leg")
     ap.add_argument("--centrality-only", action="store_true",
                     help="only the Katz/eigenvector/HITS leg")
+    ap.add_argument("--topk-only", action="store_true", help="only the top_k leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -391,6 +392,14 @@ def main():
     if args.build_only:
         print(json.dumps({"workload": c["workload"], "n": e.n, "m": e.m,
                           "graph_build": build_leg(ga, e, c["storage"])}), flush=True)
+        return
+    if args.topk_only:
+        cx = ga.Context(0)
+        dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"], ctx=cx)
+        full = ga.pagerank_weighted_run(dg, ga.PageRankConfig(0.85, c["max_it"], c["tol"]),
+                                        ctx=cx).scores
+        print(json.dumps({"workload": c["workload"], "n": e.n,
+                          "top_k": topk_leg(ga, dg, full)}), flush=True)
         return
     if args.centrality_only:
         dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"],
