@@ -338,6 +338,20 @@ def copy_probe(nbytes, reps=5):
     print(json.dumps({"copy_probe_bytes": nbytes // 4 * 4, "reps": reps}), flush=True)
 
 
+def gather_probe(dg, reps=5):
+    """gops_gather_probe: a kernel that makes only the f32 sweep's gathers on
+    this graph (every edge's source value in stored order, the col stream, a
+    sum); best of reps, HIP events, ms."""
+    import ctypes
+
+    from graphops_amd import _lib as L
+
+    ms = ctypes.c_double()
+    rc = L.lib().gops_gather_probe(dg.ctx.handle, dg.handle, reps, ctypes.byref(ms))
+    assert rc == 0, rc
+    return ms.value
+
+
 def measured_traffic(workload):
     """HBM bytes per sweep launch of this workload from the committed
     counter summary (FETCH_SIZE + WRITE_SIZE scaled by a copy of known
@@ -368,6 +382,9 @@ def main():
     ap.add_argument("--centrality-only", action="store_true",
                     help="only the Katz/eigenvector/HITS leg")
     ap.add_argument("--topk-only", action="store_true", help="only the top_k leg")
+    ap.add_argument("--no-topk", action="store_true", help="skip the top_k leg")
+    ap.add_argument("--no-gather-probe", action="store_true",
+                    help="skip the gather-only probe of the sweep's access pattern")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -456,6 +473,7 @@ def main():
                    "sweep_us": round(avg_sweep * 1e6, 2),
                    "f32_sweep_us": round(avg_f32 * 1e6, 2), "f64_sweep_us": round(avg_f64 * 1e6, 2),
                    "gteps_f32_sweep": round(m / avg_f32 / 1e9, 2) if n32 else None},
+        "gather_bound": None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
@@ -463,6 +481,15 @@ def main():
                      "kernel": f"gops::sweep_kernel<gops::{wname}>",
                      "alg_bytes_per_launch": alg_bytes},
     }
+    if not args.no_gather_probe:
+        pm = gather_probe(dg)
+        out["gather_bound"] = {
+            "probe": "gops_gather_probe: the f32 sweep's gathers alone (source values in "
+                     "stored order, the col stream, a sum)",
+            "probe_ms": round(pm, 3), "gathers_per_s": round(m / pm / 1e6, 1),
+            "f32_sweep_ms": round(avg_f32 * 1e3, 3),
+            "frac": round(pm / (avg_f32 * 1e3), 3) if n32 else None}
+        log(f"gather probe: {pm:.3f} ms for {m} gathers; f32 sweep {avg_f32 * 1e3:.3f} ms")
     if not args.no_accuracy:
         scores = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
         t0 = time.perf_counter()
@@ -479,10 +506,10 @@ def main():
         out["graph_build"] = build_leg(ga, e, c["storage"])
     if not args.no_centrality:
         out["centrality"] = centrality_leg(ga, e, dg)
-    # top_k of a weighted run's result, which stays on the device
-    full = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
-    out["top_k"] = topk_leg(ga, dg, full)
-    del full
+    if not args.no_topk:  # top_k of a weighted run's result, which stays on the device
+        full = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
+        out["top_k"] = topk_leg(ga, dg, full)
+        del full
     if not args.no_ppr:
         dg.close()
         del e

@@ -96,6 +96,7 @@ SIGNATURES = [
     ("gops_personalized_pagerank_checked_run", ctypes.c_int,
      [_vp, _vp, ctypes.POINTER(PageRankConfigC), _f64p, ctypes.c_uint64, _f64p,
       ctypes.POINTER(RunInfoC)]),
+    ("gops_gather_probe", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _f64p]),
     ("gops_top_k", ctypes.c_int,
      [_vp, _f64p, ctypes.c_uint64, ctypes.c_uint32, _u32p, _f64p, _u32p]),
     ("gops_result_top_k", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _f64p, _u32p]),

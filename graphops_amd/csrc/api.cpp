@@ -679,6 +679,27 @@ gops_status gops_personalized_pagerank_checked_run(gops_context *ctx, gops_graph
   return run_single(ctx, g, cfg, kPersonalized, p, len, scores, info);
 }
 
+gops_status gops_gather_probe(gops_context *ctx, gops_graph *g, uint32_t reps, double *best_ms) {
+  if (!ctx || !g || !best_ms) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  if (g->ctx != ctx) return fail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  gops_status st = alloc_run(ctx, g);
+  if (st != GOPS_OK) return st;
+  if ((st = ensure_events(ctx, 2)) != GOPS_OK) return st;
+  double best = std::numeric_limits<double>::infinity();
+  for (uint32_t r = 0; r <= reps; r++) {  // the first launch warms up
+    HIP_TRY(ctx, hipEventRecord(ctx->events[0], ctx->stream));
+    HIP_TRY(ctx, launch_gather_probe(ctx, g));
+    HIP_TRY(ctx, hipEventRecord(ctx->events[1], ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->events[1]));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->events[0], ctx->events[1]));
+    if (r > 0) best = std::min(best, (double)ms);
+  }
+  *best_ms = best;
+  return GOPS_OK;
+}
+
 gops_status gops_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
                        uint32_t *ids, double *values, uint32_t *count) {
   if (!ctx || !count) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");

@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void power_sweep_kernel(PowerArgs a, double
     for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x)
       do_desc(a, a.desc[i], load, epi, red, stage);
   });
-  block_sum<kPowR>(red);
+  block_sum<kPowR>(red, stage);
   if (threadIdx.x == 0) {
     a.partials[(size_t)blockIdx.x * kPowR] = red[0];
     a.partials[(size_t)blockIdx.x * kPowR + 1] = red[1];

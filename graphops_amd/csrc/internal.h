@@ -273,5 +273,6 @@ struct SweepSetup {
 hipError_t launch_init(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize);
+hipError_t launch_gather_probe(gops_context *ctx, gops_graph *g);
 
 }  // namespace gops

@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, 
     for (int i = blockIdx.x; i < a.num_desc; i += gridDim.x)
       do_desc(a, a.desc[i], load, epi, red, stage);
   }
-  block_sum<kReduce>(red);
+  block_sum<kReduce>(red, stage);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < kReduce; i++) a.partials[(size_t)blockIdx.x * kReduce + i] = red[i];
@@ -354,6 +354,35 @@ __global__ __launch_bounds__(kBlock) void scale_kernel(const RunState *st, doubl
   const double s = st->dsum_x;
   if (!(s > 0.0)) return;
   for (int o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock) out[o] /= s;
+}
+
+// The f32 sweep's gathers alone (calibration): every edge's source value is
+// gathered in stored order, kUnroll loads in flight per lane, and summed, so
+// that nothing but the col stream and the gathers touches memory.
+__global__ __launch_bounds__(kBlock) void gather_probe_kernel(const int32_t *col, const float *y,
+                                                              int m, double *partials) {
+  double acc = 0.0;
+  const int stride = gridDim.x * kBlock;
+  for (int k0 = blockIdx.x * kBlock + threadIdx.x; k0 < m; k0 += kUnroll * stride) {
+    float v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int k = k0 + u * stride;
+      v[u] = k < m ? y[__builtin_nontemporal_load(col + k)] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) acc += (double)v[u];
+  }
+  double red[1] = {acc};
+  block_sum<1>(red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+hipError_t launch_gather_probe(gops_context *ctx, gops_graph *g) {
+  const int grid = kMaxSweepBlocks;
+  gather_probe_kernel<<<grid, kBlock, 0, ctx->stream>>>(g->dev.col, g->run.yd[0], (int)g->dev.m,
+                                                        g->run.partials);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -71,22 +71,32 @@ __device__ __forceinline__ double group_sum(double v, int lg) {
   return v;
 }
 
-// Sum of `v` over the workgroup, valid in thread 0.  Fixed order.
+// Sum of `v` over the workgroup, valid in thread 0.  Fixed order.  scratch:
+// N * kBlock / 64 doubles of LDS that no other part of the workgroup uses
+// meanwhile (a sweep passes its idle stage buffer, so that the kernel's LDS
+// stays at the stage's size).
 template <int N>
-__device__ __forceinline__ void block_sum(double (&v)[N]) {
-  __shared__ double sh[N][kBlock / 64];
+__device__ __forceinline__ void block_sum(double (&v)[N], double *scratch) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int W = kBlock / 64;
 #pragma unroll
   for (int i = 0; i < N; i++) {
     const double s = group_sum(v[i], 6);
-    if (lane == 0) sh[i][wave] = s;
+    if (lane == 0) scratch[i * W + wave] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < N; i++) v[i] = (sh[i][0] + sh[i][1]) + (sh[i][2] + sh[i][3]);
+    for (int i = 0; i < N; i++)
+      v[i] = (scratch[i * W] + scratch[i * W + 1]) + (scratch[i * W + 2] + scratch[i * W + 3]);
   }
   __syncthreads();
+}
+
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N]) {
+  __shared__ double sh[N * (kBlock / 64)];
+  block_sum<N>(v, sh);
 }
 
 // Sum over edges [e0, e1) by the whole workgroup, kUnroll independent
@@ -164,14 +174,14 @@ __device__ __forceinline__ void do_desc(const A &a, const BlockDesc &bd, const L
     // one long row, the whole workgroup
     const int r = bd.y;
     double acc[1] = {stream_sum(load, a.row_ptr[r], a.row_ptr[r + 1])};
-    block_sum<1>(acc);
+    block_sum<1>(acc, stage);
     if (threadIdx.x == 0) epi(r, acc[0], red);
   } else {
     // one chunk of a split hub row; the finish kernel adds the chunks
     const int h = bd.y;
     const int chunk = bd.x & (kHubFlag - 1);
     double acc[1] = {stream_sum(load, bd.z, bd.w)};
-    block_sum<1>(acc);
+    block_sum<1>(acc, stage);
     if (threadIdx.x == 0) a.hub_partial[a.hub_first[h] + chunk] = acc[0];
   }
 }

@@ -194,6 +194,12 @@ gops_status gops_personalized_pagerank_checked_run(gops_context *ctx, gops_graph
                                                    uint64_t len, double *scores,
                                                    gops_run_info *info);
 
+/* Calibration (no reference counterpart): the best of reps timings (HIP
+ * events, ms) of a kernel that makes only the f32 sweep's gathers on g --
+ * every edge's source value in stored order, the col stream, a sum -- i.e.
+ * the bound that the source-gather pattern of this graph sets for a sweep. */
+gops_status gops_gather_probe(gops_context *ctx, gops_graph *g, uint32_t reps, double *best_ms);
+
 /* ---- top_k ----------------------------------------------------------- */
 /* top_k, topk.rs:17-42: the (id, score) pairs of the k largest finite,
  * positive scores, descending, computed on the device; *count (<= k)
