@@ -283,6 +283,15 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile *
   }
   std::memset(t->hts, 0, sizeof(TileState));
   t->hts->all_done = 1;
+  // the device copy must be valid before any admission: a server reads every
+  // tile's state back after each step (sync_state), idle tiles included, and
+  // uploads that copy again at the tile's first admission
+  e = hipMemcpyAsync(t->ts, t->hts, sizeof(TileState), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    free_tile(t);
+    return sfail(ctx, GOPS_DEVICE_ERROR, std::string("tile state: ") + hipGetErrorString(e));
+  }
   *out = t;
   return GOPS_OK;
 }
@@ -864,6 +873,9 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
   const size_t first_new = s->finished.size();
   uint32_t n_top = 0;  // results selected in this harvest
   for (Tile *t : s->tiles) {
+    bool any = false;
+    for (int q = 0; q < t->T; q++) any = any || t->used[q];
+    if (!any) continue;  // an idle tile's state has not changed
     st = sync_state(ctx, t);
     if (st != GOPS_OK) return st;
     PprArgs a = tile_args(ctx, g, t, plan, false);

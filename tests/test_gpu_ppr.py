@@ -183,6 +183,46 @@ def test_server_admission_refused_without_memory(ctx, graph13):
         ga.Server(dg, tile=16, hbm_budget_bytes=1024.0)
 
 
+def test_server_late_tiles_start_mid_flight(oracle, ctx, graph13):
+    """A server whose later tiles take their first queries while earlier ones
+    run (a few submissions per step), created after another server's run left
+    the allocator's memory dirty: every result matches the oracle.  (Idle
+    tiles' device state used to be read back uninitialised and uploaded again
+    at their first admission.)"""
+    e = graph13
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    nq = 48
+    off, seeds = generators.seed_sets(e, nq, 1, 64, seed=5)
+    cfgs = [ga.PageRankConfig([0.5, 0.85, 0.95][q % 3], 500, 1e-7) for q in range(nq)]
+
+    def serve(per_step):
+        srv = ga.Server(dg, tile=8, max_slots=32)  # four tiles of 8 queries
+        tickets, results, nxt, steps = {}, {}, 0, 0
+        while len(results) < nq:
+            for _ in range(per_step):
+                if nxt < nq:
+                    tickets[srv.submit(cfgs[nxt], seeds[off[nxt]:off[nxt + 1]])] = nxt
+                    nxt += 1
+            srv.step(2)
+            steps += 1
+            while True:
+                r = srv.poll()
+                if r is None:
+                    break
+                results[tickets[r[0]]] = r[1]
+            assert steps < 5000
+        srv.close()
+        return results
+
+    serve(nq)  # runs to completion, then frees its tiles
+    results = serve(6)
+    for q in range(nq):
+        p = dense_p(e.n, seeds[off[q]:off[q + 1]])
+        ref = oracle_ppr(oracle, e, p, cfgs[q].damping, 1e-7, 500)
+        run = results[q]
+        check(run.iterations, run.converged, run.diff_l1, run.scores, ref, 1e-7)
+
+
 def test_server_topk(oracle, ctx, graph13):
     """A top-k server selects each result's top_k on the device and returns
     only those pairs; they equal the oracle's top_k (topk.rs:17-42) of the
