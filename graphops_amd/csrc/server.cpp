@@ -2,12 +2,14 @@
 // serving loop (gops_server_*), both on resident tiles of T queries
 // (ppr_batch.hip).
 //
-// Serving: queries wait in a host queue; a tile slot is taken as soon as one
-// is free, between device steps.  Admission clears the slot's columns and
-// starts the new query with an f64 anchor sweep while the tile's running
-// queries continue, so new and running queries share every later sweep
-// (their work overlaps on the device).  Every query keeps its own sweep mode,
-// so its arithmetic does not depend on the other queries of its tile.
+// Serving: queries wait in a host queue; between device steps, a group of
+// slots that one lane serves is refilled once all of its queries have
+// finished, with queries of one damping where the queue has them.  Admission
+// clears the slots' columns and starts the new queries with an f64 anchor
+// sweep while the tile's running queries continue, so new and running queries
+// share every later sweep (their work overlaps on the device).  Every query
+// keeps its own sweep mode, so its arithmetic does not depend on the other
+// queries of its tile.
 #include <omp.h>
 
 #include <algorithm>
@@ -841,17 +843,37 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
   st = tile_plan(ctx, g, c, s->T, &plan);
   if (st != GOPS_OK) return st;
   const int n = (int)g->dev.n;
-  // 1. admission into free slots (host state is current: no kernel is queued)
+  // 1. admission (host state is current: no kernel is queued), by groups of
+  // kLaneQ slots, the queries one lane gathers for: a group takes new queries
+  // once none of its own is left, so a lane's queries start together and
+  // sweep in the same precision.  The first queued query opens the group; its
+  // other slots take the next queued queries of the same damping (which sets
+  // the iteration count most) found within kMatchWindow entries, else the
+  // next in order.  The queue head is always admitted first.
+  constexpr size_t kMatchWindow = 64;
   for (Tile *t : s->tiles) {
     if (s->queue.empty()) break;
     uint64_t mask = 0;
-    for (int q = 0; q < t->T && !s->queue.empty(); q++) {
-      if (t->used[q]) continue;
-      t->slot[q] = std::move(s->queue.front());
-      s->queue.pop_front();
-      t->used[q] = true;
-      set_slot_params(t, q, t->slot[q], ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
-      mask |= 1ull << q;
+    for (int g0 = 0; g0 < t->T && !s->queue.empty(); g0 += kLaneQ) {
+      bool group_free = true;
+      for (int q = g0; q < g0 + kLaneQ; q++) group_free = group_free && !t->used[q];
+      if (!group_free) continue;
+      const double damping = s->queue.front().cfg.damping;
+      for (int q = g0; q < g0 + kLaneQ && !s->queue.empty(); q++) {
+        size_t pick = 0;
+        const size_t window = std::min(s->queue.size(), kMatchWindow);
+        for (size_t i = 0; i < window; i++) {
+          if (s->queue[i].cfg.damping == damping) {
+            pick = i;
+            break;
+          }
+        }
+        t->slot[q] = std::move(s->queue[pick]);
+        s->queue.erase(s->queue.begin() + (std::ptrdiff_t)pick);
+        t->used[q] = true;
+        set_slot_params(t, q, t->slot[q], ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
+        mask |= 1ull << q;
+      }
     }
     if (mask) {
       st = admit(ctx, g, t, plan, mask);
