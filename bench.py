@@ -110,18 +110,11 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
     # accuracy on a sample: batched scores against the oracle's f64 run
     sample = [0, 1, 2, 3]
     sc, sruns = ga.ppr_batch_run(dg, cfg, off[:5], seeds[:off[4]], with_scores=True)
-    in_off, in_src, _ = oracle.transpose(e.offsets, e.targets)
-    deg = np.diff(e.offsets.astype(np.int64)).astype(np.float64)
-    l1 = []
-    mx = []
-    for j in sample:
-        p = np.zeros(e.n)
-        p[seeds[off[j]:off[j + 1]]] += 1.0
-        ref = oracle.pull_run(2, in_off, in_src, deg, p=p, damping=0.85, max_iterations=200,
-                              tolerance=1e-8, threads=16)
-        l1.append(float(np.abs(sc[j] - ref.scores).sum()))
-        mx.append(float(np.abs(sc[j] - ref.scores).max()))
+    refs = refs_of(oracle, e, seeds, off, sample)
+    l1 = [float(np.abs(sc[j] - refs[j]).sum()) for j in sample]
+    mx = [float(np.abs(sc[j] - refs[j]).max()) for j in sample]
     del sc
+    fast = f32_schedule(ga, dg, cfg, off, seeds, sample, refs)
     # CPU: the oracle's push form of personalized_pagerank_run, one thread
     t0 = time.perf_counter()
     for j in range(cpu_queries):
@@ -146,6 +139,7 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
         "ratio_vs_plain": round(qps / plain_qps, 2),
         "accuracy": {"sample_queries": len(sample), "l1_vs_f64_oracle_max": max(l1),
                      "max_abs": max(mx)},
+        "f32_schedule": fast,
         "roofline": {"bound": "hbm", "queries_per_s_at_peak": round(roof_qps, 1),
                      "frac": round(qps / roof_qps, 4),
                      "bytes_per_query_sweep": round(per_qsweep)},
@@ -154,6 +148,57 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
                          "sample": f"first {cpu_queries} queries by the oracle's "
                                    f"personalized_pagerank_run (f64, push), {cdt:.1f} s"},
     }
+
+
+def refs_of(oracle, e, seeds, off, sample):
+    """The oracle's f64 scores of the sample queries (d 0.85, tol 1e-8)."""
+    in_off, in_src, _ = oracle.transpose(e.offsets, e.targets)
+    deg = np.diff(e.offsets.astype(np.int64)).astype(np.float64)
+    out = []
+    for j in sample:
+        p = np.zeros(e.n)
+        p[seeds[off[j]:off[j + 1]]] += 1.0
+        out.append(oracle.pull_run(2, in_off, in_src, deg, p=p, damping=0.85, max_iterations=200,
+                                   tolerance=1e-8, threads=16).scores)
+    return out
+
+
+def f32_schedule(ga, dg, cfg, off, seeds, sample, refs, plain_sample=32):
+    """The same batch with f32 sweeps from the sweep after each query's
+    admission (GOPS_OPT_TILE_F32_BELOW = 0), against the plain path with the
+    same schedule (GOPS_OPT_PPR_F32_BELOW = 0), with the L1 it reaches."""
+    from graphops_amd import _lib as L
+
+    nq = len(off) - 1
+    ctx = dg.ctx
+    ctx.set_option(L.OPT_TILE_F32_BELOW, 0.0)
+    ctx.set_option(L.OPT_PPR_F32_BELOW, 0.0)
+    try:
+        ga.ppr_batch_run(dg, cfg, off[:65], seeds[:off[64]], with_scores=False)  # warm-up
+        t0 = time.perf_counter()
+        _, runs = ga.ppr_batch_run(dg, cfg, off, seeds, with_scores=False)
+        dt = time.perf_counter() - t0
+        k = plain_sample
+        t0 = time.perf_counter()
+        ga.ppr_batch_run(dg, cfg, off[:k + 1], seeds[:off[k]], with_scores=False, tile=-1)
+        pdt = time.perf_counter() - t0
+        n_s = len(sample)
+        sc, _ = ga.ppr_batch_run(dg, cfg, off[:n_s + 1], seeds[:off[n_s]], with_scores=True)
+        l1 = max(float(np.abs(sc[i] - refs[i]).sum()) for i in range(n_s))
+        psc, _ = ga.ppr_batch_run(dg, cfg, off[:n_s + 1], seeds[:off[n_s]], with_scores=True,
+                                  tile=-1)
+        pl1 = max(float(np.abs(psc[i] - refs[i]).sum()) for i in range(n_s))
+    finally:
+        ctx.set_option(L.OPT_TILE_F32_BELOW, 1e4)
+        ctx.set_option(L.OPT_PPR_F32_BELOW, 1e4)
+    qps, plain_qps = nq / dt, k / pdt
+    log(f"ppr f32 schedule: {qps:.1f} q/s (L1 {l1:.2e}), plain {plain_qps:.2f} q/s (L1 {pl1:.2e})")
+    return {"schedule": "f32 sweeps after each query's f64 admission sweep",
+            "queries_per_s": round(qps, 2),
+            "iterations_mean": round(float(np.mean([r.iterations for r in runs])), 3),
+            "converged": int(sum(r.converged for r in runs)),
+            "l1_vs_f64_oracle_max": l1, "plain_queries_per_s": round(plain_qps, 3),
+            "plain_l1_vs_f64_oracle_max": pl1, "ratio_vs_plain": round(qps / plain_qps, 2)}
 
 
 def serving_leg(ga, generators, e, dg, nq=1024, qps_offered=5000.0, max_slots=512, sweeps=2,

@@ -30,6 +30,7 @@ OPT_PRECISION, OPT_ANCHOR_PERIOD, OPT_NORMALIZE, OPT_TIMING, OPT_PPR_TILE = 1, 2
 OPT_ANCHOR_DROP = 6
 OPT_F32_BELOW, OPT_PPR_F32_BELOW, OPT_PPR_ANCHOR_PERIOD = 7, 8, 9
 OPT_GRAPH_BUILD = 10
+OPT_TILE_F32_BELOW = 11
 BUILD_DEVICE, BUILD_HOST = 0, 1
 PRECISION_MIXED, PRECISION_FP64 = 0, 1
 GEN_UNIT, GEN_UNIFORM, GEN_LOGNORMAL = 0, 1, 2

@@ -459,8 +459,11 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
       return GOPS_OK;
     case GOPS_OPT_F32_BELOW:
     case GOPS_OPT_PPR_F32_BELOW:
+    case GOPS_OPT_TILE_F32_BELOW:
       if (!(value >= 0.0)) return fail(ctx, GOPS_INVALID_PARAMETER, "f32 threshold must be >= 0");
-      (option == GOPS_OPT_F32_BELOW ? ctx->f32_below : ctx->ppr_f32_below) = value;
+      (option == GOPS_OPT_F32_BELOW       ? ctx->f32_below
+       : option == GOPS_OPT_PPR_F32_BELOW ? ctx->ppr_f32_below
+                                          : ctx->tile_f32_below) = value;
       return GOPS_OK;
     case GOPS_OPT_PPR_ANCHOR_PERIOD:
       if (!(value >= 2 && value <= 1e6)) return fail(ctx, GOPS_INVALID_PARAMETER, "anchor period");

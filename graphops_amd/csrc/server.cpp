@@ -619,7 +619,7 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
         t->slot[s] = qq;
         t->used[s] = true;
         slot_q[s] = q;
-        set_slot_params(t, s, qq, ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
+        set_slot_params(t, s, qq, ctx->tile_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
         mask |= 1ull << s;
         break;
       }
@@ -871,7 +871,7 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
         t->slot[q] = std::move(s->queue[pick]);
         s->queue.erase(s->queue.begin() + (std::ptrdiff_t)pick);
         t->used[q] = true;
-        set_slot_params(t, q, t->slot[q], ctx->ppr_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
+        set_slot_params(t, q, t->slot[q], ctx->tile_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
         mask |= 1ull << q;
       }
     }

@@ -91,13 +91,20 @@ enum {
                                    anchor (default 1e-2) */
     GOPS_OPT_F32_BELOW = 7,     /* mixed, global PageRank: f64 sweeps while the residual
                                    is above this times the tolerance (default 0: none) */
-    GOPS_OPT_PPR_F32_BELOW = 8, /* mixed, personalized: same (default 1e4) */
+    GOPS_OPT_PPR_F32_BELOW = 8, /* mixed, personalized, one query per device run: same
+                                   (default 1e4) */
     GOPS_OPT_PPR_ANCHOR_PERIOD = 9, /* mixed, personalized: f32 sweeps between f64
                                    anchors while the residual is above 1e3 times
                                    the tolerance (default 64) */
-    GOPS_OPT_GRAPH_BUILD = 10   /* where gops_graph_create builds the CSR of in-edges:
+    GOPS_OPT_GRAPH_BUILD = 10,  /* where gops_graph_create builds the CSR of in-edges:
                                    GOPS_BUILD_DEVICE (default) or GOPS_BUILD_HOST; the
                                    arrays, and so every result, are the same */
+    GOPS_OPT_TILE_F32_BELOW = 11 /* mixed, batched and served personalized queries (tiles):
+                                   f64 sweeps while the residual is above this times the
+                                   tolerance (default 1e4, which keeps L1 within 1e-2 of
+                                   the tolerance of the f64 reference; 0: f32 from the
+                                   sweep after the admission's f64 sweep, faster, with an
+                                   L1 error of a few 1e-8 from f32 rounding) */
 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
 enum { GOPS_BUILD_DEVICE = 0, GOPS_BUILD_HOST = 1 };

@@ -72,6 +72,33 @@ def test_batch_matches_oracle(oracle, ctx, graph13, tile):
               cfgs[q].tolerance)
 
 
+def test_batch_f32_schedule(oracle, ctx, graph13):
+    """GOPS_OPT_TILE_F32_BELOW = 0: f32 sweeps right after each query's
+    admission sweep.  Iterations and stops as the oracle's; L1 within 1e-6 of
+    the f64 oracle (the accuracy the batched path is measured at)."""
+    from graphops_amd import _lib as L
+
+    e = graph13
+    nq = 24
+    off, seeds = generators.seed_sets(e, nq, 1, 64, seed=21)
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    cfgs = [ga.PageRankConfig([0.5, 0.85, 0.95][q % 3], 400, [1e-8, 1e-6, 1e-10][(q // 3) % 3])
+            for q in range(nq)]
+    ctx.set_option(L.OPT_TILE_F32_BELOW, 0.0)
+    try:
+        scores, runs = ga.ppr_batch_run(dg, cfgs, off, seeds, ctx=ctx, tile=16)
+    finally:
+        ctx.set_option(L.OPT_TILE_F32_BELOW, 1e4)
+    worst = 0.0
+    for q in range(nq):
+        p = dense_p(e.n, seeds[off[q]:off[q + 1]])
+        ref = oracle_ppr(oracle, e, p, cfgs[q].damping, cfgs[q].tolerance, 400)
+        assert runs[q].converged == ref.converged
+        assert abs(runs[q].iterations - ref.iterations) <= 1
+        worst = max(worst, np.abs(scores[q] - ref.scores).sum())
+    assert worst <= 1e-6, worst
+
+
 def test_batch_refill_same_as_alone(ctx, graph13):
     """More queries than slots: finished slots are refilled while the others
     run on; every query's result equals the same query run alone, bit for
