@@ -48,7 +48,7 @@ hipError_t upload(gops_context *ctx, T **p, const T *src, size_t count) {
   hipError_t e = dalloc(p, count);
   if (e != hipSuccess) return e;
   if (count == 0) return hipSuccess;
-  return hipMemcpyAsync(*p, src, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+  return staged_upload(ctx, *p, src, count * sizeof(T), ctx->stream);
 }
 
 // PageRankConfig::validate, pagerank.rs:63-85 (messages verbatim)
@@ -376,7 +376,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
     }
   }
   if (scores) {
-    HIP_TRY(ctx, hipMemcpy(scores, rb.out, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, staged_download(ctx, scores, rb.out, n * sizeof(double), ctx->stream));
   }
   return GOPS_OK;
 }
@@ -427,6 +427,7 @@ void gops_context_destroy(gops_context *ctx) {
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
   free_topk(ctx->topk);
   hipFree(ctx->staging);
+  free_pinned_chunks(ctx);
   hipHostFree(ctx->host_state);
   hipHostFree(ctx->mirror_host);
   hipStreamDestroy(ctx->stream);

@@ -365,9 +365,9 @@ gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, 
     unsigned long long *d_flags = pool.get<unsigned long long>(2);  // bad id + 1, first bad weight
     unsigned long long flags[2] = {0ull, ~0ull};
     DTRY(pool.err);
-    DTRY(hipMemcpyAsync(d_off, off, (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-    if (m) DTRY(hipMemcpyAsync(d_nb, nb, m * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    if (ew && m) DTRY(hipMemcpyAsync(d_ew, ew, m * sizeof(double), hipMemcpyHostToDevice, s));
+    DTRY(staged_upload(ctx, d_off, off, (n + 1) * sizeof(uint64_t), s));
+    if (m) DTRY(staged_upload(ctx, d_nb, nb, m * sizeof(uint32_t), s));
+    if (ew && m) DTRY(staged_upload(ctx, d_ew, ew, m * sizeof(double), s));
     DTRY(hipMemcpyAsync(d_flags, flags, sizeof(flags), hipMemcpyHostToDevice, s));
     bad_ids_kernel<<<grid_for(m), kB, 0, s>>>(d_nb, m, n, d_flags);
     DTRY(hipGetLastError());
@@ -484,11 +484,10 @@ gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, 
     h.m = m;
     h.row_ptr.resize(n + 1);
     std::vector<uint32_t> order(n), rank(n);
-    DTRY(hipMemcpyAsync(h.row_ptr.data(), d_row_ptr, (n + 1) * sizeof(int32_t),
-                        hipMemcpyDeviceToHost, s));
+    DTRY(staged_download(ctx, h.row_ptr.data(), d_row_ptr, (n + 1) * sizeof(int32_t), s));
     if (n) {
-      DTRY(hipMemcpyAsync(order.data(), d_order, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-      DTRY(hipMemcpyAsync(rank.data(), d_rank, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      DTRY(staged_download(ctx, order.data(), d_order, n * sizeof(uint32_t), s));
+      DTRY(staged_download(ctx, rank.data(), d_rank, n * sizeof(uint32_t), s));
     }
     DTRY(hipMemcpyAsync(flags, d_flags, sizeof(flags), hipMemcpyDeviceToHost, s));
     DTRY(hipStreamSynchronize(s));
@@ -618,9 +617,7 @@ gops_status build_forward(gops_context *ctx, gops_graph *g, std::string &err) {
     h.n = n;
     h.m = m;
     h.row_ptr.resize(n + 1);
-    DTRY(hipMemcpyAsync(h.row_ptr.data(), d_row_ptr, (n + 1) * sizeof(int32_t),
-                        hipMemcpyDeviceToHost, s));
-    DTRY(hipStreamSynchronize(s));
+    DTRY(staged_download(ctx, h.row_ptr.data(), d_row_ptr, (n + 1) * sizeof(int32_t), s));
     Plan plan;
     plan_blocks(h, plan);
     DevGraph f;

@@ -180,6 +180,10 @@ struct gops_graph {
   std::vector<uint32_t> rank_host;     // original -> new id
 };
 
+namespace gops {
+constexpr int kPinChunks = 3;  // pinned chunks of the staged host <-> device copies
+}
+
 struct gops_context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -205,6 +209,8 @@ struct gops_context {
   uint64_t *mirror_dev = nullptr;         // its device address
   std::vector<hipEvent_t> events;         // timing pool
   int num_cus = 256;
+  void *pin[gops::kPinChunks] = {};            // staged copies (staging.cpp), on first use
+  hipEvent_t pin_done[gops::kPinChunks] = {};  // each chunk's last transfer
 };
 
 struct gops_edges {
@@ -275,5 +281,15 @@ hipError_t launch_init(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize);
 hipError_t launch_gather_probe(gops_context *ctx, gops_graph *g);
+
+// staging.cpp: copies of pageable host memory through pinned chunks on the
+// host threads.  staged_upload returns once the source has been read (the
+// transfers complete in stream order on s); staged_download returns with the
+// data in dst.
+hipError_t staged_upload(gops_context *ctx, void *dst, const void *src, size_t bytes,
+                         hipStream_t s);
+hipError_t staged_download(gops_context *ctx, void *dst, const void *src, size_t bytes,
+                           hipStream_t s);
+void free_pinned_chunks(gops_context *ctx);
 
 }  // namespace gops
