@@ -718,8 +718,7 @@ gops_status gops_top_k(gops_context *ctx, const double *scores, uint64_t n, uint
     HIP_TRY(ctx, dalloc(&ctx->staging, n));
     ctx->staging_cap = n;
   }
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->staging, scores, n * sizeof(double), hipMemcpyHostToDevice,
-                              ctx->stream));
+  HIP_TRY(ctx, staged_upload(ctx, ctx->staging, scores, n * sizeof(double), ctx->stream));
   std::string err;
   const gops_status st = device_top_k(ctx, ctx->staging, n, k, ids, values, count, err);
   return st == GOPS_OK ? st : fail(ctx, st, err);

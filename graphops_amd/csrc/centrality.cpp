@@ -136,9 +136,7 @@ gops_status download(gops_context *ctx, gops_graph *g, const double *v, double *
   if (!out) return GOPS_OK;
   double *dst = g->pow.v[3];
   HIP_TRY(ctx, launch_permute(ctx, g, v, dst));
-  HIP_TRY(ctx, hipMemcpyAsync(out, dst, g->dev.n * sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, staged_download(ctx, out, dst, g->dev.n * sizeof(double), ctx->stream));
   return GOPS_OK;
 }
 

@@ -697,9 +697,7 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
       if (scores) {
         hipError_t e = ppr_launch_gather(ctx->stream, T, a, g->dev.rank, (int)n, s, dout);
         if (e == hipSuccess)
-          e = hipMemcpyAsync(scores + (size_t)q * n, dout, n * sizeof(double),
-                             hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+          e = staged_download(ctx, scores + (size_t)q * n, dout, n * sizeof(double), ctx->stream);
         if (e != hipSuccess) st = sfail(ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
       }
       t->used[s] = false;
@@ -989,7 +987,7 @@ gops_status gops_server_poll(gops_server *s, int *ready, gops_server_result *out
   const size_t n = s->g->dev.n;
   gops_status st = GOPS_OK;
   if (scores && d.scores) {
-    hipError_t e = hipMemcpy(scores, d.scores, n * sizeof(double), hipMemcpyDeviceToHost);
+    hipError_t e = staged_download(s->ctx, scores, d.scores, n * sizeof(double), s->ctx->stream);
     if (e != hipSuccess) st = sfail(s->ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
   }
   hipFree(d.scores);
