@@ -430,6 +430,8 @@ def main():
     ap.add_argument("--no-topk", action="store_true", help="skip the top_k leg")
     ap.add_argument("--no-gather-probe", action="store_true",
                     help="skip the gather-only probe of the sweep's access pattern")
+    ap.add_argument("--delta-sweep", default="pull", choices=["pull", "blocked"],
+                    help="how f32 delta sweeps read the graph (GOPS_OPT_DELTA_SWEEP)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -470,6 +472,10 @@ def main():
                           "centrality": centrality_leg(ga, e, dg)}), flush=True)
         return
     ctx = ga.Context(0, precision="mixed", timing=True)
+    from graphops_amd import _lib as L
+
+    ctx.set_option(L.OPT_DELTA_SWEEP, {"pull": L.DELTA_SWEEP_PULL,
+                                       "blocked": L.DELTA_SWEEP_BLOCKED}[args.delta_sweep])
     t0 = time.perf_counter()
     dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"], ctx=ctx)
     log(f"graph resident on the GPU in {time.perf_counter() - t0:.1f} s")

@@ -31,6 +31,8 @@ OPT_ANCHOR_DROP = 6
 OPT_F32_BELOW, OPT_PPR_F32_BELOW, OPT_PPR_ANCHOR_PERIOD = 7, 8, 9
 OPT_GRAPH_BUILD = 10
 OPT_TILE_F32_BELOW = 11
+OPT_DELTA_SWEEP = 12
+DELTA_SWEEP_PULL, DELTA_SWEEP_BLOCKED = 0, 1
 BUILD_DEVICE, BUILD_HOST = 0, 1
 PRECISION_MIXED, PRECISION_FP64 = 0, 1
 GEN_UNIT, GEN_UNIFORM, GEN_LOGNORMAL = 0, 1, 2

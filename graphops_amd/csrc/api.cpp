@@ -233,7 +233,14 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   gops_status st = alloc_run(ctx, g);
   if (st != GOPS_OK) return st;
   RunBuffers &rb = g->run;
-  SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0};
+  SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0, 0};
+  // f32 delta sweeps in the blocked layout, when chosen (built once per graph)
+  if (ctx->precision == GOPS_PRECISION_MIXED && ctx->delta_sweep == GOPS_DELTA_SWEEP_BLOCKED) {
+    std::string err;
+    st = build_blocked(ctx, g, err);
+    if (st != GOPS_OK) return fail(ctx, st, err);
+    setup.blocked = 1;
+  }
 
   if (kind == kPersonalized) {
     // ppr.rs:104-109: p / sum(p), or uniform when the sum is not > 0
@@ -480,6 +487,12 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
         return fail(ctx, GOPS_INVALID_PARAMETER, "graph build must be GOPS_BUILD_DEVICE or _HOST");
       ctx->graph_build = (int)value;
       return GOPS_OK;
+    case GOPS_OPT_DELTA_SWEEP:
+      if (value != GOPS_DELTA_SWEEP_PULL && value != GOPS_DELTA_SWEEP_BLOCKED)
+        return fail(ctx, GOPS_INVALID_PARAMETER,
+                    "delta sweep must be GOPS_DELTA_SWEEP_PULL or _BLOCKED");
+      ctx->delta_sweep = (int)value;
+      return GOPS_OK;
     default:
       return fail(ctx, GOPS_INVALID_PARAMETER, "unknown option");
   }
@@ -587,6 +600,7 @@ void gops_graph_destroy(gops_graph *g) {
   free_power(g->pow);
   free_dev(g->dev);
   free_dev(g->fwd);
+  free_blocked(g->pb);
   delete g;
 }
 

@@ -155,6 +155,38 @@ struct PowerBuffers {
   PowerState *state = nullptr;
 };
 
+// Layout of the blocked f32 delta sweep (blocked.hip, pagerank.hip).  The
+// sweep runs as two streaming passes instead of a gather pull:
+//   scatter     per chunk of 2^kPbChunkShift sources, whose f32 deltas sit in
+//               LDS: every out-edge's product goes to its place in contrib
+//   accumulate  per bin of 2^kPbBinShift rows, whose sums sit in LDS as
+//               fixed-point int64: contrib is read in order and added
+// Entries are the edges of the CSR of in-edges in two orders: chunk-major
+// (chunk of the source, row, source) for the scatter and bin-major (bin of
+// the row, chunk of the source, row) for the accumulate pass.
+constexpr int kPbChunkShift = 14;
+constexpr int kPbBinShift = 12;
+constexpr int kPbItem1 = 1 << 18;  // entries per scatter workgroup, at most
+constexpr int kPbItem2 = 1 << 18;  // entries per accumulate workgroup, at most
+
+struct PbItem {
+  int32_t x, y, z, w;
+};
+
+struct BlockedLayout {
+  uint16_t *src = nullptr;     // chunk-major: source id - chunk base
+  void *w = nullptr;           // chunk-major: weights in the graph's storage type (or null)
+  uint32_t *pos = nullptr;     // chunk-major: index of the entry in bin-major order
+  uint16_t *dst = nullptr;     // bin-major: row id - bin base
+  float *contrib = nullptr;    // bin-major: one sweep's products
+  PbItem *items1 = nullptr;    // scatter: {chunk, first entry, end entry, 0}
+  PbItem *items2 = nullptr;    // accumulate: {bin, first entry, end entry, slot or -1}
+  PbItem *splits = nullptr;    // bins of several items: {bin, first slot, slots, 0}
+  long long *bin_partial = nullptr;  // [slot][1 << kPbBinShift] fixed-point row sums
+  double *partials = nullptr;  // [items2 + splits][kMaxReduce] sums of the epilogues
+  int num_items1 = 0, num_items2 = 0, num_splits = 0;
+};
+
 constexpr uint64_t kSweepLogCap = 1 << 16;
 constexpr size_t kMaxReduce = 8;  // partial sums per workgroup, at most
 constexpr int kMaxSweepBlocks = 2048;  // grid of the sweep kernel (grid-stride over descriptors)
@@ -172,6 +204,8 @@ struct gops_graph {
   gops::DevGraph dev;
   gops::DevGraph fwd;      // out-edge CSR in new ids (row_ptr, col, plan only), built on demand
   bool has_fwd = false;
+  gops::BlockedLayout pb;  // blocked delta sweep, built on demand
+  bool has_pb = false;
   gops::RunBuffers run;
   gops::PowerBuffers pow;
   bool weighted = false;   // created with edge weights
@@ -200,6 +234,7 @@ struct gops_context {
   int timing = 0;
   int ppr_tile = 0;
   int graph_build = GOPS_BUILD_DEVICE;
+  int delta_sweep = GOPS_DELTA_SWEEP_PULL;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
   double *staging = nullptr;              // device copy of host scores for gops_top_k
@@ -276,11 +311,16 @@ void free_ppr_cache(gops_graph *g);
 struct SweepSetup {
   int weighted;           // use edge weights and inv_ws
   int personalized;       // q vector instead of uniform
+  int blocked;            // f32 delta sweeps in the blocked layout (g->pb)
 };
 hipError_t launch_init(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize);
 hipError_t launch_gather_probe(gops_context *ctx, gops_graph *g);
+
+// blocked.hip: the layout of the blocked delta sweep, built once per graph
+gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err);
+void free_blocked(BlockedLayout &pb);
 
 // staging.cpp: copies of pageable host memory through pinned chunks on the
 // host threads.  staged_upload returns once the source has been read (the

@@ -48,6 +48,9 @@ struct SweepArgs {
   int num_hubs;
   int n_hot;  // min(n, kStageNnz): sources whose f32 deltas a workgroup keeps in LDS
   RunState *st;
+  int pb_active;     // f32 delta sweeps run in the blocked layout (pb)
+  int n;             // rows
+  BlockedLayout pb;
 };
 
 constexpr int kReduce = 5;  // |delta|, dangling delta, dangling x, delta, x
@@ -141,21 +144,37 @@ struct DeltaEpilogue {
     ddsum = d * (st->dsum_in - st->mass_in);
     uni = ddsum * st->inv_n;
   }
-  __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kReduce]) const {
-    const double delta = acc + (a.q32 ? ddsum * (double)a.q32[r] : uni);
-    const float inv = __builtin_nontemporal_load(a.inv32 + r);
-    const double x = __builtin_nontemporal_load(a.xbar + r) + delta;
+  // the row's operands, loaded apart from the update so that callers can
+  // have several rows' loads in flight
+  struct Ops {
+    double q, x, inv64;
+    float inv;
+  };
+  __device__ __forceinline__ Ops load(int r) const {
+    Ops o;
+    o.q = a.q32 ? (double)a.q32[r] : 0.0;
+    o.inv = __builtin_nontemporal_load(a.inv32 + r);
+    o.x = __builtin_nontemporal_load(a.xbar + r);
+    o.inv64 = fold ? a.inv64[r] : 0.0;
+    return o;
+  }
+  __device__ __forceinline__ void apply(int r, double acc, const Ops &o,
+                                        double (&red)[kReduce]) const {
+    const double delta = acc + (a.q32 ? ddsum * o.q : uni);
+    const double x = o.x + delta;
     __builtin_nontemporal_store(x, a.xbar + r);
-    a.yout32[r] = (float)(d * delta * (double)inv);
+    a.yout32[r] = (float)(d * delta * (double)o.inv);
     red[0] += fabs(delta);
     red[3] += delta;
-    if (inv == 0.0f) red[1] += delta;
+    if (o.inv == 0.0f) red[1] += delta;
     if (fold) {
-      const double inv64 = a.inv64[r];
-      a.yout64[r] = (d * x) * inv64;
-      if (inv64 == 0.0) red[2] += x;
+      a.yout64[r] = (d * x) * o.inv64;
+      if (o.inv64 == 0.0) red[2] += x;
       red[4] += x;
     }
+  }
+  __device__ __forceinline__ void operator()(int r, double acc, double (&red)[kReduce]) const {
+    apply(r, acc, load(r), red);
   }
 };
 
@@ -250,6 +269,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, 
                                                        double *y640, double *y641) {
   const RunState *st = a.st;
   if (st->done) return;
+  if (a.pb_active && st->mode != kModeAnchor) return;  // the blocked kernels do this sweep
   pick_buffers(a, st, yd0, yd1, y640, y641);
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
   __shared__ double stage[kStageNnz];  // a run's products (f32 or f64)
@@ -296,17 +316,214 @@ __global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float
       epi(a.hub_row[h], s, red);
     }
   };
-  if (mode == kModeAnchor) {
-    hubs(AnchorEpilogue(a, st));
-  } else {
-    hubs(DeltaEpilogue(a, st, mode == kModeDeltaFold));
-  }
-  for (int b = threadIdx.x; b < sweep_blocks; b += kBlock) {
+  if (a.pb_active && mode != kModeAnchor) {
+    // a blocked sweep: hub rows were summed in their bins
+    const int parts = a.pb.num_items2 + a.pb.num_splits;
+    for (int b = threadIdx.x; b < parts; b += kBlock) {
 #pragma unroll
-    for (int i = 0; i < kReduce; i++) red[i] += a.partials[(size_t)b * kReduce + i];
+      for (int i = 0; i < kReduce; i++) red[i] += a.pb.partials[(size_t)b * kReduce + i];
+    }
+  } else {
+    if (mode == kModeAnchor) {
+      hubs(AnchorEpilogue(a, st));
+    } else {
+      hubs(DeltaEpilogue(a, st, mode == kModeDeltaFold));
+    }
+    for (int b = threadIdx.x; b < sweep_blocks; b += kBlock) {
+#pragma unroll
+      for (int i = 0; i < kReduce; i++) red[i] += a.partials[(size_t)b * kReduce + i];
+    }
   }
   block_sum<kReduce>(red);
   if (threadIdx.x == 0) finish_sweep(st, mode, red);
+}
+
+// ---------------------------------------------------------------------------
+// Blocked f32 delta sweeps (layout: blocked.hip).  sweep_kernel returns at
+// once for them; the scatter and accumulate kernels (and the combine kernel
+// for bins split over several workgroups) do the sweep, and the finish kernel
+// reduces their partial sums.  Anchor sweeps stay pulls.
+//
+// Row sums are fixed-point int64, so that the order in which workgroups and
+// lanes add does not change a bit of the result.  The products of a delta
+// sweep add up, over all rows, to at most d * sum|delta| <= the previous
+// sweep's residual (a source's out-weights sum to its normaliser), so with
+// the scale 2^(61 - e), residual < 2^e, no row sum or partial sum reaches
+// 2^62.  One unit is 2^-62 of that residual: far below the f32 rounding of
+// the products (2^-24 of each).
+__device__ __forceinline__ int pb_exponent(const RunState *st) {
+  int e = 0;
+  const double diff = st->diff;
+  if (diff > 0.0 && diff <= 1.7e308) (void)frexp(diff, &e);
+  const int k = 61 - e;
+  return k < -1000 ? -1000 : (k > 1000 ? 1000 : k);
+}
+
+// Scatter: one chunk of sources (their f32 deltas in LDS); every edge of the
+// chunk, in chunk-major order, writes its product to its bin-major place.
+// Consecutive lanes take consecutive entries, so that the writes of a run
+// (entries of one chunk into one bin) are contiguous.
+template <typename WT>
+__global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *yd0, float *yd1) {
+  const RunState *st = a.st;
+  if (st->done || st->mode == kModeAnchor) return;
+  const float *yin = st->yd_cur ? yd1 : yd0;
+  constexpr int kChunk = 1 << kPbChunkShift;
+  __shared__ float z[kChunk];
+  const PbItem it = a.pb.items1[blockIdx.x];
+  const int base = it.x << kPbChunkShift;
+  const int len = min(kChunk, a.n - base);
+  if (len == kChunk && (base & 3) == 0) {
+    // all of the chunk's loads in flight at once (16-byte loads)
+    constexpr int V = kChunk / 4 / kBlock;
+    float4 v[V];
+#pragma unroll
+    for (int j = 0; j < V; j++)
+      v[j] = *reinterpret_cast<const float4 *>(yin + base + 4 * ((int)threadIdx.x + j * kBlock));
+#pragma unroll
+    for (int j = 0; j < V; j++)
+      *reinterpret_cast<float4 *>(z + 4 * ((int)threadIdx.x + j * kBlock)) = v[j];
+  } else {
+    for (int i = threadIdx.x; i < len; i += kBlock) z[i] = yin[base + i];
+  }
+  __syncthreads();
+  constexpr int U = 16;
+  for (int e = it.y + (int)threadIdx.x; e < it.z; e += U * kBlock) {
+    int sl[U];
+    uint32_t p[U];
+    float w[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = e + u * kBlock;
+      const int kk = k < it.z ? k : it.y;
+      sl[u] = __builtin_nontemporal_load(a.pb.src + kk);
+      p[u] = __builtin_nontemporal_load(a.pb.pos + kk);
+      w[u] = wload32<WT>(a.pb.w, kk);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (e + u * kBlock < it.z) a.pb.contrib[p[u]] = w[u] * z[sl[u]];
+    }
+  }
+}
+
+// c * 2^k as a fixed-point integer, truncated toward zero: the f32's
+// mantissa shifted by its exponent (integer operations only).  |c| * 2^k
+// stays below 2^61 (the scale's bound); truncation costs at most one unit,
+// 2^-62 of the previous residual, per product.
+__device__ __forceinline__ long long pb_fixed(float c, int k) {
+  const uint32_t b = __float_as_uint(c);
+  const int ex = (int)((b >> 23) & 0xffu);
+  if (ex == 0) return 0;  // zero or subnormal: below one unit at any scale used
+  const uint64_t mant = (b & 0x7fffffu) | 0x800000u;
+  const int sh = ex - 150 + k;  // c = mant * 2^(ex - 150)
+  uint64_t v;
+  if (sh >= 0) v = sh < 40 ? mant << sh : 0;  // sh >= 40 only for values beyond the bound
+  else v = sh > -24 ? mant >> -sh : 0;
+  return (b >> 31) ? -(long long)v : (long long)v;
+}
+
+// Adds one value per lane of a wave to acc[key] (key < 0: no value).  A wave
+// whose lanes all hold one row (a long row's run of entries) sums in
+// registers and adds once; otherwise one LDS atomic per lane.
+__device__ __forceinline__ void pb_add(long long v, int key, unsigned long long *acc) {
+  const int k0 = __builtin_amdgcn_readfirstlane(key);
+  if (__all(key == k0)) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&acc[k0], (unsigned long long)v);
+  } else if (key >= 0) {
+    atomicAdd(&acc[key], (unsigned long long)v);
+  }
+}
+
+// Accumulate: one bin of rows (or a part of its entries), fixed-point sums in
+// LDS; then the rows' epilogues, or the part's sums for the combine kernel.
+__global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, float *yd0, float *yd1,
+                                                               double *y640, double *y641) {
+  const RunState *st = a.st;
+  if (st->done || st->mode == kModeAnchor) return;
+  pick_buffers(a, st, yd0, yd1, y640, y641);
+  constexpr int kRows = 1 << kPbBinShift;
+  __shared__ unsigned long long acc[kRows];
+  const PbItem it = a.pb.items2[blockIdx.x];
+  for (int i = threadIdx.x; i < kRows; i += kBlock) acc[i] = 0ull;
+  __syncthreads();
+  const int k = pb_exponent(st);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int U = 16;
+  for (int f0 = it.y; f0 < it.z; f0 += U * kBlock) {
+    float cv[U];
+    int key[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int f = f0 + u * kBlock + wave * 64 + lane;  // 64 consecutive entries per wave
+      const bool ok = f < it.z;
+      const int ff = ok ? f : it.y;
+      cv[u] = __builtin_nontemporal_load(a.pb.contrib + ff);
+      const int d = __builtin_nontemporal_load(a.pb.dst + ff);
+      key[u] = ok ? d : -1 - lane;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) pb_add(pb_fixed(cv[u], k), key[u], acc);
+  }
+  __syncthreads();
+  double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int row0 = it.x << kPbBinShift;
+  const int rows = min(kRows, a.n - row0);
+  if (it.w < 0) {
+    const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
+    const double inv_scale = ldexp(1.0, -k);
+    constexpr int R = 4;  // rows per thread with their loads in flight together
+    for (int i0 = threadIdx.x; i0 < rows; i0 += R * kBlock) {
+      DeltaEpilogue::Ops o[R];
+#pragma unroll
+      for (int u = 0; u < R; u++) {
+        const int i = i0 + u * kBlock;
+        o[u] = epi.load(row0 + (i < rows ? i : i0));
+      }
+#pragma unroll
+      for (int u = 0; u < R; u++) {
+        const int i = i0 + u * kBlock;
+        if (i < rows) epi.apply(row0 + i, (double)(long long)acc[i] * inv_scale, o[u], red);
+      }
+    }
+  } else {
+    long long *out = a.pb.bin_partial + ((size_t)it.w << kPbBinShift);
+    for (int i = threadIdx.x; i < rows; i += kBlock) out[i] = (long long)acc[i];
+  }
+  __syncthreads();
+  block_sum<kReduce>(red, reinterpret_cast<double *>(acc));
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kReduce; i++) a.pb.partials[(size_t)blockIdx.x * kReduce + i] = red[i];
+  }
+}
+
+// Combine: kBlock rows of a bin whose entries were split over several
+// accumulate workgroups: the parts' sums in slot order, then the epilogues.
+__global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *yd0, float *yd1,
+                                                            double *y640, double *y641) {
+  const RunState *st = a.st;
+  if (st->done || st->mode == kModeAnchor) return;
+  pick_buffers(a, st, yd0, yd1, y640, y641);
+  const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
+  const double inv_scale = ldexp(1.0, -pb_exponent(st));
+  const int row0 = (sp.x << kPbBinShift) + sp.w;
+  const int i = sp.w + (int)threadIdx.x;  // row within the bin
+  double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < (1 << kPbBinShift) && row0 + (int)threadIdx.x < a.n) {
+    long long s = 0;
+    for (int j = 0; j < sp.z; j++) s += a.pb.bin_partial[((size_t)(sp.y + j) << kPbBinShift) + i];
+    const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
+    epi(row0 + (int)threadIdx.x, (double)s * inv_scale, red);
+  }
+  block_sum<kReduce>(red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < kReduce; q++)
+      a.pb.partials[(size_t)(a.pb.num_items2 + blockIdx.x) * kReduce + q] = red[q];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -408,6 +625,9 @@ static SweepArgs make_args(gops_graph *g, const SweepSetup &s) {
   a.num_hubs = dg.num_hubs;
   a.n_hot = (int)std::min<uint64_t>(dg.n, kStageNnz);
   a.st = rb.state;
+  a.pb_active = s.blocked && g->has_pb;
+  a.n = (int)dg.n;
+  if (a.pb_active) a.pb = g->pb;
   return a;
 }
 
@@ -435,6 +655,15 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
   RunBuffers &rb = g->run;
   const int grid = sweep_grid(g);
   sweep_kernel<WT><<<grid, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1]);
+  if (a.pb_active) {
+    if (a.pb.num_items1)
+      pb_scatter_kernel<WT><<<a.pb.num_items1, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1]);
+    pb_accumulate_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
+                                                                      rb.y64[0], rb.y64[1]);
+    if (a.pb.num_splits)
+      pb_combine_kernel<<<a.pb.num_splits, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
+                                                                     rb.y64[0], rb.y64[1]);
+  }
   sweep_finish_kernel<<<1, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1],
                                                      grid);
 }
@@ -454,7 +683,7 @@ hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s) {
 }
 
 hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize) {
-  SweepSetup s{0, 0};
+  SweepSetup s{0, 0, 0};
   SweepArgs a = make_args(g, s);
   const int n = (int)g->dev.n;
   const int grid = elementwise_grid(n);

@@ -99,13 +99,21 @@ enum {
     GOPS_OPT_GRAPH_BUILD = 10,  /* where gops_graph_create builds the CSR of in-edges:
                                    GOPS_BUILD_DEVICE (default) or GOPS_BUILD_HOST; the
                                    arrays, and so every result, are the same */
-    GOPS_OPT_TILE_F32_BELOW = 11 /* mixed, batched and served personalized queries (tiles):
+    GOPS_OPT_TILE_F32_BELOW = 11, /* mixed, batched and served personalized queries (tiles):
                                    f64 sweeps while the residual is above this times the
                                    tolerance (default 1e4, which keeps L1 within 1e-2 of
                                    the tolerance of the f64 reference; 0: f32 from the
                                    sweep after the admission's f64 sweep, faster, with an
                                    L1 error of a few 1e-8 from f32 rounding) */
+    GOPS_OPT_DELTA_SWEEP = 12   /* mixed, one vector per run: how f32 delta sweeps read
+                                   the graph: GOPS_DELTA_SWEEP_PULL (default; gathers over
+                                   the CSR of in-edges) or GOPS_DELTA_SWEEP_BLOCKED (a
+                                   scatter pass per chunk of sources and an accumulate
+                                   pass per bin of rows, both streaming, integer row
+                                   sums; its layout is built on the first run; slower
+                                   than the pull on R-MAT scale 24 so far) */
 };
+enum { GOPS_DELTA_SWEEP_PULL = 0, GOPS_DELTA_SWEEP_BLOCKED = 1 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
 enum { GOPS_BUILD_DEVICE = 0, GOPS_BUILD_HOST = 1 };
 

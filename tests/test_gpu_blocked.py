@@ -1,0 +1,169 @@
+"""GPU parity of the blocked f32 delta sweep (GOPS_OPT_DELTA_SWEEP =
+GOPS_DELTA_SWEEP_BLOCKED): a scatter pass per chunk of sources and an
+accumulate pass per bin of rows, fixed-point row sums.
+
+  * against the f64 oracle with the bounds of the pull sweep: L1 within
+    1e-2 * tolerance + 1e-12 for converged runs, iterations equal (or one
+    apart when a residual sits within 1e-4 of the tolerance);
+  * against the pull sweep on the same graph: the same iterations, L1 within
+    2e-2 * tolerance + 1e-12 (both are that close to the oracle);
+  * two runs give the same bits (integer sums: no dependence on scheduling);
+  * layouts that exercise every part: bins split over several workgroups
+    (a row of 140 000 in-edges), several chunks and bins with n not a multiple
+    of either, empty bins and rows, dangling nodes, every weight storage,
+    personalization, and graphs without edges.
+"""
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+import graphops_amd as ga
+from graphops_amd import _lib as L
+from graphops_amd import generators
+
+pytestmark = pytest.mark.gpu
+
+
+def make_ctx(mode):
+    c = ga.Context(0, precision="mixed")
+    c.set_option(L.OPT_DELTA_SWEEP, mode)
+    return c
+
+
+@pytest.fixture(scope="module")
+def ctx_b():
+    return make_ctx(L.DELTA_SWEEP_BLOCKED)
+
+
+@pytest.fixture(scope="module")
+def ctx_p():
+    return make_ctx(L.DELTA_SWEEP_PULL)
+
+
+def oracle_pull(oracle, e, kind, tol, max_it, p=None):
+    in_off, in_src, in_w = oracle.transpose(e.offsets, e.targets, e.weights if kind == 1 else None)
+    if kind == 1:
+        scale = oracle.out_wsum(e.offsets, e.weights)
+    else:
+        scale = np.diff(e.offsets).astype(np.float64)
+    return oracle.pull_run(kind, in_off, in_src, scale, w=in_w, p=p, damping=0.85,
+                           max_iterations=max_it, tolerance=tol, threads=0)
+
+
+def check(run, ref, tol):
+    noise = 1e-15 + 4e-18 * ref.scores.size
+    l1 = np.abs(run.scores - ref.scores).sum()
+    assert run.converged == ref.converged
+    assert l1 <= (1e-2 * tol + 1e-12 if ref.converged else 1e-7), l1
+    if run.iterations != ref.iterations:
+        assert abs(run.iterations - ref.iterations) == 1
+        assert (abs(ref.diff_l1 - tol) <= 1e-4 * tol + noise
+                or abs(run.diff_l1 - tol) <= 1e-4 * tol + noise)
+    else:
+        assert abs(run.diff_l1 - ref.diff_l1) <= 1e-4 * ref.diff_l1 + noise
+
+
+def runs(ctx_b, ctx_p, e, kind, storage, cfg, p=None):
+    out = []
+    for cx in (ctx_b, ctx_p, ctx_b):
+        dg = ga.DeviceGraph.from_edges(e, weight_storage=storage, ctx=cx)
+        if kind == 2:
+            out.append(ga.personalized_pagerank_run(dg, cfg, p, ctx=cx))
+        elif kind == 1:
+            out.append(ga.pagerank_weighted_run(dg, cfg, ctx=cx))
+        else:
+            out.append(ga.pagerank_run(dg, cfg, ctx=cx))
+        dg.close()
+    b, pl, b2 = out
+    assert np.array_equal(b.scores, b2.scores) and b.diff_l1 == b2.diff_l1  # deterministic
+    assert b.iterations == pl.iterations
+    assert np.abs(b.scores - pl.scores).sum() <= 2e-2 * cfg.tolerance + 1e-12
+    return b
+
+
+@pytest.mark.parametrize("weights,storage", [("unit", "f64"), ("uniform", "f32"),
+                                             ("lognormal", "bf16"), ("uniform", "f64")])
+def test_rmat_weights(oracle, ctx_b, ctx_p, weights, storage):
+    """R-MAT scale 16 (n = 65 536: 4 chunks, 16 bins)."""
+    if weights == "unit":
+        e = generators.rmat(16, 16, seed=7)
+    else:
+        e = generators.rmat(16, 16, seed=7, weights=weights,
+                            p0=0.1 if weights == "uniform" else 0.0,
+                            p1=1.0, rounding={"f32": "f32", "bf16": "bf16", "f64": "f64"}[storage])
+    kind = 0 if weights == "unit" else 1
+    tol = 1e-10
+    ref = oracle_pull(oracle, e, kind, tol, 300)
+    b = runs(ctx_b, ctx_p, e, kind, storage, ga.PageRankConfig(0.85, 300, tol))
+    check(b, ref, tol)
+
+
+def test_split_bins_and_ragged_sizes(oracle, ctx_b, ctx_p):
+    """n = 70 001 (not a multiple of a chunk or a bin); node 5 has 140 000
+    in-edges (its bin is split over several accumulate workgroups), node 9
+    has 70 000 out-edges, many rows and a whole bin without in-edges,
+    dangling nodes."""
+    n = 70001
+    rng = np.random.default_rng(4)
+    src = [np.arange(n, dtype=np.int64) % 60000, np.full(70000, 9),
+           rng.integers(0, 60000, size=140000)]
+    dst = [rng.integers(0, 50000, size=n), rng.integers(0, 60000, size=70000),
+           np.full(140000, 5)]
+    src = np.concatenate(src)
+    dst = np.concatenate(dst)
+    dst[(dst >= 16384) & (dst < 20480)] = 3  # rows of one bin without in-edges
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    nb = dst.astype(np.uint32)
+    w = rng.uniform(0.1, 1.0, size=nb.size).astype(np.float32).astype(np.float64)  # f32 storage
+    e = SimpleNamespace(n=n, m=nb.size, offsets=off, targets=nb, weights=w)
+    for kind, storage in ((0, "f64"), (1, "f32")):
+        tol = 1e-10
+        ref = oracle_pull(oracle, e, kind, tol, 300)
+        b = runs(ctx_b, ctx_p, e, kind, storage, ga.PageRankConfig(0.85, 300, tol))
+        check(b, ref, tol)
+
+
+def test_personalized(oracle, ctx_b, ctx_p):
+    e = generators.rmat(15, 16, seed=13)
+    rng = np.random.default_rng(8)
+    p = np.zeros(e.n)
+    p[rng.choice(e.n, size=40, replace=False)] = rng.random(40)
+    tol = 1e-9
+    ref = oracle_pull(oracle, e, 2, tol, 300, p=p)
+    b = runs(ctx_b, ctx_p, e, 2, "f64", ga.PageRankConfig(0.85, 300, tol), p=p)
+    check(b, ref, tol)
+
+
+def test_small_and_empty(oracle, ctx_b):
+    """Graphs smaller than one bin, with no edges, and a single node."""
+    for n, pairs in ((3, [(0, 1), (1, 2), (2, 0), (0, 2)]), (5, []), (1, [(0, 0)]),
+                     (4097, [(i, (i * 7) % 4097) for i in range(4097)])):
+        off = np.zeros(n + 1, np.uint64)
+        for u, _ in pairs:
+            off[u + 1] += 1
+        off = np.cumsum(off).astype(np.uint64)
+        nb = np.array([v for _, v in sorted(pairs)], dtype=np.uint32)
+        dg = ga.DeviceGraph.from_lists(n, off, nb, ctx=ctx_b)
+        r = ga.pagerank_run(dg, ga.PageRankConfig(0.85, 200, 1e-12), ctx=ctx_b)
+        ref = oracle.pagerank_run(off, nb, 0.85, 200, 1e-12)
+        assert r.converged == ref.converged
+        assert np.abs(r.scores - ref.scores).sum() <= 1e-2 * 1e-12 + 1e-12, (n, r.scores)
+        dg.close()
+
+
+def test_default_is_the_pull(ctx_p):
+    """Without the option, f32 delta sweeps are pulls: the same bits as a
+    context that chose the pull; unknown values are refused."""
+    c = ga.Context(0, precision="mixed")
+    e = generators.rmat(14, 16, seed=3)
+    cfg = ga.PageRankConfig(0.85, 200, 1e-9)
+    a = ga.pagerank_run(ga.DeviceGraph.from_edges(e, ctx=c), cfg, ctx=c)
+    pl = ga.pagerank_run(ga.DeviceGraph.from_edges(e, ctx=ctx_p), cfg, ctx=ctx_p)
+    assert a.iterations == pl.iterations and np.array_equal(a.scores, pl.scores)
+    with pytest.raises(ga.InvalidParameter):
+        c.set_option(L.OPT_DELTA_SWEEP, 7)
