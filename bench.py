@@ -8,9 +8,9 @@ f64 anchor sweeps, f64 residual / dangling sums.  One step = one complete
 pagerank_weighted_run with the graph already resident in HBM (results left
 on the device).  value = edges * sweeps / seconds (GTEPS per iteration).
 
-Also reported: the roofline of the dominant kernel (the sweep kernel over
-all its sweeps, algorithmic bytes (4 + 2) * m + 16 * n + 4 per sweep against
-8 TB/s, durations from HIP events on the library's stream), the error against
+Also reported: the roofline of the dominant kernel (the f32 delta sweep:
+algorithmic bytes (4 + 2) * m + 16 * n + 4 per sweep against 8 TB/s,
+durations from HIP events on the library's stream around its launches), the error against
 the f64 oracle, the oracle's single-core CPU baseline on a bounded sample of
 sweeps, and two more legs on an R-MAT scale-22 graph: batched personalized
 PageRank (1024 queries) against the plain one-query-per-run path, and a
@@ -397,16 +397,19 @@ def gather_probe(dg, reps=5):
     return ms.value
 
 
-def measured_traffic(workload):
-    """HBM bytes per sweep launch of this workload from the committed
-    counter summary (FETCH_SIZE + WRITE_SIZE scaled by a copy of known
-    size), or None."""
+def measured_traffic(workload, delta_sweep):
+    """HBM bytes per f32 delta sweep of this workload and sweep kind from the
+    committed counter summaries (FETCH_SIZE + WRITE_SIZE scaled by a copy of
+    known size), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
-            t = json.load(f)
+            records = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get("bytes_per_launch") if t.get("workload") == workload else None
+    for t in records if isinstance(records, list) else [records]:
+        if t.get("workload") == workload and t.get("delta_sweep", "pull") == delta_sweep:
+            return t.get("bytes_per_launch")
+    return None
 
 
 def main():
@@ -430,7 +433,7 @@ def main():
     ap.add_argument("--no-topk", action="store_true", help="skip the top_k leg")
     ap.add_argument("--no-gather-probe", action="store_true",
                     help="skip the gather-only probe of the sweep's access pattern")
-    ap.add_argument("--delta-sweep", default="pull", choices=["pull", "blocked"],
+    ap.add_argument("--delta-sweep", default="auto", choices=["auto", "pull", "blocked"],
                     help="how f32 delta sweeps read the graph (GOPS_OPT_DELTA_SWEEP)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     args = ap.parse_args()
@@ -474,7 +477,7 @@ def main():
     ctx = ga.Context(0, precision="mixed", timing=True)
     from graphops_amd import _lib as L
 
-    ctx.set_option(L.OPT_DELTA_SWEEP, {"pull": L.DELTA_SWEEP_PULL,
+    ctx.set_option(L.OPT_DELTA_SWEEP, {"auto": L.DELTA_SWEEP_AUTO, "pull": L.DELTA_SWEEP_PULL,
                                        "blocked": L.DELTA_SWEEP_BLOCKED}[args.delta_sweep])
     t0 = time.perf_counter()
     dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"], ctx=ctx)
@@ -504,10 +507,14 @@ def main():
     alg_bytes = (4 + c["w_bytes"]) * m + 16 * n + 4
     avg_f32 = f32_ms / max(n32, 1) / 1e3
     avg_f64 = f64_ms / max(n64, 1) / 1e3
-    # every sweep (f32 delta and f64 anchor) is a launch of the same kernel
     avg_sweep = (f32_ms + f64_ms) / max(n32 + n64, 1) / 1e3
-    achieved = alg_bytes / avg_sweep / 1e9 if n32 + n64 else None
+    # the dominant kernel: the f32 delta sweep (HIP events around its launches)
+    achieved = alg_bytes / avg_f32 / 1e9 if n32 else None
     wname = "Bf16W" if c["w_bytes"] == 2 else "float"
+    blocked = args.delta_sweep == "blocked" or (args.delta_sweep == "auto" and n >= (1 << 24))
+    sweep_name = (f"gops::pb_scatter_kernel<gops::{wname}> + gops::pb_accumulate_kernel + "
+                  "gops::pb_combine_kernel (blocked f32 delta sweep)" if blocked
+                  else f"gops::sweep_kernel<gops::{wname}> (f32 delta sweep)")
     out = {
         "metric": "PageRank GTEPS/iter (% HBM roofline) + batched PPR queries/s @ L1<=1e-6",
         "value": round(value, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
@@ -519,8 +526,9 @@ def main():
                    "f32_sweeps_per_run": st["sweeps_fp32"], "f64_sweeps_per_run": st["sweeps_fp64"],
                    "converged": r.converged, "diff_l1": r.diff_l1,
                    "precision": f"{'bf16' if c['w_bytes'] == 2 else 'f32'} weights, f32 delta "
-                                "sweeps with f64 sums, f64 anchor sweeps, f64 iterate, residual "
-                                "and dangling sums",
+                                "sweeps with f64 (pull) or fixed-point int64 (blocked) row sums, "
+                                "f64 anchor sweeps, f64 iterate, residual and dangling sums",
+                   "delta_sweep": "blocked" if blocked else "pull",
                    "sweep_us": round(avg_sweep * 1e6, 2),
                    "f32_sweep_us": round(avg_f32 * 1e6, 2), "f64_sweep_us": round(avg_f64 * 1e6, 2),
                    "gteps_f32_sweep": round(m / avg_f32 / 1e9, 2) if n32 else None},
@@ -528,18 +536,20 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": measured_traffic(c["workload"]),
-                     "kernel": f"gops::sweep_kernel<gops::{wname}>",
+                     "traffic": measured_traffic(c["workload"], "blocked" if blocked else "pull"),
+                     "kernel": sweep_name, "delta_sweep": "blocked" if blocked else "pull",
                      "alg_bytes_per_launch": alg_bytes},
     }
     if not args.no_gather_probe:
         pm = gather_probe(dg)
         out["gather_bound"] = {
-            "probe": "gops_gather_probe: the f32 sweep's gathers alone (source values in "
+            "probe": "gops_gather_probe: the pull sweep's gathers alone (source values in "
                      "stored order, the col stream, a sum)",
             "probe_ms": round(pm, 3), "gathers_per_s": round(m / pm / 1e6, 1),
             "f32_sweep_ms": round(avg_f32 * 1e3, 3),
-            "frac": round(pm / (avg_f32 * 1e3), 3) if n32 else None}
+            "f32_sweep": "blocked" if blocked else "pull",
+            # the probe bounds the pull; a blocked sweep does not gather
+            "frac": round(pm / (avg_f32 * 1e3), 3) if n32 and not blocked else None}
         log(f"gather probe: {pm:.3f} ms for {m} gathers; f32 sweep {avg_f32 * 1e3:.3f} ms")
     if not args.no_accuracy:
         scores = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores

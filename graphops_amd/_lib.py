@@ -32,7 +32,7 @@ OPT_F32_BELOW, OPT_PPR_F32_BELOW, OPT_PPR_ANCHOR_PERIOD = 7, 8, 9
 OPT_GRAPH_BUILD = 10
 OPT_TILE_F32_BELOW = 11
 OPT_DELTA_SWEEP = 12
-DELTA_SWEEP_PULL, DELTA_SWEEP_BLOCKED = 0, 1
+DELTA_SWEEP_AUTO, DELTA_SWEEP_PULL, DELTA_SWEEP_BLOCKED = 0, 1, 2
 BUILD_DEVICE, BUILD_HOST = 0, 1
 PRECISION_MIXED, PRECISION_FP64 = 0, 1
 GEN_UNIT, GEN_UNIFORM, GEN_LOGNORMAL = 0, 1, 2

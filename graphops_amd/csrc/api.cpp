@@ -234,8 +234,12 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   if (st != GOPS_OK) return st;
   RunBuffers &rb = g->run;
   SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0, 0};
-  // f32 delta sweeps in the blocked layout, when chosen (built once per graph)
-  if (ctx->precision == GOPS_PRECISION_MIXED && ctx->delta_sweep == GOPS_DELTA_SWEEP_BLOCKED) {
+  // f32 delta sweeps in the blocked layout (built once per graph): when chosen,
+  // or by default from 2^24 nodes, where the pull's source vector (64 MB of
+  // f32) is far beyond the L2 and the blocked passes measured faster
+  if (ctx->precision == GOPS_PRECISION_MIXED &&
+      (ctx->delta_sweep == GOPS_DELTA_SWEEP_BLOCKED ||
+       (ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO && n >= kPbAutoNodes))) {
     std::string err;
     st = build_blocked(ctx, g, err);
     if (st != GOPS_OK) return fail(ctx, st, err);
@@ -488,9 +492,10 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
       ctx->graph_build = (int)value;
       return GOPS_OK;
     case GOPS_OPT_DELTA_SWEEP:
-      if (value != GOPS_DELTA_SWEEP_PULL && value != GOPS_DELTA_SWEEP_BLOCKED)
+      if (value != GOPS_DELTA_SWEEP_AUTO && value != GOPS_DELTA_SWEEP_PULL &&
+          value != GOPS_DELTA_SWEEP_BLOCKED)
         return fail(ctx, GOPS_INVALID_PARAMETER,
-                    "delta sweep must be GOPS_DELTA_SWEEP_PULL or _BLOCKED");
+                    "delta sweep must be GOPS_DELTA_SWEEP_AUTO, _PULL or _BLOCKED");
       ctx->delta_sweep = (int)value;
       return GOPS_OK;
     default:

@@ -168,6 +168,7 @@ constexpr int kPbChunkShift = 14;
 constexpr int kPbBinShift = 12;
 constexpr int kPbItem1 = 1 << 18;  // entries per scatter workgroup, at most
 constexpr int kPbItem2 = 1 << 18;  // entries per accumulate workgroup, at most
+constexpr uint64_t kPbAutoNodes = 1ull << 24;  // GOPS_DELTA_SWEEP_AUTO: blocked from here
 
 struct PbItem {
   int32_t x, y, z, w;
@@ -234,7 +235,7 @@ struct gops_context {
   int timing = 0;
   int ppr_tile = 0;
   int graph_build = GOPS_BUILD_DEVICE;
-  int delta_sweep = GOPS_DELTA_SWEEP_PULL;
+  int delta_sweep = GOPS_DELTA_SWEEP_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
   double *staging = nullptr;              // device copy of host scores for gops_top_k

@@ -423,18 +423,38 @@ __device__ __forceinline__ long long pb_fixed(float c, int k) {
   return (b >> 31) ? -(long long)v : (long long)v;
 }
 
-// Adds one value per lane of a wave to acc[key] (key < 0: no value).  A wave
-// whose lanes all hold one row (a long row's run of entries) sums in
-// registers and adds once; otherwise one LDS atomic per lane.
+// A 64-bit DPP move (row_shr / row_bcast), 0 where the control selects no lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ long long pb_dpp(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)((unsigned long long)v >> 32), CTRL, ROWS, 0xF, true);
+  return (long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Adds one value per lane of a wave to acc[key] (key < 0: no value).  A
+// bin's entries come in runs of one row, so lanes with equal keys side by
+// side are summed first (a segmented scan: each lane adds the lanes from its
+// run's head), and only each run's last lane makes an LDS atomic.
 __device__ __forceinline__ void pb_add(long long v, int key, unsigned long long *acc) {
-  const int k0 = __builtin_amdgcn_readfirstlane(key);
-  if (__all(key == k0)) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&acc[k0], (unsigned long long)v);
-  } else if (key >= 0) {
-    atomicAdd(&acc[key], (unsigned long long)v);
-  }
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int start = 63 - __clzll(heads & upto);
+  // inclusive scan within rows of 16 lanes (row_shr 1, 2, 4, 8), then across
+  // rows (row_bcast 15: rows 1 and 3 add the last lane of the row before;
+  // row_bcast 31: rows 2 and 3 add lane 31); a lane adds only lanes of its run
+  const int r = lane & 15;
+  long long o;
+  o = pb_dpp<0x111, 0xF>(v); if (r >= 1 && lane - 1 >= start) v += o;
+  o = pb_dpp<0x112, 0xF>(v); if (r >= 2 && lane - 2 >= start) v += o;
+  o = pb_dpp<0x114, 0xF>(v); if (r >= 4 && lane - 4 >= start) v += o;
+  o = pb_dpp<0x118, 0xF>(v); if (r >= 8 && lane - 8 >= start) v += o;
+  o = pb_dpp<0x142, 0xA>(v); if (((lane >> 4) & 1) && (lane | 15) - 16 >= start) v += o;
+  o = pb_dpp<0x143, 0xC>(v); if (lane >= 32 && 31 >= start) v += o;
+  const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+  if (tail && key >= 0) atomicAdd(&acc[key], (unsigned long long)v);
 }
 
 // Accumulate: one bin of rows (or a part of its entries), fixed-point sums in

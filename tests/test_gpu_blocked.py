@@ -156,9 +156,11 @@ def test_small_and_empty(oracle, ctx_b):
         dg.close()
 
 
-def test_default_is_the_pull(ctx_p):
-    """Without the option, f32 delta sweeps are pulls: the same bits as a
-    context that chose the pull; unknown values are refused."""
+def test_default_below_2_24_nodes_is_the_pull(ctx_p):
+    """GOPS_DELTA_SWEEP_AUTO (the default) pulls below 2^24 nodes: the same
+    bits as a context that chose the pull; unknown values are refused.  (At
+    2^24 nodes it takes the blocked sweep: test_gpu_full_size.py's scale-24
+    run.)"""
     c = ga.Context(0, precision="mixed")
     e = generators.rmat(14, 16, seed=3)
     cfg = ga.PageRankConfig(0.85, 200, 1e-9)
