@@ -236,14 +236,21 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0, 0};
   // f32 delta sweeps in the blocked layout (built once per graph): when chosen,
   // or by default from 2^24 nodes, where the pull's source vector (64 MB of
-  // f32) is far beyond the L2 and the blocked passes measured faster
+  // f32) is far beyond the L2 and the blocked passes measured faster.  By
+  // default, a graph whose layout does not fit in device memory keeps the pull.
+  const bool auto_blocked = ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO && n >= kPbAutoNodes &&
+                            !g->pb_no_room;
   if (ctx->precision == GOPS_PRECISION_MIXED &&
-      (ctx->delta_sweep == GOPS_DELTA_SWEEP_BLOCKED ||
-       (ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO && n >= kPbAutoNodes))) {
+      (ctx->delta_sweep == GOPS_DELTA_SWEEP_BLOCKED || auto_blocked)) {
     std::string err;
     st = build_blocked(ctx, g, err);
-    if (st != GOPS_OK) return fail(ctx, st, err);
-    setup.blocked = 1;
+    if (st == GOPS_OUT_OF_MEMORY && ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO) {
+      g->pb_no_room = true;
+    } else if (st != GOPS_OK) {
+      return fail(ctx, st, err);
+    } else {
+      setup.blocked = 1;
+    }
   }
 
   if (kind == kPersonalized) {

@@ -207,6 +207,7 @@ struct gops_graph {
   bool has_fwd = false;
   gops::BlockedLayout pb;  // blocked delta sweep, built on demand
   bool has_pb = false;
+  bool pb_no_room = false;  // the layout did not fit in device memory: pull by default
   gops::RunBuffers run;
   gops::PowerBuffers pow;
   bool weighted = false;   // created with edge weights
