@@ -246,6 +246,8 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
     st = build_blocked(ctx, g, err);
     if (st == GOPS_OUT_OF_MEMORY && ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO) {
       g->pb_no_room = true;
+      (void)hipGetLastError();  // the failed allocation's error is not this run's
+      ctx->last_error.clear();
     } else if (st != GOPS_OK) {
       return fail(ctx, st, err);
     } else {
