@@ -164,9 +164,9 @@ struct PowerBuffers {
 // Entries are the edges of the CSR of in-edges in two orders: chunk-major
 // (chunk of the source, row, source) for the scatter and bin-major (bin of
 // the row, chunk of the source, row) for the accumulate pass.
-constexpr int kPbChunkShift = 15;
+constexpr int kPbChunkShift = 14;
 constexpr int kPbBinShift = 12;
-constexpr int kPbItem1 = 1 << 18;  // entries per scatter workgroup, at most
+constexpr int kPbItem1 = 1 << 14;  // entries per scatter workgroup, at most
 constexpr int kPbItem2 = 1 << 18;  // entries per accumulate workgroup, at most
 constexpr uint64_t kPbAutoNodes = 1ull << 24;  // GOPS_DELTA_SWEEP_AUTO: blocked from here
 

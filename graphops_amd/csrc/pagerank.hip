@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *
     for (int i = threadIdx.x; i < len; i += kBlock) z[i] = yin[base + i];
   }
   __syncthreads();
-  constexpr int U = 32;
+  constexpr int U = 24;
   for (int e = it.y + (int)threadIdx.x; e < it.z; e += U * kBlock) {
     int sl[U];
     uint32_t p[U];

@@ -85,7 +85,7 @@ def runs(ctx_b, ctx_p, e, kind, storage, cfg, p=None):
 @pytest.mark.parametrize("weights,storage", [("unit", "f64"), ("uniform", "f32"),
                                              ("lognormal", "bf16"), ("uniform", "f64")])
 def test_rmat_weights(oracle, ctx_b, ctx_p, weights, storage):
-    """R-MAT scale 16 (n = 65 536: 2 chunks, 16 bins)."""
+    """R-MAT scale 16 (n = 65 536: 4 chunks, 16 bins)."""
     if weights == "unit":
         e = generators.rmat(16, 16, seed=7)
     else:
