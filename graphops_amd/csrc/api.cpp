@@ -200,6 +200,25 @@ gops_status ensure_events(gops_context *ctx, size_t count) {
 
 namespace gops {
 
+// A graph lives while its caller's handle or a server on it does; it keeps
+// its context alive until then.
+void release_graph(gops_graph *g) {
+  if (!g || --g->refs > 0) return;
+  gops_context *ctx = g->ctx;
+  if (ctx) {
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+  }
+  free_ppr_cache(g);
+  free_run(g->run);
+  free_power(g->pow);
+  free_dev(g->dev);
+  free_dev(g->fwd);
+  free_blocked(g->pb);
+  delete g;
+  gops_context_destroy(ctx);  // the graph's reference
+}
+
 // Shared driver of pagerank_run / pagerank_weighted_run /
 // personalized_pagerank_run: host-side setup, the device sweep chain, the
 // result in the caller's ids.
@@ -442,6 +461,9 @@ gops_status gops_context_create(int device, gops_context **out) {
 
 void gops_context_destroy(gops_context *ctx) {
   if (!ctx) return;
+  // graphs made on the context keep it until they are destroyed too (Python
+  // finalizers, for one, may run in any order)
+  if (--ctx->refs > 0) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
@@ -566,6 +588,7 @@ gops_status gops_graph_create(gops_context *ctx, uint64_t n, const uint64_t *off
     delete g;
     return st;
   }
+  ctx->refs++;  // released with the graph
   *out = g;
   return GOPS_OK;
 }
@@ -603,20 +626,7 @@ gops_status gops_graph_from_dense(gops_context *ctx, uint64_t n, const double *m
                            weight_storage, out);
 }
 
-void gops_graph_destroy(gops_graph *g) {
-  if (!g) return;
-  if (g->ctx) {
-    hipSetDevice(g->ctx->device);
-    hipStreamSynchronize(g->ctx->stream);
-  }
-  free_ppr_cache(g);
-  free_run(g->run);
-  free_power(g->pow);
-  free_dev(g->dev);
-  free_dev(g->fwd);
-  free_blocked(g->pb);
-  delete g;
-}
+void gops_graph_destroy(gops_graph *g) { gops::release_graph(g); }
 
 gops_status gops_graph_info(const gops_graph *g, uint64_t *n, uint64_t *nnz) {
   if (!g) return GOPS_INVALID_PARAMETER;

@@ -201,6 +201,7 @@ struct TopkScratch;
 
 struct gops_graph {
   gops_context *ctx = nullptr;
+  int refs = 1;            // the caller's handle, plus one per live server on the graph
   gops::PprGraphCache *ppr_cache = nullptr;  // batched PPR plans (server.cpp)
   gops::DevGraph dev;
   gops::DevGraph fwd;      // out-edge CSR in new ids (row_ptr, col, plan only), built on demand
@@ -222,6 +223,7 @@ constexpr int kPinChunks = 3;  // pinned chunks of the staged host <-> device co
 
 struct gops_context {
   int device = 0;
+  int refs = 1;  // the caller's handle, plus one per live graph made on it
   hipStream_t stream = nullptr;
   std::string last_error;
   int precision = GOPS_PRECISION_MIXED;
@@ -333,5 +335,9 @@ hipError_t staged_upload(gops_context *ctx, void *dst, const void *src, size_t b
 hipError_t staged_download(gops_context *ctx, void *dst, const void *src, size_t bytes,
                            hipStream_t s);
 void free_pinned_chunks(gops_context *ctx);
+
+// api.cpp: drop one reference; the last one frees the object (a graph's
+// release drops its reference on the context).
+void release_graph(gops_graph *g);
 
 }  // namespace gops

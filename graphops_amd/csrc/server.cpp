@@ -142,6 +142,7 @@ struct TilePlan {
 struct gops_server {
   gops_context *ctx = nullptr;
   gops_graph *g = nullptr;
+  bool holds_graph = false;  // a reference on g (taken once creation succeeds)
   int T = 16;
   std::vector<Tile *> tiles;
   std::deque<Query> queue;
@@ -794,6 +795,8 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
     }
     s->tiles.push_back(t);
   }
+  g->refs++;  // the server keeps its graph (and so the context) until destroyed
+  s->holds_graph = true;
   *out = s;
   return GOPS_OK;
 }
@@ -807,7 +810,9 @@ void gops_server_destroy(gops_server *s) {
   hipFree(s->top_ids);
   hipFree(s->top_vals);
   hipFree(s->top_cnt);
+  gops_graph *g = s->holds_graph ? s->g : nullptr;
   delete s;
+  release_graph(g);
 }
 
 gops_status gops_server_submit(gops_server *s, const gops_pagerank_config *cfg,

@@ -132,6 +132,9 @@ typedef struct gops_run_stats {
 
 /* ---- context ---------------------------------------------------------- */
 gops_status gops_context_create(int device, gops_context **out);
+/* Releases the caller's handle.  Graphs made on the context keep it (its
+ * stream and memory) until the last of them is destroyed, so handles may be
+ * destroyed in any order; the context handle must not be used afterwards. */
 void gops_context_destroy(gops_context *ctx);
 /* Message of the last failing call on ctx ("" if none).  For
  * GOPS_INVALID_PARAMETER it is Error's Display text, e.g.
@@ -158,6 +161,8 @@ gops_status gops_graph_create(gops_context *ctx, uint64_t n, const uint64_t *off
  * self-loops are); edge weight = the raw entry. */
 gops_status gops_graph_from_dense(gops_context *ctx, uint64_t n, const double *matrix,
                                   int weight_storage, gops_graph **out);
+/* Releases the caller's handle; a server on the graph keeps it until the
+ * server is destroyed. */
 void gops_graph_destroy(gops_graph *g);
 gops_status gops_graph_info(const gops_graph *g, uint64_t *n, uint64_t *nnz);
 /* Host copies of the preprocessed graph (no reference counterpart: the
