@@ -461,6 +461,7 @@ gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T
   st = alloc_tile(ctx, g, T, tile_grid(g), &t);
   if (st == GOPS_OUT_OF_MEMORY) {
     ctx->last_error.clear();
+    (void)hipGetLastError();  // the failed allocation's error is not a later launch's
     return GOPS_OK;
   }
   if (st != GOPS_OK) return st;
@@ -791,6 +792,7 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
         return st;
       }
       ctx->last_error.clear();
+      (void)hipGetLastError();  // the failed allocation's error is not a later launch's
       break;  // fewer tiles than the budget suggested
     }
     s->tiles.push_back(t);
