@@ -91,6 +91,71 @@ __global__ void pb_chunkmajor_kernel(const uint32_t *vals, uint32_t m, const int
   }
 }
 
+// Segments of a scatter item: maximal runs of entries whose bin-major place
+// is the chunk-major index plus one constant (a run of one chunk into one
+// bin is one).  Thread t of an item's workgroup takes kPer consecutive entries.
+constexpr int kPer = kPbItem1 / kB;
+
+__device__ __forceinline__ bool seg_head(const PbItem &it, const uint32_t *pos, int j) {
+  return j == it.y || (int64_t)pos[j] - j != (int64_t)pos[j - 1] - (j - 1);
+}
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int *total) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int i = 0; i < kB; i++) {
+      const int x = sh[i];
+      sh[i] = acc;
+      acc += x;
+    }
+    sh[kB] = acc;
+  }
+  __syncthreads();
+  const int r = sh[threadIdx.x];
+  *total = sh[kB];
+  __syncthreads();
+  return r;
+}
+
+__global__ void pb_seg_count_kernel(const PbItem *items, const uint32_t *pos, int *count) {
+  __shared__ int sh[kB + 1];
+  const PbItem it = items[blockIdx.x];
+  const int j0 = it.y + (int)threadIdx.x * kPer;
+  int c = 0;
+  for (int q = 0; q < kPer; q++) {
+    const int j = j0 + q;
+    if (j < it.z) c += seg_head(it, pos, j);
+  }
+  int total;
+  (void)block_excl_scan(c, sh, &total);
+  if (threadIdx.x == 0) count[blockIdx.x] = total;
+}
+
+__global__ void pb_seg_write_kernel(const PbItem *items, const uint32_t *pos, uint16_t *seg,
+                                    int32_t *seg_off) {
+  __shared__ int sh[kB + 1];
+  const PbItem it = items[blockIdx.x];
+  const int j0 = it.y + (int)threadIdx.x * kPer;
+  int c = 0;
+  for (int q = 0; q < kPer; q++) {
+    const int j = j0 + q;
+    if (j < it.z) c += seg_head(it, pos, j);
+  }
+  int total;
+  int r = block_excl_scan(c, sh, &total) - 1;  // the segment before this thread's first entry
+  for (int q = 0; q < kPer; q++) {
+    const int j = j0 + q;
+    if (j >= it.z) break;
+    if (seg_head(it, pos, j)) {
+      r++;
+      seg_off[it.w + r] = (int32_t)((int64_t)pos[j] - j);
+    }
+    seg[j] = (uint16_t)r;
+  }
+}
+
 // out[j] = first index i with keys[i] >= j << shift, for j in [0, count]
 __global__ void pb_bounds_kernel(const uint64_t *keys, uint32_t m, int count, int shift,
                                  uint32_t *out) {
@@ -178,7 +243,8 @@ void split_items(const std::vector<uint32_t> &off, uint32_t most, std::vector<Pb
 void free_blocked(BlockedLayout &pb) {
   (void)hipFree(pb.src);
   (void)hipFree(pb.w);
-  (void)hipFree(pb.pos);
+  (void)hipFree(pb.seg);
+  (void)hipFree(pb.seg_off);
   (void)hipFree(pb.dst);
   (void)hipFree(pb.contrib);
   (void)hipFree(pb.items1);
@@ -213,7 +279,8 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
   } while (0)
   {
     pb.src = pool.get<uint16_t>(m);
-    pb.pos = pool.get<uint32_t>(m);
+    pb.seg = pool.get<uint16_t>(m);
+    uint32_t *pos = pool.get<uint32_t>(m);  // bin-major place of each chunk-major entry
     pb.dst = pool.get<uint16_t>(m);
     pb.contrib = pool.get<float>(m);
     if (wb) pb.w = pool.get<uint8_t>(m * wb);
@@ -262,7 +329,7 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       BTRY(hipcub::DeviceRadixSort::SortPairs(tmp, b, keys, keys_b, vals, vals_b, (int)m, 0,
                                               bits_chunk, s));
       BTRY(launch_chunkmajor(dg.weight_storage, vals_b, (uint32_t)m, dg.col, position_of,
-                             wb ? dg.w : nullptr, pb.pos, pb.src, pb.w, s));
+                             wb ? dg.w : nullptr, pos, pb.src, pb.w, s));
       pb_bounds_kernel<<<grid_for(chunks + 1), kB, 0, s>>>(keys_b, (uint32_t)m, chunks, 31,
                                                            d_chunk_off);
       BTRY(hipGetLastError());
@@ -291,14 +358,36 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
     pb.num_items2 = (int)items2.size();
     pb.num_splits = (int)splits.size();
     pb.items1 = pool.get<PbItem>(items1.size());
+    // the scatter items' segments: counted per item, then written from each
+    // item's first segment on
+    if (!items1.empty()) {
+      int *d_count = pool.get<int>(items1.size());
+      BTRY(pool.err);
+      BTRY(hipMemcpyAsync(pb.items1, items1.data(), items1.size() * sizeof(PbItem),
+                          hipMemcpyHostToDevice, s));
+      pb_seg_count_kernel<<<(int)items1.size(), kB, 0, s>>>(pb.items1, pos, d_count);
+      BTRY(hipGetLastError());
+      std::vector<int> count(items1.size());
+      BTRY(hipMemcpyAsync(count.data(), d_count, count.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, s));
+      BTRY(hipStreamSynchronize(s));
+      int64_t total = 0;
+      for (size_t i = 0; i < items1.size(); i++) {
+        items1[i].w = (int32_t)total;
+        total += count[i];
+      }
+      pb.seg_off = pool.get<int32_t>((size_t)total);
+      BTRY(pool.err);
+      BTRY(hipMemcpyAsync(pb.items1, items1.data(), items1.size() * sizeof(PbItem),
+                          hipMemcpyHostToDevice, s));
+      pb_seg_write_kernel<<<(int)items1.size(), kB, 0, s>>>(pb.items1, pos, pb.seg, pb.seg_off);
+      BTRY(hipGetLastError());
+    }
     pb.items2 = pool.get<PbItem>(items2.size());
     pb.splits = pool.get<PbItem>(splits.size());
     pb.bin_partial = pool.get<long long>((size_t)std::max(slots, 1) << kPbBinShift);
     pb.partials = pool.get<double>((items2.size() + splits.size()) * kMaxReduce);
     BTRY(pool.err);
-    if (!items1.empty())
-      BTRY(hipMemcpyAsync(pb.items1, items1.data(), items1.size() * sizeof(PbItem),
-                          hipMemcpyHostToDevice, s));
     BTRY(hipMemcpyAsync(pb.items2, items2.data(), items2.size() * sizeof(PbItem),
                         hipMemcpyHostToDevice, s));
     if (!splits.empty())
@@ -307,7 +396,8 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
     BTRY(hipStreamSynchronize(s));
   }
 #undef BTRY
-  for (void *p : {(void *)pb.src, (void *)pb.pos, (void *)pb.dst, (void *)pb.contrib, pb.w,
+  for (void *p : {(void *)pb.src, (void *)pb.seg, (void *)pb.seg_off, (void *)pb.dst,
+                  (void *)pb.contrib, pb.w,
                   (void *)pb.items1, (void *)pb.items2, (void *)pb.splits,
                   (void *)pb.bin_partial, (void *)pb.partials})
     if (p) pool.keep(p);

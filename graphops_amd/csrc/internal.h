@@ -177,10 +177,12 @@ struct PbItem {
 struct BlockedLayout {
   uint16_t *src = nullptr;     // chunk-major: source id - chunk base
   void *w = nullptr;           // chunk-major: weights in the graph's storage type (or null)
-  uint32_t *pos = nullptr;     // chunk-major: index of the entry in bin-major order
+  uint16_t *seg = nullptr;     // chunk-major: the entry's segment within its scatter item
+  int32_t *seg_off = nullptr;  // per segment: bin-major place - chunk-major index (constant
+                               // along a run of one chunk into one bin)
   uint16_t *dst = nullptr;     // bin-major: row id - bin base
   float *contrib = nullptr;    // bin-major: one sweep's products
-  PbItem *items1 = nullptr;    // scatter: {chunk, first entry, end entry, 0}
+  PbItem *items1 = nullptr;    // scatter: {chunk, first entry, end entry, first segment}
   PbItem *items2 = nullptr;    // accumulate: {bin, first entry, end entry, slot or -1}
   PbItem *splits = nullptr;    // bins of several items: {bin, first slot, slots, 0}
   long long *bin_partial = nullptr;  // [slot][1 << kPbBinShift] fixed-point row sums

@@ -397,7 +397,8 @@ __global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *
       const int k = e + u * kBlock;
       const int kk = k < it.z ? k : it.y;
       sl[u] = __builtin_nontemporal_load(a.pb.src + kk);
-      p[u] = __builtin_nontemporal_load(a.pb.pos + kk);
+      const int sg = __builtin_nontemporal_load(a.pb.seg + kk);
+      p[u] = (uint32_t)(kk + a.pb.seg_off[it.w + sg]);
       w[u] = wload32<WT>(a.pb.w, kk);
     }
 #pragma unroll
