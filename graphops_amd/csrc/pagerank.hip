@@ -388,14 +388,18 @@ __global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *
   }
   __syncthreads();
   constexpr int U = 24;
-  for (int e = it.y + (int)threadIdx.x; e < it.z; e += U * kBlock) {
+  // Waves start on a multiple of 64 entries (lanes before the item's first
+  // entry idle), so that a wave's 2-byte loads fill one 128 B line: from an
+  // unaligned start every wave touches two lines per load (at R-MAT scale 24,
+  // 0.1 GB more fetched per sweep and 3.5 % more time).
+  for (int e = (it.y & ~63) + (int)threadIdx.x; e < it.z; e += U * kBlock) {
     int sl[U];
     uint32_t p[U];
     float w[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = e + u * kBlock;
-      const int kk = k < it.z ? k : it.y;
+      const int kk = k >= it.y && k < it.z ? k : it.y;
       sl[u] = __builtin_nontemporal_load(a.pb.src + kk);
       const int sg = __builtin_nontemporal_load(a.pb.seg + kk);
       p[u] = (uint32_t)(kk + a.pb.seg_off[it.w + sg]);
@@ -403,7 +407,8 @@ __global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (e + u * kBlock < it.z) a.pb.contrib[p[u]] = w[u] * z[sl[u]];
+      const int k = e + u * kBlock;
+      if (k >= it.y && k < it.z) a.pb.contrib[p[u]] = w[u] * z[sl[u]];
     }
   }
 }
@@ -473,13 +478,16 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   const int k = pb_exponent(st);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int U = 16;
-  for (int f0 = it.y; f0 < it.z; f0 += U * kBlock) {
+  // waves start on a multiple of 64 entries, as in the scatter: whole lines
+  // per wave (unaligned, the two waves sharing each line both fetched it:
+  // 0.65 GB more per sweep at R-MAT scale 24)
+  for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kBlock) {
     float cv[U];
     int key[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int f = f0 + u * kBlock + wave * 64 + lane;  // 64 consecutive entries per wave
-      const bool ok = f < it.z;
+      const bool ok = f >= it.y && f < it.z;
       const int ff = ok ? f : it.y;
       cv[u] = __builtin_nontemporal_load(a.pb.contrib + ff);
       const int d = __builtin_nontemporal_load(a.pb.dst + ff);
