@@ -212,7 +212,11 @@ void plan_blocks(const HostCSRT &g, Plan &p) {
     while (i < n && i - s < max_rows) {
       const int32_t d = rp[i + 1] - rp[i];
       if (d > kStageNnz || lanes_for(d) != G) break;
-      if (nnz + d > kStageNnz) break;  // the run's edges must fit the LDS stage
+      // the run's edges must fit the LDS stage, and with the lanes before
+      // the run's first edge (its offset within 64 edges, where the sweep
+      // kernels' waves start) one pass of the stage's loads; a first row
+      // always fits the stage and may need a second pass
+      if (nnz > 0 && (rp[s] & 63) + nnz + d > kStageNnz) break;
       nnz += d;
       i++;
     }

@@ -101,16 +101,21 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
 
 // Sum over edges [e0, e1) by the whole workgroup, kUnroll independent
 // loads in flight per lane (out-of-range slots re-read edge e0 and count 0).
+// Waves start on a multiple of 64 edges, so that each wave's loads of the
+// col and weight streams cover whole 128 B lines: from an unaligned start a
+// wave touches one more line per load, and the line two waves share is
+// fetched by both.
 template <typename LOAD>
 __device__ __forceinline__ double stream_sum(const LOAD &load, int e0, int e1) {
   double acc = 0.0;
-  for (int k = e0 + (int)threadIdx.x; k < e1; k += kUnroll * kBlock) {
+  for (int k = (e0 & ~63) + (int)threadIdx.x; k < e1; k += kUnroll * kBlock) {
     double v[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) {
       const int kk = k + u * kBlock;
-      const double x = load(kk < e1 ? kk : e0);
-      v[u] = kk < e1 ? x : 0.0;
+      const bool ok = kk >= e0 && kk < e1;
+      const double x = load(ok ? kk : e0);
+      v[u] = ok ? x : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) acc += v[u];
@@ -135,26 +140,31 @@ __device__ __forceinline__ void do_desc(const A &a, const BlockDesc &bd, const L
     // staged in LDS (f32 or f64, one buffer); then each group of G lanes
     // reduces one row from LDS, in the order of a direct pull (lane, lane +
     // G, ...; then across the group).  stage: kStageNnz doubles of LDS.
+    // Lanes are laid over [e0 - off, e0 + nnz) with off = e0 mod 64, so that
+    // waves start on whole lines (as in stream_sum); the planner keeps off +
+    // nnz <= kStageNnz for runs of several rows, so that an f32 run still
+    // takes one pass of U loads per lane.
     using P = typename LOAD::P;
     constexpr int U = LOAD::kStageUnroll;
     P *prod = reinterpret_cast<P *>(stage);
     const int e0 = a.row_ptr[bd.y];
     const int nnz = a.row_ptr[bd.z] - e0;
-    for (int i = threadIdx.x; i < nnz; i += U * kBlock) {
+    const int off = e0 & 63;
+    for (int i = threadIdx.x; i < off + nnz; i += U * kBlock) {
       int c[U];
       P wv[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int ii = i + u * kBlock;
-        const int k = e0 + (ii < nnz ? ii : 0);
+        const int ii = i + u * kBlock - off;
+        const int k = e0 + ((unsigned)ii < (unsigned)nnz ? ii : 0);
         c[u] = __builtin_nontemporal_load(a.col + k);
         wv[u] = load.stage_weight(k);
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int ii = i + u * kBlock;
+        const int ii = i + u * kBlock - off;
         const P v = wv[u] * load.stage_gather(c[u]);
-        if (ii < nnz) prod[ii] = v;
+        if ((unsigned)ii < (unsigned)nnz) prod[ii] = v;
       }
     }
     __syncthreads();
