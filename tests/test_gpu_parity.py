@@ -11,6 +11,7 @@ Tolerances (floating point; the reference is f64 throughout):
 """
 import os
 from fractions import Fraction
+from types import SimpleNamespace
 
 import numpy as np
 import pytest
@@ -316,3 +317,41 @@ def test_ppr_batch_matches_single(ctx):
         r = ga.personalized_pagerank_run(dg, cfgs[q], p, ctx=ctx)
         assert runs[q].iterations == r.iterations and runs[q].converged == r.converged
         assert np.abs(scores[q] - r.scores).sum() <= 1e-2 * 1e-8 + 1e-12
+
+
+def test_unaligned_runs_and_hub_chunks(oracle, ctx, ctx64):
+    """Rows whose edges start off a 64-edge boundary, where the pull's waves
+    begin: two hub rows split into chunks (the second starting 48 edges past
+    a boundary), a long row with its own descriptor (1 past), a row of 4090
+    in-edges 9 past a boundary (its offset and edges overflow one pass of
+    the stage's loads, so its run takes a second pass), a row of 4000, and
+    80 000 short rows at every offset; unweighted and f32-weighted, mixed and
+    f64, against the oracle."""
+    n = 80000
+    rng = np.random.default_rng(21)
+    src = [np.arange(1, 70001), np.arange(2, 66003), np.arange(3, 5003), np.arange(4, 4094),
+           np.arange(5, 4005), rng.integers(0, n, size=2 * n)]
+    dst = [np.full(70000, 0), np.full(66001, 1), np.full(5000, 2), np.full(4090, 3),
+           np.full(4000, 4), rng.integers(5, n, size=2 * n)]
+    src = np.concatenate(src)
+    dst = np.concatenate(dst)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    w = rng.uniform(0.1, 1.0, size=dst.size).astype(np.float32).astype(np.float64)
+    e = SimpleNamespace(n=n, m=dst.size, offsets=off, targets=dst.astype(np.uint32), weights=w)
+    in_deg = np.bincount(dst, minlength=n)
+    assert in_deg[0] == 70000 and in_deg[1] == 66001 and in_deg[3] == 4090
+    tol = 1e-10
+    for kind, storage in ((0, "f64"), (1, "f32")):
+        ref = oracle_pull(oracle, e, kind, tol, 300)
+        for cx, prec in ((ctx, "mixed"), (ctx64, "fp64")):
+            dg = ga.DeviceGraph.from_edges(e, weight_storage=storage, ctx=cx)
+            cfg = ga.PageRankConfig(0.85, 300, tol)
+            r = (ga.pagerank_weighted_run if kind else ga.pagerank_run)(dg, cfg, ctx=cx)
+            check_run(r, ref, tol, prec, f"unaligned-{kind}")
+            dg.close()
