@@ -45,7 +45,8 @@ struct TkState {
   uint64_t valid;   // finite, positive scores
   uint64_t ties_in; // entries equal to T that entered the heap (j)
   uint32_t all;     // valid <= k: every valid score is selected
-  uint32_t pad;
+  uint32_t resolved;  // radix passes stopped early: T is the lower bound of a bucket
+                      // whose keys are all taken
 };
 
 __device__ __forceinline__ uint64_t skey(double s) {
@@ -56,10 +57,11 @@ __device__ __forceinline__ uint64_t skey(double s) {
 // prefix: one row of 256 bins per workgroup
 __global__ __launch_bounds__(kHB) void tk_hist_kernel(const double *s, uint64_t n, int shift,
                                                        const TkState *st, uint32_t *part) {
+  if (st->all || st->resolved) return;  // the pick kernel reads no histogram then
   __shared__ uint32_t h[256];
   if (threadIdx.x < 256) h[threadIdx.x] = 0;
   __syncthreads();
-  if (!st->all) {
+  {
     const int hi = shift + 8;
     const uint64_t pre = hi < 64 ? (st->prefix >> hi) : 0;
     for (uint64_t i = blockIdx.x * (uint64_t)kHB + threadIdx.x; i < n;
@@ -80,6 +82,7 @@ constexpr int kPickSlices = kPickThreads / 64;  // row slices (64 threads x 4 bi
 static_assert(kHGrid % kPickSlices == 0, "rows split evenly over the slices");
 __global__ __launch_bounds__(kPickThreads) void tk_pick_kernel(TkState *st, const uint32_t *part,
                                                                 int shift, uint64_t k) {
+  if (st->all || st->resolved) return;
   __shared__ uint64_t tot[kPickSlices][256];
   const int g = threadIdx.x & 63, sl = threadIdx.x >> 6;
   constexpr int kRows = kHGrid / kPickSlices;
@@ -134,6 +137,11 @@ __global__ __launch_bounds__(kPickThreads) void tk_pick_kernel(TkState *st, cons
   st->above += acc;
   st->k_rem = need - acc;
   st->prefix |= (uint64_t)d << shift;
+  // Every key of the chosen bucket is needed: the keys >= its lower bound
+  // (the prefix, remaining bits zero) are exactly the k to select, so T can
+  // stay that bound and the remaining passes return at once.  The select
+  // kernel then counts the keys equal to the bound as ties that all entered.
+  if (shift > 0 && tb[d] == st->k_rem) st->resolved = 1;
 }
 
 // exclusive ranks of two flags over a workgroup of kTB threads, and totals
@@ -254,7 +262,7 @@ __global__ __launch_bounds__(kTB) void tk_select_kernel(const double *s, uint64_
                                                          const uint64_t *off,
                                                          const uint64_t *entered, int nblocks,
                                                          uint64_t *out_key, uint32_t *out_id) {
-  const uint64_t T = st->prefix, above = st->above, c = st->k_rem;
+  const uint64_t T = st->prefix;
   __shared__ uint64_t sj[kTB];
   uint64_t tj = 0;
   for (int w = threadIdx.x; w < nblocks; w += kTB) tj += entered[w];
@@ -265,6 +273,10 @@ __global__ __launch_bounds__(kTB) void tk_select_kernel(const double *s, uint64_
     __syncthreads();
   }
   const uint64_t j = st->all ? 0 : sj[0];
+  // keys above T, and ties kept; stopped early (T a bucket's lower bound,
+  // above + k_rem = k keys >= T): every tie entered (j of them) and is kept
+  const uint64_t c = st->resolved ? j : st->k_rem;
+  const uint64_t above = st->resolved ? st->above + st->k_rem - j : st->above;
   const uint64_t first_kept = j - c;  // tie rank of the first kept tie
   uint64_t b, e;
   chunk_of(n, b, e);

@@ -56,3 +56,21 @@ def test_result_top_k_after_runs(oracle, ctx):
     p[[5, 9, 1000]] = 1.0
     r = ga.personalized_pagerank_run(dg, cfg, p, ctx=ctx)
     assert ga.result_top_k(dg, 50) == oracle.top_k(r.scores, 50)
+
+
+def test_stop_at_a_bucket_bound_with_ties(oracle, ctx):
+    """The radix passes stop once the bucket of the k-th largest holds
+    exactly the keys still to take; T is then the bucket's lower bound, and
+    keys equal to it are ties that all enter.  1.0 is such a bound (its low
+    bits are zero): 60 scores of 3.0, then 10 of exactly 1.0 and 30 in
+    (1.0, 1.0625) -- one 16-bit bucket of 40 keys, all needed for k = 100 --
+    shuffled among 5000 smaller scores; also with the bucket one key short
+    and one key over, so that the passes do not stop early."""
+    rng = np.random.default_rng(17)
+    for extra in (30, 29, 31):
+        s = np.concatenate([np.full(60, 3.0), np.full(10, 1.0),
+                            1.0 + rng.uniform(1e-9, 0.0625, size=extra),
+                            rng.uniform(0.01, 0.5, size=5000)])
+        s = s[rng.permutation(s.size)]
+        for k in (100, 70, 65, 101):
+            assert ga.top_k(s, k, ctx=ctx) == oracle.top_k(s, k), (extra, k)
