@@ -64,10 +64,32 @@ __global__ __launch_bounds__(kHB) void tk_hist_kernel(const double *s, uint64_t 
   {
     const int hi = shift + 8;
     const uint64_t pre = hi < 64 ? (st->prefix >> hi) : 0;
-    for (uint64_t i = blockIdx.x * (uint64_t)kHB + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * kHB) {
-      const uint64_t key = skey(s[i]);
-      if (key && (hi >= 64 || (key >> hi) == pre)) atomicAdd(&h[(key >> shift) & 255], 1u);
+    const int lane = threadIdx.x & 63;
+    constexpr int U = 4;  // loads in flight per lane
+    const uint64_t step = (uint64_t)gridDim.x * kHB * U;
+    for (uint64_t i0 = blockIdx.x * (uint64_t)kHB * U + threadIdx.x; i0 < n; i0 += step) {
+      uint64_t key[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint64_t i = i0 + (uint64_t)u * kHB;
+        key[u] = i < n ? skey(s[i]) : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const bool in = key[u] && (hi >= 64 || (key[u] >> hi) == pre);
+        const int dg = (int)((key[u] >> shift) & 255);
+        // Scores crowd into few digits (PageRank's share their top bits), so
+        // the lanes with the first counted lane's digit add once, by
+        // popcount; the others add one each.
+        const uint64_t act = __ballot(in);
+        if (act) {
+          const int lead = __ffsll((unsigned long long)act) - 1;
+          const int d0 = __shfl(dg, lead);
+          const uint64_t same = __ballot(in && dg == d0);
+          if (lane == lead) atomicAdd(&h[d0], (uint32_t)__popcll(same));
+          if (in && dg != d0) atomicAdd(&h[dg], 1u);
+        }
+      }
     }
   }
   __syncthreads();
