@@ -36,6 +36,7 @@ constexpr int kHB = 1024;       // threads per workgroup of the histogram passes
 constexpr int kHGrid = 256;     // workgroups (histogram rows) of the histogram passes
 constexpr int kSortMax = 4096;  // entries sorted in one workgroup's LDS
 constexpr int kSortThreads = 1024;
+constexpr int kPassU = 4;       // loads in flight per thread in the ordered passes
 static_assert(kTGrid <= 1024, "tk_scan_kernel scans one count per thread");
 
 struct TkState {
@@ -207,10 +208,18 @@ __global__ __launch_bounds__(kTB) void tk_count_kernel(const double *s, uint64_t
   uint64_t b, e;
   chunk_of(n, b, e);
   uint32_t ge = 0, eq = 0;
-  for (uint64_t i = b + threadIdx.x; i < e; i += kTB) {
-    const uint64_t key = skey(s[i]);
-    ge += key != 0 && key >= T;
-    eq += key != 0 && key == T;
+  for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += (uint64_t)kPassU * kTB) {
+    uint64_t key[kPassU];
+#pragma unroll
+    for (int u = 0; u < kPassU; u++) {
+      const uint64_t i = i0 + (uint64_t)u * kTB;
+      key[u] = i < e ? skey(s[i]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPassU; u++) {
+      ge += key[u] != 0 && key[u] >= T;
+      eq += key[u] != 0 && key[u] == T;
+    }
   }
   __shared__ uint32_t sg[kTB], se[kTB];
   sg[threadIdx.x] = ge;
@@ -259,14 +268,22 @@ __global__ __launch_bounds__(kTB) void tk_entered_kernel(const double *s, uint64
   chunk_of(n, b, e);
   uint64_t run_ge = off[2 * blockIdx.x];
   uint32_t entered = 0;
-  for (uint64_t i0 = b; i0 < e; i0 += kTB) {
-    const uint64_t i = i0 + threadIdx.x;
-    const uint64_t key = i < e ? skey(s[i]) : 0;
-    const bool ge = key != 0 && key >= T, eq = key != 0 && key == T;
-    uint32_t rg, rq, tg, tq;
-    block_rank2(ge, eq, rg, rq, tg, tq);
-    if (eq && run_ge + rg < k) entered++;
-    run_ge += tg;
+  for (uint64_t i00 = b; i00 < e; i00 += (uint64_t)kPassU * kTB) {
+    uint64_t keys[kPassU];
+#pragma unroll
+    for (int u = 0; u < kPassU; u++) {
+      const uint64_t i = i00 + (uint64_t)u * kTB + threadIdx.x;
+      keys[u] = i < e ? skey(s[i]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < kPassU; u++) {
+      const uint64_t key = keys[u];
+      const bool ge = key != 0 && key >= T, eq = key != 0 && key == T;
+      uint32_t rg, rq, tg, tq;
+      block_rank2(ge, eq, rg, rq, tg, tq);
+      if (eq && run_ge + rg < k) entered++;
+      run_ge += tg;
+    }
   }
   __shared__ uint32_t sh[kTB];
   sh[threadIdx.x] = entered;
@@ -303,25 +320,34 @@ __global__ __launch_bounds__(kTB) void tk_select_kernel(const double *s, uint64_
   uint64_t b, e;
   chunk_of(n, b, e);
   uint64_t run_ge = off[2 * blockIdx.x], run_eq = off[2 * blockIdx.x + 1];
-  for (uint64_t i0 = b; i0 < e; i0 += kTB) {
-    const uint64_t i = i0 + threadIdx.x;
-    const uint64_t key = i < e ? skey(s[i]) : 0;
-    const bool ge = key != 0 && key >= T, eq = key != 0 && key == T;
-    uint32_t rg, rq, tg, tq;
-    block_rank2(ge, eq, rg, rq, tg, tq);
-    if (ge && !eq) {
-      const uint64_t pos = (run_ge + rg) - (run_eq + rq);  // entries above T before i
-      out_key[pos] = key;
-      out_id[pos] = (uint32_t)i;
-    } else if (eq) {
-      const uint64_t tr = run_eq + rq;
-      if (tr >= first_kept && tr < j) {
-        out_key[above + (tr - first_kept)] = key;
-        out_id[above + (tr - first_kept)] = (uint32_t)i;
-      }
+  for (uint64_t i00 = b; i00 < e; i00 += (uint64_t)kPassU * kTB) {
+    uint64_t keys[kPassU];
+#pragma unroll
+    for (int u = 0; u < kPassU; u++) {
+      const uint64_t i = i00 + (uint64_t)u * kTB + threadIdx.x;
+      keys[u] = i < e ? skey(s[i]) : 0;
     }
-    run_ge += tg;
-    run_eq += tq;
+#pragma unroll
+    for (int u = 0; u < kPassU; u++) {
+      const uint64_t i = i00 + (uint64_t)u * kTB + threadIdx.x;
+      const uint64_t key = keys[u];
+      const bool ge = key != 0 && key >= T, eq = key != 0 && key == T;
+      uint32_t rg, rq, tg, tq;
+      block_rank2(ge, eq, rg, rq, tg, tq);
+      if (ge && !eq) {
+        const uint64_t pos = (run_ge + rg) - (run_eq + rq);  // entries above T before i
+        out_key[pos] = key;
+        out_id[pos] = (uint32_t)i;
+      } else if (eq) {
+        const uint64_t tr = run_eq + rq;
+        if (tr >= first_kept && tr < j) {
+          out_key[above + (tr - first_kept)] = key;
+          out_id[above + (tr - first_kept)] = (uint32_t)i;
+        }
+      }
+      run_ge += tg;
+      run_eq += tq;
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st->ties_in = j;
 }
