@@ -542,8 +542,19 @@ __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *
   const int i = sp.w + (int)threadIdx.x;  // row within the bin
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (i < (1 << kPbBinShift) && row0 + (int)threadIdx.x < a.n) {
+    // the hottest bins are split into dozens of parts: eight parts' loads in
+    // flight at a time (integer sums: the grouping leaves the bits alone)
+    const long long *bp = a.pb.bin_partial + ((size_t)sp.y << kPbBinShift) + i;
     long long s = 0;
-    for (int j = 0; j < sp.z; j++) s += a.pb.bin_partial[((size_t)(sp.y + j) << kPbBinShift) + i];
+    int j = 0;
+    for (; j + 8 <= sp.z; j += 8) {
+      long long v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = bp[(size_t)(j + u) << kPbBinShift];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; j < sp.z; j++) s += bp[(size_t)j << kPbBinShift];
     const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
     epi(row0 + (int)threadIdx.x, (double)s * inv_scale, red);
   }
