@@ -298,6 +298,28 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, 
   }
 }
 
+// red += partials[b] for b = threadIdx.x, + kBlock, ... < count, in that
+// order, with four rows' loads in flight at a time
+__device__ __forceinline__ void add_partials(const double *partials, int count,
+                                             double (&red)[kReduce]) {
+  for (int b0 = threadIdx.x; b0 < count; b0 += 4 * kBlock) {
+    double v[4][kReduce];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int b = b0 + u * kBlock;
+#pragma unroll
+      for (int i = 0; i < kReduce; i++) v[u][i] = b < count ? partials[(size_t)b * kReduce + i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (b0 + u * kBlock < count) {
+#pragma unroll
+        for (int i = 0; i < kReduce; i++) red[i] += v[u][i];
+      }
+    }
+  }
+}
+
 // One workgroup: hub-row epilogues (chunk partials summed in chunk order),
 // the fixed-order reduction of the sweep's partials, the bookkeeping.
 __global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float *yd0, float *yd1,
@@ -319,20 +341,14 @@ __global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float
   if (a.pb_active && mode != kModeAnchor) {
     // a blocked sweep: hub rows were summed in their bins
     const int parts = a.pb.num_items2 + a.pb.num_splits;
-    for (int b = threadIdx.x; b < parts; b += kBlock) {
-#pragma unroll
-      for (int i = 0; i < kReduce; i++) red[i] += a.pb.partials[(size_t)b * kReduce + i];
-    }
+    add_partials(a.pb.partials, parts, red);
   } else {
     if (mode == kModeAnchor) {
       hubs(AnchorEpilogue(a, st));
     } else {
       hubs(DeltaEpilogue(a, st, mode == kModeDeltaFold));
     }
-    for (int b = threadIdx.x; b < sweep_blocks; b += kBlock) {
-#pragma unroll
-      for (int i = 0; i < kReduce; i++) red[i] += a.partials[(size_t)b * kReduce + i];
-    }
+    add_partials(a.partials, sweep_blocks, red);
   }
   block_sum<kReduce>(red);
   if (threadIdx.x == 0) finish_sweep(st, mode, red);
