@@ -69,14 +69,26 @@ def cpu_baseline(e, cfg, n_sweeps, damping=0.85):
 
 
 def oracle_reference(e, tol, max_it):
-    """f64 result of pagerank_weighted_run by the oracle's pull restatement
-    (bit-identical to the push form; OpenMP over rows)."""
+    """The f64 result of pagerank_weighted_run by the oracle's pull
+    restatement (bit-identical to the push form; OpenMP over rows), and the
+    same iteration carried in extended precision (the exact iteration to
+    ~1e-19), from which the f64 reference drifts by its own rounding."""
     import oracle
 
     in_off, in_src, in_w = oracle.transpose(e.offsets, e.targets, e.weights)
     ws = oracle.out_wsum(e.offsets, e.weights)
-    return oracle.pull_run(1, in_off, in_src, ws, w=in_w, damping=0.85, max_iterations=max_it,
-                           tolerance=tol, threads=16)
+    f64 = oracle.pull_run(1, in_off, in_src, ws, w=in_w, damping=0.85, max_iterations=max_it,
+                          tolerance=tol, threads=16)
+    ld = oracle.pull_run_ld(1, in_off, in_src, w=in_w, damping=0.85, max_iterations=max_it,
+                            tolerance=tol, threads=16)
+    return f64, ld
+
+
+def top_agreement(a, b, k=100):
+    """|top-k(a) & top-k(b)| / k (ids by descending score, ties by id)."""
+    ta = set(np.argsort(-a, kind="stable")[:k].tolist())
+    tb = set(np.argsort(-b, kind="stable")[:k].tolist())
+    return len(ta & tb) / float(k)
 
 
 PPR_GRAPH = dict(scale=22, seed=13)  # unit weights
@@ -113,6 +125,7 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
     refs = refs_of(oracle, e, seeds, off, sample)
     l1 = [float(np.abs(sc[j] - refs[j]).sum()) for j in sample]
     mx = [float(np.abs(sc[j] - refs[j]).max()) for j in sample]
+    top = [top_agreement(sc[j], refs[j]) for j in sample]
     del sc
     fast = f32_schedule(ga, dg, cfg, off, seeds, sample, refs)
     # CPU: the oracle's push form of personalized_pagerank_run, one thread
@@ -138,7 +151,7 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
         "plain_iterations_mean": round(float(np.mean([r.iterations for r in pruns])), 3),
         "ratio_vs_plain": round(qps / plain_qps, 2),
         "accuracy": {"sample_queries": len(sample), "l1_vs_f64_oracle_max": max(l1),
-                     "max_abs": max(mx)},
+                     "max_abs": max(mx), "top100_agreement_min": min(top)},
         "f32_schedule": fast,
         "roofline": {"bound": "hbm", "queries_per_s_at_peak": round(roof_qps, 1),
                      "frac": round(qps / roof_qps, 4),
@@ -511,7 +524,8 @@ def main():
     # the dominant kernel: the f32 delta sweep (HIP events around its launches)
     achieved = alg_bytes / avg_f32 / 1e9 if n32 else None
     wname = "Bf16W" if c["w_bytes"] == 2 else "float"
-    blocked = args.delta_sweep == "blocked" or (args.delta_sweep == "auto" and n >= (1 << 24))
+    # which kind of f32 delta sweep actually ran, from the library's own count
+    blocked = st["sweeps_blocked"] > 0
     sweep_name = (f"gops::pb_scatter_kernel<gops::{wname}> + gops::pb_accumulate_kernel + "
                   "gops::pb_combine_kernel (blocked f32 delta sweep)" if blocked
                   else f"gops::sweep_kernel<gops::{wname}> (f32 delta sweep)")
@@ -554,14 +568,24 @@ def main():
     if not args.no_accuracy:
         scores = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
         t0 = time.perf_counter()
-        ref = oracle_reference(e, c["tol"], c["max_it"])
-        log(f"oracle f64 run: {ref.iterations} sweeps in {time.perf_counter() - t0:.1f} s")
+        ref, exact = oracle_reference(e, c["tol"], c["max_it"])
+        log(f"oracle runs: f64 {ref.iterations} sweeps, long double {exact.iterations} sweeps, "
+            f"{time.perf_counter() - t0:.1f} s")
         diff = np.abs(scores - ref.scores)
-        top = set(np.argsort(-scores, kind="stable")[:100].tolist())
-        top_ref = set(np.argsort(-ref.scores, kind="stable")[:100].tolist())
+        dx = np.abs(scores - exact.scores)
         out["accuracy"] = {"l1_vs_f64_oracle": float(diff.sum()), "max_abs": float(diff.max()),
-                           "top100_agreement": len(top & top_ref) / 100.0,
-                           "oracle_iterations": ref.iterations, "gpu_iterations": r.iterations}
+                           "top100_agreement": top_agreement(scores, ref.scores),
+                           "oracle_iterations": ref.iterations, "gpu_iterations": r.iterations,
+                           "gpu_diff_l1": r.diff_l1,
+                           "exact_iteration": {
+                               "what": "the same iteration in extended precision (oracle, long "
+                                       "double)",
+                               "iterations": exact.iterations, "diff_l1": exact.diff_l1,
+                               "l1_gpu_vs_exact": float(dx.sum()), "max_abs": float(dx.max()),
+                               "l1_f64_oracle_vs_exact": float(np.abs(ref.scores -
+                                                                      exact.scores).sum()),
+                               "f64_oracle_mass_drift": float(ref.scores.sum() - 1.0)}}
+        del ref, exact
     out["cpu_baseline"] = cpu_baseline(e, c, args.cpu_sweeps) if args.cpu_sweeps > 0 else None
     if not args.no_build:
         out["graph_build"] = build_leg(ga, e, c["storage"])

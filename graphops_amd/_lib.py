@@ -58,7 +58,9 @@ class RunStatsC(ctypes.Structure):
     _fields_ = [("n", ctypes.c_uint64), ("nnz", ctypes.c_uint64),
                 ("sweeps_fp32", ctypes.c_uint64), ("sweeps_fp64", ctypes.c_uint64),
                 ("device_ms", ctypes.c_double), ("fp32_sweep_ms", ctypes.c_double),
-                ("fp64_sweep_ms", ctypes.c_double), ("other_ms", ctypes.c_double)]
+                ("fp64_sweep_ms", ctypes.c_double), ("other_ms", ctypes.c_double),
+                ("sweeps_blocked", ctypes.c_uint64), ("blocked_fallback", ctypes.c_int32),
+                ("reserved2", ctypes.c_int32)]
 
 
 class ServerConfigC(ctypes.Structure):

@@ -203,7 +203,7 @@ namespace gops {
 // A graph lives while its caller's handle or a server on it does; it keeps
 // its context alive until then.
 void release_graph(gops_graph *g) {
-  if (!g || --g->refs > 0) return;
+  if (!g || g->refs.fetch_sub(1) > 1) return;
   gops_context *ctx = g->ctx;
   if (ctx) {
     hipSetDevice(ctx->device);
@@ -256,15 +256,15 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   // f32 delta sweeps in the blocked layout (built once per graph): when chosen,
   // or by default from 2^24 nodes, where the pull's source vector (64 MB of
   // f32) is far beyond the L2 and the blocked passes measured faster.  By
-  // default, a graph whose layout does not fit in device memory keeps the pull.
-  const bool auto_blocked = ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO && n >= kPbAutoNodes &&
-                            !g->pb_no_room;
+  // default, a run whose layout does not fit in free device memory keeps the
+  // pull and says so in its stats; a later run tries again.
+  const bool auto_blocked = ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO && n >= kPbAutoNodes;
   if (ctx->precision == GOPS_PRECISION_MIXED &&
       (ctx->delta_sweep == GOPS_DELTA_SWEEP_BLOCKED || auto_blocked)) {
     std::string err;
     st = build_blocked(ctx, g, err);
     if (st == GOPS_OUT_OF_MEMORY && ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO) {
-      g->pb_no_room = true;
+      ctx->stats.blocked_fallback = 1;
       (void)hipGetLastError();  // the failed allocation's error is not this run's
       ctx->last_error.clear();
     } else if (st != GOPS_OK) {
@@ -407,6 +407,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
     const bool f64 = modes[k] == kModeAnchor;
     if (f64) ctx->stats.sweeps_fp64++;
     else ctx->stats.sweeps_fp32++;
+    if (!f64 && setup.blocked) ctx->stats.sweeps_blocked++;
     if (k < timed_cap) {
       float t = 0.0f;
       HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->events[2 + 2 * k], ctx->events[3 + 2 * k]));
@@ -463,7 +464,7 @@ void gops_context_destroy(gops_context *ctx) {
   if (!ctx) return;
   // graphs made on the context keep it until they are destroyed too (Python
   // finalizers, for one, may run in any order)
-  if (--ctx->refs > 0) return;
+  if (ctx->refs.fetch_sub(1) > 1) return;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
@@ -588,7 +589,7 @@ gops_status gops_graph_create(gops_context *ctx, uint64_t n, const uint64_t *off
     delete g;
     return st;
   }
-  ctx->refs++;  // released with the graph
+  ctx->refs.fetch_add(1);  // released with the graph
   *out = g;
   return GOPS_OK;
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -203,14 +204,15 @@ struct TopkScratch;
 
 struct gops_graph {
   gops_context *ctx = nullptr;
-  int refs = 1;            // the caller's handle, plus one per live server on the graph
+  // the caller's handle, plus one per live server on the graph; atomic because
+  // finalizers may release handles from any thread
+  std::atomic<int> refs{1};
   gops::PprGraphCache *ppr_cache = nullptr;  // batched PPR plans (server.cpp)
   gops::DevGraph dev;
   gops::DevGraph fwd;      // out-edge CSR in new ids (row_ptr, col, plan only), built on demand
   bool has_fwd = false;
   gops::BlockedLayout pb;  // blocked delta sweep, built on demand
   bool has_pb = false;
-  bool pb_no_room = false;  // the layout did not fit in device memory: pull by default
   gops::RunBuffers run;
   gops::PowerBuffers pow;
   bool weighted = false;   // created with edge weights
@@ -225,7 +227,7 @@ constexpr int kPinChunks = 3;  // pinned chunks of the staged host <-> device co
 
 struct gops_context {
   int device = 0;
-  int refs = 1;  // the caller's handle, plus one per live graph made on it
+  std::atomic<int> refs{1};  // the caller's handle, plus one per live graph made on it
   hipStream_t stream = nullptr;
   std::string last_error;
   int precision = GOPS_PRECISION_MIXED;

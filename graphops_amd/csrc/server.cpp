@@ -797,7 +797,7 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
     }
     s->tiles.push_back(t);
   }
-  g->refs++;  // the server keeps its graph (and so the context) until destroyed
+  g->refs.fetch_add(1);  // the server keeps its graph (and so the context) until destroyed
   s->holds_graph = true;
   *out = s;
   return GOPS_OK;

@@ -128,6 +128,11 @@ typedef struct gops_run_stats {
     double fp32_sweep_ms;   /* summed duration of f32 sweep kernels (GOPS_OPT_TIMING) */
     double fp64_sweep_ms;   /* summed duration of f64 sweep kernels (GOPS_OPT_TIMING) */
     double other_ms;        /* reserved */
+    uint64_t sweeps_blocked; /* f32 delta sweeps that ran in the blocked layout */
+    int32_t blocked_fallback; /* 1: GOPS_DELTA_SWEEP_AUTO chose the blocked layout but it
+                                 did not fit in free device memory, so the f32 delta
+                                 sweeps ran as pulls (retried on later runs) */
+    int32_t reserved2;
 } gops_run_stats;
 
 /* ---- context ---------------------------------------------------------- */

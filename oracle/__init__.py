@@ -54,6 +54,9 @@ def lib():
         L.orc_pull_run.argtypes = [ctypes.c_int, ctypes.c_uint64, _u64p, _u32p, _f64p, _f64p, _f64p,
                                    ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int,
                                    _f64p, _u64p, _f64p, _i32p]
+        L.orc_pull_run_ld.argtypes = [ctypes.c_int, ctypes.c_uint64, _u64p, _u32p, _f64p, _f64p,
+                                      ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.c_int, _f64p, _u64p, _f64p, _i32p]
         L.orc_push_sweep.argtypes = [ctypes.c_int, ctypes.c_uint64, _u64p, _u32p, _f64p, _f64p,
                                      ctypes.c_double, _f64p, _f64p]
         L.orc_push_sweep.restype = ctypes.c_double
@@ -217,6 +220,26 @@ def pull_run(kind, in_off, in_src, scale, w=None, p=None, damping=0.85, max_iter
     lib().orc_pull_run(kind, n, _p(in_off, _u64p), _p(in_src, _u32p), _p(wc, _f64p),
                        _p(scale, _f64p), _p(pc, _f64p), damping, max_iterations, tolerance,
                        threads, _p(s, _f64p), _p(it, _u64p), _p(d, _f64p), _p(c, _i32p))
+    return _run(s, it, d, c, n)
+
+
+def pull_run_ld(kind, in_off, in_src, w=None, p=None, damping=0.85, max_iterations=100,
+                tolerance=1e-6, threads=0) -> Run:
+    """The iteration of pagerank_run (kind 0), pagerank_weighted_run (kind 1)
+    or personalized_pagerank_run (kind 2) carried in extended precision (long
+    double): a stand-in for the exact iteration at sizes fractions cannot
+    handle.  Out-degrees / out-weight sums are formed from the in-edges;
+    w = max(edge_weight, 0) per in-edge (kind 1).  Scores and the residual
+    come back rounded to f64."""
+    in_off = np.ascontiguousarray(in_off, dtype=np.uint64)
+    in_src = np.ascontiguousarray(in_src, dtype=np.uint32)
+    wc = np.ascontiguousarray(w, dtype=np.float64) if w is not None else None
+    pc = np.ascontiguousarray(p, dtype=np.float64) if p is not None else None
+    n = in_off.size - 1
+    s, it, d, c = _tail(n)
+    lib().orc_pull_run_ld(kind, n, _p(in_off, _u64p), _p(in_src, _u32p), _p(wc, _f64p),
+                          _p(pc, _f64p), damping, max_iterations, tolerance, threads,
+                          _p(s, _f64p), _p(it, _u64p), _p(d, _f64p), _p(c, _i32p))
     return _run(s, it, d, c, n)
 
 

@@ -433,6 +433,116 @@ void orc_pull_run(int kind, uint64_t n, const uint64_t *in_off, const uint32_t *
     *conv_out = converged;
 }
 
+/*
+ * The same iteration in extended precision (x87 long double, 64-bit
+ * significand): the reference's recurrence (pagerank.rs:133-179, 221-281;
+ * ppr.rs:104-149) with every quantity -- out-degree or out_wsum, dangling
+ * sum, fill, shares, row sums, residual -- carried in long double, so that
+ * its rounding (~1e-19 relative) is far below the f64 reference's own.  It
+ * stands for the exact iteration at sizes where fractions cannot run: the
+ * f64 reference drifts from it by about n * 2^-53 in total mass, so at
+ * tolerances near 1e-12 the two can stop at different sweeps.
+ *   kind 0/1/2 as orc_pull_run; w[e] = max(edge_weight, 0) per in-edge, or
+ *   NULL for kinds 0 and 2.  The out-weight sums are formed here from the
+ *   in-edges (in long double: exact for the sums this is used on).
+ * Rows run in parallel; dsum and diff are summed over a fixed partition of
+ * 256 index blocks, in block order, so results do not depend on the thread
+ * count.
+ */
+#define LD_BLOCKS 256
+static long double ld_block_sum(uint64_t n, const long double *a, const long double *b,
+                                const long double *scale, int mode, int threads) {
+    long double part[LD_BLOCKS];
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int k = 0; k < LD_BLOCKS; k++) {
+        uint64_t lo = n * (uint64_t)k / LD_BLOCKS, hi = n * (uint64_t)(k + 1) / LD_BLOCKS;
+        long double s = 0.0L;
+        for (uint64_t i = lo; i < hi; i++) {
+            if (mode == 0) s += fabsl(a[i] - b[i]);        /* residual */
+            else if (scale[i] == 0.0L) s += a[i];           /* dangling sum */
+        }
+        part[k] = s;
+    }
+    long double s = 0.0L;
+    for (int k = 0; k < LD_BLOCKS; k++) s += part[k];
+    return s;
+}
+
+void orc_pull_run_ld(int kind, uint64_t n, const uint64_t *in_off, const uint32_t *in_src,
+                     const double *w, const double *personalization, double damping,
+                     uint64_t max_iterations, double tolerance, int threads, double *scores,
+                     uint64_t *iters_out, double *diff_out, int *conv_out) {
+    if (n == 0) {
+        run_empty(diff_out, iters_out, conv_out);
+        return;
+    }
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#else
+    threads = 1;
+#endif
+    const long double d = (long double)damping, nl = (long double)n;
+    long double *cur = (long double *)malloc(n * sizeof(long double));
+    long double *nxt = (long double *)malloc(n * sizeof(long double));
+    long double *scale = (long double *)calloc(n, sizeof(long double));
+    long double *y = (long double *)malloc(n * sizeof(long double));
+    long double *q = NULL;
+    /* out-degree / out_wsum of every source (pagerank.rs:137, 227-234) */
+    for (uint64_t v = 0; v < n; v++)
+        for (uint64_t e = in_off[v]; e < in_off[v + 1]; e++)
+            scale[in_src[e]] += (kind == 1) ? (long double)w[e] : 1.0L;
+    if (kind == 2) { /* ppr.rs:104-110 */
+        q = (long double *)malloc(n * sizeof(long double));
+        long double ps = 0.0L;
+        for (uint64_t i = 0; i < n; i++) ps += (long double)personalization[i];
+        for (uint64_t i = 0; i < n; i++)
+            q[i] = ps > 0.0L ? (long double)personalization[i] / ps : 1.0L / nl;
+        memcpy(cur, q, n * sizeof(long double));
+    } else {
+        for (uint64_t i = 0; i < n; i++) cur[i] = 1.0L / nl;
+    }
+    uint64_t iters = 0;
+    long double last = INFINITY;
+    int converged = 0;
+    for (uint64_t it = 0; it < max_iterations; it++) {
+        iters++;
+        const long double dsum = ld_block_sum(n, cur, NULL, scale, 1, threads);
+        const long double fill = (1.0L - d) / nl + d * dsum / nl;
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int64_t u = 0; u < (int64_t)n; u++)
+            y[u] = scale[u] != 0.0L ? d * cur[u] / scale[u] : 0.0L;
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(threads)
+        for (int64_t vv = 0; vv < (int64_t)n; vv++) {
+            const uint64_t v = (uint64_t)vv;
+            long double acc = kind == 2 ? (1.0L - d) * q[v] + d * dsum * q[v] : fill;
+            for (uint64_t e = in_off[v]; e < in_off[v + 1]; e++) {
+                if (kind == 1) {
+                    if (w[e] > 0.0) acc += (long double)w[e] * y[in_src[e]];
+                } else {
+                    acc += y[in_src[e]];
+                }
+            }
+            nxt[v] = acc;
+        }
+        const long double diff = ld_block_sum(n, cur, nxt, NULL, 0, threads);
+        last = diff;
+        long double *t = cur; cur = nxt; nxt = t;
+        if (diff < (long double)tolerance) {
+            converged = 1;
+            break;
+        }
+    }
+    for (uint64_t i = 0; i < n; i++) scores[i] = (double)cur[i];
+    free(cur);
+    free(nxt);
+    free(scale);
+    free(y);
+    free(q);
+    *iters_out = iters;
+    *diff_out = (double)last;
+    *conv_out = converged;
+}
+
 /* One push sweep of pagerank_weighted_run / pagerank_run (pagerank.rs:239-263,
  * 142-162) from a given vector, for timing a bounded sample of the CPU path.
  * Returns the sweep's L1 residual; writes the new vector into out[n]. */

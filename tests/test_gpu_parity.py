@@ -355,3 +355,40 @@ def test_unaligned_runs_and_hub_chunks(oracle, ctx, ctx64):
             r = (ga.pagerank_weighted_run if kind else ga.pagerank_run)(dg, cfg, ctx=cx)
             check_run(r, ref, tol, prec, f"unaligned-{kind}")
             dg.close()
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_normalize_option(oracle, kind):
+    """GOPS_OPT_NORMALIZE: the f64 sum-to-one rescale of the result (the
+    reference never renormalises, lib.rs:11-12, so it is opt-in).  It must
+    leave a sum of 1 to f64 rounding and move the scores by far less than the
+    tolerance: within 1e-2 * tol of the same run without it, and within the
+    usual bound of the oracle."""
+    e = rmat_case(15, 9, "lognormal" if kind == 1 else "unit", "bf16" if kind == 1 else "f64")
+    tol = 1e-9
+    p = None
+    if kind == 2:
+        p = np.zeros(e.n)
+        p[[3, 40, 777]] = [1.0, 2.0, 0.5]
+    ref = oracle_pull(oracle, e, kind, tol, 300, p=p)
+    runs = {}
+    for norm in (False, True):
+        cx = ga.Context(0, precision="mixed", normalize=norm)
+        dg = ga.DeviceGraph.from_edges(e, weight_storage="bf16" if kind == 1 else "f64", ctx=cx)
+        cfg = ga.PageRankConfig(0.85, 300, tol)
+        if kind == 0:
+            runs[norm] = ga.pagerank_run(dg, cfg, ctx=cx)
+        elif kind == 1:
+            runs[norm] = ga.pagerank_weighted_run(dg, cfg, ctx=cx)
+        else:
+            runs[norm] = ga.personalized_pagerank_run(dg, cfg, p, ctx=cx)
+        dg.close()
+        cx.close()
+    plain, normed = runs[False], runs[True]
+    assert abs(normed.scores.sum() - 1.0) <= 64 * np.finfo(float).eps
+    assert normed.iterations == plain.iterations and normed.diff_l1 == plain.diff_l1
+    assert np.abs(normed.scores - plain.scores).sum() <= 1e-2 * tol
+    # the device divides by its own fixed-order f64 sum of the scores
+    np.testing.assert_allclose(normed.scores, plain.scores / plain.scores.sum(), rtol=1e-14,
+                               atol=0)
+    check_run(normed, ref, tol, "mixed", f"normalize-{kind}")

@@ -243,3 +243,98 @@ def test_planted_mistakes_are_caught(golden, oracle):
     m2[2, 2] = 1.0
     off, nb, _ = oracle.dense_neighbors(m2)
     assert np.abs(oracle.pagerank_run(off, nb, 0.85, 1000, 1e-13).scores - xl).sum() > 1e-3
+
+
+def ld_run_of_trajectory(oracle, t, pull=None):
+    """The trajectory's graph run through pull_run_ld (or through `pull`, an
+    alternative with the same signature, for the planted-mistake check)."""
+    off, nb, ew = oracle.dense_neighbors(np.array(t["matrix"], dtype=float))
+    kind = 2 if t["personalization"] is not None else (1 if t["weighted"] else 0)
+    in_off, in_src, in_w = oracle.transpose(off, nb, ew if kind == 1 else None)
+    return (pull or oracle.pull_run_ld)(kind, in_off, in_src, w=in_w, p=t["personalization"],
+                                        damping=t["damping"], max_iterations=t["max_iterations"],
+                                        tolerance=t["tolerance"], threads=2)
+
+
+def ld_trajectory_misses(oracle, golden, pull=None):
+    """Trajectories on which a run misses the exact iteration by more than
+    extended precision allows: iterations, stop, the last residual within
+    2e-16 relative + 1e-18 and scores within one f64 rounding of the exact
+    rationals.  (The f64 reference's iterates carry ~1e-16 of absolute noise,
+    which the residual bound does not admit.)"""
+    misses = []
+    for t in golden["trajectories"]:
+        r = ld_run_of_trajectory(oracle, t, pull)
+        ok = (r.iterations == t["iterations"] and r.converged == t["converged"]
+              and abs(r.diff_l1 - t["diff_l1"]) <= 2e-16 * t["diff_l1"] + 1e-18
+              and np.allclose(r.scores, t["scores"], rtol=2.3e-16, atol=1e-19))
+        if not ok:
+            misses.append(t["name"])
+    return misses
+
+
+def test_extended_precision_pull_is_the_exact_iteration(golden, oracle):
+    """pull_run_ld (the long double iteration the full-size parity tests
+    compare with) follows the exact-rational trajectories of pagerank_run,
+    pagerank_weighted_run and personalized_pagerank_run to extended
+    precision -- and the f64 restatement does not meet the same bound, so
+    the pin would catch the extended pull degrading to f64."""
+    assert ld_trajectory_misses(oracle, golden) == []
+
+    def f64_pull(kind, in_off, in_src, w=None, p=None, **kw):
+        n = in_off.size - 1
+        scale = np.zeros(n)
+        np.add.at(scale, in_src, w if kind == 1 else 1.0)
+        return oracle.pull_run(kind, in_off, in_src, scale, w=w, p=p, **kw)
+
+    assert len(ld_trajectory_misses(oracle, golden, f64_pull)) >= 3
+
+
+def test_extended_precision_pull_fixed_points(golden, oracle):
+    """Closed forms and the C1 dense solve: within the contraction bound of
+    the last residual, with no f64 noise floor."""
+    for case in golden["closed_forms"]:
+        mat = np.array(case["matrix"], dtype=float)
+        off, nb, ew = oracle.dense_neighbors(mat)
+        kind = 2 if case["personalization"] is not None else (1 if case["weighted"] else 0)
+        in_off, in_src, in_w = oracle.transpose(off, nb, ew if kind == 1 else None)
+        r = oracle.pull_run_ld(kind, in_off, in_src, w=in_w, p=case["personalization"],
+                               damping=case["damping"], max_iterations=10000, tolerance=1e-15)
+        x = np.array([frac(s) for s in case["fixed_point"]])
+        assert r.converged, case["name"]
+        assert np.abs(r.scores - x).sum() <= BOUND * r.diff_l1 + 4e-16, case["name"]
+    for case in golden["dense_solves"]:
+        off, nb = lists_from(case["neighbors"])
+        ew = (np.array([w for lst in case["edge_weights"] for w in lst], dtype=float)
+              if case.get("edge_weights") is not None else None)
+        kind = {"plain": 0, "weighted": 1, "ppr": 2}[case["kind"]]
+        in_off, in_src, in_w = oracle.transpose(off, nb, ew if kind == 1 else None)
+        r = oracle.pull_run_ld(kind, in_off, in_src, w=in_w, p=case.get("personalization"),
+                               damping=case["damping"], max_iterations=10000, tolerance=1e-15)
+        x = np.array(case["fixed_point"])
+        assert np.abs(r.scores - x).sum() <= BOUND * r.diff_l1 + 1e-13, case["name"]
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_push_sweep_is_the_runs_sweep(oracle, weighted):
+    """push_sweep (the CPU baseline's timed unit) from 1/n is the first sweep
+    of pagerank_run / pagerank_weighted_run bit for bit, its residual is the
+    run's diff_l1, and sweep k+1 from the run's k-sweep result is the run's
+    (k+1)-sweep result."""
+    rng = np.random.default_rng(31 + weighted)
+    for trial in range(6):
+        n = int(rng.integers(1, 200))
+        off, nb, ew = random_graph(rng, n, int(rng.integers(0, 5 * n)), weighted)
+
+        def run(k):
+            if weighted:
+                return oracle.pagerank_weighted_run(off, nb, ew, 0.85, k, 1e-300)
+            return oracle.pagerank_run(off, nb, 0.85, k, 1e-300)
+
+        ws = oracle.out_wsum(off, ew) if weighted else None
+        x = np.full(n, 1.0 / n)
+        for k in range(1, 4):
+            x, diff = oracle.push_sweep(off, nb, x, 0.85, ew=ew if weighted else None, ws=ws)
+            ref = run(k)
+            assert np.array_equal(x, ref.scores, equal_nan=True), (trial, k)
+            assert diff == ref.diff_l1 or (np.isnan(diff) and np.isnan(ref.diff_l1))
