@@ -6,17 +6,26 @@
 // HBM bandwidth, sets its rate.  The blocked sweep (kernels in pagerank.hip)
 // replaces the gathers by two passes that stream:
 //   scatter     a workgroup holds the deltas of one chunk of sources in LDS
-//               and walks that chunk's edges (chunk-major order), writing each
-//               product to the entry's place in the bin-major order;
+//               and walks that chunk's edges in (row, source) order, eight
+//               consecutive entries per lane; the products of each run of one
+//               row are summed and one f32 per run goes to the run's place in
+//               bin-major order;
 //   accumulate  a workgroup holds fixed-point sums of one bin of rows in LDS
-//               and walks the bin's products in order.
-// Both orders keep the CSR's order within a row, and the sums are integers,
-// so every result is independent of scheduling.
+//               and walks the bin's run sums in order.
+// The summation order within a run is fixed by the layout and the row sums
+// are integers, so every result is independent of scheduling.
 //
-// Entry order:
-//   chunk-major: (source >> kPbChunkShift, row, source)
-//   bin-major:   (row >> kPbBinShift, source >> kPbChunkShift, row, source)
-// Each is a stable sort of the CSR edge index by a key of at most 49 bits.
+// Orders:
+//   entries, chunk-major:  (source >> kPbChunkShift, row, source), padded so
+//                          that each segment (one chunk into one bin) starts at
+//                          a multiple of kPbLane entries and each chunk at a
+//                          multiple of kPbWin (a window, one wave's entries)
+//   runs, bin-major:       (row >> kPbBinShift, source chunk, row, run number)
+// A run is a maximal sequence of entries of one row within one window.  A
+// segment's runs are contiguous in both orders, so a run's bin-major place is
+// its chunk-major number plus its segment's offset.  The scatter finds an
+// entry's run and segment from the head bits in the source words and the
+// numbers of its window's first run and segment: no per-entry index is stored.
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>
@@ -51,109 +60,141 @@ __global__ void pb_mark_kernel(const int32_t *row_ptr, int n, int32_t *marks) {
     if (row_ptr[v] < row_ptr[v + 1]) marks[row_ptr[v]] = v;
 }
 
-// key of edge k in one of the two orders; value = k
-__global__ void pb_keys_kernel(const int32_t *col, const int32_t *row, uint32_t m, int bin_major,
-                               int sbits, uint64_t *keys, uint32_t *vals) {
+// chunk-major key of edge k (chunk of the source, row); value = k.  A stable
+// sort keeps the CSR's ascending sources within a row.
+__global__ void pb_keys_kernel(const int32_t *col, const int32_t *row, uint32_t m, uint64_t *keys,
+                               uint32_t *vals) {
   for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < m; k += gridDim.x * kB) {
     const uint64_t u = (uint32_t)col[k], v = (uint32_t)row[k];
-    const uint64_t s = u >> kPbChunkShift;
-    uint64_t key;
-    if (bin_major) {
-      key = ((v >> kPbBinShift) << (sbits + kPbBinShift)) | (s << kPbBinShift) |
-            (v & ((1u << kPbBinShift) - 1));
-    } else {
-      key = (s << 31) | v;
-    }
-    keys[k] = key;
+    keys[k] = ((u >> kPbChunkShift) << 31) | v;
     vals[k] = k;
   }
 }
 
-// bin-major: position_of[edge] = its index; dst = row - bin base
-__global__ void pb_binmajor_kernel(const uint64_t *keys, const uint32_t *vals, uint32_t m,
-                                   uint32_t *position_of, uint16_t *dst) {
-  for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < m; i += gridDim.x * kB) {
-    position_of[vals[i]] = i;
-    dst[i] = (uint16_t)(keys[i] & ((1u << kPbBinShift) - 1));
-  }
-}
+__device__ __forceinline__ uint32_t key_chunk(uint64_t key) { return (uint32_t)(key >> 31); }
+__device__ __forceinline__ uint32_t key_row(uint64_t key) { return (uint32_t)(key & 0x7fffffffu); }
 
-// chunk-major: the entry's place in bin-major order, source - chunk base, weight
-template <typename W>
-__global__ void pb_chunkmajor_kernel(const uint32_t *vals, uint32_t m, const int32_t *col,
-                                     const uint32_t *position_of, const W *w, uint32_t *pos,
-                                     uint16_t *src, W *w_out) {
+// segment starts among the sorted entries: a new chunk or a new bin of rows
+__global__ void pb_seg_flag_kernel(const uint64_t *keys, uint32_t m, int32_t *flag) {
   for (uint32_t j = blockIdx.x * kB + threadIdx.x; j < m; j += gridDim.x * kB) {
+    const uint64_t k = keys[j];
+    bool f = j == 0;
+    if (!f) {
+      const uint64_t q = keys[j - 1];
+      f = key_chunk(k) != key_chunk(q) || (key_row(k) >> kPbBinShift) != (key_row(q) >> kPbBinShift);
+    }
+    flag[j] = f;
+  }
+}
+
+// the first entry of every segment (sg: inclusive count of segment starts)
+__global__ void pb_seg_first_kernel(const int32_t *flag, const int32_t *sg, uint32_t m,
+                                    int32_t *seg_first) {
+  for (uint32_t j = blockIdx.x * kB + threadIdx.x; j < m; j += gridDim.x * kB)
+    if (flag[j]) seg_first[sg[j] - 1] = (int32_t)j;
+}
+
+// padded length of every segment (a multiple of kPbLane); 0 at [nseg]
+__global__ void pb_seg_len_kernel(const int32_t *seg_first, uint32_t nseg, int32_t *plen) {
+  for (uint32_t q = blockIdx.x * kB + threadIdx.x; q <= nseg; q += gridDim.x * kB)
+    plen[q] = q < nseg ? ((seg_first[q + 1] - seg_first[q] + kPbLane - 1) / kPbLane) * kPbLane : 0;
+}
+
+// out[i] = in[idx[i]]
+__global__ void pb_gather_kernel(const int32_t *in, const int32_t *idx, int count, int32_t *out) {
+  for (int i = blockIdx.x * kB + threadIdx.x; i < count; i += gridDim.x * kB) out[i] = in[idx[i]];
+}
+
+// Every real entry to its padded place p: source word (run and segment head
+// bits), weight; posof[j] = p.  shift[c] = padded start of chunk c minus the
+// padded start (G) of its first segment.
+template <typename W>
+__global__ void pb_place_entries_kernel(const uint64_t *keys, const uint32_t *vals, const int32_t *flag,
+                                        const int32_t *sg, const int32_t *seg_first, const int32_t *G,
+                                        const int32_t *shift, const int32_t *col, const W *w,
+                                        uint32_t m, uint16_t *src, W *w_out, uint8_t *one_out,
+                                        int32_t *posof) {
+  for (uint32_t j = blockIdx.x * kB + threadIdx.x; j < m; j += gridDim.x * kB) {
+    const uint64_t key = keys[j];
+    const int sgi = sg[j] - 1;
+    const int32_t p = shift[key_chunk(key)] + G[sgi] + ((int32_t)j - seg_first[sgi]);
+    const bool seg_head = flag[j] != 0;
+    const bool head = seg_head || (p % kPbWin) == 0 || key_row(keys[j - 1]) != key_row(key);
     const uint32_t k = vals[j];
-    pos[j] = position_of[k];
-    src[j] = (uint16_t)((uint32_t)col[k] & ((1u << kPbChunkShift) - 1));
-    if (w) w_out[j] = w[k];
+    src[p] = (uint16_t)(((uint32_t)col[k] & kPbSrcMask) | (head ? kPbRunHead : 0u) |
+                        (seg_head ? kPbSegHead : 0u));
+    if (w) w_out[p] = w[k];
+    if (one_out) one_out[p] = 1;
+    posof[j] = p;
   }
 }
 
-// Segments of a scatter item: maximal runs of entries whose bin-major place
-// is the chunk-major index plus one constant (a run of one chunk into one
-// bin is one).  Thread t of an item's workgroup takes kPer consecutive entries.
-constexpr int kPer = kPbItem1 / kB;
-
-__device__ __forceinline__ bool seg_head(const PbItem &it, const uint32_t *pos, int j) {
-  return j == it.y || (int64_t)pos[j] - j != (int64_t)pos[j - 1] - (j - 1);
-}
-
-__device__ __forceinline__ int block_excl_scan(int v, int *sh, int *total) {
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int i = 0; i < kB; i++) {
-      const int x = sh[i];
-      sh[i] = acc;
-      acc += x;
+// Padding after every segment's entries: up to the next segment's padded
+// start, or for a chunk's last segment up to the chunk's padded end; each
+// padding entry repeats the source of the segment's last entry (no head
+// bits, so it continues that run) with weight 0.
+template <typename W>
+__global__ void pb_pad_kernel(const uint64_t *keys, const int32_t *seg_first, const int32_t *G,
+                              const int32_t *shift, const int32_t *chunk_end, uint32_t nseg,
+                              uint16_t *src, W *w_out, uint8_t *one_out) {
+  for (uint32_t q = blockIdx.x * kB + threadIdx.x; q < nseg; q += gridDim.x * kB) {
+    const uint32_t c = key_chunk(keys[seg_first[q]]);
+    const int32_t e = shift[c] + G[q] + (seg_first[q + 1] - seg_first[q]);
+    const bool last = q + 1 == nseg || key_chunk(keys[seg_first[q + 1]]) != c;
+    const int32_t pend = last ? chunk_end[c] : shift[c] + G[q + 1];
+    const uint16_t s = (uint16_t)(src[e - 1] & kPbSrcMask);
+    for (int32_t p = e; p < pend; p++) {
+      src[p] = s;
+      if (w_out) w_out[p] = W(0);
+      if (one_out) one_out[p] = 0;
     }
-    sh[kB] = acc;
   }
-  __syncthreads();
-  const int r = sh[threadIdx.x];
-  *total = sh[kB];
-  __syncthreads();
-  return r;
 }
 
-__global__ void pb_seg_count_kernel(const PbItem *items, const uint32_t *pos, int *count) {
-  __shared__ int sh[kB + 1];
-  const PbItem it = items[blockIdx.x];
-  const int j0 = it.y + (int)threadIdx.x * kPer;
-  int c = 0;
-  for (int q = 0; q < kPer; q++) {
-    const int j = j0 + q;
-    if (j < it.z) c += seg_head(it, pos, j);
-  }
-  int total;
-  (void)block_excl_scan(c, sh, &total);
-  if (threadIdx.x == 0) count[blockIdx.x] = total;
+// run-head flags of the padded entries, for the scan of run numbers
+__global__ void pb_is_head_kernel(const uint16_t *src, uint32_t mp, int32_t *flag) {
+  for (uint32_t p = blockIdx.x * kB + threadIdx.x; p < mp; p += gridDim.x * kB)
+    flag[p] = (src[p] & kPbRunHead) != 0;
 }
 
-__global__ void pb_seg_write_kernel(const PbItem *items, const uint32_t *pos, uint16_t *seg,
-                                    int32_t *seg_off) {
-  __shared__ int sh[kB + 1];
-  const PbItem it = items[blockIdx.x];
-  const int j0 = it.y + (int)threadIdx.x * kPer;
-  int c = 0;
-  for (int q = 0; q < kPer; q++) {
-    const int j = j0 + q;
-    if (j < it.z) c += seg_head(it, pos, j);
-  }
-  int total;
-  int r = block_excl_scan(c, sh, &total) - 1;  // the segment before this thread's first entry
-  for (int q = 0; q < kPer; q++) {
-    const int j = j0 + q;
-    if (j >= it.z) break;
-    if (seg_head(it, pos, j)) {
-      r++;
-      seg_off[it.w + r] = (int32_t)((int64_t)pos[j] - j);
+// Per run (at its head, a real entry; runid = inclusive count of heads): the
+// bin-major key (bin, chunk, row in bin), value = run number; every
+// segment's first run; every window's first run and segment.
+__global__ void pb_run_keys_kernel(const uint64_t *keys, const int32_t *posof, const uint16_t *src,
+                                   const int32_t *runid, const int32_t *flag, const int32_t *sg,
+                                   uint32_t m, int sbits, uint64_t *rkey, uint32_t *rval,
+                                   int32_t *seg_run, int32_t *win) {
+  for (uint32_t j = blockIdx.x * kB + threadIdx.x; j < m; j += gridDim.x * kB) {
+    const int32_t p = posof[j];
+    if (!(src[p] & kPbRunHead)) continue;
+    const uint32_t r = (uint32_t)runid[p] - 1;
+    const uint64_t key = keys[j];
+    const uint64_t v = key_row(key), c = key_chunk(key);
+    rkey[r] = ((v >> kPbBinShift) << (sbits + kPbBinShift)) | (c << kPbBinShift) |
+              (v & ((1u << kPbBinShift) - 1));
+    rval[r] = r;
+    if (flag[j]) seg_run[sg[j] - 1] = (int32_t)r;
+    if (p % kPbWin == 0) {
+      win[2 * (p / kPbWin)] = (int32_t)r;
+      win[2 * (p / kPbWin) + 1] = sg[j] - 1;
     }
-    seg[j] = (uint16_t)r;
   }
+}
+
+// bin-major place of every run, and its row within the bin
+__global__ void pb_place_kernel(const uint64_t *rkey_sorted, const uint32_t *rval_sorted,
+                                uint32_t nr, int32_t *place_of, uint16_t *rrow) {
+  for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < nr; i += gridDim.x * kB) {
+    place_of[rval_sorted[i]] = (int32_t)i;
+    rrow[i] = (uint16_t)(rkey_sorted[i] & ((1u << kPbBinShift) - 1));
+  }
+}
+
+// seg_off per segment: place of its first run - that run's number
+__global__ void pb_seg_off_kernel(const int32_t *place_of, const int32_t *seg_run, uint32_t nseg,
+                                  int32_t *seg_off) {
+  for (uint32_t q = blockIdx.x * kB + threadIdx.x; q < nseg; q += gridDim.x * kB)
+    seg_off[q] = place_of[seg_run[q]] - seg_run[q];
 }
 
 // out[j] = first index i with keys[i] >= j << shift, for j in [0, count]
@@ -184,6 +225,11 @@ struct Pool {
     return static_cast<T *>(p);
   }
   void keep(void *p) { ptrs.erase(std::remove(ptrs.begin(), ptrs.end(), p), ptrs.end()); }
+  void release(void *p) {
+    if (!p) return;
+    keep(p);
+    (void)hipFree(p);
+  }
   ~Pool() {
     for (void *p : ptrs) (void)hipFree(p);
   }
@@ -194,28 +240,39 @@ size_t weight_bytes(int storage) {
                                       : storage == GOPS_WEIGHT_BF16 ? 2 : 0;
 }
 
-hipError_t launch_chunkmajor(int storage, const uint32_t *vals, uint32_t m, const int32_t *col,
-                             const uint32_t *position_of, const void *w, uint32_t *pos,
-                             uint16_t *src, void *w_out, hipStream_t s) {
-  const int g = grid_for(m);
+template <typename W>
+hipError_t place_and_pad(const DevGraph &dg, const void *w, uint32_t m, uint32_t nseg,
+                         const uint64_t *keys, const uint32_t *vals, const int32_t *flag,
+                         const int32_t *sg, const int32_t *seg_first, const int32_t *G,
+                         const int32_t *shift, const int32_t *chunk_end, uint16_t *src,
+                         void *w_out, uint8_t *one_out, int32_t *posof, hipStream_t s) {
+  pb_place_entries_kernel<W><<<grid_for(m), kB, 0, s>>>(
+      keys, vals, flag, sg, seg_first, G, shift, dg.col, static_cast<const W *>(w), m, src,
+      static_cast<W *>(w_out), one_out, posof);
+  pb_pad_kernel<W><<<grid_for(nseg), kB, 0, s>>>(keys, seg_first, G, shift, chunk_end, nseg, src,
+                                                  static_cast<W *>(w_out), one_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_place_and_pad(int storage, const DevGraph &dg, uint32_t m, uint32_t nseg,
+                                const uint64_t *keys, const uint32_t *vals, const int32_t *flag,
+                                const int32_t *sg, const int32_t *seg_first, const int32_t *G,
+                                const int32_t *shift, const int32_t *chunk_end, uint16_t *src,
+                                void *w_out, uint8_t *mask, int32_t *posof, hipStream_t s) {
   switch (storage) {
     case GOPS_WEIGHT_F64:
-      pb_chunkmajor_kernel<double><<<g, kB, 0, s>>>(vals, m, col, position_of,
-                                                    static_cast<const double *>(w), pos, src,
-                                                    static_cast<double *>(w_out));
-      break;
+      return place_and_pad<double>(dg, dg.w, m, nseg, keys, vals, flag, sg, seg_first, G, shift,
+                                   chunk_end, src, w_out, mask, posof, s);
     case GOPS_WEIGHT_F32:
-      pb_chunkmajor_kernel<float><<<g, kB, 0, s>>>(vals, m, col, position_of,
-                                                   static_cast<const float *>(w), pos, src,
-                                                   static_cast<float *>(w_out));
-      break;
-    default:
-      pb_chunkmajor_kernel<uint16_t><<<g, kB, 0, s>>>(vals, m, col, position_of,
-                                                      static_cast<const uint16_t *>(w), pos, src,
-                                                      static_cast<uint16_t *>(w_out));
-      break;
+      return place_and_pad<float>(dg, dg.w, m, nseg, keys, vals, flag, sg, seg_first, G, shift,
+                                  chunk_end, src, w_out, mask, posof, s);
+    case GOPS_WEIGHT_BF16:
+      return place_and_pad<uint16_t>(dg, dg.w, m, nseg, keys, vals, flag, sg, seg_first, G,
+                                     shift, chunk_end, src, w_out, mask, posof, s);
+    default:  // unweighted graph: the mask alone
+      return place_and_pad<uint8_t>(dg, nullptr, m, nseg, keys, vals, flag, sg, seg_first, G,
+                                    shift, chunk_end, src, nullptr, mask, posof, s);
   }
-  return hipGetLastError();
 }
 
 // ranges [off[j], off[j + 1]) split into items of at most `most` entries;
@@ -241,17 +298,11 @@ void split_items(const std::vector<uint32_t> &off, uint32_t most, std::vector<Pb
 }  // namespace
 
 void free_blocked(BlockedLayout &pb) {
-  (void)hipFree(pb.src);
-  (void)hipFree(pb.w);
-  (void)hipFree(pb.seg);
-  (void)hipFree(pb.seg_off);
-  (void)hipFree(pb.dst);
-  (void)hipFree(pb.contrib);
-  (void)hipFree(pb.items1);
-  (void)hipFree(pb.items2);
-  (void)hipFree(pb.splits);
-  (void)hipFree(pb.bin_partial);
-  (void)hipFree(pb.partials);
+  for (void *p : {(void *)pb.src, pb.w, (void *)pb.mask, (void *)pb.win, (void *)pb.seg_off,
+                  (void *)pb.rrow, (void *)pb.rsum, (void *)pb.items1, (void *)pb.items2,
+                  (void *)pb.splits, (void *)pb.bin_partial, (void *)pb.bin_flags,
+                  (void *)pb.partials})
+    (void)hipFree(p);
   pb = BlockedLayout{};
 }
 
@@ -263,10 +314,15 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
   const int chunks = (int)std::max<uint64_t>(1, (n + (1u << kPbChunkShift) - 1) >> kPbChunkShift);
   const int bins = (int)std::max<uint64_t>(1, (n + (1u << kPbBinShift) - 1) >> kPbBinShift);
   const int sbits = bits_of((uint64_t)chunks);
-  const size_t wb = weight_bytes(dg.w ? dg.weight_storage : GOPS_WEIGHT_NONE);
+  const size_t wb = dg.w ? weight_bytes(dg.weight_storage) : 0;  // weight bytes per entry
+  const int storage = dg.w ? dg.weight_storage : GOPS_WEIGHT_NONE;
   Pool pool;
   BlockedLayout pb;
   std::vector<uint32_t> chunk_off(chunks + 1, 0), bin_off(bins + 1, 0);
+  std::vector<PbItem> items1, items2, splits;
+  int slots = 0;
+  uint32_t nr = 0, nseg = 0;
+  uint64_t mp = 0;  // padded entries
   hipError_t e = hipSuccess;
   const char *what = "";
 #define BTRY(expr)        \
@@ -278,20 +334,12 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
     }                     \
   } while (0)
   {
-    pb.src = pool.get<uint16_t>(m);
-    pb.seg = pool.get<uint16_t>(m);
-    uint32_t *pos = pool.get<uint32_t>(m);  // bin-major place of each chunk-major entry
-    pb.dst = pool.get<uint16_t>(m);
-    pb.contrib = pool.get<float>(m);
-    if (wb) pb.w = pool.get<uint8_t>(m * wb);
-    int32_t *row = pool.get<int32_t>(m);
-    uint64_t *keys = pool.get<uint64_t>(m), *keys_b = pool.get<uint64_t>(m);
-    uint32_t *vals = pool.get<uint32_t>(m), *vals_b = pool.get<uint32_t>(m);
-    uint32_t *position_of = pool.get<uint32_t>(m);
-    uint32_t *d_chunk_off = pool.get<uint32_t>(chunks + 1);
-    uint32_t *d_bin_off = pool.get<uint32_t>(bins + 1);
-    BTRY(pool.err);
     if (m) {
+      int32_t *row = pool.get<int32_t>(m);
+      uint64_t *keys = pool.get<uint64_t>(m), *keys_b = pool.get<uint64_t>(m);
+      uint32_t *vals = pool.get<uint32_t>(m), *vals_b = pool.get<uint32_t>(m);
+      uint32_t *d_chunk_off = pool.get<uint32_t>(chunks + 1);
+      BTRY(pool.err);
       // rows of the edges
       BTRY(hipMemsetAsync(vals, 0, m * sizeof(int32_t), s));
       pb_mark_kernel<<<grid_for(n), kB, 0, s>>>(dg.row_ptr, (int)n, (int32_t *)vals);
@@ -300,94 +348,191 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       BTRY(hipcub::DeviceScan::InclusiveScan(nullptr, b, (int32_t *)vals, row, hipcub::Max(),
                                              (int)m, s));
       tmp_bytes = std::max(tmp_bytes, b);
-      const int bits_bin = bits_of((uint64_t)bins) + sbits + kPbBinShift;
-      const int bits_chunk = sbits + 31;
+      // sized for the largest sort (entries; runs are fewer) and scans (the
+      // padded entries are the most)
       BTRY(hipcub::DeviceRadixSort::SortPairs(nullptr, b, keys, keys_b, vals, vals_b, (int)m, 0,
-                                              std::max(bits_bin, bits_chunk), s));
+                                              63, s));
+      tmp_bytes = std::max(tmp_bytes, b);
+      const uint64_t mp_max = m + (uint64_t)(kPbLane - 1) * m + (uint64_t)kPbWin * chunks;
+      if (mp_max < (1ull << 31)) {
+        BTRY(hipcub::DeviceScan::InclusiveSum(nullptr, b, (int32_t *)vals, (int32_t *)vals_b,
+                                              (int)mp_max, s));
+        tmp_bytes = std::max(tmp_bytes, b);
+      }
+      BTRY(hipcub::DeviceScan::InclusiveSum(nullptr, b, (int32_t *)vals, (int32_t *)vals_b,
+                                            (int)std::min<uint64_t>(mp_max, (1ull << 31) - 1), s));
+      tmp_bytes = std::max(tmp_bytes, b);
+      BTRY(hipcub::DeviceScan::ExclusiveSum(nullptr, b, (int32_t *)vals, (int32_t *)vals_b,
+                                            (int)m + 1, s));
       tmp_bytes = std::max(tmp_bytes, b);
       void *tmp = pool.get<uint8_t>(tmp_bytes);
       BTRY(pool.err);
       b = tmp_bytes;
       BTRY(hipcub::DeviceScan::InclusiveScan(tmp, b, (int32_t *)vals, row, hipcub::Max(), (int)m,
                                              s));
-      // bin-major order
-      pb_keys_kernel<<<grid_for(m), kB, 0, s>>>(dg.col, row, (uint32_t)m, 1, sbits, keys, vals);
+      // chunk-major order of the edges
+      pb_keys_kernel<<<grid_for(m), kB, 0, s>>>(dg.col, row, (uint32_t)m, keys, vals);
       BTRY(hipGetLastError());
       b = tmp_bytes;
       BTRY(hipcub::DeviceRadixSort::SortPairs(tmp, b, keys, keys_b, vals, vals_b, (int)m, 0,
-                                              bits_bin, s));
-      pb_binmajor_kernel<<<grid_for(m), kB, 0, s>>>(keys_b, vals_b, (uint32_t)m, position_of,
-                                                    pb.dst);
-      BTRY(hipGetLastError());
-      pb_bounds_kernel<<<grid_for(bins + 1), kB, 0, s>>>(keys_b, (uint32_t)m, bins,
-                                                         sbits + kPbBinShift, d_bin_off);
-      BTRY(hipGetLastError());
-      // chunk-major order
-      pb_keys_kernel<<<grid_for(m), kB, 0, s>>>(dg.col, row, (uint32_t)m, 0, sbits, keys, vals);
-      BTRY(hipGetLastError());
-      b = tmp_bytes;
-      BTRY(hipcub::DeviceRadixSort::SortPairs(tmp, b, keys, keys_b, vals, vals_b, (int)m, 0,
-                                              bits_chunk, s));
-      BTRY(launch_chunkmajor(dg.weight_storage, vals_b, (uint32_t)m, dg.col, position_of,
-                             wb ? dg.w : nullptr, pos, pb.src, pb.w, s));
+                                              sbits + 31, s));
+      pool.release(row);
+      pool.release(keys);
+      keys = nullptr;
       pb_bounds_kernel<<<grid_for(chunks + 1), kB, 0, s>>>(keys_b, (uint32_t)m, chunks, 31,
                                                            d_chunk_off);
       BTRY(hipGetLastError());
       BTRY(hipMemcpyAsync(chunk_off.data(), d_chunk_off, (chunks + 1) * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
-      BTRY(hipMemcpyAsync(bin_off.data(), d_bin_off, (bins + 1) * sizeof(uint32_t),
-                          hipMemcpyDeviceToHost, s));
-    }
-    BTRY(hipStreamSynchronize(s));
-    // work items: every bin gets one, also without entries (its rows still
-    // take the epilogue); chunks without entries get none
-    std::vector<PbItem> items1, items2, splits;
-    int slots = 0;
-    {
-      std::vector<uint32_t> off;
-      for (int c = 0; c < chunks; c++) {
-        if (chunk_off[c + 1] == chunk_off[c]) continue;
-        off.assign({chunk_off[c], chunk_off[c + 1]});
+      // segments: starts, numbers (sg, from 1), first entries, padded starts G
+      int32_t *flag = pool.get<int32_t>(m);
+      int32_t *sg = reinterpret_cast<int32_t *>(vals);  // vals (unsorted) is free now
+      BTRY(pool.err);
+      pb_seg_flag_kernel<<<grid_for(m), kB, 0, s>>>(keys_b, (uint32_t)m, flag);
+      BTRY(hipGetLastError());
+      b = tmp_bytes;
+      BTRY(hipcub::DeviceScan::InclusiveSum(tmp, b, flag, sg, (int)m, s));
+      int32_t last = 0;
+      BTRY(hipMemcpyAsync(&last, sg + (m - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      BTRY(hipStreamSynchronize(s));
+      nseg = (uint32_t)last;
+      pb.num_segs = nseg;
+      int32_t *seg_first = pool.get<int32_t>((size_t)nseg + 1);
+      int32_t *plen = pool.get<int32_t>((size_t)nseg + 1);
+      int32_t *G = pool.get<int32_t>((size_t)nseg + 1);
+      BTRY(pool.err);
+      pb_seg_first_kernel<<<grid_for(m), kB, 0, s>>>(flag, sg, (uint32_t)m, seg_first);
+      BTRY(hipGetLastError());
+      const int32_t m32 = (int32_t)m;
+      BTRY(hipMemcpyAsync(seg_first + nseg, &m32, sizeof(int32_t), hipMemcpyHostToDevice, s));
+      pb_seg_len_kernel<<<grid_for((uint64_t)nseg + 1), kB, 0, s>>>(seg_first, nseg, plen);
+      BTRY(hipGetLastError());
+      b = tmp_bytes;
+      BTRY(hipcub::DeviceScan::ExclusiveSum(tmp, b, plen, G, (int)nseg + 1, s));
+      // per chunk with edges: its first segment and G there, then its padded
+      // start and end (chunks start at multiples of kPbWin)
+      std::vector<int32_t> nonempty, first_entry;
+      for (int c = 0; c < chunks; c++)
+        if (chunk_off[c + 1] > chunk_off[c]) {
+          nonempty.push_back(c);
+          first_entry.push_back((int32_t)chunk_off[c]);
+        }
+      const int ne = (int)nonempty.size();
+      int32_t *d_idx = pool.get<int32_t>((size_t)ne + 1);
+      int32_t *d_out = pool.get<int32_t>((size_t)ne + 1);
+      BTRY(pool.err);
+      BTRY(hipMemcpyAsync(d_idx, first_entry.data(), ne * sizeof(int32_t), hipMemcpyHostToDevice,
+                          s));
+      pb_gather_kernel<<<grid_for(ne), kB, 0, s>>>(sg, d_idx, ne, d_out);
+      BTRY(hipGetLastError());
+      std::vector<int32_t> first_seg(ne + 1), g_at(ne + 1);
+      BTRY(hipMemcpyAsync(first_seg.data(), d_out, ne * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      BTRY(hipStreamSynchronize(s));
+      for (int i = 0; i < ne; i++) first_seg[i] -= 1;
+      first_seg[ne] = (int32_t)nseg;
+      BTRY(hipMemcpyAsync(d_idx, first_seg.data(), (ne + 1) * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
+      pb_gather_kernel<<<grid_for(ne + 1), kB, 0, s>>>(G, d_idx, ne + 1, d_out);
+      BTRY(hipGetLastError());
+      BTRY(hipMemcpyAsync(g_at.data(), d_out, (ne + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      BTRY(hipStreamSynchronize(s));
+      std::vector<int32_t> shift(chunks, 0), chunk_end(chunks, 0);
+      uint64_t at = 0;
+      for (int i = 0; i < ne; i++) {
+        const uint64_t total = (uint64_t)(g_at[i + 1] - g_at[i]);
+        const uint64_t padded = (total + kPbWin - 1) / kPbWin * kPbWin;
+        shift[nonempty[i]] = (int32_t)((int64_t)at - g_at[i]);
+        chunk_end[nonempty[i]] = (int32_t)(at + padded);
+        // scatter items: the chunk's windows
+        const uint32_t w0 = (uint32_t)(at / kPbWin), w1 = (uint32_t)((at + padded) / kPbWin);
+        std::vector<uint32_t> off{w0, w1};
         const size_t before = items1.size();
         split_items(off, kPbItem1, items1, nullptr, nullptr);
-        for (size_t i = before; i < items1.size(); i++) items1[i].x = c;
+        for (size_t t = before; t < items1.size(); t++) items1[t] = {nonempty[i], items1[t].y, items1[t].z, 0};
+        at += padded;
+        if (at >= (1ull << 31)) {
+          e = hipErrorOutOfMemory;
+          what = "padded entries beyond 2^31";
+          goto fail;
+        }
       }
-    }
-    split_items(bin_off, kPbItem2, items2, &splits, &slots);
-    pb.num_items1 = (int)items1.size();
-    pb.num_items2 = (int)items2.size();
-    pb.num_splits = (int)splits.size();
-    pb.items1 = pool.get<PbItem>(items1.size());
-    // the scatter items' segments: counted per item, then written from each
-    // item's first segment on
-    if (!items1.empty()) {
-      int *d_count = pool.get<int>(items1.size());
+      mp = at;
+      pb.m = mp;
+      int32_t *d_shift = pool.get<int32_t>(chunks), *d_end = pool.get<int32_t>(chunks);
+      pb.src = pool.get<uint16_t>(mp);
+      if (wb) pb.w = pool.get<uint8_t>(mp * wb);
+      pb.mask = pool.get<uint8_t>(mp);
+      pb.win = pool.get<int32_t>(2 * (mp / kPbWin));
+      int32_t *posof = pool.get<int32_t>(m);
       BTRY(pool.err);
-      BTRY(hipMemcpyAsync(pb.items1, items1.data(), items1.size() * sizeof(PbItem),
-                          hipMemcpyHostToDevice, s));
-      pb_seg_count_kernel<<<(int)items1.size(), kB, 0, s>>>(pb.items1, pos, d_count);
+      BTRY(hipMemcpyAsync(d_shift, shift.data(), chunks * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      BTRY(hipMemcpyAsync(d_end, chunk_end.data(), chunks * sizeof(int32_t), hipMemcpyHostToDevice,
+                          s));
+      BTRY(launch_place_and_pad(storage, dg, (uint32_t)m, nseg, keys_b, vals_b, flag, sg, seg_first,
+                                G, d_shift, d_end, pb.src, pb.w, pb.mask, posof, s));
+      // run numbers over the padded entries
+      int32_t *hflag = pool.get<int32_t>(mp), *runid = pool.get<int32_t>(mp);
+      BTRY(pool.err);
+      pb_is_head_kernel<<<grid_for(mp), kB, 0, s>>>(pb.src, (uint32_t)mp, hflag);
       BTRY(hipGetLastError());
-      std::vector<int> count(items1.size());
-      BTRY(hipMemcpyAsync(count.data(), d_count, count.size() * sizeof(int),
+      b = tmp_bytes;
+      BTRY(hipcub::DeviceScan::InclusiveSum(tmp, b, hflag, runid, (int)mp, s));
+      BTRY(hipMemcpyAsync(&last, runid + (mp - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      BTRY(hipStreamSynchronize(s));
+      pool.release(hflag);
+      nr = (uint32_t)last;
+      pb.num_runs = nr;
+      // runs: bin-major keys, sorted; places and rows; segment offsets
+      uint64_t *rkey = pool.get<uint64_t>(nr), *rkey_b = pool.get<uint64_t>(nr);
+      uint32_t *rval = pool.get<uint32_t>(nr), *rval_b = pool.get<uint32_t>(nr);
+      int32_t *place_of = pool.get<int32_t>(nr);
+      int32_t *seg_run = pool.get<int32_t>(nseg);
+      uint32_t *d_bin_off = pool.get<uint32_t>(bins + 1);
+      pb.rrow = pool.get<uint16_t>(nr);
+      pb.rsum = pool.get<float>(nr);
+      pb.seg_off = pool.get<int32_t>(nseg);
+      BTRY(pool.err);
+      pb_run_keys_kernel<<<grid_for(m), kB, 0, s>>>(keys_b, posof, pb.src, runid, flag, sg,
+                                                    (uint32_t)m, sbits, rkey, rval, seg_run,
+                                                    pb.win);
+      BTRY(hipGetLastError());
+      const int bits_bin = bits_of((uint64_t)bins) + sbits + kPbBinShift;
+      b = tmp_bytes;
+      BTRY(hipcub::DeviceRadixSort::SortPairs(tmp, b, rkey, rkey_b, rval, rval_b, (int)nr, 0,
+                                              bits_bin, s));
+      pb_place_kernel<<<grid_for(nr), kB, 0, s>>>(rkey_b, rval_b, nr, place_of, pb.rrow);
+      BTRY(hipGetLastError());
+      pb_bounds_kernel<<<grid_for(bins + 1), kB, 0, s>>>(rkey_b, nr, bins, sbits + kPbBinShift,
+                                                         d_bin_off);
+      BTRY(hipGetLastError());
+      pb_seg_off_kernel<<<grid_for(nseg), kB, 0, s>>>(place_of, seg_run, nseg, pb.seg_off);
+      BTRY(hipGetLastError());
+      BTRY(hipMemcpyAsync(bin_off.data(), d_bin_off, (bins + 1) * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
       BTRY(hipStreamSynchronize(s));
-      int64_t total = 0;
-      for (size_t i = 0; i < items1.size(); i++) {
-        items1[i].w = (int32_t)total;
-        total += count[i];
-      }
-      pb.seg_off = pool.get<int32_t>((size_t)total);
-      BTRY(pool.err);
-      BTRY(hipMemcpyAsync(pb.items1, items1.data(), items1.size() * sizeof(PbItem),
-                          hipMemcpyHostToDevice, s));
-      pb_seg_write_kernel<<<(int)items1.size(), kB, 0, s>>>(pb.items1, pos, pb.seg, pb.seg_off);
-      BTRY(hipGetLastError());
     }
+    pb.num_items1 = (int)items1.size();
+    pb.items1 = pool.get<PbItem>(items1.size());
+    // accumulate items: every bin gets one, also without runs (its rows still
+    // take the epilogue)
+    split_items(bin_off, kPbItem2, items2, &splits, &slots);
+    pb.num_items2 = (int)items2.size();
+    pb.num_splits = (int)splits.size();
     pb.items2 = pool.get<PbItem>(items2.size());
     pb.splits = pool.get<PbItem>(splits.size());
     pb.bin_partial = pool.get<long long>((size_t)std::max(slots, 1) << kPbBinShift);
+    pb.bin_flags = pool.get<uint8_t>((size_t)std::max(slots, 1) << kPbBinShift);
     pb.partials = pool.get<double>((items2.size() + splits.size()) * kMaxReduce);
+    if (!pb.src) pb.src = pool.get<uint16_t>(1);
+    if (!pb.mask) pb.mask = pool.get<uint8_t>(8);
+    if (!pb.win) pb.win = pool.get<int32_t>(2);
+    if (!pb.rrow) pb.rrow = pool.get<uint16_t>(1);
+    if (!pb.rsum) pb.rsum = pool.get<float>(1);
+    if (!pb.seg_off) pb.seg_off = pool.get<int32_t>(1);
     BTRY(pool.err);
+    if (!items1.empty())
+      BTRY(hipMemcpyAsync(pb.items1, items1.data(), items1.size() * sizeof(PbItem),
+                          hipMemcpyHostToDevice, s));
     BTRY(hipMemcpyAsync(pb.items2, items2.data(), items2.size() * sizeof(PbItem),
                         hipMemcpyHostToDevice, s));
     if (!splits.empty())
@@ -396,10 +541,10 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
     BTRY(hipStreamSynchronize(s));
   }
 #undef BTRY
-  for (void *p : {(void *)pb.src, (void *)pb.seg, (void *)pb.seg_off, (void *)pb.dst,
-                  (void *)pb.contrib, pb.w,
-                  (void *)pb.items1, (void *)pb.items2, (void *)pb.splits,
-                  (void *)pb.bin_partial, (void *)pb.partials})
+  for (void *p : {(void *)pb.src, pb.w, (void *)pb.mask, (void *)pb.win, (void *)pb.seg_off,
+                  (void *)pb.rrow, (void *)pb.rsum, (void *)pb.items1, (void *)pb.items2,
+                  (void *)pb.splits, (void *)pb.bin_partial, (void *)pb.bin_flags,
+                  (void *)pb.partials})
     if (p) pool.keep(p);
   g->pb = pb;
   g->has_pb = true;
@@ -411,3 +556,4 @@ fail:
 }
 
 }  // namespace gops
+

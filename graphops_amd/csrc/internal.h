@@ -159,16 +159,30 @@ struct PowerBuffers {
 // Layout of the blocked f32 delta sweep (blocked.hip, pagerank.hip).  The
 // sweep runs as two streaming passes instead of a gather pull:
 //   scatter     per chunk of 2^kPbChunkShift sources, whose f32 deltas sit in
-//               LDS: every out-edge's product goes to its place in contrib
+//               LDS: the chunk's edges are walked in (row, source) order and
+//               the products of each run (consecutive edges of one row) are
+//               summed; one f32 per run goes to its place in rsum
 //   accumulate  per bin of 2^kPbBinShift rows, whose sums sit in LDS as
-//               fixed-point int64: contrib is read in order and added
-// Entries are the edges of the CSR of in-edges in two orders: chunk-major
-// (chunk of the source, row, source) for the scatter and bin-major (bin of
-// the row, chunk of the source, row) for the accumulate pass.
+//               fixed-point int64: the bin's runs are read in order and added
+// Entries are the edges of the CSR of in-edges in chunk-major order (chunk of
+// the source, row, source), padded: every segment (the entries of one chunk
+// into one bin) starts at a multiple of kPbLane entries, and every chunk at a
+// multiple of kPbWin.  A padding entry repeats the source of the entry before
+// it with weight 0 and continues its run.  A wave takes a window of kPbWin
+// entries, kPbLane consecutive entries per lane.  Runs are maximal sequences of
+// entries of one row within one window; they are numbered in chunk-major order
+// and stored in bin-major order (bin of the row, chunk, row).  The runs of one
+// segment keep their relative order, so a run's bin-major place is its
+// chunk-major number plus its segment's offset.
 constexpr int kPbChunkShift = 14;
 constexpr int kPbBinShift = 12;
-constexpr int kPbItem1 = 1 << 14;  // entries per scatter workgroup, at most
-constexpr int kPbItem2 = 1 << 18;  // entries per accumulate workgroup, at most
+constexpr int kPbLane = 8;                  // consecutive entries per lane
+constexpr int kPbWin = 64 * kPbLane;        // entries per window (one wave's)
+constexpr int kPbItem1 = 32;                // windows per scatter workgroup, at most
+constexpr int kPbItem2 = 1 << 16;           // runs per accumulate workgroup, at most
+constexpr uint16_t kPbSrcMask = (1u << kPbChunkShift) - 1;
+constexpr uint16_t kPbRunHead = 1u << 14;   // entry starts a run
+constexpr uint16_t kPbSegHead = 1u << 15;   // entry starts a segment (and a run)
 constexpr uint64_t kPbAutoNodes = 1ull << 24;  // GOPS_DELTA_SWEEP_AUTO: blocked from here
 
 struct PbItem {
@@ -176,17 +190,23 @@ struct PbItem {
 };
 
 struct BlockedLayout {
-  uint16_t *src = nullptr;     // chunk-major: source id - chunk base
-  void *w = nullptr;           // chunk-major: weights in the graph's storage type (or null)
-  uint16_t *seg = nullptr;     // chunk-major: the entry's segment within its scatter item
-  int32_t *seg_off = nullptr;  // per segment: bin-major place - chunk-major index (constant
-                               // along a run of one chunk into one bin)
-  uint16_t *dst = nullptr;     // bin-major: row id - bin base
-  float *contrib = nullptr;    // bin-major: one sweep's products
-  PbItem *items1 = nullptr;    // scatter: {chunk, first entry, end entry, first segment}
-  PbItem *items2 = nullptr;    // accumulate: {bin, first entry, end entry, slot or -1}
-  PbItem *splits = nullptr;    // bins of several items: {bin, first slot, slots, 0}
+  uint64_t m = 0;              // entries, padding included (a multiple of kPbWin)
+  uint64_t num_runs = 0;
+  uint64_t num_segs = 0;
+  uint16_t *src = nullptr;     // source id - chunk base | kPbRunHead | kPbSegHead
+  void *w = nullptr;           // weights in the graph's storage type (weighted graphs)
+  uint8_t *mask = nullptr;     // 1 per edge, 0 per padding entry (the unweighted runs'
+                               // weights)
+  int32_t *win = nullptr;      // per window: the number of the run its first entry
+                               // starts, and that entry's segment
+  int32_t *seg_off = nullptr;  // per segment: bin-major place - chunk-major run number
+  uint16_t *rrow = nullptr;    // bin-major runs: row id - bin base
+  float *rsum = nullptr;       // bin-major runs: one sweep's run sums
+  PbItem *items1 = nullptr;    // scatter: {chunk, first window, end window, 0}
+  PbItem *items2 = nullptr;    // accumulate: {bin, first run, end run, slot or -1}
+  PbItem *splits = nullptr;    // bins of several items: {bin, first slot, slots, first row}
   long long *bin_partial = nullptr;  // [slot][1 << kPbBinShift] fixed-point row sums
+  uint8_t *bin_flags = nullptr;      // [slot][1 << kPbBinShift] non-finite run sums seen
   double *partials = nullptr;  // [items2 + splits][kMaxReduce] sums of the epilogues
   int num_items1 = 0, num_items2 = 0, num_splits = 0;
 };

@@ -375,18 +375,109 @@ __device__ __forceinline__ int pb_exponent(const RunState *st) {
   return k < -1000 ? -1000 : (k > 1000 ? 1000 : k);
 }
 
-// Scatter: one chunk of sources (their f32 deltas in LDS); every edge of the
-// chunk, in chunk-major order, writes its product to its bin-major place.
-// Consecutive lanes take consecutive entries, so that the writes of a run
-// (entries of one chunk into one bin) are contiguous.
+// An f32 DPP move (row_shr / row_bcast), 0 where the control selects no lane.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float pb_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
+}
+
+// Segmented inclusive sum over a wave: every lane adds the lanes of its run
+// from the run's head (start: the highest lane at or below it that holds a
+// head).  An inclusive scan within rows of 16 lanes (row_shr 1, 2, 4, 8),
+// then across rows (row_bcast 15: rows 1 and 3 add the last lane of the row
+// before; row_bcast 31: rows 2 and 3 add lane 31); a lane adds only lanes of
+// its run.  The order of the additions depends only on the lanes' positions,
+// so a run's sum is the same bits in every sweep.
+__device__ __forceinline__ float pb_seg_scan(float v, int start) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  float o;
+  o = pb_dpp<0x111, 0xF>(v); if (r >= 1 && lane - 1 >= start) v += o;
+  o = pb_dpp<0x112, 0xF>(v); if (r >= 2 && lane - 2 >= start) v += o;
+  o = pb_dpp<0x114, 0xF>(v); if (r >= 4 && lane - 4 >= start) v += o;
+  o = pb_dpp<0x118, 0xF>(v); if (r >= 8 && lane - 8 >= start) v += o;
+  o = pb_dpp<0x142, 0xA>(v); if (((lane >> 4) & 1) && (lane | 15) - 16 >= start) v += o;
+  o = pb_dpp<0x143, 0xC>(v); if (lane >= 32 && 31 >= start) v += o;
+  return v;
+}
+
+// Inclusive sum over a wave of a 32-bit integer (the same DPP steps, no
+// segments: lanes outside a row read 0).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t pb_dppu(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, true);
+}
+__device__ __forceinline__ uint32_t pb_int_scan(uint32_t v) {
+  v += pb_dppu<0x111, 0xF>(v);
+  v += pb_dppu<0x112, 0xF>(v);
+  v += pb_dppu<0x114, 0xF>(v);
+  v += pb_dppu<0x118, 0xF>(v);
+  v += pb_dppu<0x142, 0xA>(v);
+  v += pb_dppu<0x143, 0xC>(v);
+  return v;
+}
+
+typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t pb_u32x2 __attribute__((ext_vector_type(2)));
+
+// A lane's kPbLane consecutive weights (padding entries carry 0), as f32.
 template <typename WT>
-__global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *yd0, float *yd1) {
+__device__ __forceinline__ void pb_load_w8(const void *w, size_t e, float (&out)[kPbLane]) {
+  static_assert(kPbLane == 8, "the loads below take eight entries");
+  if constexpr (std::is_same<WT, Bf16W>::value) {
+    const pb_u32x4 q = __builtin_nontemporal_load(
+        reinterpret_cast<const pb_u32x4 *>(static_cast<const uint16_t *>(w) + e));
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      out[2 * i] = __uint_as_float(q[i] << 16);
+      out[2 * i + 1] = __uint_as_float(q[i] & 0xffff0000u);
+    }
+  } else if constexpr (std::is_same<WT, float>::value) {
+    const pb_u32x4 *p = reinterpret_cast<const pb_u32x4 *>(static_cast<const float *>(w) + e);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const pb_u32x4 q = __builtin_nontemporal_load(p + h);
+#pragma unroll
+      for (int i = 0; i < 4; i++) out[4 * h + i] = __uint_as_float(q[i]);
+    }
+  } else if constexpr (std::is_same<WT, double>::value) {
+    const pb_u32x4 *p = reinterpret_cast<const pb_u32x4 *>(static_cast<const double *>(w) + e);
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+      const pb_u32x4 q = __builtin_nontemporal_load(p + h);
+      out[2 * h] = (float)__hiloint2double((int)q[1], (int)q[0]);
+      out[2 * h + 1] = (float)__hiloint2double((int)q[3], (int)q[2]);
+    }
+  } else {  // unweighted: a byte 1 per edge, 0 per padding entry
+    const pb_u32x2 q = __builtin_nontemporal_load(
+        reinterpret_cast<const pb_u32x2 *>(static_cast<const uint8_t *>(w) + e));
+#pragma unroll
+    for (int i = 0; i < 8; i++) out[i] = (float)((q[i / 4] >> (8 * (i % 4))) & 0xffu);
+  }
+}
+
+// Scatter: one chunk of sources (their f32 deltas in LDS); the item's windows
+// of kPbWin chunk-major entries, one per wave at a time, kPbLane consecutive
+// entries per lane.  A lane sums its entries' products run by run in order;
+// the sum still open at its last entry is carried into the next lanes by a
+// segmented scan over the wave.  Every run's last entry writes the run's sum
+// to its bin-major place: the runs a lane closes are consecutive run numbers,
+// from the run of its first entry (the window's first run plus the run heads
+// before that entry), all in the lane's segment (segments start at a lane's
+// first entry), whose offset one load brings.
+//
+// 64 KB of LDS per workgroup leaves room for two workgroups (8 waves) per
+// CU: the register budget of that occupancy lets all windows' loads be in
+// flight at once.
+template <typename WT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void pb_scatter_kernel(SweepArgs a, float *yd0, float *yd1) {
   const RunState *st = a.st;
   if (st->done || st->mode == kModeAnchor) return;
   const float *yin = st->yd_cur ? yd1 : yd0;
   constexpr int kChunk = 1 << kPbChunkShift;
   __shared__ float z[kChunk];
-  const PbItem it = a.pb.items1[blockIdx.x];
+  const PbItem it = a.pb.items1[blockIdx.x];  // {chunk, first window, end window}
   const int base = it.x << kPbChunkShift;
   const int len = min(kChunk, a.n - base);
   if (len == kChunk && (base & 3) == 0) {
@@ -403,28 +494,74 @@ __global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *
     for (int i = threadIdx.x; i < len; i += kBlock) z[i] = yin[base + i];
   }
   __syncthreads();
-  constexpr int U = 24;
-  // Waves start on a multiple of 64 entries (lanes before the item's first
-  // entry idle), so that a wave's 2-byte loads fill one 128 B line: from an
-  // unaligned start every wave touches two lines per load (at R-MAT scale 24,
-  // 0.1 GB more fetched per sweep and 3.5 % more time).
-  for (int e = (it.y & ~63) + (int)threadIdx.x; e < it.z; e += U * kBlock) {
-    int sl[U];
-    uint32_t p[U];
-    float w[U];
+  const int lane = threadIdx.x & 63;
+  const uint32_t nruns = (uint32_t)a.pb.num_runs, nseg = (uint32_t)a.pb.num_segs;
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  constexpr int U = 2;  // windows' loads in flight per wave
+  constexpr int kWaves = kBlock / 64;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (int w0 = it.y + wave; w0 < it.z; w0 += U * kWaves) {
+    pb_u32x4 sq[U];
+    float wv[U][kPbLane];
+    int run0[U], seg0[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int k = e + u * kBlock;
-      const int kk = k >= it.y && k < it.z ? k : it.y;
-      sl[u] = __builtin_nontemporal_load(a.pb.src + kk);
-      const int sg = __builtin_nontemporal_load(a.pb.seg + kk);
-      p[u] = (uint32_t)(kk + a.pb.seg_off[it.w + sg]);
-      w[u] = wload32<WT>(a.pb.w, kk);
+      const int wi = min(w0 + u * kWaves, it.z - 1);  // clamped: every load in bounds
+      const size_t e = (size_t)wi * kPbWin + (size_t)lane * kPbLane;
+      sq[u] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(a.pb.src + e));
+      pb_load_w8<WT>(std::is_same<WT, NoW>::value ? (const void *)a.pb.mask : a.pb.w, e, wv[u]);
+      run0[u] = a.pb.win[2 * wi];
+      seg0[u] = a.pb.win[2 * wi + 1];
     }
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads together, ahead of their uses
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int k = e + u * kBlock;
-      if (k >= it.y && k < it.z) a.pb.contrib[p[u]] = w[u] * z[sl[u]];
+      if (w0 + u * kWaves >= it.z) break;  // the same in every lane
+      uint32_t sw[kPbLane];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        sw[2 * i] = sq[u][i] & 0xffffu;
+        sw[2 * i + 1] = sq[u][i] >> 16;
+      }
+      // products, and the in-lane sums run by run (q[j]: the sum of the
+      // entries of j's run up to j within the lane)
+      float q[kPbLane];
+      uint32_t hm = 0;
+#pragma unroll
+      for (int j = 0; j < kPbLane; j++) {
+        const float pj = wv[u][j] * z[sw[j] & kPbSrcMask];
+        const bool h = (sw[j] & kPbRunHead) != 0;
+        hm |= (h ? 1u : 0u) << j;
+        q[j] = (j == 0 || h) ? pj : q[j - 1] + pj;
+      }
+      // the open run carried across lanes
+      const unsigned long long lheads = __ballot(hm != 0);  // bit 0: the window's first entry
+      const int start = 63 - __clzll(lheads & upto);
+      const float S = pb_seg_scan(q[kPbLane - 1], start);
+      const float X = __shfl_up(S, 1);  // the open run's sum up to the previous lane
+      // run heads before and segment heads up to each lane (packed counts)
+      const uint32_t sh = (sw[0] & kPbSegHead) ? 1u : 0u;
+      const uint32_t cnt = (uint32_t)__popc(hm) | (sh << 16);
+      const uint32_t before = pb_int_scan(cnt) - cnt;
+      const uint32_t sh0 = __builtin_amdgcn_readfirstlane(sh);
+      const uint32_t run = (uint32_t)run0[u] + (before & 0xffffu) + (hm & 1u) - 1u;
+      const uint32_t seg = (uint32_t)seg0[u] + (before >> 16) + sh - sh0;
+      const int32_t off = a.pb.seg_off[seg < nseg ? seg : 0];
+      // the ballot runs in every lane, ahead of the test that may skip it
+      const unsigned long long first_heads = __ballot((hm & 1u) != 0);
+      const bool next_head = lane == 63 || ((first_heads >> (lane + 1)) & 1ull);
+      uint32_t place = run + (uint32_t)off;
+#pragma unroll
+      for (int j = 0; j < kPbLane; j++) {
+        const bool tail = j + 1 < kPbLane ? ((hm >> (j + 1)) & 1u) != 0 : next_head;
+        const float t = (hm & ((2u << j) - 1u)) ? q[j] : q[j] + X;
+        if (tail) {
+          // the layout keeps both in range; the checks keep a damaged layout
+          // from writing outside rsum
+          if (seg < nseg && place < nruns) a.pb.rsum[place] = t;
+          place++;
+        }
+      }
     }
   }
 }
@@ -432,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void pb_scatter_kernel(SweepArgs a, float *
 // c * 2^k as a fixed-point integer, truncated toward zero: the f32's
 // mantissa shifted by its exponent (integer operations only).  |c| * 2^k
 // stays below 2^61 (the scale's bound); truncation costs at most one unit,
-// 2^-62 of the previous residual, per product.
+// 2^-62 of the previous residual, per run.  Finite c only.
 __device__ __forceinline__ long long pb_fixed(float c, int k) {
   const uint32_t b = __float_as_uint(c);
   const int ex = (int)((b >> 23) & 0xffu);
@@ -445,42 +582,24 @@ __device__ __forceinline__ long long pb_fixed(float c, int k) {
   return (b >> 31) ? -(long long)v : (long long)v;
 }
 
-// A 64-bit DPP move (row_shr / row_bcast), 0 where the control selects no lane.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ long long pb_dpp(long long v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWS, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)((unsigned long long)v >> 32), CTRL, ROWS, 0xF, true);
-  return (long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo);
+// Non-finite run sums (an infinite or NaN weight or delta) carry through as
+// they would in a pull's f64 sum: per row, bit 0 NaN, bit 1 +inf, bit 2 -inf.
+__device__ __forceinline__ uint32_t pb_nonfinite_bits(float c) {
+  const uint32_t b = __float_as_uint(c);
+  return (b & 0x7fffffu) ? 1u : ((b >> 31) ? 4u : 2u);
 }
 
-// Adds one value per lane of a wave to acc[key] (key < 0: no value).  A
-// bin's entries come in runs of one row, so lanes with equal keys side by
-// side are summed first (a segmented scan: each lane adds the lanes from its
-// run's head), and only each run's last lane makes an LDS atomic.
-__device__ __forceinline__ void pb_add(long long v, int key, unsigned long long *acc) {
-  const int lane = threadIdx.x & 63;
-  const int prev = __shfl_up(key, 1);
-  const bool head = lane == 0 || prev != key;
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const int start = 63 - __clzll(heads & upto);
-  // inclusive scan within rows of 16 lanes (row_shr 1, 2, 4, 8), then across
-  // rows (row_bcast 15: rows 1 and 3 add the last lane of the row before;
-  // row_bcast 31: rows 2 and 3 add lane 31); a lane adds only lanes of its run
-  const int r = lane & 15;
-  long long o;
-  o = pb_dpp<0x111, 0xF>(v); if (r >= 1 && lane - 1 >= start) v += o;
-  o = pb_dpp<0x112, 0xF>(v); if (r >= 2 && lane - 2 >= start) v += o;
-  o = pb_dpp<0x114, 0xF>(v); if (r >= 4 && lane - 4 >= start) v += o;
-  o = pb_dpp<0x118, 0xF>(v); if (r >= 8 && lane - 8 >= start) v += o;
-  o = pb_dpp<0x142, 0xA>(v); if (((lane >> 4) & 1) && (lane | 15) - 16 >= start) v += o;
-  o = pb_dpp<0x143, 0xC>(v); if (lane >= 32 && 31 >= start) v += o;
-  const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-  if (tail && key >= 0) atomicAdd(&acc[key], (unsigned long long)v);
+__device__ __forceinline__ double pb_row_value(long long s, uint32_t flags, double inv_scale) {
+  if (flags) {
+    if ((flags & 1u) || (flags & 6u) == 6u) return __builtin_nan("");
+    return (flags & 2u) ? __builtin_inf() : -__builtin_inf();
+  }
+  return (double)s * inv_scale;
 }
 
-// Accumulate: one bin of rows (or a part of its entries), fixed-point sums in
-// LDS; then the rows' epilogues, or the part's sums for the combine kernel.
+// Accumulate: one bin of rows (or a part of its runs), fixed-point sums in
+// LDS, one atomic per run; then the rows' epilogues, or the part's sums for
+// the combine kernel.
 __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, float *yd0, float *yd1,
                                                                double *y640, double *y641) {
   const RunState *st = a.st;
@@ -488,29 +607,38 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   pick_buffers(a, st, yd0, yd1, y640, y641);
   constexpr int kRows = 1 << kPbBinShift;
   __shared__ unsigned long long acc[kRows];
+  __shared__ uint32_t nonfin[kRows / 4];  // a byte of pb_nonfinite_bits per row
   const PbItem it = a.pb.items2[blockIdx.x];
   for (int i = threadIdx.x; i < kRows; i += kBlock) acc[i] = 0ull;
+  for (int i = threadIdx.x; i < kRows / 4; i += kBlock) nonfin[i] = 0u;
   __syncthreads();
   const int k = pb_exponent(st);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int U = 16;
-  // waves start on a multiple of 64 entries, as in the scatter: whole lines
-  // per wave (unaligned, the two waves sharing each line both fetched it:
-  // 0.65 GB more per sweep at R-MAT scale 24)
-  for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kBlock) {
-    float cv[U];
-    int key[U];
+  // waves start on a multiple of 64 runs: whole lines per wave
+  if (it.y < it.z) {
+    for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kBlock) {
+      float cv[U];
+      int key[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int f = f0 + u * kBlock + wave * 64 + lane;  // 64 consecutive entries per wave
-      const bool ok = f >= it.y && f < it.z;
-      const int ff = ok ? f : it.y;
-      cv[u] = __builtin_nontemporal_load(a.pb.contrib + ff);
-      const int d = __builtin_nontemporal_load(a.pb.dst + ff);
-      key[u] = ok ? d : -1 - lane;
+      for (int u = 0; u < U; u++) {
+        const int f = f0 + u * kBlock + wave * 64 + lane;  // 64 consecutive runs per wave
+        const bool ok = f >= it.y && f < it.z;
+        const int ff = ok ? f : it.y;
+        cv[u] = __builtin_nontemporal_load(a.pb.rsum + ff);
+        const int d = __builtin_nontemporal_load(a.pb.rrow + ff);
+        key[u] = ok ? d : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (key[u] < 0) continue;
+        const float c = cv[u];
+        if (((__float_as_uint(c) >> 23) & 0xffu) == 0xffu)
+          atomicOr(&nonfin[key[u] >> 2], pb_nonfinite_bits(c) << (8 * (key[u] & 3)));
+        else
+          atomicAdd(&acc[key[u]], (unsigned long long)pb_fixed(c, k));
+      }
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) pb_add(pb_fixed(cv[u], k), key[u], acc);
   }
   __syncthreads();
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -530,12 +658,19 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
 #pragma unroll
       for (int u = 0; u < R; u++) {
         const int i = i0 + u * kBlock;
-        if (i < rows) epi.apply(row0 + i, (double)(long long)acc[i] * inv_scale, o[u], red);
+        if (i < rows) {
+          const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
+          epi.apply(row0 + i, pb_row_value((long long)acc[i], fl, inv_scale), o[u], red);
+        }
       }
     }
   } else {
     long long *out = a.pb.bin_partial + ((size_t)it.w << kPbBinShift);
-    for (int i = threadIdx.x; i < rows; i += kBlock) out[i] = (long long)acc[i];
+    uint8_t *fout = a.pb.bin_flags + ((size_t)it.w << kPbBinShift);
+    for (int i = threadIdx.x; i < rows; i += kBlock) {
+      out[i] = (long long)acc[i];
+      fout[i] = (uint8_t)((nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu);
+    }
   }
   __syncthreads();
   block_sum<kReduce>(red, reinterpret_cast<double *>(acc));
@@ -545,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   }
 }
 
-// Combine: kBlock rows of a bin whose entries were split over several
+// Combine: kBlock rows of a bin whose runs were split over several
 // accumulate workgroups: the parts' sums in slot order, then the epilogues.
 __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *yd0, float *yd1,
                                                             double *y640, double *y641) {
@@ -560,19 +695,32 @@ __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *
   if (i < (1 << kPbBinShift) && row0 + (int)threadIdx.x < a.n) {
     // the hottest bins are split into dozens of parts: eight parts' loads in
     // flight at a time (integer sums: the grouping leaves the bits alone)
-    const long long *bp = a.pb.bin_partial + ((size_t)sp.y << kPbBinShift) + i;
+    const size_t at = ((size_t)sp.y << kPbBinShift) + i;
+    const long long *bp = a.pb.bin_partial + at;
+    const uint8_t *fp = a.pb.bin_flags + at;
     long long s = 0;
+    uint32_t fl = 0;
     int j = 0;
     for (; j + 8 <= sp.z; j += 8) {
       long long v[8];
+      uint32_t f[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = bp[(size_t)(j + u) << kPbBinShift];
+      for (int u = 0; u < 8; u++) {
+        v[u] = bp[(size_t)(j + u) << kPbBinShift];
+        f[u] = fp[(size_t)(j + u) << kPbBinShift];
+      }
 #pragma unroll
-      for (int u = 0; u < 8; u++) s += v[u];
+      for (int u = 0; u < 8; u++) {
+        s += v[u];
+        fl |= f[u];
+      }
     }
-    for (; j < sp.z; j++) s += bp[(size_t)j << kPbBinShift];
+    for (; j < sp.z; j++) {
+      s += bp[(size_t)j << kPbBinShift];
+      fl |= fp[(size_t)j << kPbBinShift];
+    }
     const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
-    epi(row0 + (int)threadIdx.x, (double)s * inv_scale, red);
+    epi(row0 + (int)threadIdx.x, pb_row_value(s, fl, inv_scale), red);
   }
   block_sum<kReduce>(red);
   if (threadIdx.x == 0) {

@@ -169,3 +169,31 @@ def test_default_below_2_24_nodes_is_the_pull(ctx_p):
     assert a.iterations == pl.iterations and np.array_equal(a.scores, pl.scores)
     with pytest.raises(ga.InvalidParameter):
         c.set_option(L.OPT_DELTA_SWEEP, 7)
+
+
+def test_nonfinite_weight_carries_through(oracle, ctx_b, ctx_p):
+    """An unchecked weighted run with one +inf edge weight: the reference's
+    share (d*x_u)*(w/ws_u) is inf/inf = NaN there (pagerank.rs:255-259) and
+    the NaN spreads along the edges.  The blocked sweep's run sums carry the
+    non-finite values through its integer row sums, so its NaNs land where the
+    pull's and the oracle's do."""
+    e0 = generators.rmat(14, 16, seed=21, weights="uniform", p0=0.1, p1=1.0, rounding="f32")
+    w = e0.weights.copy()
+    w[int(e0.offsets[7])] = np.inf  # the first out-edge of node 7
+    e = SimpleNamespace(n=e0.n, m=e0.m, offsets=e0.offsets, targets=e0.targets, weights=w)
+    cfg = ga.PageRankConfig(0.85, 40, 1e-10)
+    got = {}
+    for name, cx in (("blocked", ctx_b), ("pull", ctx_p)):
+        dg = ga.DeviceGraph.from_edges(e, weight_storage="f32", ctx=cx)
+        got[name] = ga.pagerank_weighted_run(dg, cfg, ctx=cx)
+        if name == "blocked":
+            assert cx.last_stats()["sweeps_blocked"] > 0
+        dg.close()
+    ref = oracle.pagerank_weighted_run(e.offsets, e.targets, w, 0.85, 40, 1e-10)
+    b, pl = got["blocked"], got["pull"]
+    assert np.isnan(ref.scores).any() and not ref.converged
+    for r in (b, pl):
+        assert not r.converged and r.iterations == 40 and np.isnan(r.diff_l1)
+        assert np.array_equal(np.isnan(r.scores), np.isnan(ref.scores))
+    fin = ~np.isnan(ref.scores)
+    assert np.abs(b.scores[fin] - pl.scores[fin]).sum() <= 1e-9
