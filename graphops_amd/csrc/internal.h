@@ -178,7 +178,7 @@ constexpr int kPbChunkShift = 14;
 constexpr int kPbBinShift = 12;
 constexpr int kPbLane = 8;                  // consecutive entries per lane
 constexpr int kPbWin = 64 * kPbLane;        // entries per window (one wave's)
-constexpr int kPbItem1 = 32;                // windows per scatter workgroup, at most
+constexpr int kPbItem1 = 64;               // windows per scatter workgroup, at most
 constexpr int kPbItem2 = 1 << 16;           // runs per accumulate workgroup, at most
 constexpr uint16_t kPbSrcMask = (1u << kPbChunkShift) - 1;
 constexpr uint16_t kPbRunHead = 1u << 14;   // entry starts a run

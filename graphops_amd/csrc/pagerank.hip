@@ -466,11 +466,14 @@ __device__ __forceinline__ void pb_load_w8(const void *w, size_t e, float (&out)
 // before that entry), all in the lane's segment (segments start at a lane's
 // first entry), whose offset one load brings.
 //
-// 64 KB of LDS per workgroup leaves room for two workgroups (8 waves) per
-// CU: the register budget of that occupancy lets all windows' loads be in
-// flight at once.
+// 1024 threads per scatter workgroup: the 64 KB of LDS allows two workgroups
+// per CU, and a larger workgroup puts more waves beside each other's loads
+// (measured at R-MAT scale 24: 256 threads 579 us, 512 553 us, 1024 543 us
+// per f32 sweep); the register budget of that occupancy lets all of a wave's
+// windows' loads be in flight at once.
+constexpr int kScatterBlock = 1024;
 template <typename WT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
+__global__ __launch_bounds__(kScatterBlock) __attribute__((amdgpu_waves_per_eu(2 * kScatterBlock / 256, 2 * kScatterBlock / 256)))
 void pb_scatter_kernel(SweepArgs a, float *yd0, float *yd1) {
   const RunState *st = a.st;
   if (st->done || st->mode == kModeAnchor) return;
@@ -482,23 +485,23 @@ void pb_scatter_kernel(SweepArgs a, float *yd0, float *yd1) {
   const int len = min(kChunk, a.n - base);
   if (len == kChunk && (base & 3) == 0) {
     // all of the chunk's loads in flight at once (16-byte loads)
-    constexpr int V = kChunk / 4 / kBlock;
+    constexpr int V = kChunk / 4 / kScatterBlock;
     float4 v[V];
 #pragma unroll
     for (int j = 0; j < V; j++)
-      v[j] = *reinterpret_cast<const float4 *>(yin + base + 4 * ((int)threadIdx.x + j * kBlock));
+      v[j] = *reinterpret_cast<const float4 *>(yin + base + 4 * ((int)threadIdx.x + j * kScatterBlock));
 #pragma unroll
     for (int j = 0; j < V; j++)
-      *reinterpret_cast<float4 *>(z + 4 * ((int)threadIdx.x + j * kBlock)) = v[j];
+      *reinterpret_cast<float4 *>(z + 4 * ((int)threadIdx.x + j * kScatterBlock)) = v[j];
   } else {
-    for (int i = threadIdx.x; i < len; i += kBlock) z[i] = yin[base + i];
+    for (int i = threadIdx.x; i < len; i += kScatterBlock) z[i] = yin[base + i];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const uint32_t nruns = (uint32_t)a.pb.num_runs, nseg = (uint32_t)a.pb.num_segs;
   const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  constexpr int U = 2;  // windows' loads in flight per wave
-  constexpr int kWaves = kBlock / 64;
+  constexpr int U = 4;  // windows' loads in flight per wave
+  constexpr int kWaves = kScatterBlock / 64;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   for (int w0 = it.y + wave; w0 < it.z; w0 += U * kWaves) {
     pb_u32x4 sq[U];
@@ -861,7 +864,7 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
   sweep_kernel<WT><<<grid, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1]);
   if (a.pb_active) {
     if (a.pb.num_items1)
-      pb_scatter_kernel<WT><<<a.pb.num_items1, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1]);
+      pb_scatter_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1]);
     pb_accumulate_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
                                                                       rb.y64[0], rb.y64[1]);
     if (a.pb.num_splits)
