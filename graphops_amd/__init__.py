@@ -33,7 +33,7 @@ __all__ = [
     "pagerank_weighted", "pagerank_weighted_run", "pagerank_weighted_checked",
     "pagerank_weighted_checked_run", "personalized_pagerank", "personalized_pagerank_run",
     "personalized_pagerank_checked", "personalized_pagerank_checked_run", "ppr_batch_run",
-    "top_k", "result_top_k", "normalize", "Server", "KatzConfig", "KatzRun", "katz_centrality",
+    "top_k", "result_top_k", "result_top_k_noalloc", "normalize", "Server", "KatzConfig", "KatzRun", "katz_centrality",
     "katz_centrality_run", "katz_centrality_checked", "EigenvectorRun",
     "eigenvector_centrality", "eigenvector_centrality_run", "hits", "hits_run",
 ]
@@ -696,6 +696,21 @@ def result_top_k(graph: "DeviceGraph", k):
     c = graph.ctx.handle
     _check(L.lib().gops_result_top_k(c, graph.handle, k, L.ptr(ids, L._u32p), L.ptr(vals, L._f64p),
                                      ctypes.byref(cnt)), c, "gops_result_top_k")
+    return [(int(i), float(v)) for i, v in zip(ids[:cnt.value], vals[:cnt.value])]
+
+
+def result_top_k_noalloc(graph: "DeviceGraph", k):
+    """result_top_k without allocating device memory
+    (gops_result_top_k_noalloc): the same pairs, k <= 4096, scratch in the
+    library's file-scope device array (calls that overlap on one GPU race)."""
+    k = int(k)
+    ids = np.zeros(max(k, 1), dtype=np.uint32)
+    vals = np.zeros(max(k, 1), dtype=np.float64)
+    cnt = ctypes.c_uint32()
+    c = graph.ctx.handle
+    _check(L.lib().gops_result_top_k_noalloc(c, graph.handle, k, L.ptr(ids, L._u32p),
+                                             L.ptr(vals, L._f64p), ctypes.byref(cnt)),
+           c, "gops_result_top_k_noalloc")
     return [(int(i), float(v)) for i, v in zip(ids[:cnt.value], vals[:cnt.value])]
 
 

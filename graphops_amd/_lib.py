@@ -105,6 +105,7 @@ SIGNATURES = [
     ("gops_top_k", ctypes.c_int,
      [_vp, _f64p, ctypes.c_uint64, ctypes.c_uint32, _u32p, _f64p, _u32p]),
     ("gops_result_top_k", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _f64p, _u32p]),
+    ("gops_result_top_k_noalloc", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _f64p, _u32p]),
     ("gops_katz_centrality_run", ctypes.c_int,
      [_vp, _vp, ctypes.POINTER(KatzConfigC), _f64p, ctypes.POINTER(RunInfoC)]),
     ("gops_katz_centrality_checked_run", ctypes.c_int,

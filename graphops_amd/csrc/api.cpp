@@ -456,6 +456,12 @@ gops_status gops_context_create(int device, gops_context **out) {
   *ctx->mirror_host = 0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  // set-up of the allocation-free top_k: its scratch array's address on this
+  // device (looked up once per device; a failure here is retried by the call)
+  if (gops::noalloc_scratch(device, &ctx->noalloc_scratch) != hipSuccess) {
+    ctx->noalloc_scratch = nullptr;
+    (void)hipGetLastError();
+  }
   *out = ctx;
   return GOPS_OK;
 }
@@ -773,6 +779,28 @@ gops_status gops_result_top_k(gops_context *ctx, gops_graph *g, uint32_t k, uint
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   std::string err;
   const gops_status st = device_top_k(ctx, g->run.out, g->dev.n, k, ids, values, count, err);
+  return st == GOPS_OK ? st : fail(ctx, st, err);
+}
+
+gops_status gops_result_top_k_noalloc(gops_context *ctx, gops_graph *g, uint32_t k, uint32_t *ids,
+                                      double *values, uint32_t *count) {
+  if (!ctx || !g || !count) return fail(ctx, GOPS_INVALID_PARAMETER, "null argument");
+  if (g->ctx != ctx) return fail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  *count = 0;
+  if (!g->run.has_result)
+    return fail(ctx, GOPS_INVALID_PARAMETER, "no PageRank result on this graph yet");
+  if (k > kNoAllocTopK)
+    return fail(ctx, GOPS_INVALID_PARAMETER,
+                "top_k without allocation: k must be at most " + std::to_string(kNoAllocTopK) +
+                    " (got " + std::to_string(k) + ")");
+  const uint64_t n = g->dev.n;
+  if (k == 0 || n == 0) return GOPS_OK;  // topk.rs:18-20
+  if (n >= (1ull << 32)) return fail(ctx, GOPS_NOT_SUPPORTED, "top_k: at most 2^32 - 1 scores");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (!ctx->noalloc_scratch) HIP_TRY(ctx, noalloc_scratch(ctx->device, &ctx->noalloc_scratch));
+  std::string err;
+  const gops_status st = top_k_noalloc(ctx, static_cast<unsigned char *>(ctx->noalloc_scratch),
+                                       g->run.out, n, k, ids, values, count, err);
   return st == GOPS_OK ? st : fail(ctx, st, err);
 }
 

@@ -265,6 +265,7 @@ struct gops_context {
   int delta_sweep = GOPS_DELTA_SWEEP_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
+  void *noalloc_scratch = nullptr;       // device address of topk.hip's file-scope scratch
   double *staging = nullptr;              // device copy of host scores for gops_top_k
   size_t staging_cap = 0;
   gops::RunState *host_state = nullptr;  // pinned, 2 slots
@@ -317,6 +318,18 @@ hipError_t launch_permute(gops_context *ctx, gops_graph *g, const double *src, d
 gops_status device_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
                          uint32_t *ids, double *vals, uint32_t *count, std::string &err);
 void free_topk(TopkScratch *t);
+// topk.hip: the allocation-free top_k.  noalloc_scratch gives the device
+// address of the library's file-scope scratch array on `device` (the current
+// device), looked up once per device; top_k_noalloc selects top_k (k <=
+// kNoAllocTopK, n < 2^32) of n device scores into host ids / vals with that
+// scratch at `base`, allocating nothing.  Calls that overlap on one GPU share
+// the array and race.
+constexpr int kMaxDevices = 64;
+constexpr uint32_t kNoAllocTopK = 4096;
+hipError_t noalloc_scratch(int device, void **out);
+gops_status top_k_noalloc(gops_context *ctx, unsigned char *base, const double *scores,
+                          uint64_t n, uint32_t k, uint32_t *ids, double *vals, uint32_t *count,
+                          std::string &err);
 // the same, queued on ctx's stream with the results left on the device
 // (ids / vals: k entries, *count); n < 2^32
 hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "internal.h"
 
@@ -420,6 +421,15 @@ __global__ __launch_bounds__(kTB) void tk_emit_kernel(const uint64_t *key, const
 
 }  // namespace
 
+// the selection state (and, on request, the result count) set to zero by a
+// kernel rather than a memset, so that every step of a selection is a kernel
+__global__ void tk_clear_kernel(TkState *st, uint32_t *count) {
+  if (threadIdx.x == 0) {
+    if (st) *st = TkState{};
+    if (count) *count = 0;
+  }
+}
+
 struct TopkScratch {
   uint32_t *part = nullptr;     // kHGrid * 256 histogram bins
   uint64_t *cnt = nullptr;      // 2 * kTGrid chunk counts
@@ -490,19 +500,13 @@ static hipError_t topk_scratch(gops_context *ctx, uint32_t k) {
   return hipSuccess;
 }
 
-// Queues the selection of top_k of n device scores on ctx's stream: ids /
-// vals (k entries) and *count on the device.  Calls on one context share its
-// scratch and run in stream order.
-hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
-                        uint32_t *ids, double *vals, uint32_t *count) {
-  hipStream_t s = ctx->stream;
-  if (k == 0 || n == 0) return hipMemsetAsync(count, 0, sizeof(uint32_t), s);
-  const uint64_t kk = std::min<uint64_t>(k, n);
-  hipError_t e = topk_scratch(ctx, (uint32_t)kk);
-  if (e != hipSuccess) return e;
-  TopkScratch &t = *ctx->topk;
-  e = hipMemsetAsync(t.st, 0, sizeof(TkState), s);
-  if (e != hipSuccess) return e;
+// Queues the selection of top_k of n device scores on stream s with the
+// given scratch (kk = min(k, n) > 0 entries; kk <= t.cap, and t.tmp sized
+// when kk > kSortMax): ids / vals (kk entries) and *count on the device.
+static hipError_t launch_top_k_in(hipStream_t s, const double *scores, uint64_t n, uint64_t kk,
+                                  uint32_t *ids, double *vals, uint32_t *count,
+                                  const TopkScratch &t) {
+  tk_clear_kernel<<<1, 64, 0, s>>>(t.st, nullptr);
   for (int shift = 56; shift >= 0; shift -= 8) {
     tk_hist_kernel<<<kHGrid, kHB, 0, s>>>(scores, n, shift, t.st, t.part);
     tk_pick_kernel<<<1, kPickThreads, 0, s>>>(t.st, t.part, shift, kk);
@@ -519,13 +523,112 @@ hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uin
   // the sort is stable and the entries are in index order within equal keys
   const int g = (int)std::min<uint64_t>((kk + kTB - 1) / kTB, 4096);
   tk_pad_kernel<<<g, kTB, 0, s>>>(t.key[0], t.st, kk);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
   size_t b = t.tmp_bytes;
   e = hipcub::DeviceRadixSort::SortPairsDescending(t.tmp, b, t.key[0], t.key[1], t.id[0], t.id[1],
                                                    (int)kk, 0, 64, s);
   if (e != hipSuccess) return e;
   tk_emit_kernel<<<g, kTB, 0, s>>>(t.key[1], t.id[1], t.st, kk, ids, vals, count);
   return hipGetLastError();
+}
+
+// Queues the selection of top_k of n device scores on ctx's stream: ids /
+// vals (k entries) and *count on the device.  Calls on one context share its
+// scratch and run in stream order.
+hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
+                        uint32_t *ids, double *vals, uint32_t *count) {
+  hipStream_t s = ctx->stream;
+  if (k == 0 || n == 0) {
+    tk_clear_kernel<<<1, 64, 0, s>>>(nullptr, count);
+    return hipGetLastError();
+  }
+  const uint64_t kk = std::min<uint64_t>(k, n);
+  hipError_t e = topk_scratch(ctx, (uint32_t)kk);
+  if (e != hipSuccess) return e;
+  return launch_top_k_in(s, scores, n, kk, ids, vals, count, *ctx->topk);
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+// The allocation-free variant's scratch: one file-scope array, which every
+// GPU the library runs on holds (its own copy) until the process ends.  The
+// selection's state, histogram rows, chunk counts and offsets, the kNoAllocK
+// selected keys and ids, and the results before they cross to the host.
+constexpr uint32_t kNoAllocK = kNoAllocTopK;  // k at most: the one-workgroup sort
+static_assert(kNoAllocK <= (uint32_t)kSortMax, "the selected entries are sorted in one workgroup");
+constexpr size_t na_round(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaPart = 0;
+constexpr size_t kNaCnt = kNaPart + na_round((size_t)kHGrid * 256 * sizeof(uint32_t));
+constexpr size_t kNaOff = kNaCnt + na_round(2 * (size_t)kTGrid * sizeof(uint64_t));
+constexpr size_t kNaEntered = kNaOff + na_round(2 * (size_t)kTGrid * sizeof(uint64_t));
+constexpr size_t kNaState = kNaEntered + na_round((size_t)kTGrid * sizeof(uint64_t));
+constexpr size_t kNaKey = kNaState + na_round(sizeof(TkState));
+constexpr size_t kNaId = kNaKey + na_round((size_t)kNoAllocK * sizeof(uint64_t));
+constexpr size_t kNaIds = kNaId + na_round((size_t)kNoAllocK * sizeof(uint32_t));
+constexpr size_t kNaVals = kNaIds + na_round((size_t)kNoAllocK * sizeof(uint32_t));
+constexpr size_t kNaCount = kNaVals + na_round((size_t)kNoAllocK * sizeof(double));
+constexpr size_t kNoAllocBytes = kNaCount + na_round(sizeof(uint32_t));  // about 400 KB
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+}  // namespace
+
+// The device address of `scratch` on the current device.  In host code the
+// array's name stands for no device address, so kernels and copies take this
+// pointer; it is looked up once per device and kept.
+hipError_t noalloc_scratch(int device, void **out) {
+  static std::mutex mu;
+  static void *ptr[kMaxDevices] = {};
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    ptr[device] = p;
+  }
+  *out = ptr[device];
+  return hipSuccess;
+}
+
+// top_k of n device scores into host ids / vals with the scratch at `base`
+// (the array above on ctx's device); kk = min(k, n) <= kNoAllocK.
+gops_status top_k_noalloc(gops_context *ctx, unsigned char *base, const double *scores,
+                          uint64_t n, uint32_t k, uint32_t *ids, double *vals, uint32_t *count,
+                          std::string &err) {
+  hipStream_t s = ctx->stream;
+  TopkScratch t;
+  t.part = reinterpret_cast<uint32_t *>(base + kNaPart);
+  t.cnt = reinterpret_cast<uint64_t *>(base + kNaCnt);
+  t.off = reinterpret_cast<uint64_t *>(base + kNaOff);
+  t.entered = reinterpret_cast<uint64_t *>(base + kNaEntered);
+  t.st = reinterpret_cast<TkState *>(base + kNaState);
+  t.key[0] = reinterpret_cast<uint64_t *>(base + kNaKey);
+  t.id[0] = reinterpret_cast<uint32_t *>(base + kNaId);
+  t.cap = kNoAllocK;
+  uint32_t *d_ids = reinterpret_cast<uint32_t *>(base + kNaIds);
+  double *d_vals = reinterpret_cast<double *>(base + kNaVals);
+  uint32_t *d_count = reinterpret_cast<uint32_t *>(base + kNaCount);
+  const uint64_t kk = std::min<uint64_t>(k, n);
+  uint32_t cnt = 0;
+  hipError_t e = launch_top_k_in(s, scores, n, kk, d_ids, d_vals, d_count, t);
+  if (e == hipSuccess) e = hipMemcpyAsync(&cnt, d_count, sizeof(cnt), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && cnt) {
+    if (ids) e = hipMemcpyAsync(ids, d_ids, cnt * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && vals)
+      e = hipMemcpyAsync(vals, d_vals, cnt * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  if (e != hipSuccess) {
+    err = std::string("top_k without allocation: ") + hipGetErrorString(e);
+    return GOPS_DEVICE_ERROR;
+  }
+  *count = cnt;
+  return GOPS_OK;
 }
 
 // top_k of n device scores into host ids / values (count entries, <= k)

@@ -241,6 +241,17 @@ gops_status gops_top_k(gops_context *ctx, const double *scores, uint64_t n, uint
  * run and top_k). */
 gops_status gops_result_top_k(gops_context *ctx, gops_graph *g, uint32_t k, uint32_t *ids,
                               double *values, uint32_t *count);
+/* gops_result_top_k without allocating.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results, bit for bit; k is at most 4096
+ * (GOPS_INVALID_PARAMETER beyond).  The selection's scratch is one
+ * file-scope device array of about 400 KB, which every GPU the library runs
+ * on holds until the process ends; every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (from several contexts or threads)
+ * race. */
+gops_status gops_result_top_k_noalloc(gops_context *ctx, gops_graph *g, uint32_t k,
+                                      uint32_t *ids, double *values, uint32_t *count);
 
 /* ---- Katz, eigenvector and HITS centrality ----------------------------- */
 /* KatzConfig, katz.rs:25-53 (defaults alpha 0.1, beta 1.0, 100, 1e-6). */
