@@ -56,6 +56,8 @@ def cpu_baseline(e, cfg, n_sweeps, damping=0.85):
     bounded sample of sweeps of the same workload."""
     import oracle
 
+    info = machine_info()
+
     ws = oracle.out_wsum(e.offsets, e.weights)
     x = np.full(e.n, 1.0 / e.n)
     t0 = time.perf_counter()
@@ -63,7 +65,8 @@ def cpu_baseline(e, cfg, n_sweeps, damping=0.85):
         x, _ = oracle.push_sweep(e.offsets, e.targets, x, damping, e.weights, ws)
     dt = time.perf_counter() - t0
     return {"value": round(e.m * n_sweeps / dt / 1e9, 4), "unit": "GTEPS", "cores": 1,
-            "kind": "port",
+            "kind": "port", "cpu_model": info["cpu_model"], "nproc": info["nproc"],
+            "cores_usable": info["cores_usable"],
             "sample": f"{n_sweeps} push sweeps of the f64 oracle (single thread) on the same "
                       f"graph, {dt:.1f} s"}
 
@@ -77,11 +80,13 @@ def oracle_reference(e, tol, max_it):
 
     in_off, in_src, in_w = oracle.transpose(e.offsets, e.targets, e.weights)
     ws = oracle.out_wsum(e.offsets, e.weights)
+    t0 = time.perf_counter()
     f64 = oracle.pull_run(1, in_off, in_src, ws, w=in_w, damping=0.85, max_iterations=max_it,
-                          tolerance=tol, threads=16)
+                          tolerance=tol, threads=0)
+    f64_s = time.perf_counter() - t0
     ld = oracle.pull_run_ld(1, in_off, in_src, w=in_w, damping=0.85, max_iterations=max_it,
-                            tolerance=tol, threads=16)
-    return f64, ld
+                            tolerance=tol, threads=0)
+    return f64, ld, f64_s
 
 
 def top_agreement(a, b, k=100):
@@ -89,6 +94,192 @@ def top_agreement(a, b, k=100):
     ta = set(np.argsort(-a, kind="stable")[:k].tolist())
     tb = set(np.argsort(-b, kind="stable")[:k].tolist())
     return len(ta & tb) / float(k)
+
+
+def machine_info():
+    """The host the CPU figures ran on: model, cores visible, cores this
+    process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS", usable) or usable)
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cores_usable": usable,
+            "omp_threads": threads}
+
+
+def copy_peak(nbytes=1 << 30, reps=5):
+    """Measured device copy bandwidth: hipMemcpyAsync of nbytes device to
+    device (the runtime's copy kernel), HIP events around it; read + write
+    bytes over the best of reps."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    a, b = vp(), vp()
+    e0, e1 = vp(), vp()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    try:
+        assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        best = None
+        for _ in range(reps):
+            hip.hipEventRecord(e0, None)
+            assert hip.hipMemcpyAsync(b, a, ctypes.c_size_t(nbytes), 3, None) == 0  # D2D
+            hip.hipEventRecord(e1, None)
+            hip.hipEventSynchronize(e1)
+            ms = ctypes.c_float()
+            hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+            best = ms.value if best is None else min(best, ms.value)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    gbs = 2 * nbytes / (best / 1e3) / 1e9
+    return {"what": "device-to-device copy of 1 GiB (hipMemcpyAsync), read + write bytes over "
+                    "the best of 5, HIP events",
+            "gbs": round(gbs, 1), "frac_of_nominal": round(gbs / HBM_PEAK_GBS, 3)}
+
+
+def c1_leg(ga, generators, reps=20):
+    """C1: directed G(2000, 0.005) from seed 42 as a dense AdjacencyMatrix,
+    pagerank_run at d 0.85, tol 1e-10, max 100 (pagerank.rs:122-187): the GPU
+    (graph resident, scores downloaded each run) against the oracle, whose
+    CPU time includes the dense adapter's O(n^2) neighbour materialisation
+    (graph.rs:128-135)."""
+    import oracle
+
+    e = generators.erdos_renyi(2000, 0.005, 42)
+    n = e.n
+    mat = np.zeros((n, n))
+    for u in range(n):
+        mat[u, e.targets[e.offsets[u]:e.offsets[u + 1]]] = 1.0
+    cfg = ga.PageRankConfig(0.85, 100, 1e-10)
+    ctx = ga.Context(0)
+    dg = ga.DeviceGraph.from_dense(mat, weighted=False, ctx=ctx)
+    r = ga.pagerank_run(dg, cfg, ctx=ctx)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = ga.pagerank_run(dg, cfg, ctx=ctx)
+    gpu_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    off, nb, _ = oracle.dense_neighbors(mat)
+    ref = oracle.pagerank_run(off, nb, 0.85, 100, 1e-10)
+    cpu_s = time.perf_counter() - t0
+    d = np.abs(r.scores - ref.scores)
+    m = int(off[-1])
+    dg.close()
+    ctx.close()
+    return {"workload": "er2000 p0.005 seed42 dense AdjacencyMatrix, pagerank_run d0.85 tol1e-10",
+            "n": n, "m": m, "sweeps": r.iterations, "oracle_sweeps": ref.iterations,
+            "gpu_ms_per_run": round(gpu_s * 1e3, 3),
+            "gpu_gteps": round(m * r.iterations / gpu_s / 1e9, 3),
+            "accuracy": {"l1_vs_f64_oracle": float(d.sum()), "max_abs": float(d.max()),
+                         "top100_agreement": top_agreement(r.scores, ref.scores)},
+            "cpu_baseline": {"value": round(cpu_s * 1e3, 2), "unit": "ms per run", "cores": 1,
+                             "kind": "port",
+                             "sample": "the oracle's pagerank_run with the dense adapter's "
+                                       "neighbour lists built from the matrix (one run)"},
+            "speedup_vs_cpu": round(cpu_s / gpu_s, 1)}
+
+
+def c2_leg(ga, generators, reps=10, push_sweeps=3, find_edge_frac=0.02):
+    """C2: R-MAT scale 20, weights U[0.1, 1) stored f32, seed 7,
+    pagerank_weighted_run at d 0.85, tol 1e-9, max 200 (pagerank.rs:210-289):
+    GTEPS per iteration, the f32 sweep's roofline, accuracy against the f64
+    oracle and the extended-precision iteration; CPU: the oracle's push sweep
+    on one core, its OpenMP pull on every usable core, and the reference's
+    per-edge find_edge lookup (graph.rs:205-211) timed on a sample of
+    sources."""
+    import oracle
+
+    c = CONFIGS["c2"]
+    e = generators.rmat(c["scale"], 16, 0.57, 0.19, 0.19, c["seed"], c["weights"], c["p0"],
+                        c["p1"], c["rounding"])
+    n, m = e.n, e.m
+    cfg = ga.PageRankConfig(0.85, c["max_it"], c["tol"])
+    ctx = ga.Context(0, timing=True)
+    dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"], ctx=ctx)
+    ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
+    sweeps = 0
+    f32_ms = 0.0
+    n32 = 0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
+        sweeps += r.iterations
+        f32_ms += r.stats["fp32_sweep_ms"]
+        n32 += r.stats["sweeps_fp32"]
+    wall = time.perf_counter() - t0
+    scores = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
+    dg.close()
+    ctx.close()
+    alg = (4 + c["w_bytes"]) * m + 16 * n + 4
+    avg_f32 = f32_ms / max(n32, 1) / 1e3
+    in_off, in_src, in_w = oracle.transpose(e.offsets, e.targets, e.weights)
+    ws = oracle.out_wsum(e.offsets, e.weights)
+    t0 = time.perf_counter()
+    ref = oracle.pull_run(1, in_off, in_src, ws, w=in_w, damping=0.85, max_iterations=c["max_it"],
+                          tolerance=c["tol"], threads=0)
+    all_core_s = time.perf_counter() - t0
+    exact = oracle.pull_run_ld(1, in_off, in_src, w=in_w, damping=0.85,
+                               max_iterations=c["max_it"], tolerance=c["tol"], threads=0)
+    d = np.abs(scores - ref.scores)
+    x = np.full(n, 1.0 / n)
+    t0 = time.perf_counter()
+    for _ in range(push_sweeps):
+        x, _ = oracle.push_sweep(e.offsets, e.targets, x, 0.85, e.weights, ws)
+    push_s = (time.perf_counter() - t0) / push_sweeps
+    # find_edge: the scatter of a random sample of sources with each entry's
+    # weight found by a scan of its source's list; extrapolated to every
+    # source by the sum of squared out-degrees (the scan cost)
+    deg = np.diff(e.offsets.astype(np.int64))
+    rng = np.random.default_rng(3)
+    us = np.sort(rng.choice(n, size=max(1, int(find_edge_frac * n)), replace=False)).astype(np.uint32)
+    t0 = time.perf_counter()
+    oracle.push_find_edge(us, e.offsets, e.targets, e.weights, ws, np.full(n, 1.0 / n))
+    fe_s = time.perf_counter() - t0
+    scale_up = float((deg.astype(np.float64) ** 2).sum() / max((deg[us].astype(np.float64) ** 2).sum(), 1.0))
+    info = machine_info()
+    return {"workload": c["workload"], "n": n, "m": m, "sweeps_per_run": r.iterations,
+            "gteps": round(m * sweeps / wall / 1e9, 2), "ms_per_run": round(wall / reps * 1e3, 3),
+            "delta_sweep": "blocked" if r.stats["sweeps_blocked"] else "pull",
+            "f32_sweep_us": round(avg_f32 * 1e6, 2),
+            "roofline": {"bound": "hbm", "alg_bytes_per_launch": alg,
+                         "achieved": round(alg / avg_f32 / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(alg / avg_f32 / 1e9 / HBM_PEAK_GBS, 4)},
+            "accuracy": {"l1_vs_f64_oracle": float(d.sum()), "max_abs": float(d.max()),
+                         "top100_agreement": top_agreement(scores, ref.scores),
+                         "oracle_iterations": ref.iterations, "gpu_iterations": r.iterations,
+                         "l1_gpu_vs_exact": float(np.abs(scores - exact.scores).sum()),
+                         "exact_iterations": exact.iterations},
+            "cpu_baseline": {"value": round(m / push_s / 1e9, 4), "unit": "GTEPS", "cores": 1,
+                             "kind": "port",
+                             "sample": f"{push_sweeps} push sweeps of the f64 oracle, one thread "
+                                       "(adjacency lists with the weights beside the targets)"},
+            "best_cpu": {"value": round(m * ref.iterations / all_core_s / 1e9, 4),
+                         "unit": "GTEPS", "cores": info["omp_threads"], "kind": "port",
+                         "sample": f"the oracle's OpenMP pull, the whole run ({ref.iterations} "
+                                   "sweeps)"},
+            "find_edge_footnote": {
+                "what": "the reference's per-entry edge_weight through petgraph's find_edge "
+                        "(graph.rs:205-211, pagerank.rs:257): a scan of the source's list",
+                "sample_sources": int(us.size), "sample_s": round(fe_s, 3),
+                "projected_s_per_sweep": round(fe_s * scale_up, 2),
+                "ns_per_edge": round(fe_s * scale_up / m * 1e9, 1),
+                "cores": 1}}
 
 
 PPR_GRAPH = dict(scale=22, seed=13)  # unit weights
@@ -449,6 +640,7 @@ def main():
     ap.add_argument("--delta-sweep", default="auto", choices=["auto", "pull", "blocked"],
                     help="how f32 delta sweeps read the graph (GOPS_OPT_DELTA_SWEEP)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--no-c1c2", action="store_true", help="skip the C1 and C2 legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -515,6 +707,13 @@ def main():
     wall = time.perf_counter() - t0
     log(f"timed {args.steps} runs: {wall:.3f} s, {sweeps} sweeps ({n32} f32 + {n64} f64)")
 
+    # the same runs with every result downloaded (n f64 over PCIe)
+    t0 = time.perf_counter()
+    pcie_runs = 2
+    for _ in range(pcie_runs):
+        rp = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=True)
+    pcie_wall = (time.perf_counter() - t0) / pcie_runs
+    log(f"with the result downloaded: {pcie_wall * 1e3:.2f} ms per run")
     n, m = e.n, e.m
     value = m * sweeps / wall / 1e9
     alg_bytes = (4 + c["w_bytes"]) * m + 16 * n + 4
@@ -553,7 +752,14 @@ def main():
                      "traffic": measured_traffic(c["workload"], "blocked" if blocked else "pull"),
                      "kernel": sweep_name, "delta_sweep": "blocked" if blocked else "pull",
                      "alg_bytes_per_launch": alg_bytes},
+        "pcie_inclusive": {"what": "whole runs with the n f64 scores downloaded to the host",
+                           "ms_per_run": round(pcie_wall * 1e3, 3),
+                           "gteps": round(m * rp.iterations / pcie_wall / 1e9, 3)},
     }
+    try:
+        out["roofline"]["measured_copy_peak"] = copy_peak()
+    except Exception as exc:  # no HIP runtime: the nominal peak alone
+        out["roofline"]["measured_copy_peak"] = {"error": str(exc)[:80]}
     if not args.no_gather_probe:
         pm = gather_probe(dg)
         out["gather_bound"] = {
@@ -568,7 +774,7 @@ def main():
     if not args.no_accuracy:
         scores = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
         t0 = time.perf_counter()
-        ref, exact = oracle_reference(e, c["tol"], c["max_it"])
+        ref, exact, ref_s = oracle_reference(e, c["tol"], c["max_it"])
         log(f"oracle runs: f64 {ref.iterations} sweeps, long double {exact.iterations} sweeps, "
             f"{time.perf_counter() - t0:.1f} s")
         diff = np.abs(scores - ref.scores)
@@ -585,8 +791,17 @@ def main():
                                "l1_f64_oracle_vs_exact": float(np.abs(ref.scores -
                                                                       exact.scores).sum()),
                                "f64_oracle_mass_drift": float(ref.scores.sum() - 1.0)}}
+        out["best_cpu"] = {"value": round(m * ref.iterations / ref_s / 1e9, 4), "unit": "GTEPS",
+                           "cores": machine_info()["omp_threads"], "kind": "port",
+                           "sample": f"the oracle's OpenMP pull (f64), the whole run "
+                                     f"({ref.iterations} sweeps, {ref_s:.1f} s)"}
         del ref, exact
     out["cpu_baseline"] = cpu_baseline(e, c, args.cpu_sweeps) if args.cpu_sweeps > 0 else None
+    if not args.no_c1c2:
+        out["c1"] = c1_leg(ga, generators)
+        log(f"C1: {out['c1']['gpu_ms_per_run']} ms per run on the GPU")
+        out["c2"] = c2_leg(ga, generators)
+        log(f"C2: {out['c2']['gteps']} GTEPS, f32 sweep {out['c2']['f32_sweep_us']} us")
     if not args.no_build:
         out["graph_build"] = build_leg(ga, e, c["storage"])
     if not args.no_centrality:

@@ -60,6 +60,8 @@ def lib():
         L.orc_push_sweep.argtypes = [ctypes.c_int, ctypes.c_uint64, _u64p, _u32p, _f64p, _f64p,
                                      ctypes.c_double, _f64p, _f64p]
         L.orc_push_sweep.restype = ctypes.c_double
+        L.orc_push_find_edge.argtypes = [ctypes.c_uint64, _u32p, _u64p, _u32p, _f64p, _f64p,
+                                         ctypes.c_double, _f64p, _f64p]
         L.orc_transpose.argtypes = [ctypes.c_uint64, _u64p, _u32p, _f64p, _u64p, _u32p, _f64p]
         L.orc_top_k.argtypes = [ctypes.c_uint64, _f64p, ctypes.c_uint64, _u64p, _f64p]
         L.orc_top_k.restype = ctypes.c_uint64
@@ -257,6 +259,23 @@ def push_sweep(off, nb, cur, damping=0.85, ew=None, ws=None):
     r = lib().orc_push_sweep(int(weighted), n, _p(off, _u64p), _p(nb, _u32p), _p(ewc, _f64p),
                              _p(wsc, _f64p), damping, _p(cur, _f64p), _p(out, _f64p))
     return out, float(r)
+
+
+def push_find_edge(us, off, nb, ew, ws, cur, damping=0.85, out=None):
+    """The weighted sweep's scatter for the sources `us`, each entry's weight
+    looked up by a find_edge scan (graph.rs:205-211); adds into out."""
+    us = np.ascontiguousarray(us, dtype=np.uint32)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    nb = np.ascontiguousarray(nb, dtype=np.uint32)
+    ew = np.ascontiguousarray(ew, dtype=np.float64)
+    ws = np.ascontiguousarray(ws, dtype=np.float64)
+    cur = np.ascontiguousarray(cur, dtype=np.float64)
+    n = off.size - 1
+    if out is None:
+        out = np.zeros(n, dtype=np.float64)
+    lib().orc_push_find_edge(us.size, _p(us, _u32p), _p(off, _u64p), _p(nb, _u32p), _p(ew, _f64p),
+                             _p(ws, _f64p), damping, _p(cur, _f64p), _p(out, _f64p))
+    return out
 
 
 def validate_katz(alpha: float, beta: float, max_iterations: int, tolerance: float) -> None:

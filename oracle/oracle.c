@@ -580,6 +580,29 @@ double orc_push_sweep(int weighted, uint64_t n, const uint64_t *off, const uint3
     return l1_seq(n, cur, out);
 }
 
+/* The scatter of pagerank_weighted_run's sweep (pagerank.rs:252-263) for the
+ * sources listed in us[0..nu), with the weight of every neighbour entry
+ * looked up the way the petgraph adapter's edge_weight does (graph.rs:
+ * 205-211): find_edge(u, v) scans u's edges for the first one to v, so every
+ * entry costs a scan of u's list up to v's first occurrence.  For timing the
+ * reference's per-edge lookup on a sample of sources; adds into out[n]. */
+void orc_push_find_edge(uint64_t nu, const uint32_t *us, const uint64_t *off, const uint32_t *nb,
+                        const double *ew, const double *ws, double damping, const double *cur,
+                        double *out) {
+    for (uint64_t i = 0; i < nu; i++) {
+        uint64_t u = us[i];
+        double w_s = ws[u];
+        if (!(w_s > 0.0)) continue;
+        for (uint64_t j = off[u]; j < off[u + 1]; j++) {
+            uint32_t v = nb[j];
+            uint64_t f = off[u];
+            while (nb[f] != v) f++; /* find_edge: the first entry to v */
+            double w = fmax(ew[f], 0.0);
+            if (w > 0.0) out[v] += damping * cur[u] * (w / w_s);
+        }
+    }
+}
+
 /* Transpose neighbour lists (adapter order) into in-edge lists sorted by
  * ascending source; duplicates keep their adapter order.  Sequential. */
 void orc_transpose(uint64_t n, const uint64_t *off, const uint32_t *nb, const double *ew,

@@ -338,3 +338,28 @@ def test_push_sweep_is_the_runs_sweep(oracle, weighted):
             ref = run(k)
             assert np.array_equal(x, ref.scores, equal_nan=True), (trial, k)
             assert diff == ref.diff_l1 or (np.isnan(diff) and np.isnan(ref.diff_l1))
+
+
+def test_find_edge_scatter_is_the_push_scatter(oracle):
+    """push_find_edge (the timing restatement of the petgraph adapter's
+    per-entry edge_weight) equals the push sweep's scatter when every list
+    holds each target once; with a duplicate target, every entry to it reads
+    the first entry's weight (find_edge returns the first edge)."""
+    rng = np.random.default_rng(9)
+    n = 60
+    nbrs = [sorted(set(rng.integers(0, n, size=int(rng.integers(0, 8))).tolist())) for _ in range(n)]
+    off, nb = lists_from(nbrs)
+    ew = rng.uniform(0.1, 2.0, size=nb.size)
+    ws = oracle.out_wsum(off, ew)
+    x = rng.random(n)
+    full, _ = oracle.push_sweep(off, nb, x, 0.85, ew=ew, ws=ws)
+    dsum = sum(x[i] for i in range(n) if ws[i] == 0.0)
+    fill = (1 - 0.85) / n + 0.85 * dsum / n
+    got = oracle.push_find_edge(np.arange(n), off, nb, ew, ws, x, 0.85, out=np.full(n, fill))
+    assert np.array_equal(got, full)
+    # a duplicate: node 0 -> 1 twice with different weights
+    off2, nb2 = lists_from([[1, 1], [0]])
+    ew2 = np.array([2.0, 5.0, 1.0])
+    ws2 = oracle.out_wsum(off2, ew2)
+    got2 = oracle.push_find_edge(np.arange(2), off2, nb2, ew2, ws2, np.array([0.5, 0.5]), 0.85)
+    assert got2[1] == 2 * (0.85 * 0.5 * (2.0 / 7.0))
