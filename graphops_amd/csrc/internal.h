@@ -183,7 +183,9 @@ constexpr int kPbItem2 = 1 << 16;           // runs per accumulate workgroup, at
 constexpr uint16_t kPbSrcMask = (1u << kPbChunkShift) - 1;
 constexpr uint16_t kPbRunHead = 1u << 14;   // entry starts a run
 constexpr uint16_t kPbSegHead = 1u << 15;   // entry starts a segment (and a run)
-constexpr uint64_t kPbAutoNodes = 1ull << 24;  // GOPS_DELTA_SWEEP_AUTO: blocked from here
+// GOPS_DELTA_SWEEP_AUTO: blocked from here (f32 sweep, bf16 weights, R-MAT:
+// scale 20 pull 105 us / blocked 101 us, scale 21 194 / 119, 22 357 / 172)
+constexpr uint64_t kPbAutoNodes = 1ull << 21;
 
 struct PbItem {
   int32_t x, y, z, w;

@@ -107,12 +107,13 @@ enum {
                                    L1 error of a few 1e-8 from f32 rounding) */
     GOPS_OPT_DELTA_SWEEP = 12   /* mixed, one vector per run: how f32 delta sweeps read
                                    the graph: GOPS_DELTA_SWEEP_AUTO (default: blocked
-                                   from 2^24 nodes, where the f32 source vector is far
-                                   beyond the L2; pull below), _PULL (gathers over the
-                                   CSR of in-edges) or _BLOCKED (a scatter pass per
-                                   chunk of sources and an accumulate pass per bin of
-                                   rows, both streaming, integer row sums; its layout is
-                                   built on the graph's first such run) */
+                                   from 2^21 nodes, where the f32 source vector outgrows
+                                   the L2; pull below), _PULL (gathers over the CSR of
+                                   in-edges) or _BLOCKED (a scatter pass per chunk of
+                                   sources that sums each row's run of edges, and an
+                                   accumulate pass per bin of rows, both streaming,
+                                   integer row sums; its layout is built on the graph's
+                                   first such run) */
 };
 enum { GOPS_DELTA_SWEEP_AUTO = 0, GOPS_DELTA_SWEEP_PULL = 1, GOPS_DELTA_SWEEP_BLOCKED = 2 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };

@@ -156,10 +156,10 @@ def test_small_and_empty(oracle, ctx_b):
         dg.close()
 
 
-def test_default_below_2_24_nodes_is_the_pull(ctx_p):
-    """GOPS_DELTA_SWEEP_AUTO (the default) pulls below 2^24 nodes: the same
-    bits as a context that chose the pull; unknown values are refused.  (At
-    2^24 nodes it takes the blocked sweep: test_gpu_full_size.py's scale-24
+def test_default_below_2_21_nodes_is_the_pull(ctx_p):
+    """GOPS_DELTA_SWEEP_AUTO (the default) pulls below 2^21 nodes: the same
+    bits as a context that chose the pull; unknown values are refused.  (From
+    2^21 nodes it takes the blocked sweep: test_gpu_full_size.py's scale-24
     run.)"""
     c = ga.Context(0, precision="mixed")
     e = generators.rmat(14, 16, seed=3)
