@@ -166,14 +166,21 @@ def c1_leg(ga, generators, reps=20):
     mat = np.zeros((n, n))
     for u in range(n):
         mat[u, e.targets[e.offsets[u]:e.offsets[u + 1]]] = 1.0
+    from graphops_amd import _lib as L
+
     cfg = ga.PageRankConfig(0.85, 100, 1e-10)
     ctx = ga.Context(0)
     dg = ga.DeviceGraph.from_dense(mat, weighted=False, ctx=ctx)
-    r = ga.pagerank_run(dg, cfg, ctx=ctx)
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    launch_ms = {}
+    for name, mode in (("queued", L.LAUNCH_QUEUED), ("graph", L.LAUNCH_GRAPH)):
+        ctx.set_option(L.OPT_LAUNCH, mode)
         r = ga.pagerank_run(dg, cfg, ctx=ctx)
-    gpu_s = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = ga.pagerank_run(dg, cfg, ctx=ctx)
+        launch_ms[name] = (time.perf_counter() - t0) / reps * 1e3
+    ctx.set_option(L.OPT_LAUNCH, L.LAUNCH_AUTO)
+    gpu_s = min(launch_ms.values()) / 1e3
     t0 = time.perf_counter()
     off, nb, _ = oracle.dense_neighbors(mat)
     ref = oracle.pagerank_run(off, nb, 0.85, 100, 1e-10)
@@ -185,6 +192,7 @@ def c1_leg(ga, generators, reps=20):
     return {"workload": "er2000 p0.005 seed42 dense AdjacencyMatrix, pagerank_run d0.85 tol1e-10",
             "n": n, "m": m, "sweeps": r.iterations, "oracle_sweeps": ref.iterations,
             "gpu_ms_per_run": round(gpu_s * 1e3, 3),
+            "launch_ms_per_run": {k: round(v, 3) for k, v in launch_ms.items()},
             "gpu_gteps": round(m * r.iterations / gpu_s / 1e9, 3),
             "accuracy": {"l1_vs_f64_oracle": float(d.sum()), "max_abs": float(d.max()),
                          "top100_agreement": top_agreement(r.scores, ref.scores)},
@@ -692,20 +700,40 @@ def main():
     for i in range(args.warmup):
         r = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
         log(f"warmup {i}: {r.iterations} sweeps, diff {r.diff_l1:.3e}")
+    # the timed runs take the library's default launch path; per-sweep kernel
+    # times come from separate runs with every sweep timed, and the replay of
+    # captured sweep chains (GOPS_LAUNCH_GRAPH) is timed beside them
+    ctx.set_option(L.OPT_TIMING, 0)
     sweeps = 0
-    f32_ms = f64_ms = 0.0
-    n32 = n64 = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         r = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
-        st = r.stats
         sweeps += r.iterations
+    wall = time.perf_counter() - t0
+    launch = "graph" if r.stats["graph_replay"] else "queued"
+    ctx.set_option(L.OPT_LAUNCH, L.LAUNCH_GRAPH)
+    ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)  # captures the chain
+    t0 = time.perf_counter()
+    for i in range(2):
+        ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
+    graph_wall = (time.perf_counter() - t0) / 2
+    ctx.set_option(L.OPT_LAUNCH, L.LAUNCH_AUTO)
+    ctx.set_option(L.OPT_TIMING, 1)
+    f32_ms = f64_ms = 0.0
+    n32 = n64 = 0
+    qsweeps = 0
+    t0 = time.perf_counter()
+    for i in range(2):
+        r = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
+        st = r.stats
+        qsweeps += r.iterations
         f32_ms += st["fp32_sweep_ms"]
         f64_ms += st["fp64_sweep_ms"]
         n32 += st["sweeps_fp32"]
         n64 += st["sweeps_fp64"]
-    wall = time.perf_counter() - t0
-    log(f"timed {args.steps} runs: {wall:.3f} s, {sweeps} sweeps ({n32} f32 + {n64} f64)")
+    qwall = time.perf_counter() - t0
+    log(f"timed {args.steps} runs ({launch}): {wall:.3f} s, {sweeps} sweeps; graph replay "
+        f"{graph_wall * 1e3:.2f} ms per run; every sweep timed {qwall / 2 * 1e3:.2f} ms per run")
 
     # the same runs with every result downloaded (n f64 over PCIe)
     t0 = time.perf_counter()
@@ -744,7 +772,10 @@ def main():
                    "delta_sweep": "blocked" if blocked else "pull",
                    "sweep_us": round(avg_sweep * 1e6, 2),
                    "f32_sweep_us": round(avg_f32 * 1e6, 2), "f64_sweep_us": round(avg_f64 * 1e6, 2),
-                   "gteps_f32_sweep": round(m / avg_f32 / 1e9, 2) if n32 else None},
+                   "gteps_f32_sweep": round(m / avg_f32 / 1e9, 2) if n32 else None,
+                   "launch": launch,
+                   "graph_replay_ms_per_run": round(graph_wall * 1e3, 3),
+                   "timed_kernels_ms_per_run": round(qwall / 2 * 1e3, 3)},
         "gather_bound": None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -32,6 +32,8 @@ OPT_F32_BELOW, OPT_PPR_F32_BELOW, OPT_PPR_ANCHOR_PERIOD = 7, 8, 9
 OPT_GRAPH_BUILD = 10
 OPT_TILE_F32_BELOW = 11
 OPT_DELTA_SWEEP = 12
+OPT_LAUNCH = 13
+LAUNCH_AUTO, LAUNCH_QUEUED, LAUNCH_GRAPH = 0, 1, 2
 DELTA_SWEEP_AUTO, DELTA_SWEEP_PULL, DELTA_SWEEP_BLOCKED = 0, 1, 2
 BUILD_DEVICE, BUILD_HOST = 0, 1
 PRECISION_MIXED, PRECISION_FP64 = 0, 1
@@ -60,7 +62,7 @@ class RunStatsC(ctypes.Structure):
                 ("device_ms", ctypes.c_double), ("fp32_sweep_ms", ctypes.c_double),
                 ("fp64_sweep_ms", ctypes.c_double), ("other_ms", ctypes.c_double),
                 ("sweeps_blocked", ctypes.c_uint64), ("blocked_fallback", ctypes.c_int32),
-                ("reserved2", ctypes.c_int32)]
+                ("graph_replay", ctypes.c_int32)]
 
 
 class ServerConfigC(ctypes.Structure):

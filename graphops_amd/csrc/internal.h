@@ -128,6 +128,9 @@ struct RunBuffers {
   uint8_t *sweep_log = nullptr;
   size_t partial_slots = 0;
   bool has_result = false;       // out holds the last run's scores
+  // captured chains of kChainSweeps sweeps, one per kind of run
+  // (weighted, personalized, blocked: the bits of the index)
+  hipGraphExec_t chain[8] = {};
 };
 
 // State of a Katz / eigenvector / HITS power iteration in device memory.
@@ -214,6 +217,14 @@ struct BlockedLayout {
 };
 
 constexpr uint64_t kSweepLogCap = 1 << 16;
+#ifndef GOPS_TUNE_CHAIN
+#define GOPS_TUNE_CHAIN 4
+#endif
+#ifndef GOPS_TUNE_CHAIN_AHEAD
+#define GOPS_TUNE_CHAIN_AHEAD 1
+#endif
+constexpr int kChainSweeps = GOPS_TUNE_CHAIN;       // sweeps per captured chain
+constexpr int kChainAhead = GOPS_TUNE_CHAIN_AHEAD;  // chains queued beyond the last seen sweep
 constexpr size_t kMaxReduce = 8;  // partial sums per workgroup, at most
 constexpr int kMaxSweepBlocks = 2048;  // grid of the sweep kernel (grid-stride over descriptors)
 
@@ -269,6 +280,7 @@ struct gops_context {
   int ppr_tile = 0;
   int graph_build = GOPS_BUILD_DEVICE;
   int delta_sweep = GOPS_DELTA_SWEEP_AUTO;
+  int launch = GOPS_LAUNCH_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
   void *noalloc_scratch = nullptr;       // device address of topk.hip's file-scope scratch

@@ -105,6 +105,14 @@ enum {
                                    the tolerance of the f64 reference; 0: f32 from the
                                    sweep after the admission's f64 sweep, faster, with an
                                    L1 error of a few 1e-8 from f32 rounding) */
+    GOPS_OPT_LAUNCH = 13,       /* how a run's sweeps are queued: GOPS_LAUNCH_AUTO (default:
+                                   queued; measured no slower than the replays on every
+                                   benchmark graph), _QUEUED (the sweeps' kernels
+                                   launched one by one, a
+                                   few ahead of the last finished sweep) or _GRAPH (a
+                                   HIP graph of a few sweeps, captured once per graph
+                                   and kind of run, replayed until the device says
+                                   stop); the results are the same bits */
     GOPS_OPT_DELTA_SWEEP = 12   /* mixed, one vector per run: how f32 delta sweeps read
                                    the graph: GOPS_DELTA_SWEEP_AUTO (default: blocked
                                    from 2^21 nodes, where the f32 source vector outgrows
@@ -116,6 +124,7 @@ enum {
                                    first such run) */
 };
 enum { GOPS_DELTA_SWEEP_AUTO = 0, GOPS_DELTA_SWEEP_PULL = 1, GOPS_DELTA_SWEEP_BLOCKED = 2 };
+enum { GOPS_LAUNCH_AUTO = 0, GOPS_LAUNCH_QUEUED = 1, GOPS_LAUNCH_GRAPH = 2 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
 enum { GOPS_BUILD_DEVICE = 0, GOPS_BUILD_HOST = 1 };
 
@@ -133,7 +142,7 @@ typedef struct gops_run_stats {
     int32_t blocked_fallback; /* 1: GOPS_DELTA_SWEEP_AUTO chose the blocked layout but it
                                  did not fit in free device memory, so the f32 delta
                                  sweeps ran as pulls (retried on later runs) */
-    int32_t reserved2;
+    int32_t graph_replay;   /* 1: the sweeps ran as replays of a captured HIP graph */
 } gops_run_stats;
 
 /* ---- context ---------------------------------------------------------- */

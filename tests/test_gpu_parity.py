@@ -392,3 +392,41 @@ def test_normalize_option(oracle, kind):
     np.testing.assert_allclose(normed.scores, plain.scores / plain.scores.sum(), rtol=1e-14,
                                atol=0)
     check_run(normed, ref, tol, "mixed", f"normalize-{kind}")
+
+
+@pytest.mark.parametrize("delta", ["pull", "blocked"])
+def test_graph_replay_is_the_queued_run(oracle, delta):
+    """GOPS_LAUNCH_GRAPH (the sweep chain captured once as a HIP graph of
+    a few sweeps and replayed until the device stops it) gives
+    the same bits as GOPS_LAUNCH_QUEUED (kernel by kernel), for plain,
+    weighted and personalized runs, with the iteration cap falling inside a
+    replay; and replays the captured chain on later runs."""
+    from graphops_amd import _lib as L
+
+    e = rmat_case(14, 5, "lognormal", "bf16")
+    p = np.zeros(e.n)
+    p[[1, 50, 900]] = [1.0, 0.5, 2.0]
+    out = {}
+    for launch in (L.LAUNCH_QUEUED, L.LAUNCH_GRAPH):
+        cx = ga.Context(0)
+        cx.set_option(L.OPT_LAUNCH, launch)
+        cx.set_option(L.OPT_DELTA_SWEEP, {"pull": L.DELTA_SWEEP_PULL,
+                                          "blocked": L.DELTA_SWEEP_BLOCKED}[delta])
+        dg = ga.DeviceGraph.from_edges(e, weight_storage="bf16", ctx=cx)
+        runs = []
+        for rep in range(2):
+            for cfg in (ga.PageRankConfig(0.85, 300, 1e-10), ga.PageRankConfig(0.85, 7, 1e-14)):
+                runs.append(ga.pagerank_weighted_run(dg, cfg, ctx=cx))
+                assert cx.last_stats()["graph_replay"] == (launch == L.LAUNCH_GRAPH)
+                runs.append(ga.pagerank_run(dg, cfg, ctx=cx))
+                runs.append(ga.personalized_pagerank_run(dg, cfg, p, ctx=cx))
+        out[launch] = runs
+        dg.close()
+        cx.close()
+    for a, b in zip(out[L.LAUNCH_QUEUED], out[L.LAUNCH_GRAPH]):
+        assert a.iterations == b.iterations and a.converged == b.converged
+        assert a.diff_l1 == b.diff_l1 and np.array_equal(a.scores, b.scores)
+    capped = out[L.LAUNCH_GRAPH][3]  # the weighted run capped at 7 sweeps
+    assert capped.iterations == 7 and not capped.converged
+    ref = oracle_pull(oracle, e, 1, 1e-10, 300)
+    check_run(out[L.LAUNCH_GRAPH][0], ref, 1e-10, "mixed", f"graph-{delta}")
