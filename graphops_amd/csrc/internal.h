@@ -272,7 +272,7 @@ struct gops_context {
   double anchor_drop = 1e-2;
   double f32_below = 0.0;          // global PageRank: f32 sweeps from the start
   double ppr_f32_below = 1e4;      // personalized: f64 until diff <= 1e4 * tolerance
-  double tile_f32_below = 1e4;     // batched / served personalized (tiles): the same
+  double tile_f32_below = 0.0;     // batched / served personalized (tiles): f32 with carried rounding
   double ppr_anchor_min = 1e3;     // personalized: no anchor once diff <= 1e3 * tolerance
   int ppr_anchor_period = 64;
   int normalize = 0;

@@ -22,7 +22,7 @@ struct TileState {
   uint64_t anchor_period;
   double anchor_drop;    // 0: no residual-triggered anchors
   double f32_below;      // f32 sweeps only once diff <= f32_below * tol
-  double anchor_min;     // periodic anchors only while diff > anchor_min * tol
+  double anchor_min;     // anchors only while diff > anchor_min * tol
   // per query slot
   double damping[kMaxTile];
   double tol[kMaxTile];
@@ -30,6 +30,7 @@ struct TileState {
   double mass_in[kMaxTile];
   double dsum_x[kMaxTile];
   double init_dsum[kMaxTile];  // dangling part of the personalization (admission)
+  double init_mass[kMaxTile];  // sum of the personalization (admission)
   double diff[kMaxTile];
   double diff_anchor[kMaxTile];
   double x_scale[kMaxTile];
@@ -52,6 +53,9 @@ struct PprArgs {
   const double *inv64;
   double *x;            // [n][T] iterate
   float *yd0, *yd1;     // [n][T] prescaled deltas (f32), ping-pong
+  float *carry;         // [n][T] what the row's last f32 delta rounded off, added
+                        // to its next one (so the sums the sweeps propagate lag
+                        // the f64 deltas by one rounding, not by all of them)
   const float *yin32;   // picked per sweep
   float *yout32;
   double *y64a, *y64b;  // [n][T] prescaled iterate for anchor sweeps, ping-pong

@@ -86,7 +86,10 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *
     delta = t - sc * a.x[at];
     a.x[at] = t;
     const double inv = a.inv64[r];
-    a.yout32[at] = (float)((d * delta) * inv);
+    const double v = (d * delta) * inv;  // the delta chain restarts here: no carry
+    const float yo = (float)v;
+    a.yout32[at] = yo;
+    a.carry[at] = (float)(v - (double)yo);
     if (ts->f64_phase[q]) (c64 ? a.y64a : a.y64b)[at] = (d * t) * inv;  // next anchor's input
     dangling = inv == 0.0;
     dang_x = dangling ? t : 0.0;
@@ -95,13 +98,17 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *
     delta = acc + d * (ts->dsum_in[q] - ts->mass_in[q]) * seed_p;
     const double x = __builtin_nontemporal_load(a.x + at) + delta;
     __builtin_nontemporal_store(x, a.x + at);
-    const float inv = a.inv32[r];
-    a.yout32[at] = (float)(d * delta * (double)inv);
-    dangling = inv == 0.0f;
+    const double inv = a.inv64[r];
+    // the f32 delta carries the rounding of the row's previous one: summed
+    // over the sweeps, the roundings cancel but for the last
+    const double v = (d * delta) * inv + (double)a.carry[at];
+    const float yo = (float)v;
+    a.yout32[at] = yo;
+    a.carry[at] = (float)(v - (double)yo);
+    dangling = inv == 0.0;
     if (mode == kModeDeltaFold) {
-      const double inv64 = a.inv64[r];
-      (c64 ? a.y64a : a.y64b)[at] = (d * x) * inv64;  // the anchor that follows
-      dang_x = inv64 == 0.0 ? x : 0.0;
+      (c64 ? a.y64a : a.y64b)[at] = (d * x) * inv;  // the anchor that follows
+      dang_x = dangling ? x : 0.0;
       sum_x = x;
     }
   }
@@ -156,6 +163,7 @@ struct Q4 {
 struct LaneModes {
   bool a0, a1, a2, a3;  // query j of the lane is in an f64 anchor sweep
   bool need32, need64;  // some query of the lane gathers f32 deltas / the f64 iterate
+  bool live;            // some query of the lane is live (else the lane gathers nothing)
 };
 
 template <int T>
@@ -198,6 +206,7 @@ template <int T>
 __device__ __forceinline__ Q4 row_slice_sum(const PprArgs &a, const double *y64,
                                             const LaneModes &m, int b, int e, int step, int q0) {
   Q4 acc;
+  if (!m.live) return acc;  // stopped queries and free slots: no loads
   int k = b;
   if (!m.need64) {
     for (; k + 3 * step < e; k += 4 * step) {
@@ -286,6 +295,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
   m.a3 = ts->mode[q0 + 3] == kModeAnchor;
   m.need64 = m.a0 || m.a1 || m.a2 || m.a3;
   m.need32 = !(m.a0 && m.a1 && m.a2 && m.a3);
+  m.live = ts->live[q0] || ts->live[q0 + 1] || ts->live[q0 + 2] || ts->live[q0 + 3];
   // epilogue phase: query q of this lane
   const int q = threadIdx.x % T;
   const int mode = ts->mode[q];
@@ -431,8 +441,11 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
     } else {
       const uint64_t since = ts->since_anchor[qq] + 1;
       ts->since_anchor[qq] = since;
-      const bool anchor = diff <= ts->anchor_drop * ts->diff_anchor[qq] ||
-                          (since + 1 >= ts->anchor_period && diff > ts->anchor_min * ts->tol[qq]);
+      // no anchors once the residual is within anchor_min of the tolerance:
+      // the f32 rounding of the sweeps left is far below the tolerance then
+      const bool anchor = diff > ts->anchor_min * ts->tol[qq] &&
+                          (diff <= ts->anchor_drop * ts->diff_anchor[qq] ||
+                           since + 1 >= ts->anchor_period);
       next = anchor ? kModeDeltaFold : kModeDelta;
     }
     ts->mode[qq] = next;
@@ -466,32 +479,43 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
   }
 }
 
-// Admission, step 1: clear the admitted queries' columns (x = 0, and 0 in
-// the y64 buffer the next sweep's anchors gather).  One thread per row and
-// group of four queries; the tile's other queries are not touched.
+// Admission, step 1: clear the admitted queries' columns (x, the carry and
+// the f32 deltas the next sweep gathers = 0, and, for an anchor admission,
+// 0 in the y64 buffer the next sweep's anchors gather).  One thread per row and group of four queries;
+// the tile's other queries are not touched.
 template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uint64_t clear_mask) {
   constexpr int Q = T / kLaneQ;
   const int qg = threadIdx.x % Q;
   const unsigned m4 = (unsigned)(clear_mask >> (qg * kLaneQ)) & 0xFu;
   if (!m4) return;
-  double *y64 = a.ts->y64_par ? a.y64b : a.y64a;
+  // y64 matters only to an anchor admission: otherwise a fold sweep writes
+  // the query's whole y64 column before any anchor reads it
+  double *y64 = a.ts->f32_below > 0.0 ? (a.ts->y64_par ? a.y64b : a.y64a) : nullptr;
+  float *yin = a.ts->yd_cur ? a.yd1 : a.yd0;
   const int rows_per_block = kBlock / Q;
   for (int r = blockIdx.x * rows_per_block + threadIdx.x / Q; r < n;
        r += gridDim.x * rows_per_block) {
     const size_t at = (size_t)r * T + qg * kLaneQ;
     if (m4 == 0xFu) {
       const double2 z = {0.0, 0.0};
+      const float4 zf = {0.0f, 0.0f, 0.0f, 0.0f};
       *reinterpret_cast<double2 *>(a.x + at) = z;
       *reinterpret_cast<double2 *>(a.x + at + 2) = z;
-      *reinterpret_cast<double2 *>(y64 + at) = z;
-      *reinterpret_cast<double2 *>(y64 + at + 2) = z;
+      if (y64) {
+        *reinterpret_cast<double2 *>(y64 + at) = z;
+        *reinterpret_cast<double2 *>(y64 + at + 2) = z;
+      }
+      *reinterpret_cast<float4 *>(yin + at) = zf;
+      *reinterpret_cast<float4 *>(a.carry + at) = zf;
     } else {
 #pragma unroll
       for (int j = 0; j < kLaneQ; j++) {
         if ((m4 >> j) & 1u) {
           a.x[at + j] = 0.0;
-          y64[at + j] = 0.0;
+          if (y64) y64[at + j] = 0.0;
+          yin[at + j] = 0.0f;
+          a.carry[at + j] = 0.0f;
         }
       }
     }
@@ -499,11 +523,17 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
 }
 
 // Admission, step 2: the admitted queries start from their personalization
-// (x = p-hat on the seeds, ppr.rs:110) with an f64 anchor sweep.
+// (x = p-hat on the seeds, ppr.rs:110).  Their first sweep computes
+// T(p-hat) - p-hat.  With a leading f64 phase (f32_below > 0) it is an f64
+// anchor sweep over y64 = d p-hat / deg.  Without, it is a delta sweep whose
+// input delta is p-hat itself: d A p-hat + d (dangling(p-hat) - sum(p-hat))
+// p-hat is T(p-hat) - p-hat, as sum(p-hat) = 1 (ppr.rs:125-137).
 template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t admit_mask) {
   TileState *ts = a.ts;
+  const bool anchored = ts->f32_below > 0.0;
   double *y64 = ts->y64_par ? a.y64b : a.y64a;
+  float *yin = ts->yd_cur ? a.yd1 : a.yd0;
   for (int j = threadIdx.x; j < a.num_seed_entries; j += kBlock) {
     const int qq = a.seed_q[j];
     if (!((admit_mask >> qq) & 1ull)) continue;
@@ -511,7 +541,14 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
     const size_t at = (size_t)r * T + qq;
     const double p = a.seed_p[j];
     a.x[at] = p;  // one entry per (row, query): the host merges duplicate seeds
-    y64[at] = (ts->damping[qq] * p) * a.inv64[r];
+    const double v = (ts->damping[qq] * p) * a.inv64[r];
+    if (anchored) {
+      y64[at] = v;
+    } else {
+      const float yo = (float)v;
+      yin[at] = yo;
+      a.carry[at] = (float)(v - (double)yo);
+    }
   }
   __syncthreads();
   if (threadIdx.x < T) {
@@ -523,9 +560,12 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
       ts->iter[qq] = 0;
       ts->converged[qq] = 0;
       ts->diff[qq] = 0.0;
-      ts->mode[qq] = kModeAnchor;
+      ts->mode[qq] = anchored ? kModeAnchor : kModeDelta;
       ts->since_anchor[qq] = 0;
-      ts->f64_phase[qq] = ts->f32_below > 0.0;
+      ts->f64_phase[qq] = anchored;
+      ts->dsum_in[qq] = ts->init_dsum[qq];
+      ts->mass_in[qq] = ts->init_mass[qq];
+      ts->diff_anchor[qq] = ts->init_mass[qq];
     }
   }
   __syncthreads();

@@ -27,6 +27,13 @@
 
 using namespace gops;
 
+#ifndef GOPS_TUNE_TILE_AHEAD
+#define GOPS_TUNE_TILE_AHEAD 4
+#endif
+#ifndef GOPS_TUNE_TILE_REFILL
+#define GOPS_TUNE_TILE_REFILL 4
+#endif
+
 namespace {
 
 gops_status sfail(gops_context *ctx, gops_status st, const std::string &msg) {
@@ -107,6 +114,7 @@ struct Query {
   gops_pagerank_config cfg{};
   std::vector<std::pair<uint32_t, double>> pers;  // (new id, normalised value)
   double init_dsum = 0.0;
+  double init_mass = 0.0;
   double submit_ms = 0.0;
   bool dense_fallback = false;  // personalization sum <= 0 -> uniform (ppr.rs:105-109)
 };
@@ -117,7 +125,7 @@ struct Tile {
   size_t n = 0;
   double *x = nullptr, *y64a = nullptr, *y64b = nullptr, *partials = nullptr,
          *hub_partial = nullptr;
-  float *yd0 = nullptr, *yd1 = nullptr;
+  float *yd0 = nullptr, *yd1 = nullptr, *carry = nullptr;
   int32_t *seed_idx = nullptr;
   int32_t *seed_row = nullptr, *seed_q = nullptr;
   double *seed_p = nullptr;
@@ -243,6 +251,7 @@ void free_tile(Tile *t) {
   hipFree(t->hub_partial);
   hipFree(t->yd0);
   hipFree(t->yd1);
+  hipFree(t->carry);
   hipFree(t->seed_idx);
   hipFree(t->seed_row);
   hipFree(t->seed_q);
@@ -252,7 +261,7 @@ void free_tile(Tile *t) {
   delete t;
 }
 
-size_t tile_bytes(size_t n, int T) { return n * (size_t)T * (8 + 8 + 8 + 4 + 4) + n * 4; }
+size_t tile_bytes(size_t n, int T) { return n * (size_t)T * (8 + 8 + 8 + 4 + 4 + 4) + n * 4; }
 
 gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile **out) {
   auto *t = new (std::nothrow) Tile;
@@ -270,6 +279,7 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile *
   chk(hipMalloc((void **)&t->y64b, std::max<size_t>(nt, 1) * 8));
   chk(hipMalloc((void **)&t->yd0, std::max<size_t>(nt, 1) * 4));
   chk(hipMalloc((void **)&t->yd1, std::max<size_t>(nt, 1) * 4));
+  chk(hipMalloc((void **)&t->carry, std::max<size_t>(nt, 1) * 4));
   chk(hipMalloc((void **)&t->partials, (size_t)(kMaxSweepBlocks + kTileReduce) * T * 5 * 8));
   chk(hipMalloc((void **)&t->hub_partial, (size_t)std::max(g->dev.num_hub_chunks, 1) * T * 8));
   chk(hipMalloc((void **)&t->seed_idx, std::max<size_t>(t->n, 1) * 4));
@@ -278,6 +288,7 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile *
   if (e == hipSuccess) chk(hipMemsetAsync(t->seed_idx, 0xff, std::max<size_t>(t->n, 1) * 4, ctx->stream));
   if (e == hipSuccess) chk(hipMemsetAsync(t->yd0, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
   if (e == hipSuccess) chk(hipMemsetAsync(t->yd1, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->carry, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
   if (e == hipSuccess) chk(hipStreamSynchronize(ctx->stream));
   if (e != hipSuccess) {
     free_tile(t);
@@ -311,6 +322,7 @@ PprArgs tile_args(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, 
   a.x = t->x;
   a.yd0 = t->yd0;
   a.yd1 = t->yd1;
+  a.carry = t->carry;
   a.y64a = t->y64a;
   a.y64b = t->y64b;
   a.partials = t->partials;
@@ -345,6 +357,7 @@ gops_status make_query(gops_context *ctx, gops_graph *g, PprGraphCache *c,
   q.cfg = *cfg;
   q.pers.clear();
   q.init_dsum = 0.0;
+  q.init_mass = 0.0;
   q.dense_fallback = !(psum > 0.0);
   if (q.dense_fallback) return GOPS_OK;
   for (auto &kv : acc) {
@@ -352,6 +365,7 @@ gops_status make_query(gops_context *ctx, gops_graph *g, PprGraphCache *c,
     const double v = kv.second / psum;
     q.pers.push_back({r, v});
     if (c->dangling[r]) q.init_dsum += v;
+    q.init_mass += v;
   }
   return GOPS_OK;
 }
@@ -413,10 +427,12 @@ void set_slot_params(Tile *t, int s, const Query &q, double f32_below, uint64_t 
   h->tol[s] = q.cfg.tolerance;
   h->max_iter[s] = q.cfg.max_iterations;
   h->init_dsum[s] = q.init_dsum;
+  h->init_mass[s] = q.init_mass;
   h->dsum_in[s] = 0.0;
   h->mass_in[s] = 0.0;
   h->diff_anchor[s] = 0.0;
-  h->anchor_drop = 0.0;  // personalized schedule (pagerank.hip): no residual-triggered anchors
+  h->anchor_drop = 0.0;  // no residual-triggered anchors: the carried f32 rounding does not
+                         // build up (ppr_epilogue)
   h->f32_below = f32_below;
   h->anchor_min = 1e3;
   h->anchor_period = period;
@@ -448,10 +464,11 @@ int tile_grid(const gops_graph *g) {
 }
 
 // Time per query-sweep of tile width T on this graph: T single-seed trial
-// queries (tolerance 0, four sweeps: one f64 anchor, then three f32 delta
-// sweeps), timed with events.  The estimate weights the two sweep kinds
-// equally, as the personalized schedule spends about half of its sweeps in
-// each.  Infinity if the tile does not fit in device memory.
+// queries (tolerance 0, four sweeps in the context's tile schedule), timed
+// with events.  With a leading f64 phase the first sweep is an f64 anchor and
+// the estimate weights the two sweep kinds equally, as such a schedule spends
+// about half of its sweeps in each; otherwise all four are f32 delta sweeps.
+// Infinity if the tile does not fit in device memory.
 gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, double *cost) {
   *cost = std::numeric_limits<double>::infinity();
   TilePlan *plan = nullptr;
@@ -474,9 +491,10 @@ gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T
     const uint32_t r = (uint32_t)(((uint64_t)s * 2654435761ull + 12345) % n);
     q.pers.push_back({r, 1.0});
     q.init_dsum = c->dangling[r] ? 1.0 : 0.0;
+    q.init_mass = 1.0;
     t->slot[s] = q;
     t->used[s] = true;
-    set_slot_params(t, s, q, 0.0, 64);
+    set_slot_params(t, s, q, ctx->tile_f32_below, 64);
     mask |= 1ull << s;
   }
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -487,7 +505,7 @@ gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T
     if (st == GOPS_OK) {
       PprArgs a = tile_args(ctx, g, t, plan, false);
       e = hipEventRecord(ev[0], ctx->stream);
-      if (e == hipSuccess) e = ppr_launch_sweep(ctx->stream, T, a, t->grid);  // f64 anchor
+      if (e == hipSuccess) e = ppr_launch_sweep(ctx->stream, T, a, t->grid);  // admission sweep
       if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
       for (int k = 0; k < 3 && e == hipSuccess; k++) e = ppr_launch_sweep(ctx->stream, T, a, t->grid);
       if (e == hipSuccess) e = hipEventRecord(ev[2], ctx->stream);
@@ -495,7 +513,9 @@ gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T
       float ms_anchor = 0.f, ms_delta = 0.f;
       if (e == hipSuccess) e = hipEventElapsedTime(&ms_anchor, ev[0], ev[1]);
       if (e == hipSuccess) e = hipEventElapsedTime(&ms_delta, ev[1], ev[2]);
-      if (e == hipSuccess) *cost = 0.5 * ((double)ms_anchor + (double)ms_delta / 3.0) / T;
+      if (e == hipSuccess)
+        *cost = (ctx->tile_f32_below > 0.0 ? 0.5 * ((double)ms_anchor + (double)ms_delta / 3.0)
+                                           : ((double)ms_anchor + (double)ms_delta) / 4.0) / T;
     }
   }
   for (hipEvent_t x : ev)
@@ -597,8 +617,8 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
   t->hts->all_done = 1;
   for (int s = 0; s < T; s++) t->used[s] = false;
   uint32_t next = 0;
-  const int refill_groups = std::max(1, T / kLaneQ / 4);
-  const uint64_t kAhead = 4;  // sweeps queued beyond the last one seen to finish
+  const int refill_groups = std::max(1, T / kLaneQ / GOPS_TUNE_TILE_REFILL);
+  const uint64_t kAhead = GOPS_TUNE_TILE_AHEAD;  // sweeps queued beyond the last one seen to finish
   while (st == GOPS_OK) {
     // 1. admission into free slots (the host copy of the tile state is current)
     uint64_t mask = 0;

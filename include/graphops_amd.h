@@ -101,10 +101,10 @@ enum {
                                    arrays, and so every result, are the same */
     GOPS_OPT_TILE_F32_BELOW = 11, /* mixed, batched and served personalized queries (tiles):
                                    f64 sweeps while the residual is above this times the
-                                   tolerance (default 1e4, which keeps L1 within 1e-2 of
-                                   the tolerance of the f64 reference; 0: f32 from the
-                                   sweep after the admission's f64 sweep, faster, with an
-                                   L1 error of a few 1e-8 from f32 rounding) */
+                                   tolerance (default 0: f32 delta sweeps from the first,
+                                   each row carrying its f32 rounding into its next
+                                   delta, which keeps L1 near 1e-14 of the f64 reference;
+                                   the f64 phase only adds time) */
     GOPS_OPT_LAUNCH = 13,       /* how a run's sweeps are queued: GOPS_LAUNCH_AUTO (default:
                                    queued; measured no slower than the replays on every
                                    benchmark graph), _QUEUED (the sweeps' kernels
