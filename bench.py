@@ -354,7 +354,9 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
         "f64_phase_schedule": alt,
         "roofline": {"bound": "hbm", "queries_per_s_at_peak": round(roof_qps, 1),
                      "frac": round(qps / roof_qps, 4),
-                     "bytes_per_query_sweep": round(per_qsweep)},
+                     "bytes_per_query_sweep": round(per_qsweep),
+                     "traffic_per_tile_sweep": measured_traffic("rmat22-unit ppr tile64", "tile")
+                     if tile == 64 else None},
         "cpu_baseline": {"value": round(cpu_queries / cdt, 4), "unit": "queries/s", "cores": 1,
                          "kind": "port",
                          "sample": f"first {cpu_queries} queries by the oracle's "
@@ -587,7 +589,8 @@ def ppr_legs(args, ga, generators):
     dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
     log(f"ppr graph n={e.n} m={e.m} resident in {time.perf_counter() - t0:.1f} s")
     out = {"ppr_batch": ppr_batch_leg(ga, generators, e, dg, nq=args.ppr_queries)}
-    out["serving"] = serving_leg(ga, generators, e, dg, nq=args.serve_queries)
+    if not args.no_serving:
+        out["serving"] = serving_leg(ga, generators, e, dg, nq=args.serve_queries)
     dg.close()
     return out
 
@@ -645,6 +648,7 @@ def main():
     ap.add_argument("--no-ppr", action="store_true", help="skip the batched PPR and serving legs")
     ap.add_argument("--ppr-queries", type=int, default=1024)
     ap.add_argument("--serve-queries", type=int, default=1024)
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
     ap.add_argument("--ppr-only", action="store_true", help="only the PPR and serving legs")
     ap.add_argument("--no-build", action="store_true", help="skip the graph build timing leg")
     ap.add_argument("--build-only", action="store_true", help="only the graph build timing leg")
