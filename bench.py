@@ -661,6 +661,9 @@ def main():
                     help="skip the gather-only probe of the sweep's access pattern")
     ap.add_argument("--delta-sweep", default="auto", choices=["auto", "pull", "blocked"],
                     help="how f32 delta sweeps read the graph (GOPS_OPT_DELTA_SWEEP)")
+    ap.add_argument("--anchor-sweep", default="auto", choices=["auto", "pull"],
+                    help="how f64 anchor sweeps read the graph with blocked delta sweeps "
+                         "(GOPS_OPT_ANCHOR_SWEEP)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES")
     ap.add_argument("--no-c1c2", action="store_true", help="skip the C1 and C2 legs")
     args = ap.parse_args()
@@ -706,6 +709,8 @@ def main():
 
     ctx.set_option(L.OPT_DELTA_SWEEP, {"auto": L.DELTA_SWEEP_AUTO, "pull": L.DELTA_SWEEP_PULL,
                                        "blocked": L.DELTA_SWEEP_BLOCKED}[args.delta_sweep])
+    ctx.set_option(L.OPT_ANCHOR_SWEEP, {"auto": L.ANCHOR_SWEEP_AUTO,
+                                        "pull": L.ANCHOR_SWEEP_PULL}[args.anchor_sweep])
     t0 = time.perf_counter()
     dg = ga.DeviceGraph.from_edges(e, weight_storage=c["storage"], ctx=ctx)
     log(f"graph resident on the GPU in {time.perf_counter() - t0:.1f} s")
@@ -746,6 +751,13 @@ def main():
         n32 += st["sweeps_fp32"]
         n64 += st["sweeps_fp64"]
     qwall = time.perf_counter() - t0
+    anchors_blocked = st["sweeps_blocked"] == st["sweeps_fp32"] + st["sweeps_fp64"] > 0
+    pull_anchor_us = None
+    if anchors_blocked and st["sweeps_fp64"]:  # the same run with pull anchors, for comparison
+        ctx.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_PULL)
+        rq = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
+        ctx.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_AUTO)
+        pull_anchor_us = rq.stats["fp64_sweep_ms"] / max(rq.stats["sweeps_fp64"], 1) * 1e3
     log(f"timed {args.steps} runs ({launch}): {wall:.3f} s, {sweeps} sweeps; graph replay "
         f"{graph_wall * 1e3:.2f} ms per run; every sweep timed {qwall / 2 * 1e3:.2f} ms per run")
 
@@ -782,8 +794,11 @@ def main():
                    "converged": r.converged, "diff_l1": r.diff_l1,
                    "precision": f"{'bf16' if c['w_bytes'] == 2 else 'f32'} weights, f32 delta "
                                 "sweeps with f64 (pull) or fixed-point int64 (blocked) row sums, "
-                                "f64 anchor sweeps, f64 iterate, residual and dangling sums",
+                                "f64 anchor sweeps (blocked: f64 run sums added as 128-bit "
+                                "fixed point), f64 iterate, residual and dangling sums",
                    "delta_sweep": "blocked" if blocked else "pull",
+                   "anchor_sweep": "blocked" if anchors_blocked else "pull",
+                   "f64_pull_anchor_us": round(pull_anchor_us, 2) if pull_anchor_us else None,
                    "sweep_us": round(avg_sweep * 1e6, 2),
                    "f32_sweep_us": round(avg_f32 * 1e6, 2), "f64_sweep_us": round(avg_f64 * 1e6, 2),
                    "gteps_f32_sweep": round(m / avg_f32 / 1e9, 2) if n32 else None,

@@ -33,6 +33,9 @@ OPT_GRAPH_BUILD = 10
 OPT_TILE_F32_BELOW = 11
 OPT_DELTA_SWEEP = 12
 OPT_LAUNCH = 13
+OPT_ANCHOR_SWEEP = 14
+ANCHOR_SWEEP_AUTO = 0
+ANCHOR_SWEEP_PULL = 1
 LAUNCH_AUTO, LAUNCH_QUEUED, LAUNCH_GRAPH = 0, 1, 2
 DELTA_SWEEP_AUTO, DELTA_SWEEP_PULL, DELTA_SWEEP_BLOCKED = 0, 1, 2
 BUILD_DEVICE, BUILD_HOST = 0, 1

@@ -254,7 +254,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   gops_status st = alloc_run(ctx, g);
   if (st != GOPS_OK) return st;
   RunBuffers &rb = g->run;
-  SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0, 0};
+  SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0, 0, 0};
   // f32 delta sweeps in the blocked layout (built once per graph): when chosen,
   // or by default from 2^24 nodes, where the pull's source vector (64 MB of
   // f32) is far beyond the L2 and the blocked passes measured faster.  By
@@ -273,6 +273,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
       return fail(ctx, st, err);
     } else {
       setup.blocked = 1;
+      setup.anchors_blocked = ctx->anchor_sweep != GOPS_ANCHOR_SWEEP_PULL;
     }
   }
 
@@ -449,7 +450,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
     const bool f64 = modes[k] == kModeAnchor;
     if (f64) ctx->stats.sweeps_fp64++;
     else ctx->stats.sweeps_fp32++;
-    if (!f64 && setup.blocked) ctx->stats.sweeps_blocked++;
+    if (setup.blocked && (!f64 || setup.anchors_blocked)) ctx->stats.sweeps_blocked++;
     if (k < timed_cap) {
       float t = 0.0f;
       HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->events[2 + 2 * k], ctx->events[3 + 2 * k]));
@@ -576,6 +577,12 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
         return fail(ctx, GOPS_INVALID_PARAMETER,
                     "launch must be GOPS_LAUNCH_AUTO, _QUEUED or _GRAPH");
       ctx->launch = (int)value;
+      return GOPS_OK;
+    case GOPS_OPT_ANCHOR_SWEEP:
+      if (value != GOPS_ANCHOR_SWEEP_AUTO && value != GOPS_ANCHOR_SWEEP_PULL)
+        return fail(ctx, GOPS_INVALID_PARAMETER,
+                    "anchor sweep must be GOPS_ANCHOR_SWEEP_AUTO or _PULL");
+      ctx->anchor_sweep = (int)value;
       return GOPS_OK;
     case GOPS_OPT_DELTA_SWEEP:
       if (value != GOPS_DELTA_SWEEP_AUTO && value != GOPS_DELTA_SWEEP_PULL &&

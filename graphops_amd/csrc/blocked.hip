@@ -489,7 +489,8 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       int32_t *seg_run = pool.get<int32_t>(nseg);
       uint32_t *d_bin_off = pool.get<uint32_t>(bins + 1);
       pb.rrow = pool.get<uint16_t>(nr);
-      pb.rsum = pool.get<float>(nr);
+      pb.rsum64 = pool.get<double>(nr);
+      pb.rsum = reinterpret_cast<float *>(pb.rsum64);
       pb.seg_off = pool.get<int32_t>(nseg);
       BTRY(pool.err);
       pb_run_keys_kernel<<<grid_for(m), kB, 0, s>>>(keys_b, posof, pb.src, runid, flag, sg,
@@ -520,14 +521,17 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
     pb.num_splits = (int)splits.size();
     pb.items2 = pool.get<PbItem>(items2.size());
     pb.splits = pool.get<PbItem>(splits.size());
-    pb.bin_partial = pool.get<long long>((size_t)std::max(slots, 1) << kPbBinShift);
+    pb.bin_partial = pool.get<long long>((size_t)std::max(slots, 1) << (kPbBinShift + 1));
     pb.bin_flags = pool.get<uint8_t>((size_t)std::max(slots, 1) << kPbBinShift);
     pb.partials = pool.get<double>((items2.size() + splits.size()) * kMaxReduce);
     if (!pb.src) pb.src = pool.get<uint16_t>(1);
     if (!pb.mask) pb.mask = pool.get<uint8_t>(8);
     if (!pb.win) pb.win = pool.get<int32_t>(2);
     if (!pb.rrow) pb.rrow = pool.get<uint16_t>(1);
-    if (!pb.rsum) pb.rsum = pool.get<float>(1);
+    if (!pb.rsum) {
+      pb.rsum64 = pool.get<double>(1);
+      pb.rsum = reinterpret_cast<float *>(pb.rsum64);
+    }
     if (!pb.seg_off) pb.seg_off = pool.get<int32_t>(1);
     BTRY(pool.err);
     if (!items1.empty())

@@ -206,11 +206,13 @@ struct BlockedLayout {
                                // starts, and that entry's segment
   int32_t *seg_off = nullptr;  // per segment: bin-major place - chunk-major run number
   uint16_t *rrow = nullptr;    // bin-major runs: row id - bin base
-  float *rsum = nullptr;       // bin-major runs: one sweep's run sums
+  float *rsum = nullptr;       // bin-major runs: one sweep's run sums (8 bytes a run)
+  double *rsum64 = nullptr;    // the same memory: an anchor sweep's run sums
   PbItem *items1 = nullptr;    // scatter: {chunk, first window, end window, 0}
   PbItem *items2 = nullptr;    // accumulate: {bin, first run, end run, slot or -1}
   PbItem *splits = nullptr;    // bins of several items: {bin, first slot, slots, first row}
-  long long *bin_partial = nullptr;  // [slot][1 << kPbBinShift] fixed-point row sums
+  long long *bin_partial = nullptr;  // [slot][1 << kPbBinShift] fixed-point row sums, or
+                                     // [slot][1 << kPbBinShift][2] (128-bit, anchors)
   uint8_t *bin_flags = nullptr;      // [slot][1 << kPbBinShift] non-finite run sums seen
   double *partials = nullptr;  // [items2 + splits][kMaxReduce] sums of the epilogues
   int num_items1 = 0, num_items2 = 0, num_splits = 0;
@@ -280,6 +282,7 @@ struct gops_context {
   int ppr_tile = 0;
   int graph_build = GOPS_BUILD_DEVICE;
   int delta_sweep = GOPS_DELTA_SWEEP_AUTO;
+  int anchor_sweep = GOPS_ANCHOR_SWEEP_AUTO;
   int launch = GOPS_LAUNCH_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
@@ -371,6 +374,7 @@ struct SweepSetup {
   int weighted;           // use edge weights and inv_ws
   int personalized;       // q vector instead of uniform
   int blocked;            // f32 delta sweeps in the blocked layout (g->pb)
+  int anchors_blocked;    // and the f64 anchor sweeps
 };
 hipError_t launch_init(gops_context *ctx, gops_graph *g, const SweepSetup &s);
 hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s);

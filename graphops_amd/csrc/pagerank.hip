@@ -49,6 +49,7 @@ struct SweepArgs {
   int n_hot;  // min(n, kStageNnz): sources whose f32 deltas a workgroup keeps in LDS
   RunState *st;
   int pb_active;     // f32 delta sweeps run in the blocked layout (pb)
+  int pb_anchor;     // so do the f64 anchor sweeps
   int n;             // rows
   BlockedLayout pb;
 };
@@ -269,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, 
                                                        double *y640, double *y641) {
   const RunState *st = a.st;
   if (st->done) return;
-  if (a.pb_active && st->mode != kModeAnchor) return;  // the blocked kernels do this sweep
+  if (a.pb_active && (st->mode != kModeAnchor || a.pb_anchor)) return;  // the blocked kernels do it
   pick_buffers(a, st, yd0, yd1, y640, y641);
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
   __shared__ double stage[kStageNnz];  // a run's products (f32 or f64)
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float
       epi(a.hub_row[h], s, red);
     }
   };
-  if (a.pb_active && mode != kModeAnchor) {
+  if (a.pb_active && (mode != kModeAnchor || a.pb_anchor)) {
     // a blocked sweep: hub rows were summed in their bins
     const int parts = a.pb.num_items2 + a.pb.num_splits;
     add_partials(a.pb.partials, parts, red);
@@ -358,7 +359,8 @@ __global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float
 // Blocked f32 delta sweeps (layout: blocked.hip).  sweep_kernel returns at
 // once for them; the scatter and accumulate kernels (and the combine kernel
 // for bins split over several workgroups) do the sweep, and the finish kernel
-// reduces their partial sums.  Anchor sweeps stay pulls.
+// reduces their partial sums.  Anchor sweeps are pulls, or blocked too (the
+// f64 kernels below).
 //
 // Row sums are fixed-point int64, so that the order in which workgroups and
 // lanes add does not change a bit of the result.  The products of a delta
@@ -734,6 +736,326 @@ __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *
 }
 
 // ---------------------------------------------------------------------------
+// Blocked f64 anchor sweeps: the same layout, the iterate's prescaled f64
+// values y64 = d * x * inv in place of the f32 deltas.  The scatter holds a
+// chunk's 16 384 values in 128 KB of LDS (one workgroup per CU) and sums each
+// run in f64, in the same lane order as the f32 scatter; the accumulate adds
+// the runs of a row as 128-bit fixed-point integers with 2^-100 units, so the
+// row sum is one rounding of the exact sum of the run sums, whatever the
+// order of the atomics.  An anchor's products are w * d * x_u / wsum_u <= d *
+// x_u, so every row sum is at most d * sum(x) ~ 1, far below the 2^26 that
+// the format holds; a run sum beyond it (a layout or input fault) makes the
+// row NaN rather than wrap.
+constexpr int kAnchorFix = 100;  // fixed-point units of the anchors' row sums: 2^-100
+
+// An f64 DPP move: both halves with the same control (0 where it selects no lane).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double pb_dpp64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+// pb_seg_scan in f64 (the same steps and order).
+__device__ __forceinline__ double pb_seg_scan64(double v, int start) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  double o;
+  o = pb_dpp64<0x111, 0xF>(v); if (r >= 1 && lane - 1 >= start) v += o;
+  o = pb_dpp64<0x112, 0xF>(v); if (r >= 2 && lane - 2 >= start) v += o;
+  o = pb_dpp64<0x114, 0xF>(v); if (r >= 4 && lane - 4 >= start) v += o;
+  o = pb_dpp64<0x118, 0xF>(v); if (r >= 8 && lane - 8 >= start) v += o;
+  o = pb_dpp64<0x142, 0xA>(v); if (((lane >> 4) & 1) && (lane | 15) - 16 >= start) v += o;
+  o = pb_dpp64<0x143, 0xC>(v); if (lane >= 32 && 31 >= start) v += o;
+  return v;
+}
+
+// A lane's kPbLane consecutive f64 weights (for the anchors' products; the
+// other storage types are exact in f32 and take pb_load_w8).
+__device__ __forceinline__ void pb_load_w8(const void *w, size_t e, double (&out)[kPbLane]) {
+  const pb_u32x4 *p = reinterpret_cast<const pb_u32x4 *>(static_cast<const double *>(w) + e);
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const pb_u32x4 q = __builtin_nontemporal_load(p + h);
+    out[2 * h] = __hiloint2double((int)q[1], (int)q[0]);
+    out[2 * h + 1] = __hiloint2double((int)q[3], (int)q[2]);
+  }
+}
+
+// Scatter of an anchor sweep: pb_scatter_kernel's windows and run logic on
+// f64 products (f64 weights as stored, the others converted exactly) with
+// f64 run sums.
+template <typename WT>
+__global__ __launch_bounds__(kScatterBlock) __attribute__((amdgpu_waves_per_eu(kScatterBlock / 256, kScatterBlock / 256)))
+void pb_scatter64_kernel(SweepArgs a, double *y640, double *y641) {
+  const RunState *st = a.st;
+  if (st->done || st->mode != kModeAnchor) return;
+  const double *yin = st->y64_cur ? y641 : y640;
+  constexpr int kChunk = 1 << kPbChunkShift;
+  __shared__ double z[kChunk];
+  const PbItem it = a.pb.items1[blockIdx.x];  // {chunk, first window, end window}
+  const int base = it.x << kPbChunkShift;
+  const int len = min(kChunk, a.n - base);
+  if (len == kChunk && (base & 1) == 0) {
+    constexpr int V = kChunk / 2 / kScatterBlock;
+    double2 v[V];
+#pragma unroll
+    for (int j = 0; j < V; j++)
+      v[j] = *reinterpret_cast<const double2 *>(yin + base + 2 * ((int)threadIdx.x + j * kScatterBlock));
+#pragma unroll
+    for (int j = 0; j < V; j++)
+      *reinterpret_cast<double2 *>(z + 2 * ((int)threadIdx.x + j * kScatterBlock)) = v[j];
+  } else {
+    for (int i = threadIdx.x; i < len; i += kScatterBlock) z[i] = yin[base + i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint32_t nruns = (uint32_t)a.pb.num_runs, nseg = (uint32_t)a.pb.num_segs;
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  constexpr int U = 4;  // windows' loads in flight per wave
+  constexpr int kWaves = kScatterBlock / 64;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (int w0 = it.y + wave; w0 < it.z; w0 += U * kWaves) {
+    using WV = typename std::conditional<std::is_same<WT, double>::value, double, float>::type;
+    pb_u32x4 sq[U];
+    WV wv[U][kPbLane];
+    int run0[U], seg0[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int wi = min(w0 + u * kWaves, it.z - 1);  // clamped: every load in bounds
+      const size_t e = (size_t)wi * kPbWin + (size_t)lane * kPbLane;
+      sq[u] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(a.pb.src + e));
+      if constexpr (std::is_same<WT, double>::value)
+        pb_load_w8(a.pb.w, e, wv[u]);
+      else
+        pb_load_w8<WT>(std::is_same<WT, NoW>::value ? (const void *)a.pb.mask : a.pb.w, e, wv[u]);
+      run0[u] = a.pb.win[2 * wi];
+      seg0[u] = a.pb.win[2 * wi + 1];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (w0 + u * kWaves >= it.z) break;  // the same in every lane
+      uint32_t sw[kPbLane];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        sw[2 * i] = sq[u][i] & 0xffffu;
+        sw[2 * i + 1] = sq[u][i] >> 16;
+      }
+      double q[kPbLane];
+      uint32_t hm = 0;
+#pragma unroll
+      for (int j = 0; j < kPbLane; j++) {
+        const double pj = (double)wv[u][j] * z[sw[j] & kPbSrcMask];
+        const bool h = (sw[j] & kPbRunHead) != 0;
+        hm |= (h ? 1u : 0u) << j;
+        q[j] = (j == 0 || h) ? pj : q[j - 1] + pj;
+      }
+      const unsigned long long lheads = __ballot(hm != 0);
+      const int start = 63 - __clzll(lheads & upto);
+      const double S = pb_seg_scan64(q[kPbLane - 1], start);
+      const double X = __shfl_up(S, 1);
+      const uint32_t sh = (sw[0] & kPbSegHead) ? 1u : 0u;
+      const uint32_t cnt = (uint32_t)__popc(hm) | (sh << 16);
+      const uint32_t before = pb_int_scan(cnt) - cnt;
+      const uint32_t sh0 = __builtin_amdgcn_readfirstlane(sh);
+      const uint32_t run = (uint32_t)run0[u] + (before & 0xffffu) + (hm & 1u) - 1u;
+      const uint32_t seg = (uint32_t)seg0[u] + (before >> 16) + sh - sh0;
+      const int32_t off = a.pb.seg_off[seg < nseg ? seg : 0];
+      const unsigned long long first_heads = __ballot((hm & 1u) != 0);
+      const bool next_head = lane == 63 || ((first_heads >> (lane + 1)) & 1ull);
+      uint32_t place = run + (uint32_t)off;
+#pragma unroll
+      for (int j = 0; j < kPbLane; j++) {
+        const bool tail = j + 1 < kPbLane ? ((hm >> (j + 1)) & 1u) != 0 : next_head;
+        const double t = (hm & ((2u << j) - 1u)) ? q[j] : q[j] + X;
+        if (tail) {
+          if (seg < nseg && place < nruns) a.pb.rsum64[place] = t;
+          place++;
+        }
+      }
+    }
+  }
+}
+
+// c * 2^kAnchorFix as a signed 128-bit integer (lo, hi), truncated toward
+// zero.  Returns false for |c| >= 2^26 (beyond the anchors' bound).  Finite c.
+__device__ __forceinline__ bool pb_fixed128(double c, unsigned long long &lo, unsigned long long &hi) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(c);
+  const int ex = (int)((b >> 52) & 0x7ffu);
+  lo = 0ull;
+  hi = 0ull;
+  if (ex == 0) return true;  // zero or subnormal: below one unit
+  const unsigned long long mant = (b & 0xfffffffffffffull) | (1ull << 52);
+  const int sh = ex - 1075 + kAnchorFix;  // c = mant * 2^(ex - 1075)
+  if (sh > 126 - 53) return false;
+  if (sh >= 64) {
+    hi = mant << (sh - 64);
+  } else if (sh > 0) {
+    lo = mant << sh;
+    hi = mant >> (64 - sh);
+  } else if (sh > -53) {
+    lo = mant >> -sh;
+  }
+  if (b >> 63) {  // two's complement
+    lo = ~lo + 1ull;
+    hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+  }
+  return true;
+}
+
+// The 128-bit sum (lo, hi) in units of 2^-kAnchorFix as a double.
+__device__ __forceinline__ double pb_value128(unsigned long long lo, unsigned long long hi) {
+  const bool neg = (long long)hi < 0;
+  if (neg) {
+    lo = ~lo + 1ull;
+    hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+  }
+  const double v = ldexp((double)hi, 64 - kAnchorFix) + ldexp((double)lo, -kAnchorFix);
+  return neg ? -v : v;
+}
+
+__device__ __forceinline__ double pb_row_value128(unsigned long long lo, unsigned long long hi,
+                                                  uint32_t flags) {
+  if (flags) {
+    if ((flags & 1u) || (flags & 6u) == 6u) return __builtin_nan("");
+    return (flags & 2u) ? __builtin_inf() : -__builtin_inf();
+  }
+  return pb_value128(lo, hi);
+}
+
+// Accumulate of an anchor sweep: one bin of rows (or a part of its runs),
+// 128-bit fixed-point sums in LDS (an atomic on the low word, its carry into
+// the high word), then the rows' anchor epilogues or the part's sums.
+__global__ __launch_bounds__(kBlock) void pb_accumulate64_kernel(SweepArgs a, float *yd0, float *yd1,
+                                                                 double *y640, double *y641) {
+  const RunState *st = a.st;
+  if (st->done || st->mode != kModeAnchor) return;
+  pick_buffers(a, st, yd0, yd1, y640, y641);
+  constexpr int kRows = 1 << kPbBinShift;
+  __shared__ unsigned long long acc_lo[kRows], acc_hi[kRows];
+  __shared__ uint32_t nonfin[kRows / 4];
+  const PbItem it = a.pb.items2[blockIdx.x];
+  for (int i = threadIdx.x; i < kRows; i += kBlock) {
+    acc_lo[i] = 0ull;
+    acc_hi[i] = 0ull;
+  }
+  for (int i = threadIdx.x; i < kRows / 4; i += kBlock) nonfin[i] = 0u;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int U = 8;
+  if (it.y < it.z) {
+    for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kBlock) {
+      double cv[U];
+      int key[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int f = f0 + u * kBlock + wave * 64 + lane;
+        const bool ok = f >= it.y && f < it.z;
+        const int ff = ok ? f : it.y;
+        cv[u] = __builtin_nontemporal_load(a.pb.rsum64 + ff);
+        const int d = __builtin_nontemporal_load(a.pb.rrow + ff);
+        key[u] = ok ? d : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (key[u] < 0) continue;
+        const double c = cv[u];
+        unsigned long long lo, hi;
+        uint32_t bad = 0u;
+        if (!isfinite(c)) bad = isnan(c) ? 1u : (c > 0.0 ? 2u : 4u);
+        else if (!pb_fixed128(c, lo, hi)) bad = 1u;
+        if (bad) {
+          atomicOr(&nonfin[key[u] >> 2], bad << (8 * (key[u] & 3)));
+        } else {
+          const unsigned long long old = atomicAdd(&acc_lo[key[u]], lo);
+          atomicAdd(&acc_hi[key[u]], hi + (old + lo < old ? 1ull : 0ull));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int row0 = it.x << kPbBinShift;
+  const int rows = min(kRows, a.n - row0);
+  if (it.w < 0) {
+    const AnchorEpilogue epi(a, st);
+    for (int i = threadIdx.x; i < rows; i += kBlock) {
+      const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
+      epi(row0 + i, pb_row_value128(acc_lo[i], acc_hi[i], fl), red);
+    }
+  } else {
+    long long *out = a.pb.bin_partial + ((size_t)it.w << (kPbBinShift + 1));
+    uint8_t *fout = a.pb.bin_flags + ((size_t)it.w << kPbBinShift);
+    for (int i = threadIdx.x; i < rows; i += kBlock) {
+      out[2 * i] = (long long)acc_lo[i];
+      out[2 * i + 1] = (long long)acc_hi[i];
+      fout[i] = (uint8_t)((nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu);
+    }
+  }
+  __syncthreads();
+  block_sum<kReduce>(red, reinterpret_cast<double *>(acc_lo));
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < kReduce; i++) a.pb.partials[(size_t)blockIdx.x * kReduce + i] = red[i];
+  }
+}
+
+// Combine of an anchor sweep: the parts' 128-bit sums in slot order, then the
+// anchor epilogues.
+__global__ __launch_bounds__(kBlock) void pb_combine64_kernel(SweepArgs a, float *yd0, float *yd1,
+                                                              double *y640, double *y641) {
+  const RunState *st = a.st;
+  if (st->done || st->mode != kModeAnchor) return;
+  pick_buffers(a, st, yd0, yd1, y640, y641);
+  const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
+  const int row0 = (sp.x << kPbBinShift) + sp.w;
+  const int i = sp.w + (int)threadIdx.x;
+  double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < (1 << kPbBinShift) && row0 + (int)threadIdx.x < a.n) {
+    const long long *bp = a.pb.bin_partial + ((size_t)sp.y << (kPbBinShift + 1)) + 2 * (size_t)i;
+    const uint8_t *fp = a.pb.bin_flags + ((size_t)sp.y << kPbBinShift) + i;
+    unsigned long long lo = 0ull, hi = 0ull;
+    uint32_t fl = 0;
+    int j = 0;
+    for (; j + 4 <= sp.z; j += 4) {
+      unsigned long long vl[4], vh[4];
+      uint32_t f[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const size_t at = (size_t)(j + u) << (kPbBinShift + 1);
+        vl[u] = (unsigned long long)bp[at];
+        vh[u] = (unsigned long long)bp[at + 1];
+        f[u] = fp[(size_t)(j + u) << kPbBinShift];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const unsigned long long old = lo;
+        lo += vl[u];
+        hi += vh[u] + (lo < old ? 1ull : 0ull);
+        fl |= f[u];
+      }
+    }
+    for (; j < sp.z; j++) {
+      const size_t at = (size_t)j << (kPbBinShift + 1);
+      const unsigned long long old = lo;
+      lo += (unsigned long long)bp[at];
+      hi += (unsigned long long)bp[at + 1] + (lo < old ? 1ull : 0ull);
+      fl |= fp[(size_t)j << kPbBinShift];
+    }
+    const AnchorEpilogue epi(a, st);
+    epi(row0 + (int)threadIdx.x, pb_row_value128(lo, hi, fl), red);
+  }
+  block_sum<kReduce>(red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < kReduce; q++)
+      a.pb.partials[(size_t)(a.pb.num_items2 + blockIdx.x) * kReduce + q] = red[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // init: x0 = 1/n (pagerank.rs:133) or the normalised personalization
 // (ppr.rs:104-110); x = x0, y64[0] = d*x0*inv; partial dangling sums of x0.
 __global__ __launch_bounds__(kBlock) void init_kernel(SweepArgs a, int n) {
@@ -833,6 +1155,7 @@ static SweepArgs make_args(gops_graph *g, const SweepSetup &s) {
   a.n_hot = (int)std::min<uint64_t>(dg.n, kStageNnz);
   a.st = rb.state;
   a.pb_active = s.blocked && g->has_pb;
+  a.pb_anchor = a.pb_active && s.anchors_blocked;
   a.n = (int)dg.n;
   if (a.pb_active) a.pb = g->pb;
   return a;
@@ -861,7 +1184,8 @@ template <typename WT>
 static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
   RunBuffers &rb = g->run;
   const int grid = sweep_grid(g);
-  sweep_kernel<WT><<<grid, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1]);
+  if (!a.pb_anchor)  // pull anchors (and every sweep when not blocked)
+    sweep_kernel<WT><<<grid, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1]);
   if (a.pb_active) {
     if (a.pb.num_items1)
       pb_scatter_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1]);
@@ -870,6 +1194,16 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
     if (a.pb.num_splits)
       pb_combine_kernel<<<a.pb.num_splits, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
                                                                      rb.y64[0], rb.y64[1]);
+  }
+  if (a.pb_anchor) {
+    if (a.pb.num_items1)
+      pb_scatter64_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.y64[0],
+                                                                                 rb.y64[1]);
+    pb_accumulate64_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
+                                                                        rb.y64[0], rb.y64[1]);
+    if (a.pb.num_splits)
+      pb_combine64_kernel<<<a.pb.num_splits, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
+                                                                       rb.y64[0], rb.y64[1]);
   }
   sweep_finish_kernel<<<1, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1],
                                                      grid);
@@ -890,7 +1224,7 @@ hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s) {
 }
 
 hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize) {
-  SweepSetup s{0, 0, 0};
+  SweepSetup s{0, 0, 0, 0};
   SweepArgs a = make_args(g, s);
   const int n = (int)g->dev.n;
   const int grid = elementwise_grid(n);

@@ -121,9 +121,15 @@ enum {
                                    sources that sums each row's run of edges, and an
                                    accumulate pass per bin of rows, both streaming,
                                    integer row sums; its layout is built on the graph's
-                                   first such run) */
+                                   first such run) */,
+    GOPS_OPT_ANCHOR_SWEEP = 14  /* mixed, one vector per run, with blocked delta sweeps:
+                                   how the f64 anchor sweeps read the graph:
+                                   GOPS_ANCHOR_SWEEP_AUTO (default: blocked as well,
+                                   f64 run sums added as 128-bit fixed point) or _PULL
+                                   (gathers of the f64 iterate over the CSR) */
 };
 enum { GOPS_DELTA_SWEEP_AUTO = 0, GOPS_DELTA_SWEEP_PULL = 1, GOPS_DELTA_SWEEP_BLOCKED = 2 };
+enum { GOPS_ANCHOR_SWEEP_AUTO = 0, GOPS_ANCHOR_SWEEP_PULL = 1 };
 enum { GOPS_LAUNCH_AUTO = 0, GOPS_LAUNCH_QUEUED = 1, GOPS_LAUNCH_GRAPH = 2 };
 enum { GOPS_PRECISION_MIXED = 0, GOPS_PRECISION_FP64 = 1 };
 enum { GOPS_BUILD_DEVICE = 0, GOPS_BUILD_HOST = 1 };
@@ -138,7 +144,8 @@ typedef struct gops_run_stats {
     double fp32_sweep_ms;   /* summed duration of f32 sweep kernels (GOPS_OPT_TIMING) */
     double fp64_sweep_ms;   /* summed duration of f64 sweep kernels (GOPS_OPT_TIMING) */
     double other_ms;        /* reserved */
-    uint64_t sweeps_blocked; /* f32 delta sweeps that ran in the blocked layout */
+    uint64_t sweeps_blocked; /* sweeps that ran in the blocked layout (f32 deltas, and
+                                f64 anchors unless GOPS_ANCHOR_SWEEP_PULL) */
     int32_t blocked_fallback; /* 1: GOPS_DELTA_SWEEP_AUTO chose the blocked layout but it
                                  did not fit in free device memory, so the f32 delta
                                  sweeps ran as pulls (retried on later runs) */

@@ -8,6 +8,9 @@ accumulate pass per bin of rows, fixed-point row sums.
   * against the pull sweep on the same graph: the same iterations, L1 within
     2e-2 * tolerance + 1e-12 (both are that close to the oracle);
   * two runs give the same bits (integer sums: no dependence on scheduling);
+  * the f64 anchor sweeps run blocked too (128-bit fixed-point row sums):
+    against pull anchors on the same graph, the same iterations and L1
+    within f64 rounding;
   * layouts that exercise every part: bins split over several workgroups
     (a row of 140 000 in-edges), several chunks and bins with n not a multiple
     of either, empty bins and rows, dangling nodes, every weight storage,
@@ -197,3 +200,53 @@ def test_nonfinite_weight_carries_through(oracle, ctx_b, ctx_p):
         assert np.array_equal(np.isnan(r.scores), np.isnan(ref.scores))
     fin = ~np.isnan(ref.scores)
     assert np.abs(b.scores[fin] - pl.scores[fin]).sum() <= 1e-9
+
+
+@pytest.fixture(scope="module")
+def ctx_bp():
+    c = make_ctx(L.DELTA_SWEEP_BLOCKED)
+    c.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_PULL)
+    return c
+
+
+def test_blocked_anchors_match_pull_anchors(oracle, ctx_b, ctx_bp):
+    """Blocked f64 anchors (the default with blocked delta sweeps) against
+    pull anchors: every sweep of the first context runs blocked, the
+    anchors' row sums differ from the pull's only by f64 rounding, so the
+    iterations are the same, the residuals agree as closely as f64 rounding
+    of the iterate lets a residual near 1e-12 agree (1e-4 of it, or 1e-15)
+    and the scores to 1e-13 in L1; two runs give the same bits.  The graph has split bins and
+    ragged chunks (n = 70 001, a row of 140 000 in-edges)."""
+    n = 70001
+    rng = np.random.default_rng(12)
+    src = np.concatenate([np.arange(n, dtype=np.int64) % 65000, rng.integers(0, 65000, 140000)])
+    dst = np.concatenate([rng.integers(0, 60000, n), np.full(140000, 7)])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    w = bf16_exact(np.exp(rng.normal(0.0, 1.0, dst.size)))  # stored as bf16 without rounding
+    e = SimpleNamespace(n=n, m=dst.size, offsets=off, targets=dst.astype(np.uint32), weights=w)
+    tol = 1e-12
+    cfg = ga.PageRankConfig(0.85, 300, tol)
+    out = []
+    for cx in (ctx_b, ctx_bp, ctx_b):
+        dg = ga.DeviceGraph.from_edges(e, weight_storage="bf16", ctx=cx)
+        out.append((ga.pagerank_weighted_run(dg, cfg, ctx=cx), cx.last_stats()))
+        dg.close()
+    (b, sb), (pl, sp), (b2, _) = out
+    assert sb["sweeps_fp64"] > 0 and sb["sweeps_blocked"] == b.iterations
+    assert sp["sweeps_blocked"] == sp["sweeps_fp32"]
+    assert np.array_equal(b.scores, b2.scores) and b.diff_l1 == b2.diff_l1
+    assert b.iterations == pl.iterations and b.converged and pl.converged
+    assert abs(b.diff_l1 - pl.diff_l1) <= 1e-4 * pl.diff_l1 + 1e-15
+    assert np.abs(b.scores - pl.scores).sum() <= 1e-13
+    check(b, oracle_pull(oracle, e, 1, tol, 300), tol)
+
+
+def bf16_exact(w):
+    """w with its f32 mantissa cut to bf16's 8 bits: values that bf16 holds
+    exactly, whatever rounding the storage conversion applies."""
+    u = np.asarray(w, dtype=np.float32).view(np.uint32) & np.uint32(0xFFFF0000)
+    return u.view(np.float32).astype(np.float64)
