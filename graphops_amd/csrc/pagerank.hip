@@ -813,7 +813,11 @@ void pb_scatter64_kernel(SweepArgs a, double *y640, double *y641) {
   const int lane = threadIdx.x & 63;
   const uint32_t nruns = (uint32_t)a.pb.num_runs, nseg = (uint32_t)a.pb.num_segs;
   const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  constexpr int U = 4;  // windows' loads in flight per wave
+#ifndef GOPS_TUNE_S64U
+#define GOPS_TUNE_S64U 4
+#endif
+  // windows' loads in flight per wave (f64 weights take twice the registers)
+  constexpr int U = std::is_same<WT, double>::value ? 4 : GOPS_TUNE_S64U;
   constexpr int kWaves = kScatterBlock / 64;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   for (int w0 = it.y + wave; w0 < it.z; w0 += U * kWaves) {
@@ -928,7 +932,8 @@ __device__ __forceinline__ double pb_row_value128(unsigned long long lo, unsigne
 // Accumulate of an anchor sweep: one bin of rows (or a part of its runs),
 // 128-bit fixed-point sums in LDS (an atomic on the low word, its carry into
 // the high word), then the rows' anchor epilogues or the part's sums.
-__global__ __launch_bounds__(kBlock) void pb_accumulate64_kernel(SweepArgs a, float *yd0, float *yd1,
+constexpr int kAcc64Block = 1024;  // 68 KB of LDS: two workgroups, 32 waves per CU
+__global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs a, float *yd0, float *yd1,
                                                                  double *y640, double *y641) {
   const RunState *st = a.st;
   if (st->done || st->mode != kModeAnchor) return;
@@ -937,21 +942,21 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate64_kernel(SweepArgs a, fl
   __shared__ unsigned long long acc_lo[kRows], acc_hi[kRows];
   __shared__ uint32_t nonfin[kRows / 4];
   const PbItem it = a.pb.items2[blockIdx.x];
-  for (int i = threadIdx.x; i < kRows; i += kBlock) {
+  for (int i = threadIdx.x; i < kRows; i += kAcc64Block) {
     acc_lo[i] = 0ull;
     acc_hi[i] = 0ull;
   }
-  for (int i = threadIdx.x; i < kRows / 4; i += kBlock) nonfin[i] = 0u;
+  for (int i = threadIdx.x; i < kRows / 4; i += kAcc64Block) nonfin[i] = 0u;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int U = 8;
   if (it.y < it.z) {
-    for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kBlock) {
+    for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kAcc64Block) {
       double cv[U];
       int key[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int f = f0 + u * kBlock + wave * 64 + lane;
+        const int f = f0 + u * kAcc64Block + wave * 64 + lane;
         const bool ok = f >= it.y && f < it.z;
         const int ff = ok ? f : it.y;
         cv[u] = __builtin_nontemporal_load(a.pb.rsum64 + ff);
@@ -981,21 +986,21 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate64_kernel(SweepArgs a, fl
   const int rows = min(kRows, a.n - row0);
   if (it.w < 0) {
     const AnchorEpilogue epi(a, st);
-    for (int i = threadIdx.x; i < rows; i += kBlock) {
+    for (int i = threadIdx.x; i < rows; i += kAcc64Block) {
       const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
       epi(row0 + i, pb_row_value128(acc_lo[i], acc_hi[i], fl), red);
     }
   } else {
     long long *out = a.pb.bin_partial + ((size_t)it.w << (kPbBinShift + 1));
     uint8_t *fout = a.pb.bin_flags + ((size_t)it.w << kPbBinShift);
-    for (int i = threadIdx.x; i < rows; i += kBlock) {
+    for (int i = threadIdx.x; i < rows; i += kAcc64Block) {
       out[2 * i] = (long long)acc_lo[i];
       out[2 * i + 1] = (long long)acc_hi[i];
       fout[i] = (uint8_t)((nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu);
     }
   }
   __syncthreads();
-  block_sum<kReduce>(red, reinterpret_cast<double *>(acc_lo));
+  block_sum_waves<kReduce, kAcc64Block / 64>(red, reinterpret_cast<double *>(acc_lo));
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < kReduce; i++) a.pb.partials[(size_t)blockIdx.x * kReduce + i] = red[i];
@@ -1199,8 +1204,8 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
     if (a.pb.num_items1)
       pb_scatter64_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.y64[0],
                                                                                  rb.y64[1]);
-    pb_accumulate64_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
-                                                                        rb.y64[0], rb.y64[1]);
+    pb_accumulate64_kernel<<<a.pb.num_items2, kAcc64Block, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
+                                                                             rb.y64[0], rb.y64[1]);
     if (a.pb.num_splits)
       pb_combine64_kernel<<<a.pb.num_splits, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
                                                                        rb.y64[0], rb.y64[1]);

@@ -93,6 +93,27 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double *scratch) {
   __syncthreads();
 }
 
+// block_sum for a workgroup of W waves (fixed order: wave 0, 1, ...).
+template <int N, int W>
+__device__ __forceinline__ void block_sum_waves(double (&v)[N], double *scratch) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    const double s = group_sum(v[i], 6);
+    if (lane == 0) scratch[i * W + wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      double t = scratch[i * W];
+      for (int w = 1; w < W; w++) t += scratch[i * W + w];
+      v[i] = t;
+    }
+  }
+  __syncthreads();
+}
+
 template <int N>
 __device__ __forceinline__ void block_sum(double (&v)[N]) {
   __shared__ double sh[N * (kBlock / 64)];
