@@ -232,6 +232,21 @@ def c2_leg(ga, generators, reps=10, push_sweeps=3, find_edge_frac=0.02):
         n32 += r.stats["sweeps_fp32"]
     wall = time.perf_counter() - t0
     scores = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
+    # the row plan's autotuner (gops_graph_tune_rows), then the same runs
+    tune = ga.graph_tune_rows(dg)
+    t_sweeps = 0
+    t_f32 = 0.0
+    t_n32 = 0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rt = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
+        t_sweeps += rt.iterations
+        t_f32 += rt.stats["fp32_sweep_ms"]
+        t_n32 += rt.stats["sweeps_fp32"]
+    t_wall = time.perf_counter() - t0
+    tune.update({"gteps": round(m * t_sweeps / t_wall / 1e9, 2),
+                 "f32_sweep_us_in_runs": round(t_f32 / max(t_n32, 1) * 1e3, 2)})
+    log(f"c2 row plan: {tune}")
     dg.close()
     ctx.close()
     alg = (4 + c["w_bytes"]) * m + 16 * n + 4
@@ -265,6 +280,7 @@ def c2_leg(ga, generators, reps=10, push_sweeps=3, find_edge_frac=0.02):
             "gteps": round(m * sweeps / wall / 1e9, 2), "ms_per_run": round(wall / reps * 1e3, 3),
             "delta_sweep": "blocked" if r.stats["sweeps_blocked"] else "pull",
             "f32_sweep_us": round(avg_f32 * 1e6, 2),
+            "row_plan_autotune": tune,
             "roofline": {"bound": "hbm", "alg_bytes_per_launch": alg,
                          "achieved": round(alg / avg_f32 / 1e9, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(alg / avg_f32 / 1e9 / HBM_PEAK_GBS, 4)},

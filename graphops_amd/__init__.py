@@ -33,7 +33,8 @@ __all__ = [
     "pagerank_weighted", "pagerank_weighted_run", "pagerank_weighted_checked",
     "pagerank_weighted_checked_run", "personalized_pagerank", "personalized_pagerank_run",
     "personalized_pagerank_checked", "personalized_pagerank_checked_run", "ppr_batch_run",
-    "top_k", "result_top_k", "result_top_k_noalloc", "normalize", "Server", "KatzConfig", "KatzRun", "katz_centrality",
+    "top_k", "result_top_k", "result_top_k_noalloc", "normalize", "Server", "graph_tune_rows", "graph_set_rows",
+    "ppr_tile_info", "KatzConfig", "KatzRun", "katz_centrality",
     "katz_centrality_run", "katz_centrality_checked", "EigenvectorRun",
     "eigenvector_centrality", "eigenvector_centrality_run", "hits", "hits_run",
 ]
@@ -540,6 +541,27 @@ def ppr_tile_info(graph, num_queries, ctx: Context = None):
     _check(L.lib().gops_ppr_tile_info(dg.ctx.handle, dg.handle, int(num_queries), ctypes.byref(t),
                                       L.ptr(cost, L._f64p)), dg.ctx.handle, "gops_ppr_tile_info")
     return int(t.value), {w: float(c) for w, c in zip((8, 16, 32, 64), cost)}
+
+
+def graph_tune_rows(graph, ctx: Context = None) -> dict:
+    """Autotune the pull sweeps' row plan of a device graph
+    (gops_graph_tune_rows): the chosen cut-offs (row_min, hub_chunk,
+    row_iters), the number of plans timed, and the f32 pull sweep's time
+    with the default and the chosen cut-offs (us)."""
+    dg = _device_graph(graph, False, ctx)
+    r = L.RowPlanC()
+    _check(L.lib().gops_graph_tune_rows(dg.ctx.handle, dg.handle, ctypes.byref(r)), dg.ctx.handle,
+           "gops_graph_tune_rows")
+    return {"row_min": r.row_min, "hub_chunk": r.hub_chunk, "row_iters": r.row_iters,
+            "candidates": r.candidates, "default_us": r.default_us, "tuned_us": r.tuned_us}
+
+
+def graph_set_rows(graph, row_min=4096, hub_chunk=65536, row_iters=8, ctx: Context = None):
+    """Set the pull sweeps' row plan of a device graph (gops_graph_set_rows)."""
+    dg = _device_graph(graph, False, ctx)
+    r = L.RowPlanC(int(row_min), int(hub_chunk), int(row_iters), 0, 0.0, 0.0)
+    _check(L.lib().gops_graph_set_rows(dg.ctx.handle, dg.handle, ctypes.byref(r)), dg.ctx.handle,
+           "gops_graph_set_rows")
 
 
 # ---------------------------------------------------------------- ranking

@@ -68,6 +68,12 @@ class RunStatsC(ctypes.Structure):
                 ("graph_replay", ctypes.c_int32)]
 
 
+class RowPlanC(ctypes.Structure):
+    _fields_ = [("row_min", ctypes.c_int32), ("hub_chunk", ctypes.c_int32),
+                ("row_iters", ctypes.c_int32), ("candidates", ctypes.c_int32),
+                ("default_us", ctypes.c_double), ("tuned_us", ctypes.c_double)]
+
+
 class ServerConfigC(ctypes.Structure):
     _fields_ = [("max_slots", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_double),
                 ("tile", ctypes.c_uint32), ("topk", ctypes.c_uint32)]
@@ -123,6 +129,8 @@ SIGNATURES = [
      [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(PageRankConfigC), _u64p, _u32p, _f64p, _f64p,
       ctypes.POINTER(RunInfoC)]),
     ("gops_ppr_tile_info", ctypes.c_int, [_vp, _vp, ctypes.c_uint32, _u32p, _f64p]),
+    ("gops_graph_tune_rows", ctypes.c_int, [_vp, _vp, ctypes.POINTER(RowPlanC)]),
+    ("gops_graph_set_rows", ctypes.c_int, [_vp, _vp, ctypes.POINTER(RowPlanC)]),
     ("gops_server_create", ctypes.c_int,
      [_vp, _vp, ctypes.POINTER(ServerConfigC), ctypes.POINTER(_vp)]),
     ("gops_server_destroy", None, [_vp]),

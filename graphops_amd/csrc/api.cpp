@@ -596,6 +596,148 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
   }
 }
 
+// Replace the graph's row plan (the pull sweeps' work descriptors) by p.
+static gops_status replace_plan(gops_context *ctx, gops_graph *g, const Plan &p) {
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  DevGraph &dg = g->dev;
+  for (void *q : {(void *)dg.desc, (void *)dg.hub_row, (void *)dg.hub_nchunks, (void *)dg.hub_first})
+    (void)hipFree(q);
+  dg.desc = nullptr;
+  dg.hub_row = dg.hub_nchunks = dg.hub_first = nullptr;
+  HIP_TRY(ctx, upload_plan_to(ctx->stream, dg, p));
+  RunBuffers &rb = g->run;
+  if (rb.state) {  // the run buffers' hub partials follow the plan's chunk count
+    (void)hipFree(rb.hub_partial);
+    rb.hub_partial = nullptr;
+    HIP_TRY(ctx, dalloc(&rb.hub_partial, std::max(dg.num_hub_chunks, 1)));
+  }
+  return GOPS_OK;
+}
+
+static gops_status download_rows(gops_context *ctx, gops_graph *g, HostCSRT &h) {
+  h.n = g->dev.n;
+  h.m = g->dev.m;
+  h.row_ptr.resize(h.n + 1);
+  HIP_TRY(ctx, staged_download(ctx, h.row_ptr.data(), g->dev.row_ptr,
+                               (h.n + 1) * sizeof(int32_t), ctx->stream));
+  return GOPS_OK;
+}
+
+gops_status gops_graph_set_rows(gops_context *ctx, gops_graph *g, const gops_row_plan *plan) {
+  if (!ctx || !g || !plan) return GOPS_INVALID_PARAMETER;
+  if (g->ctx != ctx) return fail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  ctx->last_error.clear();
+  if (!(plan->row_min >= 64 && plan->row_min <= kStageNnz))
+    return fail(ctx, GOPS_INVALID_PARAMETER, "row_min must be in [64, 4096]");
+  if (!(plan->hub_chunk >= plan->row_min && plan->hub_chunk <= (1 << 26)))
+    return fail(ctx, GOPS_INVALID_PARAMETER, "hub_chunk must be in [row_min, 2^26]");
+  if (!(plan->row_iters >= 1 && plan->row_iters <= 64))
+    return fail(ctx, GOPS_INVALID_PARAMETER, "row_iters must be in [1, 64]");
+  const PlanParams pp{plan->row_min, plan->hub_chunk, plan->row_iters};
+  if (g->dev.n) {
+    HostCSRT h;
+    gops_status st = download_rows(ctx, g, h);
+    if (st != GOPS_OK) return st;
+    Plan p;
+    plan_blocks(h, p, pp);
+    st = replace_plan(ctx, g, p);
+    if (st != GOPS_OK) return st;
+  }
+  g->plan_params = pp;
+  return GOPS_OK;
+}
+
+gops_status gops_graph_tune_rows(gops_context *ctx, gops_graph *g, gops_row_plan *out) {
+  if (!ctx || !g) return GOPS_INVALID_PARAMETER;
+  if (g->ctx != ctx) return fail(ctx, GOPS_INVALID_PARAMETER, "graph belongs to another context");
+  ctx->last_error.clear();
+  const uint64_t n = g->dev.n;
+  gops_row_plan res{g->plan_params.row_min, g->plan_params.hub_chunk, g->plan_params.row_iters,
+                    0, 0.0, 0.0};
+  if (n == 0 || g->dev.m == 0) {
+    if (out) *out = res;
+    return GOPS_OK;
+  }
+  HostCSRT h;
+  gops_status st = download_rows(ctx, g, h);
+  if (st != GOPS_OK) return st;
+  // timed runs: f32 pull sweeps with every kernel timed, the context's
+  // options and last statistics restored afterwards
+  const int saved_prec = ctx->precision, saved_ds = ctx->delta_sweep, saved_launch = ctx->launch;
+  const bool saved_timing = ctx->timing;
+  const gops_run_stats saved_stats = ctx->stats;
+  ctx->precision = GOPS_PRECISION_MIXED;
+  ctx->delta_sweep = GOPS_DELTA_SWEEP_PULL;
+  ctx->launch = GOPS_LAUNCH_QUEUED;
+  ctx->timing = true;
+  const RunKind kind = g->weighted ? kWeighted : kPlain;
+  const gops_pagerank_config cfg{0.85, 10, 0.0};
+  auto measure = [&](const PlanParams &pp, double *us) {
+    Plan p;
+    plan_blocks(h, p, pp);
+    st = replace_plan(ctx, g, p);
+    double best = std::numeric_limits<double>::infinity();
+    for (int rep = 0; rep < 3 && st == GOPS_OK; rep++) {
+      gops_run_info info{};
+      st = run_single(ctx, g, &cfg, kind, nullptr, 0, nullptr, &info);
+      if (st == GOPS_OK && ctx->stats.sweeps_fp32)
+        best = std::min(best, ctx->stats.fp32_sweep_ms * 1e3 / (double)ctx->stats.sweeps_fp32);
+    }
+    *us = best;
+    res.candidates++;
+  };
+  // coordinate search from the defaults: the run cut-off, then the hub
+  // chunk, then the rows per lane group; a change must gain 1 %
+  PlanParams best_pp{}, pp{};
+  double best_us = 0.0;
+  measure(best_pp, &best_us);
+  res.default_us = best_us;
+  const int32_t row_mins[] = {2048, 1024, 512};
+  const int32_t hub_chunks[] = {32768, 16384};
+  const int32_t row_iters[] = {4, 16};
+  for (int32_t v : row_mins) {
+    if (st != GOPS_OK) break;
+    pp = best_pp;
+    pp.row_min = v;
+    double us;
+    measure(pp, &us);
+    if (us < 0.99 * best_us) best_pp = pp, best_us = us;
+  }
+  for (int32_t v : hub_chunks) {
+    if (st != GOPS_OK) break;
+    pp = best_pp;
+    pp.hub_chunk = v;
+    double us;
+    measure(pp, &us);
+    if (us < 0.99 * best_us) best_pp = pp, best_us = us;
+  }
+  for (int32_t v : row_iters) {
+    if (st != GOPS_OK) break;
+    pp = best_pp;
+    pp.row_iters = v;
+    double us;
+    measure(pp, &us);
+    if (us < 0.99 * best_us) best_pp = pp, best_us = us;
+  }
+  ctx->precision = saved_prec;
+  ctx->delta_sweep = saved_ds;
+  ctx->launch = saved_launch;
+  ctx->timing = saved_timing;
+  ctx->stats = saved_stats;
+  if (st != GOPS_OK) return st;
+  Plan p;
+  plan_blocks(h, p, best_pp);
+  st = replace_plan(ctx, g, p);
+  if (st != GOPS_OK) return st;
+  g->plan_params = best_pp;
+  res.row_min = best_pp.row_min;
+  res.hub_chunk = best_pp.hub_chunk;
+  res.row_iters = best_pp.row_iters;
+  res.tuned_us = best_us;
+  if (out) *out = res;
+  return GOPS_OK;
+}
+
 gops_status gops_last_run_stats(const gops_context *ctx, gops_run_stats *out) {
   if (!ctx || !out) return GOPS_INVALID_PARAMETER;
   *out = ctx->stats;

@@ -173,7 +173,7 @@ static inline int lanes_for(int32_t deg) {
   return g;
 }
 
-void plan_blocks(const HostCSRT &g, Plan &p) {
+void plan_blocks(const HostCSRT &g, Plan &p, const PlanParams &pp) {
   p.desc.clear();
   p.hub_row.clear();
   p.hub_nchunks.clear();
@@ -184,20 +184,20 @@ void plan_blocks(const HostCSRT &g, Plan &p) {
   int32_t chunks = 0;
   while (i < n) {
     const int32_t deg = rp[i + 1] - rp[i];
-    if (deg > kStageNnz && deg <= kHubChunk) {
+    if (deg > pp.row_min && deg <= pp.hub_chunk) {
       p.desc.push_back({kRowFlag, (int32_t)i, 0, 0});
       i++;
       continue;
     }
-    if (deg > kHubChunk) {
+    if (deg > pp.hub_chunk) {
       const int32_t h = (int32_t)p.hub_row.size();
-      const int32_t nch = (deg + kHubChunk - 1) / kHubChunk;
+      const int32_t nch = (deg + pp.hub_chunk - 1) / pp.hub_chunk;
       p.hub_row.push_back((int32_t)i);
       p.hub_nchunks.push_back(nch);
       p.hub_first.push_back(chunks);
       for (int32_t c = 0; c < nch; c++) {
-        const int32_t b = rp[i] + c * kHubChunk;
-        const int32_t e = std::min(b + kHubChunk, rp[i + 1]);
+        const int32_t b = rp[i] + c * pp.hub_chunk;
+        const int32_t e = std::min(b + pp.hub_chunk, rp[i + 1]);
         p.desc.push_back({kHubFlag | c, h, b, e});
       }
       chunks += nch;
@@ -206,12 +206,12 @@ void plan_blocks(const HostCSRT &g, Plan &p) {
     }
     const int G = lanes_for(deg);
     const int groups = kBlock / G;
-    const int64_t max_rows = (int64_t)groups * kRowIters;
+    const int64_t max_rows = (int64_t)groups * pp.row_iters;
     const int64_t s = i;
     int64_t nnz = 0;
     while (i < n && i - s < max_rows) {
       const int32_t d = rp[i + 1] - rp[i];
-      if (d > kStageNnz || lanes_for(d) != G) break;
+      if (d > pp.row_min || lanes_for(d) != G) break;
       // the run's edges must fit the LDS stage, and with the lanes before
       // the run's first edge (its offset within 64 edges, where the sweep
       // kernels' waves start) one pass of the stage's loads; a first row

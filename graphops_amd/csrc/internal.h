@@ -87,6 +87,16 @@ struct HostCSRT {
   int first_bad_weight = 0;
 };
 
+// Cut-offs of the pull sweeps' row plan (plan_blocks): rows of more than
+// row_min in-edges (<= kStageNnz, the stage) take a workgroup of their own,
+// rows of more than hub_chunk are split into chunks of it, and a run of short
+// rows gives each group of lanes at most row_iters rows.
+struct PlanParams {
+  int32_t row_min = kStageNnz;
+  int32_t hub_chunk = kHubChunk;
+  int32_t row_iters = kRowIters;
+};
+
 struct Plan {
   std::vector<BlockDesc> desc;
   std::vector<int32_t> hub_row, hub_nchunks, hub_first;
@@ -254,6 +264,7 @@ struct gops_graph {
   int first_bad_weight = 0;
   std::vector<uint32_t> order_host;    // new -> original id
   std::vector<uint32_t> rank_host;     // original -> new id
+  gops::PlanParams plan_params;        // cut-offs of dev's row plan
 };
 
 namespace gops {
@@ -310,7 +321,7 @@ namespace gops {
 // host.cpp
 gops_status build_csrt(uint64_t n, const uint64_t *off, const uint32_t *nb, const double *ew,
                        HostCSRT &out, std::string &err);
-void plan_blocks(const HostCSRT &g, Plan &plan);
+void plan_blocks(const HostCSRT &g, Plan &plan, const PlanParams &pp = PlanParams{});
 uint16_t f32_to_bf16(float f);
 
 // graph_build.hip: build_csrt's arrays made on the device and kept there

@@ -319,6 +319,34 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t num_qu
                                const double *seed_weights, double *scores,
                                gops_run_info *infos);
 
+/* The pull sweeps' row plan: rows of more than row_min in-edges take a
+ * workgroup each, rows of more than hub_chunk are split into chunks of it,
+ * and runs of shorter rows give each group of lanes at most row_iters rows.
+ * Defaults 4096 / 65536 / 8. */
+typedef struct gops_row_plan {
+    int32_t row_min;
+    int32_t hub_chunk;
+    int32_t row_iters;
+    int32_t candidates;  /* plans timed by gops_graph_tune_rows */
+    double default_us;   /* f32 pull sweep with the default cut-offs (us) */
+    double tuned_us;     /* f32 pull sweep with the chosen ones (us) */
+} gops_row_plan;
+
+/* Autotune g's row plan: time f32 pull sweeps (10-sweep runs, best of 3,
+ * HIP events) under a coordinate search over row_min {4096, 2048, 1024, 512},
+ * hub_chunk {65536, 32768, 16384} and row_iters {8, 4, 16}, and keep the
+ * fastest (a change must gain 1 %) for every later pull sweep on g, f64
+ * ones included.  Results change only by f64 rounding (the order in which
+ * a row's products are added).  The context's options and last run
+ * statistics are left as they were.  No reference counterpart (tuning of
+ * the loop of pagerank.rs:154-162). */
+gops_status gops_graph_tune_rows(gops_context *ctx, gops_graph *g, gops_row_plan *out);
+
+/* Set g's row plan to the given cut-offs (row_min in [64, 4096], hub_chunk
+ * in [row_min, 2^26], row_iters in [1, 64]; the other fields are ignored).
+ * No reference counterpart. */
+gops_status gops_graph_set_rows(gops_context *ctx, gops_graph *g, const gops_row_plan *plan);
+
 /* Tile width a batch of num_queries queries runs with on g under ctx's
  * options (0: the one-query-per-run path), and the autotuner's measured cost
  * of widths 8, 16, 32 and 64 in ms per query-sweep (nullable, 4 doubles;

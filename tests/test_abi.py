@@ -33,6 +33,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(L.PageRankConfigC) == 24
     assert ctypes.sizeof(L.RunInfoC) == 24
     assert ctypes.sizeof(L.RunStatsC) == 80
+    assert ctypes.sizeof(L.RowPlanC) == 32
 
 
 def test_rmat_properties_and_determinism():

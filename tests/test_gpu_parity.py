@@ -430,3 +430,62 @@ def test_graph_replay_is_the_queued_run(oracle, delta):
     assert capped.iterations == 7 and not capped.converged
     ref = oracle_pull(oracle, e, 1, 1e-10, 300)
     check_run(out[L.LAUNCH_GRAPH][0], ref, 1e-10, "mixed", f"graph-{delta}")
+
+
+@pytest.mark.parametrize("plan", [(4096, 65536, 8), (512, 16384, 4), (1024, 32768, 16),
+                                  (64, 64, 1)])
+def test_row_plans(oracle, ctx, ctx64, plan):
+    """Every row plan the autotuner can choose (and the extreme one: a
+    workgroup per row of more than 64 in-edges, hubs in chunks of 64, one row
+    per lane group) gives the oracle's result, mixed and f64, on the graph
+    with hub rows, long rows and unaligned runs."""
+    n = 30000
+    rng = np.random.default_rng(5)
+    src = [np.arange(1, 20001), np.arange(2, 9003), np.arange(3, 1503), rng.integers(0, n, 3 * n)]
+    dst = [np.full(20000, 0), np.full(9001, 1), np.full(1500, 2), rng.integers(3, n, 3 * n)]
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    w = rng.uniform(0.1, 1.0, size=dst.size).astype(np.float32).astype(np.float64)
+    e = SimpleNamespace(n=n, m=dst.size, offsets=off, targets=dst.astype(np.uint32), weights=w)
+    tol = 1e-10
+    ref = oracle_pull(oracle, e, 1, tol, 300)
+    for cx, prec in ((ctx, "mixed"), (ctx64, "fp64")):
+        dg = ga.DeviceGraph.from_edges(e, weight_storage="f32", ctx=cx)
+        ga.graph_set_rows(dg, *plan)
+        r = ga.pagerank_weighted_run(dg, ga.PageRankConfig(0.85, 300, tol), ctx=cx)
+        check_run(r, ref, tol, prec, f"plan{plan}")
+        dg.close()
+
+
+def test_row_plan_autotune(oracle, ctx):
+    """gops_graph_tune_rows on R-MAT scale 16: the chosen plan is one of the
+    candidates, no slower than the default as measured, and the runs after
+    it still match the oracle; the context's last statistics are the ones
+    of the run before the tuning.  Bad cut-offs are refused."""
+    e = rmat_case(16, 7, "uniform", "f32")
+    tol = 1e-9
+    ref = oracle_pull(oracle, e, 1, tol, 200)
+    dg = ga.DeviceGraph.from_edges(e, weight_storage="f32", ctx=ctx)
+    cfg = ga.PageRankConfig(0.85, 200, tol)
+    r0 = ga.pagerank_weighted_run(dg, cfg, ctx=ctx)
+    before = ctx.last_stats()
+    t = ga.graph_tune_rows(dg)
+    assert ctx.last_stats() == before
+    assert t["candidates"] == 8
+    assert t["row_min"] in (4096, 2048, 1024, 512) and t["hub_chunk"] in (65536, 32768, 16384)
+    assert t["row_iters"] in (8, 4, 16)
+    assert 0.0 < t["tuned_us"] <= t["default_us"]
+    r1 = ga.pagerank_weighted_run(dg, cfg, ctx=ctx)
+    check_run(r1, ref, tol, "mixed", "tuned")
+    assert r1.iterations == r0.iterations
+    assert np.abs(r1.scores - r0.scores).sum() <= 1e-12
+    for bad in ((32, 65536, 8), (8192, 65536, 8), (4096, 1024, 8), (4096, 65536, 0)):
+        with pytest.raises(ga.InvalidParameter):
+            ga.graph_set_rows(dg, *bad)
+    dg.close()
