@@ -518,6 +518,7 @@ void gops_context_destroy(gops_context *ctx) {
   hipStreamSynchronize(ctx->stream);
   for (hipEvent_t e : ctx->events) hipEventDestroy(e);
   free_topk(ctx->topk);
+  free_topk_batch(ctx->topk_batch);
   hipFree(ctx->staging);
   free_pinned_chunks(ctx);
   hipHostFree(ctx->host_state);

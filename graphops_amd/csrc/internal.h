@@ -245,6 +245,7 @@ constexpr int kMaxSweepBlocks = 2048;  // grid of the sweep kernel (grid-stride 
 namespace gops {
 struct PprGraphCache;
 struct TopkScratch;
+struct TopkBatchScratch;
 }
 
 struct gops_graph {
@@ -297,6 +298,7 @@ struct gops_context {
   int launch = GOPS_LAUNCH_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
+  gops::TopkBatchScratch *topk_batch = nullptr;  // batched top_k scratch, on first use
   void *noalloc_scratch = nullptr;       // device address of topk.hip's file-scope scratch
   double *staging = nullptr;              // device copy of host scores for gops_top_k
   size_t staging_cap = 0;
@@ -350,6 +352,7 @@ hipError_t launch_permute(gops_context *ctx, gops_graph *g, const double *src, d
 gops_status device_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
                          uint32_t *ids, double *vals, uint32_t *count, std::string &err);
 void free_topk(TopkScratch *t);
+void free_topk_batch(TopkBatchScratch *t);
 // topk.hip: the allocation-free top_k.  noalloc_scratch gives the device
 // address of the library's file-scope scratch array on `device` (the current
 // device), looked up once per device; top_k_noalloc selects top_k (k <=
@@ -366,6 +369,8 @@ gops_status top_k_noalloc(gops_context *ctx, unsigned char *base, const double *
 // (ids / vals: k entries, *count); n < 2^32
 hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uint32_t k,
                         uint32_t *ids, double *vals, uint32_t *count);
+hipError_t launch_top_k_batch(gops_context *ctx, const double *scores, uint64_t n, uint32_t nq,
+                              uint32_t k, uint32_t *ids, double *vals, uint32_t *count);
 
 // api.cpp
 enum RunKind { kPlain = 0, kWeighted = 1, kPersonalized = 2 };

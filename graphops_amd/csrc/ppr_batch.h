@@ -78,6 +78,8 @@ hipError_t ppr_launch_admit(hipStream_t s, int T, const PprArgs &a, int n, uint6
                             uint64_t admit_mask);
 hipError_t ppr_launch_gather(hipStream_t s, int T, const PprArgs &a, const uint32_t *rank, int n,
                              int q, double *out);
+hipError_t ppr_launch_gather_mask(hipStream_t s, int T, const PprArgs &a, const uint32_t *rank,
+                                  int n, uint64_t mask, double *out);
 hipError_t ppr_launch_seed_mark(hipStream_t s, int32_t *seed_idx, const int32_t *rows, int count,
                                 int set);
 

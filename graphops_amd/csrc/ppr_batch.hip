@@ -96,12 +96,25 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *
   } else {
     // delta' = A delta: the dangling and (removed) total sums go to the seeds
     delta = acc + d * (ts->dsum_in[q] - ts->mass_in[q]) * seed_p;
-    const double x = __builtin_nontemporal_load(a.x + at) + delta;
-    __builtin_nontemporal_store(x, a.x + at);
     const double inv = a.inv64[r];
+    // A query's first sweep (admitted with p-hat as its input delta) starts
+    // from x = p-hat and the carry of p-hat's f32 rounding on its seeds, 0
+    // elsewhere: neither column is read (nor cleared at admission) for it.
+    const bool first = ts->iter[q] == 0;
+    double x_old, c_old;
+    if (first) {
+      x_old = seed_p;
+      const double v0 = (d * seed_p) * inv;
+      c_old = v0 - (double)(float)v0;
+    } else {
+      x_old = __builtin_nontemporal_load(a.x + at);
+      c_old = (double)a.carry[at];
+    }
+    const double x = x_old + delta;
+    __builtin_nontemporal_store(x, a.x + at);
     // the f32 delta carries the rounding of the row's previous one: summed
     // over the sweeps, the roundings cancel but for the last
-    const double v = (d * delta) * inv + (double)a.carry[at];
+    const double v = (d * delta) * inv + c_old;
     const float yo = (float)v;
     a.yout32[at] = yo;
     a.carry[at] = (float)(v - (double)yo);
@@ -479,9 +492,9 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
   }
 }
 
-// Admission, step 1: clear the admitted queries' columns (x, the carry and
-// the f32 deltas the next sweep gathers = 0, and, for an anchor admission,
-// 0 in the y64 buffer the next sweep's anchors gather).  One thread per row and group of four queries;
+// Admission, step 1: clear the admitted queries' columns of the f32 deltas
+// the next sweep gathers, and, for an anchor admission, of x and of the y64
+// buffer the next sweep's anchors gather.  One thread per row and group of four queries;
 // the tile's other queries are not touched.
 template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uint64_t clear_mask) {
@@ -489,9 +502,12 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
   const int qg = threadIdx.x % Q;
   const unsigned m4 = (unsigned)(clear_mask >> (qg * kLaneQ)) & 0xFu;
   if (!m4) return;
-  // y64 matters only to an anchor admission: otherwise a fold sweep writes
-  // the query's whole y64 column before any anchor reads it
-  double *y64 = a.ts->f32_below > 0.0 ? (a.ts->y64_par ? a.y64b : a.y64a) : nullptr;
+  // An anchor admission's first sweep reads x and y64.  A delta admission's
+  // first sweep reads only the gathered deltas: it takes x and the carry as
+  // p-hat and its rounding (ppr_epilogue), and a fold sweep writes the whole
+  // y64 column before any anchor reads it.
+  const bool anchored = a.ts->f32_below > 0.0;
+  double *y64 = anchored ? (a.ts->y64_par ? a.y64b : a.y64a) : nullptr;
   float *yin = a.ts->yd_cur ? a.yd1 : a.yd0;
   const int rows_per_block = kBlock / Q;
   for (int r = blockIdx.x * rows_per_block + threadIdx.x / Q; r < n;
@@ -500,22 +516,22 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
     if (m4 == 0xFu) {
       const double2 z = {0.0, 0.0};
       const float4 zf = {0.0f, 0.0f, 0.0f, 0.0f};
-      *reinterpret_cast<double2 *>(a.x + at) = z;
-      *reinterpret_cast<double2 *>(a.x + at + 2) = z;
-      if (y64) {
+      if (anchored) {
+        *reinterpret_cast<double2 *>(a.x + at) = z;
+        *reinterpret_cast<double2 *>(a.x + at + 2) = z;
         *reinterpret_cast<double2 *>(y64 + at) = z;
         *reinterpret_cast<double2 *>(y64 + at + 2) = z;
       }
       *reinterpret_cast<float4 *>(yin + at) = zf;
-      *reinterpret_cast<float4 *>(a.carry + at) = zf;
     } else {
 #pragma unroll
       for (int j = 0; j < kLaneQ; j++) {
         if ((m4 >> j) & 1u) {
-          a.x[at + j] = 0.0;
-          if (y64) y64[at + j] = 0.0;
+          if (anchored) {
+            a.x[at + j] = 0.0;
+            y64[at + j] = 0.0;
+          }
           yin[at + j] = 0.0f;
-          a.carry[at + j] = 0.0f;
         }
       }
     }
@@ -540,14 +556,12 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
     const int r = a.seed_row[j];
     const size_t at = (size_t)r * T + qq;
     const double p = a.seed_p[j];
-    a.x[at] = p;  // one entry per (row, query): the host merges duplicate seeds
     const double v = (ts->damping[qq] * p) * a.inv64[r];
     if (anchored) {
+      a.x[at] = p;  // one entry per (row, query): the host merges duplicate seeds
       y64[at] = v;
     } else {
-      const float yo = (float)v;
-      yin[at] = yo;
-      a.carry[at] = (float)(v - (double)yo);
+      yin[at] = (float)v;  // x and the carry: ppr_epilogue's first sweep
     }
   }
   __syncthreads();
@@ -582,6 +596,20 @@ __global__ __launch_bounds__(kBlock) void ppr_gather_kernel(PprArgs a, const uin
                                                             int q, double *out) {
   for (int o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock)
     out[o] = a.x[(size_t)rank[o] * T + q];
+}
+
+// out[j * n + o] = x[rank[o]][q] for the queries q of mask (j: q's rank
+// among them), in the caller's ids: one pass over the rows for all of them,
+// each row's values read from the lines of one 8 T-byte row.
+template <int T>
+__global__ __launch_bounds__(kBlock) void ppr_gather_mask_kernel(PprArgs a, const uint32_t *rank,
+                                                                 int n, uint64_t mask,
+                                                                 double *out) {
+  for (int o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock) {
+    const double *row = a.x + (size_t)rank[o] * T;
+    uint64_t m = mask;
+    for (int j = 0; m; j++, m &= m - 1) out[(size_t)j * n + o] = row[__ffsll((long long)m) - 1];
+  }
 }
 
 __global__ void seed_mark_kernel(int32_t *seed_idx, const int32_t *rows, int count, int set) {
@@ -642,6 +670,18 @@ hipError_t ppr_launch_gather(hipStream_t s, int T, const PprArgs &a, const uint3
     case 16: ppr_gather_kernel<16><<<grid, kBlock, 0, s>>>(a, rank, n, q, out); break;
     case 32: ppr_gather_kernel<32><<<grid, kBlock, 0, s>>>(a, rank, n, q, out); break;
     default: ppr_gather_kernel<64><<<grid, kBlock, 0, s>>>(a, rank, n, q, out); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t ppr_launch_gather_mask(hipStream_t s, int T, const PprArgs &a, const uint32_t *rank,
+                                  int n, uint64_t mask, double *out) {
+  const int grid = std::max(1, std::min((n + kBlock - 1) / kBlock, 4096));
+  switch (T) {
+    case 8: ppr_gather_mask_kernel<8><<<grid, kBlock, 0, s>>>(a, rank, n, mask, out); break;
+    case 16: ppr_gather_mask_kernel<16><<<grid, kBlock, 0, s>>>(a, rank, n, mask, out); break;
+    case 32: ppr_gather_mask_kernel<32><<<grid, kBlock, 0, s>>>(a, rank, n, mask, out); break;
+    default: ppr_gather_mask_kernel<64><<<grid, kBlock, 0, s>>>(a, rank, n, mask, out); break;
   }
   return hipGetLastError();
 }

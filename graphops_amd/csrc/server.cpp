@@ -162,7 +162,8 @@ struct gops_server {
     std::vector<uint32_t> top_ids;  // topk > 0: the result's top_k, made on the device
     std::vector<double> top_vals;
   };
-  double *gather = nullptr;  // topk > 0: one query's scores, reused by every harvest
+  double *gather = nullptr;  // topk > 0: gather_cap queries' scores, reused by every harvest
+  uint32_t gather_cap = 0;
   uint32_t *top_ids = nullptr;  // topk > 0: a harvest's selections on the device,
   double *top_vals = nullptr;   // top_cap results of topk pairs, and their counts
   uint32_t *top_cnt = nullptr;
@@ -914,11 +915,41 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
     for (uint32_t k = 0; k < sweeps; k++) STRY(ctx, ppr_launch_sweep(ctx->stream, t->T, a, t->grid));
   }
   STRY(ctx, hipStreamSynchronize(ctx->stream));
-  // 3. harvest finished queries (with topk: each result's top_k selected on
-  // the device, all of this step's pairs copied back once at the end)
+  // 3. harvest finished queries.  With topk: a tile's finished queries are
+  // gathered in one pass over its rows and their top_k selected together
+  // (every selection kernel launched once for all of them); all of this
+  // step's pairs are copied back once at the end.
   uint32_t live_total = 0;
   const size_t first_new = s->finished.size();
   uint32_t n_top = 0;  // results selected in this harvest
+  auto ensure_top = [&](uint32_t need) -> gops_status {
+    if (need <= s->top_cap) return GOPS_OK;
+    const uint32_t cap = std::max<uint32_t>(std::max<uint32_t>(64, 2 * s->top_cap), need);
+    const size_t k = s->topk;
+    uint32_t *ids = nullptr, *cnt = nullptr;
+    double *vals = nullptr;
+    STRY(ctx, hipMalloc((void **)&ids, (size_t)cap * k * sizeof(uint32_t)));
+    STRY(ctx, hipMalloc((void **)&vals, (size_t)cap * k * sizeof(double)));
+    STRY(ctx, hipMalloc((void **)&cnt, (size_t)cap * sizeof(uint32_t)));
+    if (n_top) {  // this harvest's earlier selections
+      STRY(ctx, hipMemcpyAsync(ids, s->top_ids, (size_t)n_top * k * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+      STRY(ctx, hipMemcpyAsync(vals, s->top_vals, (size_t)n_top * k * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+      STRY(ctx, hipMemcpyAsync(cnt, s->top_cnt, (size_t)n_top * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    STRY(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(s->top_ids);
+    hipFree(s->top_vals);
+    hipFree(s->top_cnt);
+    s->top_ids = ids;
+    s->top_vals = vals;
+    s->top_cnt = cnt;
+    s->top_cap = cap;
+    return GOPS_OK;
+  };
+  const bool batched_topk = s->topk > 0 && s->topk <= (uint32_t)kNoAllocTopK;
   for (Tile *t : s->tiles) {
     bool any = false;
     for (int q = 0; q < t->T; q++) any = any || t->used[q];
@@ -926,46 +957,68 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
     st = sync_state(ctx, t);
     if (st != GOPS_OK) return st;
     PprArgs a = tile_args(ctx, g, t, plan, false);
+    uint64_t done_mask = 0;
     for (int q = 0; q < t->T; q++) {
       if (!t->used[q]) continue;
       if (t->hts->live[q]) {
         live_total++;
         continue;
       }
+      done_mask |= 1ull << q;
+    }
+    if (!done_mask) continue;
+    const uint32_t F = (uint32_t)__builtin_popcountll(done_mask);
+    if (batched_topk) {
+      if (s->gather_cap < F && s->gather_cap < (uint32_t)t->T) {
+        // room for a tile's worth of columns, or as many as fit (at least one)
+        hipFree(s->gather);
+        s->gather = nullptr;
+        s->gather_cap = 0;
+        for (uint32_t c = (uint32_t)t->T; c >= 1 && !s->gather; c /= 2) {
+          if (hipMalloc((void **)&s->gather, (size_t)c * n * sizeof(double)) == hipSuccess)
+            s->gather_cap = c;
+          else
+            (void)hipGetLastError();  // a failed allocation is not a later launch's error
+        }
+        if (!s->gather) return sfail(ctx, GOPS_OUT_OF_MEMORY, "harvest buffer: out of device memory");
+      }
+      st = ensure_top(n_top + F);
+      if (st != GOPS_OK) return st;
+      // groups of at most gather_cap queries, in slot order
+      uint64_t rest = done_mask;
+      uint32_t at = n_top;
+      while (rest) {
+        uint64_t grp = 0;
+        for (uint32_t c = 0; c < s->gather_cap && rest; c++) {
+          grp |= rest & (~rest + 1);
+          rest &= rest - 1;
+        }
+        const uint32_t fg = (uint32_t)__builtin_popcountll(grp);
+        STRY(ctx, ppr_launch_gather_mask(ctx->stream, t->T, a, g->dev.rank, n, grp, s->gather));
+        const size_t off = (size_t)at * s->topk;
+        STRY(ctx, launch_top_k_batch(ctx, s->gather, n, fg, s->topk, s->top_ids + off,
+                                     s->top_vals + off, s->top_cnt + at));
+        at += fg;
+      }
+    }
+    for (int q = 0; q < t->T; q++) {
+      if (!((done_mask >> q) & 1ull)) continue;
       gops_server::Done d;
       d.q = std::move(t->slot[q]);
       d.info = gops_run_info{t->hts->iter[q], t->hts->diff[q], t->hts->converged[q], 0};
       d.finish_ms = s->now_ms();
       d.scores = nullptr;
-      if (s->topk) {
-        // the gather buffer is reused: stream order puts each query's
-        // selection before the next query's gather
-        if (!s->gather) STRY(ctx, hipMalloc((void **)&s->gather, (size_t)n * sizeof(double)));
-        if (n_top == s->top_cap) {
-          const uint32_t cap = std::max<uint32_t>(64, 2 * s->top_cap);
-          const size_t k = s->topk;
-          uint32_t *ids = nullptr, *cnt = nullptr;
-          double *vals = nullptr;
-          STRY(ctx, hipMalloc((void **)&ids, (size_t)cap * k * sizeof(uint32_t)));
-          STRY(ctx, hipMalloc((void **)&vals, (size_t)cap * k * sizeof(double)));
-          STRY(ctx, hipMalloc((void **)&cnt, (size_t)cap * sizeof(uint32_t)));
-          if (n_top) {  // this harvest's earlier selections
-            STRY(ctx, hipMemcpyAsync(ids, s->top_ids, (size_t)n_top * k * sizeof(uint32_t),
-                                     hipMemcpyDeviceToDevice, ctx->stream));
-            STRY(ctx, hipMemcpyAsync(vals, s->top_vals, (size_t)n_top * k * sizeof(double),
-                                     hipMemcpyDeviceToDevice, ctx->stream));
-            STRY(ctx, hipMemcpyAsync(cnt, s->top_cnt, (size_t)n_top * sizeof(uint32_t),
-                                     hipMemcpyDeviceToDevice, ctx->stream));
-            STRY(ctx, hipStreamSynchronize(ctx->stream));
-          }
-          hipFree(s->top_ids);
-          hipFree(s->top_vals);
-          hipFree(s->top_cnt);
-          s->top_ids = ids;
-          s->top_vals = vals;
-          s->top_cnt = cnt;
-          s->top_cap = cap;
+      if (batched_topk) {
+        n_top++;  // selected above, in slot order
+      } else if (s->topk) {
+        // k beyond the one-workgroup sort: one selection per query, the
+        // gather buffer reused in stream order
+        if (!s->gather) {
+          STRY(ctx, hipMalloc((void **)&s->gather, (size_t)n * sizeof(double)));
+          s->gather_cap = 1;
         }
+        st = ensure_top(n_top + 1);
+        if (st != GOPS_OK) return st;
         STRY(ctx, ppr_launch_gather(ctx->stream, t->T, a, g->dev.rank, n, q, s->gather));
         const size_t off = (size_t)n_top * s->topk;
         STRY(ctx, launch_top_k(ctx, s->gather, n, s->topk, s->top_ids + off, s->top_vals + off,

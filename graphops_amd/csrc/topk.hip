@@ -51,6 +51,14 @@ struct TkState {
                       // whose keys are all taken
 };
 
+// A batch of selections runs one per blockIdx.y: query y's scores are
+// y * score doubles in, its selected keys and ids y * sel entries in, its
+// results y * out entries in; state, histograms, counts and offsets are
+// per query at their own sizes.  A single selection has y = 0.
+struct TkBatch {
+  uint64_t score, sel, out;
+};
+
 __device__ __forceinline__ uint64_t skey(double s) {
   return (isfinite(s) && s > 0.0) ? (uint64_t)__double_as_longlong(s) : 0ull;
 }
@@ -58,7 +66,11 @@ __device__ __forceinline__ uint64_t skey(double s) {
 // digit histogram of the valid keys whose bits above `shift + 8` match the
 // prefix: one row of 256 bins per workgroup
 __global__ __launch_bounds__(kHB) void tk_hist_kernel(const double *s, uint64_t n, int shift,
-                                                       const TkState *st, uint32_t *part) {
+                                                       const TkState *st, uint32_t *part,
+                                                       TkBatch bt) {
+  s += blockIdx.y * bt.score;
+  st += blockIdx.y;
+  part += (size_t)blockIdx.y * kHGrid * 256;
   if (st->all || st->resolved) return;  // the pick kernel reads no histogram then
   __shared__ uint32_t h[256];
   if (threadIdx.x < 256) h[threadIdx.x] = 0;
@@ -106,6 +118,8 @@ constexpr int kPickSlices = kPickThreads / 64;  // row slices (64 threads x 4 bi
 static_assert(kHGrid % kPickSlices == 0, "rows split evenly over the slices");
 __global__ __launch_bounds__(kPickThreads) void tk_pick_kernel(TkState *st, const uint32_t *part,
                                                                 int shift, uint64_t k) {
+  st += blockIdx.y;
+  part += (size_t)blockIdx.y * kHGrid * 256;
   if (st->all || st->resolved) return;
   __shared__ uint64_t tot[kPickSlices][256];
   const int g = threadIdx.x & 63, sl = threadIdx.x >> 6;
@@ -204,7 +218,11 @@ __device__ __forceinline__ void chunk_of(uint64_t n, uint64_t &b, uint64_t &e) {
 
 // per chunk: entries >= T and == T (valid ones)
 __global__ __launch_bounds__(kTB) void tk_count_kernel(const double *s, uint64_t n,
-                                                        const TkState *st, uint64_t *cnt) {
+                                                        const TkState *st, uint64_t *cnt,
+                                                        TkBatch bt) {
+  s += blockIdx.y * bt.score;
+  st += blockIdx.y;
+  cnt += (size_t)blockIdx.y * 2 * kTGrid;
   const uint64_t T = st->prefix;
   uint64_t b, e;
   chunk_of(n, b, e);
@@ -240,6 +258,8 @@ __global__ __launch_bounds__(kTB) void tk_count_kernel(const double *s, uint64_t
 // one workgroup of nblocks (<= 1024) threads: exclusive scans of the chunk counts
 __global__ __launch_bounds__(1024) void tk_scan_kernel(const uint64_t *cnt, int nblocks,
                                                         uint64_t *off) {
+  cnt += (size_t)blockIdx.y * 2 * kTGrid;
+  off += (size_t)blockIdx.y * 2 * kTGrid;
   __shared__ uint64_t sg[1024], sq[1024];
   const int t = threadIdx.x;
   const uint64_t g = t < nblocks ? cnt[2 * t] : 0, q = t < nblocks ? cnt[2 * t + 1] : 0;
@@ -263,7 +283,11 @@ __global__ __launch_bounds__(1024) void tk_scan_kernel(const uint64_t *cnt, int 
 // >= T before them)
 __global__ __launch_bounds__(kTB) void tk_entered_kernel(const double *s, uint64_t n, uint64_t k,
                                                           const TkState *st, const uint64_t *off,
-                                                          uint64_t *part) {
+                                                          uint64_t *part, TkBatch bt) {
+  s += blockIdx.y * bt.score;
+  st += blockIdx.y;
+  off += (size_t)blockIdx.y * 2 * kTGrid;
+  part += (size_t)blockIdx.y * kTGrid;
   const uint64_t T = st->prefix;
   uint64_t b, e;
   chunk_of(n, b, e);
@@ -301,7 +325,14 @@ __global__ __launch_bounds__(kTB) void tk_entered_kernel(const double *s, uint64
 __global__ __launch_bounds__(kTB) void tk_select_kernel(const double *s, uint64_t n, TkState *st,
                                                          const uint64_t *off,
                                                          const uint64_t *entered, int nblocks,
-                                                         uint64_t *out_key, uint32_t *out_id) {
+                                                         uint64_t *out_key, uint32_t *out_id,
+                                                         TkBatch bt) {
+  s += blockIdx.y * bt.score;
+  st += blockIdx.y;
+  off += (size_t)blockIdx.y * 2 * kTGrid;
+  entered += (size_t)blockIdx.y * kTGrid;
+  out_key += blockIdx.y * bt.sel;
+  out_id += blockIdx.y * bt.sel;
   const uint64_t T = st->prefix;
   __shared__ uint64_t sj[kTB];
   uint64_t tj = 0;
@@ -359,7 +390,13 @@ __global__ __launch_bounds__(kSortThreads) void tk_sort_kernel(const uint64_t *k
                                                                 const uint32_t *id_in,
                                                                 const TkState *st, uint64_t k,
                                                                 uint32_t *ids, double *vals,
-                                                                uint32_t *count) {
+                                                                uint32_t *count, TkBatch bt) {
+  key_in += blockIdx.y * bt.sel;
+  id_in += blockIdx.y * bt.sel;
+  st += blockIdx.y;
+  ids += blockIdx.y * bt.out;
+  vals += blockIdx.y * bt.out;
+  count += blockIdx.y;
   __shared__ uint64_t sk[kSortMax];
   __shared__ uint32_t si[kSortMax];
   const uint64_t K = st->all ? st->valid : k;
@@ -425,8 +462,8 @@ __global__ __launch_bounds__(kTB) void tk_emit_kernel(const uint64_t *key, const
 // kernel rather than a memset, so that every step of a selection is a kernel
 __global__ void tk_clear_kernel(TkState *st, uint32_t *count) {
   if (threadIdx.x == 0) {
-    if (st) *st = TkState{};
-    if (count) *count = 0;
+    if (st) st[blockIdx.y] = TkState{};
+    if (count) count[blockIdx.y] = 0;
   }
 }
 
@@ -506,18 +543,19 @@ static hipError_t topk_scratch(gops_context *ctx, uint32_t k) {
 static hipError_t launch_top_k_in(hipStream_t s, const double *scores, uint64_t n, uint64_t kk,
                                   uint32_t *ids, double *vals, uint32_t *count,
                                   const TopkScratch &t) {
+  const TkBatch one{0, 0, 0};
   tk_clear_kernel<<<1, 64, 0, s>>>(t.st, nullptr);
   for (int shift = 56; shift >= 0; shift -= 8) {
-    tk_hist_kernel<<<kHGrid, kHB, 0, s>>>(scores, n, shift, t.st, t.part);
+    tk_hist_kernel<<<kHGrid, kHB, 0, s>>>(scores, n, shift, t.st, t.part, one);
     tk_pick_kernel<<<1, kPickThreads, 0, s>>>(t.st, t.part, shift, kk);
   }
-  tk_count_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.cnt);
+  tk_count_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.cnt, one);
   tk_scan_kernel<<<1, 1024, 0, s>>>(t.cnt, kTGrid, t.off);
-  tk_entered_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, kk, t.st, t.off, t.entered);
+  tk_entered_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, kk, t.st, t.off, t.entered, one);
   tk_select_kernel<<<kTGrid, kTB, 0, s>>>(scores, n, t.st, t.off, t.entered, kTGrid, t.key[0],
-                                         t.id[0]);
+                                         t.id[0], one);
   if (kk <= (uint64_t)kSortMax) {
-    tk_sort_kernel<<<1, kSortThreads, 0, s>>>(t.key[0], t.id[0], t.st, kk, ids, vals, count);
+    tk_sort_kernel<<<1, kSortThreads, 0, s>>>(t.key[0], t.id[0], t.st, kk, ids, vals, count, one);
     return hipGetLastError();
   }
   // the sort is stable and the entries are in index order within equal keys
@@ -547,6 +585,73 @@ hipError_t launch_top_k(gops_context *ctx, const double *scores, uint64_t n, uin
   hipError_t e = topk_scratch(ctx, (uint32_t)kk);
   if (e != hipSuccess) return e;
   return launch_top_k_in(s, scores, n, kk, ids, vals, count, *ctx->topk);
+}
+
+// Scratch of batched selections: per query a state, histogram rows, chunk
+// counts, offsets and tie counts, and k selected keys and ids.
+struct TopkBatchScratch {
+  uint32_t cap_q = 0, cap_k = 0;
+  uint32_t *part = nullptr;
+  uint64_t *cnt = nullptr, *off = nullptr, *entered = nullptr;
+  TkState *st = nullptr;
+  uint64_t *key = nullptr;
+  uint32_t *id = nullptr;
+};
+
+void free_topk_batch(TopkBatchScratch *t) {
+  if (!t) return;
+  for (void *p : {(void *)t->part, (void *)t->cnt, (void *)t->off, (void *)t->entered,
+                  (void *)t->st, (void *)t->key, (void *)t->id})
+    (void)hipFree(p);
+  delete t;
+}
+
+// Queues top_k of nq score vectors (n doubles each, n apart in scores) on
+// ctx's stream, every kernel launched once for all of them (blockIdx.y is
+// the query): query y's ids / vals at y * k and its count at count[y].
+// k <= kSortMax (the one-workgroup sort).
+hipError_t launch_top_k_batch(gops_context *ctx, const double *scores, uint64_t n, uint32_t nq,
+                              uint32_t k, uint32_t *ids, double *vals, uint32_t *count) {
+  hipStream_t s = ctx->stream;
+  if (nq == 0) return hipSuccess;
+  if (k > (uint32_t)kSortMax) return hipErrorInvalidValue;
+  if (k == 0 || n == 0) {
+    tk_clear_kernel<<<dim3(1, nq), 64, 0, s>>>(nullptr, count);
+    return hipGetLastError();
+  }
+  if (!ctx->topk_batch) ctx->topk_batch = new TopkBatchScratch;
+  TopkBatchScratch &t = *ctx->topk_batch;
+  const uint64_t kk = std::min<uint64_t>(k, n);
+  if (t.cap_q < nq || t.cap_k < kk) {
+    for (void *p : {(void *)t.part, (void *)t.cnt, (void *)t.off, (void *)t.entered, (void *)t.st,
+                    (void *)t.key, (void *)t.id})
+      (void)hipFree(p);
+    t = TopkBatchScratch{};
+    const size_t q = nq;
+    hipError_t e = hipMalloc((void **)&t.part, q * kHGrid * 256 * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&t.cnt, q * 2 * kTGrid * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&t.off, q * 2 * kTGrid * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&t.entered, q * kTGrid * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&t.st, q * sizeof(TkState));
+    if (e == hipSuccess) e = hipMalloc((void **)&t.key, q * kk * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&t.id, q * kk * sizeof(uint32_t));
+    if (e != hipSuccess) return e;
+    t.cap_q = nq;
+    t.cap_k = (uint32_t)kk;
+  }
+  const TkBatch bt{n, t.cap_k, k};
+  tk_clear_kernel<<<dim3(1, nq), 64, 0, s>>>(t.st, nullptr);
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    tk_hist_kernel<<<dim3(kHGrid, nq), kHB, 0, s>>>(scores, n, shift, t.st, t.part, bt);
+    tk_pick_kernel<<<dim3(1, nq), kPickThreads, 0, s>>>(t.st, t.part, shift, kk);
+  }
+  tk_count_kernel<<<dim3(kTGrid, nq), kTB, 0, s>>>(scores, n, t.st, t.cnt, bt);
+  tk_scan_kernel<<<dim3(1, nq), 1024, 0, s>>>(t.cnt, kTGrid, t.off);
+  tk_entered_kernel<<<dim3(kTGrid, nq), kTB, 0, s>>>(scores, n, kk, t.st, t.off, t.entered, bt);
+  tk_select_kernel<<<dim3(kTGrid, nq), kTB, 0, s>>>(scores, n, t.st, t.off, t.entered, kTGrid,
+                                                   t.key, t.id, bt);
+  tk_sort_kernel<<<dim3(1, nq), kSortThreads, 0, s>>>(t.key, t.id, t.st, kk, ids, vals, count, bt);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
