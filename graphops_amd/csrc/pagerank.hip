@@ -299,21 +299,26 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, 
   }
 }
 
-// red += partials[b] for b = threadIdx.x, + kBlock, ... < count, in that
-// order, with four rows' loads in flight at a time
+// The finish kernel's workgroup: enough threads that the partials of a
+// sweep's workgroups (about 4600 blocked, 2048 pulled) load in one round.
+constexpr int kFinishBlock = 1024;
+
+// red += partials[b] for b = threadIdx.x, + kFinishBlock, ... < count, in
+// that order, with eight rows' loads in flight at a time
 __device__ __forceinline__ void add_partials(const double *partials, int count,
                                              double (&red)[kReduce]) {
-  for (int b0 = threadIdx.x; b0 < count; b0 += 4 * kBlock) {
-    double v[4][kReduce];
+  constexpr int U = 8;
+  for (int b0 = threadIdx.x; b0 < count; b0 += U * kFinishBlock) {
+    double v[U][kReduce];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int b = b0 + u * kBlock;
+    for (int u = 0; u < U; u++) {
+      const int b = b0 + u * kFinishBlock;
 #pragma unroll
       for (int i = 0; i < kReduce; i++) v[u][i] = b < count ? partials[(size_t)b * kReduce + i] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      if (b0 + u * kBlock < count) {
+    for (int u = 0; u < U; u++) {
+      if (b0 + u * kFinishBlock < count) {
 #pragma unroll
         for (int i = 0; i < kReduce; i++) red[i] += v[u][i];
       }
@@ -323,16 +328,17 @@ __device__ __forceinline__ void add_partials(const double *partials, int count,
 
 // One workgroup: hub-row epilogues (chunk partials summed in chunk order),
 // the fixed-order reduction of the sweep's partials, the bookkeeping.
-__global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float *yd0, float *yd1,
-                                                              double *y640, double *y641,
-                                                              int sweep_blocks) {
+__global__ __launch_bounds__(kFinishBlock) void sweep_finish_kernel(SweepArgs a, float *yd0,
+                                                                    float *yd1, double *y640,
+                                                                    double *y641,
+                                                                    int sweep_blocks) {
   RunState *st = a.st;
   if (st->done) return;
   pick_buffers(a, st, yd0, yd1, y640, y641);
   const int mode = st->mode;
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
   auto hubs = [&](const auto &epi) {
-    for (int h = threadIdx.x; h < a.num_hubs; h += kBlock) {
+    for (int h = threadIdx.x; h < a.num_hubs; h += kFinishBlock) {
       const int first = a.hub_first[h], nch = a.hub_nchunks[h];
       double s = 0.0;
       for (int c = 0; c < nch; c++) s += a.hub_partial[first + c];
@@ -351,7 +357,8 @@ __global__ __launch_bounds__(kBlock) void sweep_finish_kernel(SweepArgs a, float
     }
     add_partials(a.partials, sweep_blocks, red);
   }
-  block_sum<kReduce>(red);
+  __shared__ double scratch[kReduce * (kFinishBlock / 64)];
+  block_sum_waves<kReduce, kFinishBlock / 64>(red, scratch);
   if (threadIdx.x == 0) finish_sweep(st, mode, red);
 }
 
@@ -685,13 +692,20 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   }
 }
 
+// the parts of a bin split over several workgroups in a blocked anchor (below)
+__device__ void pb_combine64(const SweepArgs &a, const RunState *st);
+
 // Combine: kBlock rows of a bin whose runs were split over several
 // accumulate workgroups: the parts' sums in slot order, then the epilogues.
 __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *yd0, float *yd1,
                                                             double *y640, double *y641) {
   const RunState *st = a.st;
-  if (st->done || st->mode == kModeAnchor) return;
+  if (st->done || (st->mode == kModeAnchor && !a.pb_anchor)) return;
   pick_buffers(a, st, yd0, yd1, y640, y641);
+  if (st->mode == kModeAnchor) {  // a blocked anchor's parts
+    pb_combine64(a, st);
+    return;
+  }
   const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
   const double inv_scale = ldexp(1.0, -pb_exponent(st));
   const int row0 = (sp.x << kPbBinShift) + sp.w;
@@ -1009,11 +1023,7 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
 
 // Combine of an anchor sweep: the parts' 128-bit sums in slot order, then the
 // anchor epilogues.
-__global__ __launch_bounds__(kBlock) void pb_combine64_kernel(SweepArgs a, float *yd0, float *yd1,
-                                                              double *y640, double *y641) {
-  const RunState *st = a.st;
-  if (st->done || st->mode != kModeAnchor) return;
-  pick_buffers(a, st, yd0, yd1, y640, y641);
+__device__ void pb_combine64(const SweepArgs &a, const RunState *st) {
   const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
   const int row0 = (sp.x << kPbBinShift) + sp.w;
   const int i = sp.w + (int)threadIdx.x;
@@ -1196,21 +1206,19 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
       pb_scatter_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1]);
     pb_accumulate_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
                                                                       rb.y64[0], rb.y64[1]);
+    if (a.pb_anchor) {
+      if (a.pb.num_items1)
+        pb_scatter64_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.y64[0],
+                                                                                   rb.y64[1]);
+      pb_accumulate64_kernel<<<a.pb.num_items2, kAcc64Block, 0, ctx->stream>>>(
+          a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1]);
+    }
+    // the parts of split bins, of either kind: after both accumulates
     if (a.pb.num_splits)
       pb_combine_kernel<<<a.pb.num_splits, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
                                                                      rb.y64[0], rb.y64[1]);
   }
-  if (a.pb_anchor) {
-    if (a.pb.num_items1)
-      pb_scatter64_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.y64[0],
-                                                                                 rb.y64[1]);
-    pb_accumulate64_kernel<<<a.pb.num_items2, kAcc64Block, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
-                                                                             rb.y64[0], rb.y64[1]);
-    if (a.pb.num_splits)
-      pb_combine64_kernel<<<a.pb.num_splits, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
-                                                                       rb.y64[0], rb.y64[1]);
-  }
-  sweep_finish_kernel<<<1, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1],
+  sweep_finish_kernel<<<1, kFinishBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1],
                                                      grid);
 }
 

@@ -102,11 +102,10 @@ def test_rmat_weights(oracle, ctx_b, ctx_p, weights, storage):
     check(b, ref, tol)
 
 
-def test_split_bins_and_ragged_sizes(oracle, ctx_b, ctx_p):
+def test_ragged_sizes_and_long_rows(oracle, ctx_b, ctx_p):
     """n = 70 001 (not a multiple of a chunk or a bin); node 5 has 140 000
-    in-edges (its bin is split over several accumulate workgroups), node 9
-    has 70 000 out-edges, many rows and a whole bin without in-edges,
-    dangling nodes."""
+    in-edges, node 9 has 70 000 out-edges, many rows and a whole bin without
+    in-edges, dangling nodes."""
     n = 70001
     rng = np.random.default_rng(4)
     src = [np.arange(n, dtype=np.int64) % 60000, np.full(70000, 9),
@@ -250,3 +249,59 @@ def bf16_exact(w):
     exactly, whatever rounding the storage conversion applies."""
     u = np.asarray(w, dtype=np.float32).view(np.uint32) & np.uint32(0xFFFF0000)
     return u.view(np.float32).astype(np.float64)
+
+
+def bin_runs(e, chunk=16384, bin_rows=4096):
+    """Runs (a row's in-edges from one chunk of sources) per bin of rows, in
+    the library's ids: nodes renumbered by in-degree, descending, ties by id."""
+    src = np.repeat(np.arange(e.n, dtype=np.int64), np.diff(e.offsets.astype(np.int64)))
+    dst = e.targets.astype(np.int64)
+    indeg = np.bincount(dst, minlength=e.n)
+    order = np.lexsort((np.arange(e.n), -indeg))
+    rank = np.empty(e.n, np.int64)
+    rank[order] = np.arange(e.n)
+    pairs = np.unique(rank[dst] * (e.n // chunk + 1) + rank[src] // chunk)
+    rows = pairs // (e.n // chunk + 1)
+    return np.bincount(rows // bin_rows)
+
+
+def test_split_bins(oracle, ctx_b, ctx_p):
+    """Bins whose runs exceed one accumulate item (65 536 runs) are split over
+    several workgroups whose partial sums the combine kernel adds, 64-bit in
+    delta sweeps and 128-bit in anchors.  n = 2^19 + 3 (33 chunks): 4096 rows
+    take an in-edge from 33 sources spread over the chunks, which puts 87 203
+    runs in the first bin (counted here in the library's ids); the tolerance
+    makes the run take several anchors.  Against the oracle, the pull, and
+    pull anchors."""
+    n = (1 << 19) + 3
+    rng = np.random.default_rng(19)
+    hot = np.arange(4096)
+    src = [(hot[:, None] + 16384 * np.arange(33)[None, :] + 7).ravel() % n,
+           rng.integers(0, n, size=3 * n)]
+    dst = [np.repeat(hot, 33), rng.integers(4096, n, size=3 * n)]
+    src, dst = np.concatenate(src), np.concatenate(dst)
+    keep = src != dst
+    src, dst = src[keep], dst[keep]
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    off = np.cumsum(off).astype(np.uint64)
+    w = bf16_exact(rng.uniform(0.1, 1.0, size=dst.size))
+    e = SimpleNamespace(n=n, m=dst.size, offsets=off, targets=dst.astype(np.uint32), weights=w)
+    runs_per_bin = bin_runs(e)
+    assert runs_per_bin.max() > 65536  # at least two accumulate items in one bin
+    tol = 1e-12
+    ref = oracle_pull(oracle, e, 1, tol, 300)
+    b = runs(ctx_b, ctx_p, e, 1, "bf16", ga.PageRankConfig(0.85, 300, tol))
+    check(b, ref, tol)
+    assert ctx_b.last_stats()["sweeps_fp64"] >= 3
+    dg = ga.DeviceGraph.from_edges(e, weight_storage="bf16", ctx=ctx_b)
+    ctx_b.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_PULL)
+    try:
+        pa = ga.pagerank_weighted_run(dg, ga.PageRankConfig(0.85, 300, tol), ctx=ctx_b)
+    finally:
+        ctx_b.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_AUTO)
+    dg.close()
+    assert pa.iterations == b.iterations
+    assert np.abs(pa.scores - b.scores).sum() <= 1e-13
