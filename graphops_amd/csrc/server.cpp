@@ -869,18 +869,29 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
   st = tile_plan(ctx, g, c, s->T, &plan);
   if (st != GOPS_OK) return st;
   const int n = (int)g->dev.n;
-  // 1. admission (host state is current: no kernel is queued), by groups of
-  // kLaneQ slots, the queries one lane gathers for: a group takes new queries
-  // once none of its own is left, so a lane's queries start together and
-  // sweep in the same precision.  The first queued query opens the group; its
-  // other slots take the next queued queries of the same damping (which sets
-  // the iteration count most) found within kMatchWindow entries, else the
-  // next in order.  The queue head is always admitted first.
+  // 1. admission (host state is current: no kernel is queued).  With the
+  // default all-f32 schedule every free slot takes the queue head.  With a
+  // leading f64 phase (anchored admissions), by groups of kLaneQ slots, the
+  // queries one lane gathers for: a group takes new queries once none of its
+  // own is left, so a lane's queries start together and sweep in the same
+  // precision.  The first queued query opens the group; its other slots take
+  // the next queued queries of the same damping (which sets the iteration
+  // count most) found within kMatchWindow entries, else the next in order.
+  // The queue head is always admitted first.
   constexpr size_t kMatchWindow = 64;
+  const bool by_group = ctx->tile_f32_below > 0.0;
   for (Tile *t : s->tiles) {
     if (s->queue.empty()) break;
     uint64_t mask = 0;
-    for (int g0 = 0; g0 < t->T && !s->queue.empty(); g0 += kLaneQ) {
+    for (int q = 0; q < t->T && !by_group && !s->queue.empty(); q++) {
+      if (t->used[q]) continue;
+      t->slot[q] = std::move(s->queue.front());
+      s->queue.pop_front();
+      t->used[q] = true;
+      set_slot_params(t, q, t->slot[q], ctx->tile_f32_below, (uint64_t)std::max(2, ctx->ppr_anchor_period));
+      mask |= 1ull << q;
+    }
+    for (int g0 = 0; g0 < t->T && by_group && !s->queue.empty(); g0 += kLaneQ) {
       bool group_free = true;
       for (int q = g0; q < g0 + kLaneQ; q++) group_free = group_free && !t->used[q];
       if (!group_free) continue;
