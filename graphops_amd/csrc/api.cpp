@@ -256,8 +256,8 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   RunBuffers &rb = g->run;
   SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0, 0, 0};
   // f32 delta sweeps in the blocked layout (built once per graph): when chosen,
-  // or by default from 2^24 nodes, where the pull's source vector (64 MB of
-  // f32) is far beyond the L2 and the blocked passes measured faster.  By
+  // or by default from 2^21 nodes, where the pull's source vector (8 MB of
+  // f32 and more) outgrows the L2 and the blocked passes measured faster.  By
   // default, a run whose layout does not fit in free device memory keeps the
   // pull and says so in its stats; a later run tries again.
   const bool auto_blocked = ctx->delta_sweep == GOPS_DELTA_SWEEP_AUTO && n >= kPbAutoNodes;
@@ -584,6 +584,11 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
         return fail(ctx, GOPS_INVALID_PARAMETER,
                     "anchor sweep must be GOPS_ANCHOR_SWEEP_AUTO or _PULL");
       ctx->anchor_sweep = (int)value;
+      return GOPS_OK;
+    case GOPS_OPT_TILE_DELTA:
+      if (value != GOPS_TILE_DELTA_F32 && value != GOPS_TILE_DELTA_F16)
+        return fail(ctx, GOPS_INVALID_PARAMETER, "tile delta must be GOPS_TILE_DELTA_F32 or _F16");
+      ctx->tile_delta = (int)value;
       return GOPS_OK;
     case GOPS_OPT_DELTA_SWEEP:
       if (value != GOPS_DELTA_SWEEP_AUTO && value != GOPS_DELTA_SWEEP_PULL &&

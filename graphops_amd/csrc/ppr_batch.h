@@ -34,6 +34,10 @@ struct TileState {
   double diff[kMaxTile];
   double diff_anchor[kMaxTile];
   double x_scale[kMaxTile];
+  // f16 delta storage: the power-of-two scale of the stored deltas the next
+  // sweep gathers (ysc_in) and of the ones it writes (ysc_out); 1 for f32
+  double ysc_in[kMaxTile];
+  double ysc_out[kMaxTile];
   uint64_t iter[kMaxTile];
   uint64_t max_iter[kMaxTile];
   uint64_t since_anchor[kMaxTile];
@@ -52,12 +56,14 @@ struct PprArgs {
   const float *inv32;   // 1/out-degree (0 = dangling), new ids
   const double *inv64;
   double *x;            // [n][T] iterate
-  float *yd0, *yd1;     // [n][T] prescaled deltas (f32), ping-pong
-  float *carry;         // [n][T] what the row's last f32 delta rounded off, added
+  void *yd0, *yd1;      // [n][T] prescaled deltas, ping-pong: f32, or f16 times the
+                        // query's scale (delta16)
+  float *carry;         // [n][T] what the row's last stored delta rounded off, added
                         // to its next one (so the sums the sweeps propagate lag
                         // the f64 deltas by one rounding, not by all of them)
-  const float *yin32;   // picked per sweep
-  float *yout32;
+  const void *yin;      // picked per sweep
+  void *yout;
+  int delta16;          // storage of yd0 / yd1: 0 f32, 1 f16 (GOPS_OPT_TILE_DELTA)
   double *y64a, *y64b;  // [n][T] prescaled iterate for anchor sweeps, ping-pong
   double *partials;     // [grid][T][5] per sweep workgroup, then [kTileReduce][T][5]
                         // after the first reduction stage (at kMaxSweepBlocks * T * 5)

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.h"
 #include "ppr_batch.h"
@@ -62,12 +63,84 @@ struct Red {
 
 }  // namespace
 
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+
+// Storage of the prescaled deltas a sweep gathers.  f32 as is; f16 times a
+// per-query power of two s (TileState::ysc_in / ysc_out), chosen from the
+// query's previous residual R so that every stored value, at most about
+// 3 R in magnitude, stays below the f16 range, clamped there all the same.
+// What a value rounds off (or loses to the clamp or to f16's underflow) goes
+// into the row's carry, so it reaches the next sweep instead of being lost.
+template <typename DT>
+struct DeltaStore;
+
+template <>
+struct DeltaStore<float> {
+  __device__ static float put(double v, double) { return (float)v; }
+  __device__ static double back(float y, double) { return (double)y; }
+  __device__ static float4 load4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+  __device__ static void zero4(float *p) { *reinterpret_cast<float4 *>(p) = float4{0.f, 0.f, 0.f, 0.f}; }
+};
+
+template <>
+struct DeltaStore<_Float16> {
+  __device__ static _Float16 put(double v, double s) {
+    const float f = (float)(v * s);  // s is a power of two: exact
+    return (_Float16)fminf(fmaxf(f, -65504.0f), 65504.0f);
+  }
+  __device__ static double back(_Float16 y, double s) { return (double)(float)y / s; }
+  __device__ static float4 load4(const _Float16 *p) {
+    const half4_t h = *reinterpret_cast<const half4_t *>(p);
+    return float4{(float)h.x, (float)h.y, (float)h.z, (float)h.w};
+  }
+  __device__ static void zero4(_Float16 *p) { *reinterpret_cast<uint2 *>(p) = uint2{0u, 0u}; }
+};
+
+// The f16 scale for values bounded by about 3 R: 2^(14 - e) with R < 2^e,
+// so that they stay below 3 * 2^14 = 49152 (f16's largest finite value is
+// 65504) and their relative rounding is f16's 2^-11 down to 2^-28 R.
+__device__ __forceinline__ double f16_scale(double R) {
+  int e = 0;
+  if (!(R > 0.0) || !isfinite(R)) return 16384.0;
+  (void)frexp(R, &e);  // R < 2^e
+  e = e < -900 ? -900 : (e > 900 ? 900 : e);
+  return ldexp(1.0, 14 - e);
+}
+
+// What a sweep's epilogue needs of its query, read from the tile state once
+// per thread (lane q of the epilogue always handles query q): reloading it
+// for every row would put a chain of global loads on each row's path.
+struct QState {
+  bool live, first, f64_phase;
+  int c64;  // this sweep's anchors gather y64[c64]; the next gathers the other
+  double d, s_in, s_out, x_scale, dsum_x, dsum_in, mass_in;
+};
+
+template <typename DT>
+__device__ __forceinline__ QState load_qstate(const TileState *ts, int q) {
+  QState z;
+  z.live = ts->live[q] != 0;
+  z.first = ts->iter[q] == 0;
+  z.f64_phase = ts->f64_phase[q] != 0;
+  z.c64 = ts->y64_par;
+  z.d = ts->damping[q];
+  const bool scaled = !std::is_same<DT, float>::value;
+  z.s_in = scaled ? ts->ysc_in[q] : 1.0;
+  z.s_out = scaled ? ts->ysc_out[q] : 1.0;
+  z.x_scale = ts->x_scale[q];
+  z.dsum_x = ts->dsum_x[q];
+  z.dsum_in = ts->dsum_in[q];
+  z.mass_in = ts->mass_in[q];
+  return z;
+}
+
 // Per (row, query) epilogue.  mode: kModeDelta / kModeDeltaFold / kModeAnchor.
-template <int T>
-__device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *ts, int mode,
+// acc is the row's gathered sum as stored (delta modes: times ysc_in).
+template <int T, typename DT>
+__device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs, int mode,
                                              int r, int q, double acc, Red &red) {
-  if (!ts->live[q]) return;
-  const double d = ts->damping[q];
+  if (!qs.live) return;
+  const double d = qs.d;
   double seed_p = 0.0;
   const int sidx = a.seed_idx[r];
   if (sidx >= 0) {
@@ -75,49 +148,53 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const TileState *
       if (a.seed_q[j] == q) seed_p += a.seed_p[j];
   }
   const size_t at = (size_t)r * T + q;
-  const int c64 = ts->y64_par;  // this sweep's anchors gather y64[c64]; the next gathers the other
+  const int c64 = qs.c64;
+  DT *yout = static_cast<DT *>(a.yout);
+  const double s_out = qs.s_out;
   // every path sets all five contributions; one update of red at the end
   double delta, dang_x = 0.0, sum_x = 0.0;
   bool dangling;
   if (mode == kModeAnchor) {
-    const double sc = ts->x_scale[q];
-    const double dsum = ts->dsum_x[q] * sc;
+    const double sc = qs.x_scale;
+    const double dsum = qs.dsum_x * sc;
     const double t = sc * acc + ((1.0 - d) * seed_p + (d * dsum) * seed_p);  // ppr.rs:126-127
     delta = t - sc * a.x[at];
     a.x[at] = t;
     const double inv = a.inv64[r];
     const double v = (d * delta) * inv;  // the delta chain restarts here: no carry
-    const float yo = (float)v;
-    a.yout32[at] = yo;
-    a.carry[at] = (float)(v - (double)yo);
-    if (ts->f64_phase[q]) (c64 ? a.y64a : a.y64b)[at] = (d * t) * inv;  // next anchor's input
+    const DT yo = DeltaStore<DT>::put(v, s_out);
+    yout[at] = yo;
+    a.carry[at] = (float)(v - DeltaStore<DT>::back(yo, s_out));
+    if (qs.f64_phase) (c64 ? a.y64a : a.y64b)[at] = (d * t) * inv;  // next anchor's input
     dangling = inv == 0.0;
     dang_x = dangling ? t : 0.0;
   } else {
     // delta' = A delta: the dangling and (removed) total sums go to the seeds
-    delta = acc + d * (ts->dsum_in[q] - ts->mass_in[q]) * seed_p;
+    if (!std::is_same<DT, float>::value) acc /= qs.s_in;  // a power of two: exact
+    delta = acc + d * (qs.dsum_in - qs.mass_in) * seed_p;
     const double inv = a.inv64[r];
     // A query's first sweep (admitted with p-hat as its input delta) starts
-    // from x = p-hat and the carry of p-hat's f32 rounding on its seeds, 0
-    // elsewhere: neither column is read (nor cleared at admission) for it.
-    const bool first = ts->iter[q] == 0;
+    // from x = p-hat and the carry of p-hat's rounding on its seeds (stored
+    // by the admission at scale ysc_in), 0 elsewhere: neither column is read
+    // (nor cleared at admission) for it.
+    const bool first = qs.first;
     double x_old, c_old;
     if (first) {
       x_old = seed_p;
       const double v0 = (d * seed_p) * inv;
-      c_old = v0 - (double)(float)v0;
+      c_old = v0 - DeltaStore<DT>::back(DeltaStore<DT>::put(v0, qs.s_in), qs.s_in);
     } else {
       x_old = __builtin_nontemporal_load(a.x + at);
       c_old = (double)a.carry[at];
     }
     const double x = x_old + delta;
     __builtin_nontemporal_store(x, a.x + at);
-    // the f32 delta carries the rounding of the row's previous one: summed
-    // over the sweeps, the roundings cancel but for the last
+    // the stored delta carries the rounding of the row's previous one:
+    // summed over the sweeps, the roundings cancel but for the last
     const double v = (d * delta) * inv + c_old;
-    const float yo = (float)v;
-    a.yout32[at] = yo;
-    a.carry[at] = (float)(v - (double)yo);
+    const DT yo = DeltaStore<DT>::put(v, s_out);
+    yout[at] = yo;
+    a.carry[at] = (float)(v - DeltaStore<DT>::back(yo, s_out));
     dangling = inv == 0.0;
     if (mode == kModeDeltaFold) {
       (c64 ? a.y64a : a.y64b)[at] = (d * x) * inv;  // the anchor that follows
@@ -179,13 +256,12 @@ struct LaneModes {
   bool live;            // some query of the lane is live (else the lane gathers nothing)
 };
 
-template <int T>
-__device__ __forceinline__ void add_f32(const PprArgs &a, int c, int q0, Q4 &acc) {
-  const float4 f = *reinterpret_cast<const float4 *>(a.yin32 + (size_t)c * T + q0);
-  acc.v0 += (double)f.x;
-  acc.v1 += (double)f.y;
-  acc.v2 += (double)f.z;
-  acc.v3 += (double)f.w;
+// acc += f in edge order, or nothing (f belongs to a missing edge)
+__device__ __forceinline__ void add4(Q4 &acc, const float4 &f, bool have) {
+  acc.v0 = have ? acc.v0 + (double)f.x : acc.v0;
+  acc.v1 = have ? acc.v1 + (double)f.y : acc.v1;
+  acc.v2 = have ? acc.v2 + (double)f.z : acc.v2;
+  acc.v3 = have ? acc.v3 + (double)f.w : acc.v3;
 }
 
 template <int T>
@@ -199,11 +275,11 @@ __device__ __forceinline__ void add_f64(const double *y64, int c, int q0, Q4 &ac
 }
 
 // queries of the lane in different modes: each takes its own vector
-template <int T>
-__device__ __forceinline__ void add_mixed(const PprArgs &a, const double *y64, const LaneModes &m,
+template <int T, typename DT>
+__device__ __forceinline__ void add_mixed(const DT *yin, const double *y64, const LaneModes &m,
                                           int c, int q0, Q4 &acc) {
   const size_t off = (size_t)c * T + q0;
-  const float4 f = *reinterpret_cast<const float4 *>(a.yin32 + off);
+  const float4 f = DeltaStore<DT>::load4(yin + off);
   const double2 lo = *reinterpret_cast<const double2 *>(y64 + off);
   const double2 hi = *reinterpret_cast<const double2 *>(y64 + off + 2);
   acc.v0 += m.a0 ? lo.x : (double)f.x;
@@ -213,24 +289,33 @@ __device__ __forceinline__ void add_mixed(const PprArgs &a, const double *y64, c
 }
 
 // Sums of one lane's slice of a row: edges b, b + step, ... < e, added in
-// edge order.  f32 lanes keep four edges' loads in flight, f64 and mixed
+// edge order.  Delta lanes keep four edges' loads in flight, f64 and mixed
 // lanes two.
-template <int T>
-__device__ __forceinline__ Q4 row_slice_sum(const PprArgs &a, const double *y64,
+template <int T, typename DT>
+__device__ __forceinline__ Q4 row_slice_sum(const PprArgs &a, const DT *yin, const double *y64,
                                             const LaneModes &m, int b, int e, int step, int q0) {
   Q4 acc;
   if (!m.live) return acc;  // stopped queries and free slots: no loads
   int k = b;
   if (!m.need64) {
-    for (; k + 3 * step < e; k += 4 * step) {
-      const int c0 = a.col[k], c1 = a.col[k + step], c2 = a.col[k + 2 * step],
-                c3 = a.col[k + 3 * step];
-      add_f32<T>(a, c0, q0, acc);
-      add_f32<T>(a, c1, q0, acc);
-      add_f32<T>(a, c2, q0, acc);
-      add_f32<T>(a, c3, q0, acc);
+    // batches of four edges, the last one partial: every load of a batch is
+    // issued before the first add (a slice of a short row would otherwise
+    // wait out two dependent loads per edge); a missing edge repeats the
+    // batch's first one, and its value is not added
+    for (; k < e; k += 4 * step) {
+      const int k1 = k + step, k2 = k + 2 * step, k3 = k + 3 * step;
+      const bool h1 = k1 < e, h2 = k2 < e, h3 = k3 < e;
+      const int c0 = a.col[k];
+      const int c1 = a.col[h1 ? k1 : k], c2 = a.col[h2 ? k2 : k], c3 = a.col[h3 ? k3 : k];
+      const float4 f0 = DeltaStore<DT>::load4(yin + (size_t)c0 * T + q0);
+      const float4 f1 = DeltaStore<DT>::load4(yin + (size_t)c1 * T + q0);
+      const float4 f2 = DeltaStore<DT>::load4(yin + (size_t)c2 * T + q0);
+      const float4 f3 = DeltaStore<DT>::load4(yin + (size_t)c3 * T + q0);
+      add4(acc, f0, true);
+      add4(acc, f1, h1);
+      add4(acc, f2, h2);
+      add4(acc, f3, h3);
     }
-    for (; k < e; k += step) add_f32<T>(a, a.col[k], q0, acc);
   } else if (!m.need32) {
     for (; k + step < e; k += 2 * step) {
       const int c0 = a.col[k], c1 = a.col[k + step];
@@ -241,10 +326,10 @@ __device__ __forceinline__ Q4 row_slice_sum(const PprArgs &a, const double *y64,
   } else {
     for (; k + step < e; k += 2 * step) {
       const int c0 = a.col[k], c1 = a.col[k + step];
-      add_mixed<T>(a, y64, m, c0, q0, acc);
-      add_mixed<T>(a, y64, m, c1, q0, acc);
+      add_mixed<T>(yin, y64, m, c0, q0, acc);
+      add_mixed<T>(yin, y64, m, c1, q0, acc);
     }
-    for (; k < e; k += step) add_mixed<T>(a, y64, m, a.col[k], q0, acc);
+    for (; k < e; k += step) add_mixed<T>(yin, y64, m, a.col[k], q0, acc);
   }
   return acc;
 }
@@ -288,15 +373,16 @@ __device__ __forceinline__ Q4 block_slices_sum(Q4 v) {
 // Row sums leave the gather phase in the slice-0 lanes, four queries each;
 // they go through LDS so that the epilogue runs one query per lane (lane
 // q = tid % T always handles query q), with one query's state in registers.
-template <int T>
+template <int T, typename DT>
 __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
   constexpr int Q = T / kLaneQ;  // lanes per edge
   __shared__ double rs[kBlock * kLaneQ];  // row sums of a pass: rows (<= kBlock / Q) x T
   const TileState *ts = a.ts;
   if (ts->all_done) return;
   const int ycur = ts->yd_cur;
-  a.yin32 = ycur ? a.yd1 : a.yd0;
-  a.yout32 = ycur ? a.yd0 : a.yd1;
+  a.yin = ycur ? a.yd1 : a.yd0;
+  a.yout = ycur ? a.yd0 : a.yd1;
+  const DT *yin = static_cast<const DT *>(a.yin);
   const double *y64 = ts->y64_par ? a.y64b : a.y64a;
   // gather phase: queries q0 .. q0 + 3 of this lane
   const int qg = threadIdx.x % Q;
@@ -312,6 +398,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
   // epilogue phase: query q of this lane
   const int q = threadIdx.x % T;
   const int mode = ts->mode[q];
+  const QState qs = load_qstate<DT>(ts, q);
   int lq = 0;
   while ((1 << lq) < Q) lq++;
   Red red;
@@ -328,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
         const int r = base + g;
         Q4 acc;
         if (r < bd.z)
-          acc = row_slice_sum<T>(a, y64, m, a.row_ptr[r] + slice, a.row_ptr[r + 1], 1 << ls, q0);
+          acc = row_slice_sum<T>(a, yin, y64, m, a.row_ptr[r] + slice, a.row_ptr[r + 1], 1 << ls, q0);
         q4_lane_sum(acc, lq, lq + ls < 6 ? lq + ls : 6);  // Q << ls <= 64
         if (slice == 0) {
           double *o = rs + g * T + q0;
@@ -340,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
         __syncthreads();
         const int rows = bd.z - base < ngrp ? bd.z - base : ngrp;
         for (int p = threadIdx.x; p < rows * T; p += kBlock)
-          ppr_epilogue<T>(a, ts, mode, base + p / T, q, rs[p], red);
+          ppr_epilogue<T, DT>(a, qs, mode, base + p / T, q, rs[p], red);
         __syncthreads();
       }
     } else {
@@ -355,7 +442,8 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
         b = bd.z;
         e = bd.w;
       }
-      const Q4 acc = block_slices_sum<T>(row_slice_sum<T>(a, y64, m, b + slice, e, kBlock / Q, q0));
+      const Q4 acc =
+          block_slices_sum<T>(row_slice_sum<T>(a, yin, y64, m, b + slice, e, kBlock / Q, q0));
       if (threadIdx.x < Q) {
         double *o = rs + q0;
         o[0] = acc.v0;
@@ -366,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
       __syncthreads();
       if (threadIdx.x < T) {
         if (r >= 0) {
-          ppr_epilogue<T>(a, ts, mode, r, q, rs[q], red);
+          ppr_epilogue<T, DT>(a, qs, mode, r, q, rs[q], red);
         } else {
           const int h = bd.y, chunk = bd.x & (kHubFlag - 1);
           a.hub_partial[(size_t)(a.hub_first[h] + chunk) * T + q] = rs[q];
@@ -397,15 +485,16 @@ __global__ __launch_bounds__(kBlock) void ppr_reduce_kernel(PprArgs a, int sweep
 // Per-query bookkeeping (pagerank.rs:163-179 for every query) and each
 // query's next mode (the single-vector schedule of pagerank.hip).  One
 // workgroup.
-template <int T>
+template <int T, typename DT>
 __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep_blocks) {
   TileState *ts = a.ts;
   if (ts->all_done) return;
   const int ycur = ts->yd_cur;
-  a.yin32 = ycur ? a.yd1 : a.yd0;
-  a.yout32 = ycur ? a.yd0 : a.yd1;
+  a.yin = ycur ? a.yd1 : a.yd0;
+  a.yout = ycur ? a.yd0 : a.yd1;
   const int q = threadIdx.x % T;
   const int mode = ts->mode[q];
+  const QState qs = load_qstate<DT>(ts, q);
   Red red;
   // hub rows: (hub, query) pairs; thread t always gets query t % T
   for (int p = threadIdx.x; p < a.num_hubs * T; p += kBlock) {
@@ -413,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
     const int first = a.hub_first[h], nch = a.hub_nchunks[h];
     double s = 0.0;
     for (int c = 0; c < nch; c++) s += a.hub_partial[(size_t)(first + c) * T + q];
-    ppr_epilogue<T>(a, ts, mode, a.hub_row[h], q, s, red);
+    ppr_epilogue<T, DT>(a, qs, mode, a.hub_row[h], q, s, red);
   }
   const double *stage = a.partials + (size_t)kMaxSweepBlocks * T * 5;
   for (int p = threadIdx.x; p < kTileReduce * T; p += kBlock) {
@@ -462,6 +551,12 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
       next = anchor ? kModeDeltaFold : kModeDelta;
     }
     ts->mode[qq] = next;
+    if (!std::is_same<DT, float>::value) {
+      // the next sweep gathers what this one stored, and stores at the
+      // scale of this sweep's residual
+      ts->ysc_in[qq] = ts->ysc_out[qq];
+      ts->ysc_out[qq] = f16_scale(diff);
+    }
     const uint64_t it = ts->iter[qq] + 1;
     ts->iter[qq] = it;
     ts->diff[qq] = diff;
@@ -496,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
 // the next sweep gathers, and, for an anchor admission, of x and of the y64
 // buffer the next sweep's anchors gather.  One thread per row and group of four queries;
 // the tile's other queries are not touched.
-template <int T>
+template <int T, typename DT>
 __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uint64_t clear_mask) {
   constexpr int Q = T / kLaneQ;
   const int qg = threadIdx.x % Q;
@@ -508,21 +603,20 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
   // y64 column before any anchor reads it.
   const bool anchored = a.ts->f32_below > 0.0;
   double *y64 = anchored ? (a.ts->y64_par ? a.y64b : a.y64a) : nullptr;
-  float *yin = a.ts->yd_cur ? a.yd1 : a.yd0;
+  DT *yin = static_cast<DT *>(a.ts->yd_cur ? a.yd1 : a.yd0);
   const int rows_per_block = kBlock / Q;
   for (int r = blockIdx.x * rows_per_block + threadIdx.x / Q; r < n;
        r += gridDim.x * rows_per_block) {
     const size_t at = (size_t)r * T + qg * kLaneQ;
     if (m4 == 0xFu) {
       const double2 z = {0.0, 0.0};
-      const float4 zf = {0.0f, 0.0f, 0.0f, 0.0f};
       if (anchored) {
         *reinterpret_cast<double2 *>(a.x + at) = z;
         *reinterpret_cast<double2 *>(a.x + at + 2) = z;
         *reinterpret_cast<double2 *>(y64 + at) = z;
         *reinterpret_cast<double2 *>(y64 + at + 2) = z;
       }
-      *reinterpret_cast<float4 *>(yin + at) = zf;
+      DeltaStore<DT>::zero4(yin + at);
     } else {
 #pragma unroll
       for (int j = 0; j < kLaneQ; j++) {
@@ -531,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
             a.x[at + j] = 0.0;
             y64[at + j] = 0.0;
           }
-          yin[at + j] = 0.0f;
+          yin[at + j] = (DT)0.0f;
         }
       }
     }
@@ -544,12 +638,15 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
 // anchor sweep over y64 = d p-hat / deg.  Without, it is a delta sweep whose
 // input delta is p-hat itself: d A p-hat + d (dangling(p-hat) - sum(p-hat))
 // p-hat is T(p-hat) - p-hat, as sum(p-hat) = 1 (ppr.rs:125-137).
-template <int T>
+template <int T, typename DT>
 __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t admit_mask) {
   TileState *ts = a.ts;
   const bool anchored = ts->f32_below > 0.0;
   double *y64 = ts->y64_par ? a.y64b : a.y64a;
-  float *yin = ts->yd_cur ? a.yd1 : a.yd0;
+  DT *yin = static_cast<DT *>(ts->yd_cur ? a.yd1 : a.yd0);
+  // p-hat sums to 1: its deltas are at most 1, their first successors at
+  // most 3 d (the scale of a residual of 1)
+  const double s0 = std::is_same<DT, float>::value ? 1.0 : f16_scale(1.0);
   for (int j = threadIdx.x; j < a.num_seed_entries; j += kBlock) {
     const int qq = a.seed_q[j];
     if (!((admit_mask >> qq) & 1ull)) continue;
@@ -561,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
       a.x[at] = p;  // one entry per (row, query): the host merges duplicate seeds
       y64[at] = v;
     } else {
-      yin[at] = (float)v;  // x and the carry: ppr_epilogue's first sweep
+      yin[at] = DeltaStore<DT>::put(v, s0);  // x and the carry: ppr_epilogue's first sweep
     }
   }
   __syncthreads();
@@ -570,6 +667,8 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
     if ((admit_mask >> qq) & 1ull) {
       ts->dsum_x[qq] = ts->init_dsum[qq];
       ts->x_scale[qq] = 1.0;
+      ts->ysc_in[qq] = s0;
+      ts->ysc_out[qq] = s0;
       ts->live[qq] = 1;
       ts->iter[qq] = 0;
       ts->converged[qq] = 0;
@@ -624,42 +723,54 @@ __global__ void seed_mark_kernel(int32_t *seed_idx, const int32_t *rows, int cou
 }
 
 // ---------------------------------------------------------------------------
-template <int T>
+template <int T, typename DT>
 static hipError_t launch_sweep_t(hipStream_t s, const PprArgs &a, int grid) {
-  ppr_sweep_kernel<T><<<grid, kBlock, 0, s>>>(a);
+  ppr_sweep_kernel<T, DT><<<grid, kBlock, 0, s>>>(a);
   ppr_reduce_kernel<T><<<kTileReduce, kBlock, 0, s>>>(a, grid);
-  ppr_finish_kernel<T><<<1, kBlock, 0, s>>>(a, grid);
+  ppr_finish_kernel<T, DT><<<1, kBlock, 0, s>>>(a, grid);
   return hipGetLastError();
 }
 
-hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid) {
+template <typename DT>
+static hipError_t launch_sweep_dt(hipStream_t s, int T, const PprArgs &a, int grid) {
   switch (T) {
-    case 8: return launch_sweep_t<8>(s, a, grid);
-    case 16: return launch_sweep_t<16>(s, a, grid);
-    case 32: return launch_sweep_t<32>(s, a, grid);
-    default: return launch_sweep_t<64>(s, a, grid);
+    case 8: return launch_sweep_t<8, DT>(s, a, grid);
+    case 16: return launch_sweep_t<16, DT>(s, a, grid);
+    case 32: return launch_sweep_t<32, DT>(s, a, grid);
+    default: return launch_sweep_t<64, DT>(s, a, grid);
   }
 }
 
-template <int T>
+hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid) {
+  return a.delta16 ? launch_sweep_dt<_Float16>(s, T, a, grid) : launch_sweep_dt<float>(s, T, a, grid);
+}
+
+template <int T, typename DT>
 static hipError_t launch_admit_t(hipStream_t s, const PprArgs &a, int n, uint64_t clear_mask,
                                  uint64_t admit_mask) {
   const int rows_per_block = kBlock / (T / kLaneQ);
   int grid = (n + rows_per_block - 1) / rows_per_block;
   grid = std::max(1, std::min(grid, 4096));
-  ppr_clear_kernel<T><<<grid, kBlock, 0, s>>>(a, n, clear_mask);
-  ppr_admit_kernel<T><<<1, kBlock, 0, s>>>(a, admit_mask);
+  ppr_clear_kernel<T, DT><<<grid, kBlock, 0, s>>>(a, n, clear_mask);
+  ppr_admit_kernel<T, DT><<<1, kBlock, 0, s>>>(a, admit_mask);
   return hipGetLastError();
+}
+
+template <typename DT>
+static hipError_t launch_admit_dt(hipStream_t s, int T, const PprArgs &a, int n,
+                                  uint64_t clear_mask, uint64_t admit_mask) {
+  switch (T) {
+    case 8: return launch_admit_t<8, DT>(s, a, n, clear_mask, admit_mask);
+    case 16: return launch_admit_t<16, DT>(s, a, n, clear_mask, admit_mask);
+    case 32: return launch_admit_t<32, DT>(s, a, n, clear_mask, admit_mask);
+    default: return launch_admit_t<64, DT>(s, a, n, clear_mask, admit_mask);
+  }
 }
 
 hipError_t ppr_launch_admit(hipStream_t s, int T, const PprArgs &a, int n, uint64_t clear_mask,
                             uint64_t admit_mask) {
-  switch (T) {
-    case 8: return launch_admit_t<8>(s, a, n, clear_mask, admit_mask);
-    case 16: return launch_admit_t<16>(s, a, n, clear_mask, admit_mask);
-    case 32: return launch_admit_t<32>(s, a, n, clear_mask, admit_mask);
-    default: return launch_admit_t<64>(s, a, n, clear_mask, admit_mask);
-  }
+  return a.delta16 ? launch_admit_dt<_Float16>(s, T, a, n, clear_mask, admit_mask)
+                   : launch_admit_dt<float>(s, T, a, n, clear_mask, admit_mask);
 }
 
 hipError_t ppr_launch_gather(hipStream_t s, int T, const PprArgs &a, const uint32_t *rank, int n,

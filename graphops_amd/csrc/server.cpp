@@ -125,7 +125,9 @@ struct Tile {
   size_t n = 0;
   double *x = nullptr, *y64a = nullptr, *y64b = nullptr, *partials = nullptr,
          *hub_partial = nullptr;
-  float *yd0 = nullptr, *yd1 = nullptr, *carry = nullptr;
+  void *yd0 = nullptr, *yd1 = nullptr;  // f32 or f16 (delta16)
+  float *carry = nullptr;
+  int delta16 = 0;
   int32_t *seed_idx = nullptr;
   int32_t *seed_row = nullptr, *seed_q = nullptr;
   double *seed_p = nullptr;
@@ -152,6 +154,7 @@ struct gops_server {
   gops_graph *g = nullptr;
   bool holds_graph = false;  // a reference on g (taken once creation succeeds)
   int T = 16;
+  int delta16 = 0;  // the tiles' delta storage, fixed at creation (GOPS_OPT_TILE_DELTA)
   std::vector<Tile *> tiles;
   std::deque<Query> queue;
   struct Done {
@@ -186,8 +189,9 @@ struct PprGraphCache {
   std::vector<uint8_t> dangling;  // new ids, out-degree 0
   std::vector<int32_t> hub_row, hub_nchunks;
   TilePlan plans[4];
-  int tuned_T = 0;          // autotuned tile width (0: not measured yet)
-  double tile_cost[4] = {}; // measured ms per query-sweep of widths 8/16/32/64
+  int tuned_T[2] = {};          // autotuned tile width per delta storage (f32, f16; 0: not
+                                // measured yet)
+  double tile_cost[2][4] = {};  // measured ms per query-sweep of widths 8/16/32/64
 };
 
 }  // namespace gops
@@ -262,15 +266,21 @@ void free_tile(Tile *t) {
   delete t;
 }
 
-size_t tile_bytes(size_t n, int T) { return n * (size_t)T * (8 + 8 + 8 + 4 + 4 + 4) + n * 4; }
+// x, y64a, y64b (f64), the two stored deltas (f32 or f16) and the carry (f32)
+// per (node, query), plus seed_idx per node
+size_t tile_bytes(size_t n, int T, int delta16) {
+  return n * (size_t)T * (8 + 8 + 8 + (delta16 ? 2 + 2 : 4 + 4) + 4) + n * 4;
+}
 
-gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile **out) {
+gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, int delta16, Tile **out) {
   auto *t = new (std::nothrow) Tile;
   if (!t) return sfail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
   t->T = T;
   t->n = g->dev.n;
   t->grid = grid;
+  t->delta16 = delta16;
   const size_t nt = t->n * (size_t)T;
+  const size_t yb = delta16 ? 2 : 4;  // bytes per stored delta
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) {
     if (r != hipSuccess && e == hipSuccess) e = r;
@@ -278,8 +288,8 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile *
   chk(hipMalloc((void **)&t->x, std::max<size_t>(nt, 1) * 8));
   chk(hipMalloc((void **)&t->y64a, std::max<size_t>(nt, 1) * 8));
   chk(hipMalloc((void **)&t->y64b, std::max<size_t>(nt, 1) * 8));
-  chk(hipMalloc((void **)&t->yd0, std::max<size_t>(nt, 1) * 4));
-  chk(hipMalloc((void **)&t->yd1, std::max<size_t>(nt, 1) * 4));
+  chk(hipMalloc(&t->yd0, std::max<size_t>(nt, 1) * yb));
+  chk(hipMalloc(&t->yd1, std::max<size_t>(nt, 1) * yb));
   chk(hipMalloc((void **)&t->carry, std::max<size_t>(nt, 1) * 4));
   chk(hipMalloc((void **)&t->partials, (size_t)(kMaxSweepBlocks + kTileReduce) * T * 5 * 8));
   chk(hipMalloc((void **)&t->hub_partial, (size_t)std::max(g->dev.num_hub_chunks, 1) * T * 8));
@@ -287,8 +297,8 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile *
   chk(hipMalloc((void **)&t->ts, sizeof(TileState)));
   chk(hipHostMalloc((void **)&t->hts, sizeof(TileState), hipHostMallocDefault));
   if (e == hipSuccess) chk(hipMemsetAsync(t->seed_idx, 0xff, std::max<size_t>(t->n, 1) * 4, ctx->stream));
-  if (e == hipSuccess) chk(hipMemsetAsync(t->yd0, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
-  if (e == hipSuccess) chk(hipMemsetAsync(t->yd1, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->yd0, 0, std::max<size_t>(nt, 1) * yb, ctx->stream));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->yd1, 0, std::max<size_t>(nt, 1) * yb, ctx->stream));
   if (e == hipSuccess) chk(hipMemsetAsync(t->carry, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
   if (e == hipSuccess) chk(hipStreamSynchronize(ctx->stream));
   if (e != hipSuccess) {
@@ -323,6 +333,7 @@ PprArgs tile_args(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, 
   a.x = t->x;
   a.yd0 = t->yd0;
   a.yd1 = t->yd1;
+  a.delta16 = t->delta16;
   a.carry = t->carry;
   a.y64a = t->y64a;
   a.y64b = t->y64b;
@@ -476,7 +487,7 @@ gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T
   gops_status st = tile_plan(ctx, g, c, T, &plan);
   if (st != GOPS_OK) return st;
   Tile *t = nullptr;
-  st = alloc_tile(ctx, g, T, tile_grid(g), &t);
+  st = alloc_tile(ctx, g, T, tile_grid(g), ctx->tile_delta == GOPS_TILE_DELTA_F16, &t);
   if (st == GOPS_OUT_OF_MEMORY) {
     ctx->last_error.clear();
     (void)hipGetLastError();  // the failed allocation's error is not a later launch's
@@ -536,24 +547,25 @@ gops_status pick_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int nq
     *T = ctx->ppr_tile;
     return GOPS_OK;
   }
-  if (!c->tuned_T) {
+  const int sk = ctx->tile_delta == GOPS_TILE_DELTA_F16;  // widths are tuned per storage
+  if (!c->tuned_T[sk]) {
     double best = std::numeric_limits<double>::infinity();
     int best_T = 0;
     for (int w : {8, 16, 32, 64}) {
       double cost;
       gops_status st = trial_tile(ctx, g, c, w, &cost);
       if (st != GOPS_OK) return st;
-      c->tile_cost[plan_slot(w)] = cost;
+      c->tile_cost[sk][plan_slot(w)] = cost;
       if (cost < best) {
         best = cost;
         best_T = w;
       }
     }
     if (!best_T) return sfail(ctx, GOPS_OUT_OF_MEMORY, "no tile width fits in device memory");
-    c->tuned_T = best_T;
+    c->tuned_T[sk] = best_T;
   }
   int w = 8;
-  while (w < nq && w < c->tuned_T) w <<= 1;
+  while (w < nq && w < c->tuned_T[sk]) w <<= 1;
   *T = w;
   return GOPS_OK;
 }
@@ -599,7 +611,7 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
   st = tile_plan(ctx, g, c, T, &plan);
   if (st != GOPS_OK) return st;
   Tile *t = nullptr;
-  st = alloc_tile(ctx, g, T, tile_grid(g), &t);
+  st = alloc_tile(ctx, g, T, tile_grid(g), ctx->tile_delta == GOPS_TILE_DELTA_F16, &t);
   if (st != GOPS_OK) return st;
   double *dout = nullptr;  // one query's scores in the caller's ids
   if (scores && hipMalloc((void **)&dout, n * sizeof(double)) != hipSuccess) {
@@ -762,7 +774,8 @@ gops_status gops_ppr_tile_info(gops_context *ctx, gops_graph *g, uint32_t nq, ui
   if (saved < 0) T = 0;
   if (tile) *tile = (uint32_t)T;
   if (ms_per_query_sweep)
-    for (int i = 0; i < 4; i++) ms_per_query_sweep[i] = c->tile_cost[i];
+    for (int i = 0; i < 4; i++)
+      ms_per_query_sweep[i] = c->tile_cost[ctx->tile_delta == GOPS_TILE_DELTA_F16][i];
   return GOPS_OK;
 }
 
@@ -790,7 +803,8 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
   size_t free_b = 0, total_b = 0;
   STRY(ctx, hipMemGetInfo(&free_b, &total_b));
   double budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : 0.9 * (double)free_b;
-  const size_t per_tile = tile_bytes(g->dev.n, T);
+  const int delta16 = ctx->tile_delta == GOPS_TILE_DELTA_F16;
+  const size_t per_tile = tile_bytes(g->dev.n, T, delta16);
   uint32_t ntiles = (uint32_t)std::floor(budget / (double)per_tile);
   if (cfg && cfg->max_slots) ntiles = std::min<uint32_t>(ntiles, (cfg->max_slots + T - 1) / T);
   if (ntiles == 0) return sfail(ctx, GOPS_OUT_OF_MEMORY, "HBM budget holds no query tile");
@@ -802,11 +816,12 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
   s->ctx = ctx;
   s->g = g;
   s->T = T;
+  s->delta16 = delta16;
   s->topk = cfg ? cfg->topk : 0;
   s->t0 = std::chrono::steady_clock::now();
   for (uint32_t i = 0; i < ntiles; i++) {
     Tile *t = nullptr;
-    st = alloc_tile(ctx, g, T, tile_grid(g), &t);
+    st = alloc_tile(ctx, g, T, tile_grid(g), delta16, &t);
     if (st != GOPS_OK) {
       if (i == 0) {
         delete s;
