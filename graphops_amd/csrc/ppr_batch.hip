@@ -134,15 +134,38 @@ __device__ __forceinline__ QState load_qstate(const TileState *ts, int q) {
   return z;
 }
 
+// The loads of one (row, query) epilogue, issued together for all of a
+// thread's rows before any of them is finished (ppr_epilogue).
+struct EpiLoad {
+  int sidx = -1;
+  double inv = 0.0, x = 0.0;
+  float c = 0.0f;
+};
+
+template <int T>
+__device__ __forceinline__ EpiLoad epi_load(const PprArgs &a, const QState &qs, int mode, int r,
+                                            int q) {
+  EpiLoad ld;
+  const size_t at = (size_t)r * T + q;
+  ld.sidx = a.seed_idx[r];
+  ld.inv = a.inv64[r];
+  // a query's first delta sweep reads neither x nor the carry (ppr_epilogue)
+  if (mode == kModeAnchor || !qs.first) ld.x = __builtin_nontemporal_load(a.x + at);
+  if (mode != kModeAnchor && !qs.first) ld.c = a.carry[at];
+  return ld;
+}
+
 // Per (row, query) epilogue.  mode: kModeDelta / kModeDeltaFold / kModeAnchor.
-// acc is the row's gathered sum as stored (delta modes: times ysc_in).
+// acc is the row's gathered sum as stored (delta modes: times ysc_in); ld
+// holds the row's loads (epi_load).
 template <int T, typename DT>
 __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs, int mode,
-                                             int r, int q, double acc, Red &red) {
+                                             int r, int q, double acc, const EpiLoad &ld,
+                                             Red &red) {
   if (!qs.live) return;
   const double d = qs.d;
   double seed_p = 0.0;
-  const int sidx = a.seed_idx[r];
+  const int sidx = ld.sidx;
   if (sidx >= 0) {
     for (int j = sidx; j < a.num_seed_entries && a.seed_row[j] == r; j++)
       if (a.seed_q[j] == q) seed_p += a.seed_p[j];
@@ -158,9 +181,9 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs,
     const double sc = qs.x_scale;
     const double dsum = qs.dsum_x * sc;
     const double t = sc * acc + ((1.0 - d) * seed_p + (d * dsum) * seed_p);  // ppr.rs:126-127
-    delta = t - sc * a.x[at];
+    delta = t - sc * ld.x;
     a.x[at] = t;
-    const double inv = a.inv64[r];
+    const double inv = ld.inv;
     const double v = (d * delta) * inv;  // the delta chain restarts here: no carry
     const DT yo = DeltaStore<DT>::put(v, s_out);
     yout[at] = yo;
@@ -172,7 +195,7 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs,
     // delta' = A delta: the dangling and (removed) total sums go to the seeds
     if (!std::is_same<DT, float>::value) acc /= qs.s_in;  // a power of two: exact
     delta = acc + d * (qs.dsum_in - qs.mass_in) * seed_p;
-    const double inv = a.inv64[r];
+    const double inv = ld.inv;
     // A query's first sweep (admitted with p-hat as its input delta) starts
     // from x = p-hat and the carry of p-hat's rounding on its seeds (stored
     // by the admission at scale ysc_in), 0 elsewhere: neither column is read
@@ -184,8 +207,8 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs,
       const double v0 = (d * seed_p) * inv;
       c_old = v0 - DeltaStore<DT>::back(DeltaStore<DT>::put(v0, qs.s_in), qs.s_in);
     } else {
-      x_old = __builtin_nontemporal_load(a.x + at);
-      c_old = (double)a.carry[at];
+      x_old = ld.x;
+      c_old = (double)ld.c;
     }
     const double x = x_old + delta;
     __builtin_nontemporal_store(x, a.x + at);
@@ -374,7 +397,8 @@ __device__ __forceinline__ Q4 block_slices_sum(Q4 v) {
 // they go through LDS so that the epilogue runs one query per lane (lane
 // q = tid % T always handles query q), with one query's state in registers.
 template <int T, typename DT>
-__global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void ppr_sweep_kernel(PprArgs a) {
   constexpr int Q = T / kLaneQ;  // lanes per edge
   __shared__ double rs[kBlock * kLaneQ];  // row sums of a pass: rows (<= kBlock / Q) x T
   const TileState *ts = a.ts;
@@ -411,11 +435,23 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
       const int g = threadIdx.x / L;
       const int slice = (threadIdx.x % L) / Q;
       const int ngrp = kBlock / L;
+      // each pass's row bounds are loaded one pass ahead
+      int rb = 0, re = 0;
+      if (bd.y + g < bd.z) {
+        rb = a.row_ptr[bd.y + g];
+        re = a.row_ptr[bd.y + g + 1];
+      }
       for (int base = bd.y; base < bd.z; base += ngrp) {
         const int r = base + g;
+        int nb = 0, ne = 0;
+        if (r + ngrp < bd.z) {
+          nb = a.row_ptr[r + ngrp];
+          ne = a.row_ptr[r + ngrp + 1];
+        }
         Q4 acc;
-        if (r < bd.z)
-          acc = row_slice_sum<T>(a, yin, y64, m, a.row_ptr[r] + slice, a.row_ptr[r + 1], 1 << ls, q0);
+        if (r < bd.z) acc = row_slice_sum<T>(a, yin, y64, m, rb + slice, re, 1 << ls, q0);
+        rb = nb;
+        re = ne;
         q4_lane_sum(acc, lq, lq + ls < 6 ? lq + ls : 6);  // Q << ls <= 64
         if (slice == 0) {
           double *o = rs + g * T + q0;
@@ -425,9 +461,20 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
           o[3] = acc.v3;
         }
         __syncthreads();
-        const int rows = bd.z - base < ngrp ? bd.z - base : ngrp;
-        for (int p = threadIdx.x; p < rows * T; p += kBlock)
-          ppr_epilogue<T, DT>(a, qs, mode, base + p / T, q, rs[p], red);
+        // at most kLaneQ (row, query) pairs per thread, all with query q:
+        // their loads first, then the epilogues
+        const int pairs = (bd.z - base < ngrp ? bd.z - base : ngrp) * T;
+        EpiLoad ld[kLaneQ];
+#pragma unroll
+        for (int j = 0; j < kLaneQ; j++) {
+          const int p = (int)threadIdx.x + j * kBlock;
+          if (p < pairs && qs.live) ld[j] = epi_load<T>(a, qs, mode, base + p / T, q);
+        }
+#pragma unroll
+        for (int j = 0; j < kLaneQ; j++) {
+          const int p = (int)threadIdx.x + j * kBlock;
+          if (p < pairs) ppr_epilogue<T, DT>(a, qs, mode, base + p / T, q, rs[p], ld[j], red);
+        }
         __syncthreads();
       }
     } else {
@@ -454,7 +501,8 @@ __global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
       __syncthreads();
       if (threadIdx.x < T) {
         if (r >= 0) {
-          ppr_epilogue<T, DT>(a, qs, mode, r, q, rs[q], red);
+          const EpiLoad ld = qs.live ? epi_load<T>(a, qs, mode, r, q) : EpiLoad{};
+          ppr_epilogue<T, DT>(a, qs, mode, r, q, rs[q], ld, red);
         } else {
           const int h = bd.y, chunk = bd.x & (kHubFlag - 1);
           a.hub_partial[(size_t)(a.hub_first[h] + chunk) * T + q] = rs[q];
@@ -502,7 +550,8 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
     const int first = a.hub_first[h], nch = a.hub_nchunks[h];
     double s = 0.0;
     for (int c = 0; c < nch; c++) s += a.hub_partial[(size_t)(first + c) * T + q];
-    ppr_epilogue<T, DT>(a, qs, mode, a.hub_row[h], q, s, red);
+    const EpiLoad ld = qs.live ? epi_load<T>(a, qs, mode, a.hub_row[h], q) : EpiLoad{};
+    ppr_epilogue<T, DT>(a, qs, mode, a.hub_row[h], q, s, ld, red);
   }
   const double *stage = a.partials + (size_t)kMaxSweepBlocks * T * 5;
   for (int p = threadIdx.x; p < kTileReduce * T; p += kBlock) {
