@@ -456,9 +456,17 @@ def f16_delta_storage(ga, oracle, e, dg, cfg, off, seeds, n_sample=16):
     del sc
     log(f"ppr f16 deltas: tile {tile}, {qps:.1f} q/s (L1 max {max(l1):.2e}, "
         f"{it_diff} of {n_sample} iteration counts differ)")
+    n, m = e.n, e.m
+    per_qsweep = 8 * n + (4 * m + 8 * n) / tile
+    its = float(np.mean([r.iterations for r in runs]))
+    roof_qps = HBM_PEAK_GBS * 1e9 / (its * per_qsweep)
     return {"storage": "f16 deltas, per-query power-of-two scale, carried rounding",
             "tile": tile, "tile_ms_per_query_sweep": {str(w): round(c, 5) for w, c in costs.items()},
             "queries_per_s": round(qps, 2),
+            "roofline": {"bound": "hbm", "frac": round(qps / roof_qps, 4),
+                         "bytes_per_query_sweep": round(per_qsweep),
+                         "traffic_per_tile_sweep": measured_traffic("rmat22-unit ppr tile64", "tile_f16")
+                         if tile == 64 else None},
             "iterations_mean": round(float(np.mean([r.iterations for r in runs])), 3),
             "converged": int(sum(r.converged for r in runs)),
             "accuracy": {"sample_queries": n_sample, "l1_vs_f64_oracle_max": max(l1),

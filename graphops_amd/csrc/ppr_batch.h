@@ -7,7 +7,15 @@
 namespace gops {
 
 constexpr int kMaxTile = 64;  // queries per tile, at most
-constexpr int kLaneQ = 4;     // queries per lane in the tile sweep (one 16-byte gather)
+constexpr int kLaneQ = 4;     // queries per refill group (and per lane with f32 deltas)
+
+// Queries per lane in the tile sweep.  The host's work descriptors depend
+// on it (Q = T / this lanes per edge).  With f16 deltas 8 per lane (one
+// 16-byte gather, 8 lanes per 64-query edge) measured slower than 4 (0.0624
+// against about 0.058 ms per query-sweep at T = 64, scale 22): it spills
+// registers at five waves per SIMD and halves the rows a pass can spread
+// over.
+constexpr int tile_lane_queries(bool delta16) { return delta16 ? 4 : 4; }
 constexpr int kTileReduce = 64;  // workgroups of the first stage of the per-query sums
 
 // Device state of one tile (T <= kMaxTile queries).  The host writes it while

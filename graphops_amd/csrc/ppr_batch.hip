@@ -262,62 +262,74 @@ __device__ __forceinline__ void tile_block_sum(Red &red, double *out) {
 }
 
 // ---------------------------------------------------------------------------
-// The tile sweep.  Lane layout: kLaneQ (4) consecutive queries per lane, so
-// Q = T / 4 lanes cover one edge's T queries with one 16-byte gather each
-// (a float4 of deltas, or two double2 of the f64 iterate); a wave gathers
-// 64 / Q edges per load instruction.  A row gets 2^s slices of Q lanes (slice
-// i sums edges i, i + 2^s, ...), and the slices are summed by lane shuffles.
+// The tile sweep.  Lane layout: LQ (tile_lane_queries) consecutive queries
+// per lane, so Q = T / LQ lanes cover one edge's T queries with one gather
+// each (16 bytes of f32 deltas, 8 of f16); a wave gathers 64 / Q edges per
+// load instruction.  A row gets 2^s slices of Q lanes (slice i sums edges
+// i, i + 2^s, ...), and the slices are summed by lane shuffles.
 
-// The four queries of a lane: sums, modes.
-struct Q4 {
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+// The LQ queries of a lane: sums, modes.
+template <int LQ>
+struct QV {
+  double v[LQ];
 };
 
+template <int LQ>
 struct LaneModes {
-  bool a0, a1, a2, a3;  // query j of the lane is in an f64 anchor sweep
-  bool need32, need64;  // some query of the lane gathers f32 deltas / the f64 iterate
+  bool a[LQ];           // query j of the lane is in an f64 anchor sweep
+  bool need32, need64;  // some query of the lane gathers deltas / the f64 iterate
   bool live;            // some query of the lane is live (else the lane gathers nothing)
 };
 
-// acc += f in edge order, or nothing (f belongs to a missing edge)
-__device__ __forceinline__ void add4(Q4 &acc, const float4 &f, bool have) {
-  acc.v0 = have ? acc.v0 + (double)f.x : acc.v0;
-  acc.v1 = have ? acc.v1 + (double)f.y : acc.v1;
-  acc.v2 = have ? acc.v2 + (double)f.z : acc.v2;
-  acc.v3 = have ? acc.v3 + (double)f.w : acc.v3;
+// LQ consecutive stored deltas, as loaded (one load per lane)
+template <typename DT, int LQ>
+using DeltaVec = DT __attribute__((ext_vector_type(LQ)));
+
+template <int LQ, typename DT>
+__device__ __forceinline__ DeltaVec<DT, LQ> load_deltas(const DT *p) {
+  return *reinterpret_cast<const DeltaVec<DT, LQ> *>(p);
 }
 
-template <int T>
-__device__ __forceinline__ void add_f64(const double *y64, int c, int q0, Q4 &acc) {
-  const double2 lo = *reinterpret_cast<const double2 *>(y64 + (size_t)c * T + q0);
-  const double2 hi = *reinterpret_cast<const double2 *>(y64 + (size_t)c * T + q0 + 2);
-  acc.v0 += lo.x;
-  acc.v1 += lo.y;
-  acc.v2 += hi.x;
-  acc.v3 += hi.y;
+// acc += f in edge order, or nothing (f belongs to a missing edge)
+template <int LQ, typename V>
+__device__ __forceinline__ void add_deltas(QV<LQ> &acc, const V &f, bool have) {
+#pragma unroll
+  for (int j = 0; j < LQ; j++) acc.v[j] = have ? acc.v[j] + (double)(float)f[j] : acc.v[j];
+}
+
+template <int T, int LQ>
+__device__ __forceinline__ void add_f64(const double *y64, int c, int q0, QV<LQ> &acc) {
+  const double *p = y64 + (size_t)c * T + q0;
+#pragma unroll
+  for (int j = 0; j < LQ; j += 2) {
+    const double2 v = *reinterpret_cast<const double2 *>(p + j);
+    acc.v[j] += v.x;
+    acc.v[j + 1] += v.y;
+  }
 }
 
 // queries of the lane in different modes: each takes its own vector
-template <int T, typename DT>
-__device__ __forceinline__ void add_mixed(const DT *yin, const double *y64, const LaneModes &m,
-                                          int c, int q0, Q4 &acc) {
+template <int T, int LQ, typename DT>
+__device__ __forceinline__ void add_mixed(const DT *yin, const double *y64, const LaneModes<LQ> &m,
+                                          int c, int q0, QV<LQ> &acc) {
   const size_t off = (size_t)c * T + q0;
-  const float4 f = DeltaStore<DT>::load4(yin + off);
-  const double2 lo = *reinterpret_cast<const double2 *>(y64 + off);
-  const double2 hi = *reinterpret_cast<const double2 *>(y64 + off + 2);
-  acc.v0 += m.a0 ? lo.x : (double)f.x;
-  acc.v1 += m.a1 ? lo.y : (double)f.y;
-  acc.v2 += m.a2 ? hi.x : (double)f.z;
-  acc.v3 += m.a3 ? hi.y : (double)f.w;
+  const DeltaVec<DT, LQ> f = load_deltas<LQ>(yin + off);
+#pragma unroll
+  for (int j = 0; j < LQ; j += 2) {
+    const double2 v = *reinterpret_cast<const double2 *>(y64 + off + j);
+    acc.v[j] += m.a[j] ? v.x : (double)(float)f[j];
+    acc.v[j + 1] += m.a[j + 1] ? v.y : (double)(float)f[j + 1];
+  }
 }
 
 // Sums of one lane's slice of a row: edges b, b + step, ... < e, added in
 // edge order.  Delta lanes keep four edges' loads in flight, f64 and mixed
 // lanes two.
-template <int T, typename DT>
-__device__ __forceinline__ Q4 row_slice_sum(const PprArgs &a, const DT *yin, const double *y64,
-                                            const LaneModes &m, int b, int e, int step, int q0) {
-  Q4 acc;
+template <int T, int LQ, typename DT>
+__device__ __forceinline__ QV<LQ> row_slice_sum(const PprArgs &a, const DT *yin, const double *y64,
+                                                const LaneModes<LQ> &m, int b, int e, int step,
+                                                int q0) {
+  QV<LQ> acc = {};
   if (!m.live) return acc;  // stopped queries and free slots: no loads
   int k = b;
   if (!m.need64) {
@@ -330,77 +342,78 @@ __device__ __forceinline__ Q4 row_slice_sum(const PprArgs &a, const DT *yin, con
       const bool h1 = k1 < e, h2 = k2 < e, h3 = k3 < e;
       const int c0 = a.col[k];
       const int c1 = a.col[h1 ? k1 : k], c2 = a.col[h2 ? k2 : k], c3 = a.col[h3 ? k3 : k];
-      const float4 f0 = DeltaStore<DT>::load4(yin + (size_t)c0 * T + q0);
-      const float4 f1 = DeltaStore<DT>::load4(yin + (size_t)c1 * T + q0);
-      const float4 f2 = DeltaStore<DT>::load4(yin + (size_t)c2 * T + q0);
-      const float4 f3 = DeltaStore<DT>::load4(yin + (size_t)c3 * T + q0);
-      add4(acc, f0, true);
-      add4(acc, f1, h1);
-      add4(acc, f2, h2);
-      add4(acc, f3, h3);
+      const DeltaVec<DT, LQ> f0 = load_deltas<LQ>(yin + (size_t)c0 * T + q0);
+      const DeltaVec<DT, LQ> f1 = load_deltas<LQ>(yin + (size_t)c1 * T + q0);
+      const DeltaVec<DT, LQ> f2 = load_deltas<LQ>(yin + (size_t)c2 * T + q0);
+      const DeltaVec<DT, LQ> f3 = load_deltas<LQ>(yin + (size_t)c3 * T + q0);
+      add_deltas<LQ>(acc, f0, true);
+      add_deltas<LQ>(acc, f1, h1);
+      add_deltas<LQ>(acc, f2, h2);
+      add_deltas<LQ>(acc, f3, h3);
     }
   } else if (!m.need32) {
-    for (; k + step < e; k += 2 * step) {
-      const int c0 = a.col[k], c1 = a.col[k + step];
-      add_f64<T>(y64, c0, q0, acc);
-      add_f64<T>(y64, c1, q0, acc);
+    if (LQ == 4) {  // 8 lanes' worth of f64 per edge at LQ = 8: one edge at a time
+      for (; k + step < e; k += 2 * step) {
+        const int c0 = a.col[k], c1 = a.col[k + step];
+        add_f64<T>(y64, c0, q0, acc);
+        add_f64<T>(y64, c1, q0, acc);
+      }
     }
     for (; k < e; k += step) add_f64<T>(y64, a.col[k], q0, acc);
   } else {
-    for (; k + step < e; k += 2 * step) {
-      const int c0 = a.col[k], c1 = a.col[k + step];
-      add_mixed<T>(yin, y64, m, c0, q0, acc);
-      add_mixed<T>(yin, y64, m, c1, q0, acc);
+    if (LQ == 4) {
+      for (; k + step < e; k += 2 * step) {
+        const int c0 = a.col[k], c1 = a.col[k + step];
+        add_mixed<T>(yin, y64, m, c0, q0, acc);
+        add_mixed<T>(yin, y64, m, c1, q0, acc);
+      }
     }
     for (; k < e; k += step) add_mixed<T>(yin, y64, m, a.col[k], q0, acc);
   }
   return acc;
 }
 
-__device__ __forceinline__ void q4_lane_sum(Q4 &v, int lo_bit, int hi_bit) {
-  v.v0 = lane_bits_sum(v.v0, lo_bit, hi_bit);
-  v.v1 = lane_bits_sum(v.v1, lo_bit, hi_bit);
-  v.v2 = lane_bits_sum(v.v2, lo_bit, hi_bit);
-  v.v3 = lane_bits_sum(v.v3, lo_bit, hi_bit);
+template <int LQ>
+__device__ __forceinline__ void qv_lane_sum(QV<LQ> &v, int lo_bit, int hi_bit) {
+#pragma unroll
+  for (int j = 0; j < LQ; j++) v.v[j] = lane_bits_sum(v.v[j], lo_bit, hi_bit);
 }
 
 // Sum over the block's slices of Q lanes (threads with the same tid % Q);
 // valid in threads < Q.
-template <int T>
-__device__ __forceinline__ Q4 block_slices_sum(Q4 v) {
-  constexpr int Q = T / kLaneQ;
-  __shared__ double sh[kBlock / 64][Q][kLaneQ];
+template <int T, int LQ>
+__device__ __forceinline__ QV<LQ> block_slices_sum(QV<LQ> v) {
+  constexpr int Q = T / LQ;
+  __shared__ double sh[kBlock / 64][Q][LQ];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int lq = 0;
   while ((1 << lq) < Q) lq++;
-  q4_lane_sum(v, lq, 6);
+  qv_lane_sum(v, lq, 6);
   if (lane < Q) {
-    sh[wave][lane][0] = v.v0;
-    sh[wave][lane][1] = v.v1;
-    sh[wave][lane][2] = v.v2;
-    sh[wave][lane][3] = v.v3;
+#pragma unroll
+    for (int j = 0; j < LQ; j++) sh[wave][lane][j] = v.v[j];
   }
   __syncthreads();
-  Q4 r;
+  QV<LQ> r = {};
   if (threadIdx.x < Q) {
     const int t = threadIdx.x;
-    r.v0 = (sh[0][t][0] + sh[1][t][0]) + (sh[2][t][0] + sh[3][t][0]);
-    r.v1 = (sh[0][t][1] + sh[1][t][1]) + (sh[2][t][1] + sh[3][t][1]);
-    r.v2 = (sh[0][t][2] + sh[1][t][2]) + (sh[2][t][2] + sh[3][t][2]);
-    r.v3 = (sh[0][t][3] + sh[1][t][3]) + (sh[2][t][3] + sh[3][t][3]);
+#pragma unroll
+    for (int j = 0; j < LQ; j++) r.v[j] = (sh[0][t][j] + sh[1][t][j]) + (sh[2][t][j] + sh[3][t][j]);
   }
   __syncthreads();
   return r;
 }
 
-// Row sums leave the gather phase in the slice-0 lanes, four queries each;
+// Row sums leave the gather phase in the slice-0 lanes, LQ queries each;
 // they go through LDS so that the epilogue runs one query per lane (lane
 // q = tid % T always handles query q), with one query's state in registers.
 template <int T, typename DT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void ppr_sweep_kernel(PprArgs a) {
-  constexpr int Q = T / kLaneQ;  // lanes per edge
-  __shared__ double rs[kBlock * kLaneQ];  // row sums of a pass: rows (<= kBlock / Q) x T
+  constexpr int LQ = tile_lane_queries(sizeof(DT) == 2);
+  constexpr int Q = T / LQ;  // lanes per edge
+  static_assert(Q >= 1 && kBlock % T == 0, "lane layout");
+  __shared__ double rs[kBlock * LQ];  // row sums of a pass: rows (<= kBlock / Q) x T
   const TileState *ts = a.ts;
   if (ts->all_done) return;
   const int ycur = ts->yd_cur;
@@ -408,17 +421,20 @@ void ppr_sweep_kernel(PprArgs a) {
   a.yout = ycur ? a.yd0 : a.yd1;
   const DT *yin = static_cast<const DT *>(a.yin);
   const double *y64 = ts->y64_par ? a.y64b : a.y64a;
-  // gather phase: queries q0 .. q0 + 3 of this lane
+  // gather phase: queries q0 .. q0 + LQ - 1 of this lane
   const int qg = threadIdx.x % Q;
-  const int q0 = qg * kLaneQ;
-  LaneModes m;
-  m.a0 = ts->mode[q0] == kModeAnchor;
-  m.a1 = ts->mode[q0 + 1] == kModeAnchor;
-  m.a2 = ts->mode[q0 + 2] == kModeAnchor;
-  m.a3 = ts->mode[q0 + 3] == kModeAnchor;
-  m.need64 = m.a0 || m.a1 || m.a2 || m.a3;
-  m.need32 = !(m.a0 && m.a1 && m.a2 && m.a3);
-  m.live = ts->live[q0] || ts->live[q0 + 1] || ts->live[q0 + 2] || ts->live[q0 + 3];
+  const int q0 = qg * LQ;
+  LaneModes<LQ> m;
+  m.need64 = false;
+  m.need32 = false;
+  m.live = false;
+#pragma unroll
+  for (int j = 0; j < LQ; j++) {
+    m.a[j] = ts->mode[q0 + j] == kModeAnchor;
+    m.need64 = m.need64 || m.a[j];
+    m.need32 = m.need32 || !m.a[j];
+    m.live = m.live || ts->live[q0 + j] != 0;
+  }
   // epilogue phase: query q of this lane
   const int q = threadIdx.x % T;
   const int mode = ts->mode[q];
@@ -448,32 +464,34 @@ void ppr_sweep_kernel(PprArgs a) {
           nb = a.row_ptr[r + ngrp];
           ne = a.row_ptr[r + ngrp + 1];
         }
-        Q4 acc;
+        QV<LQ> acc = {};
         if (r < bd.z) acc = row_slice_sum<T>(a, yin, y64, m, rb + slice, re, 1 << ls, q0);
         rb = nb;
         re = ne;
-        q4_lane_sum(acc, lq, lq + ls < 6 ? lq + ls : 6);  // Q << ls <= 64
+        qv_lane_sum(acc, lq, lq + ls < 6 ? lq + ls : 6);  // Q << ls <= 64
         if (slice == 0) {
           double *o = rs + g * T + q0;
-          o[0] = acc.v0;
-          o[1] = acc.v1;
-          o[2] = acc.v2;
-          o[3] = acc.v3;
+#pragma unroll
+          for (int j = 0; j < LQ; j++) o[j] = acc.v[j];
         }
         __syncthreads();
-        // at most kLaneQ (row, query) pairs per thread, all with query q:
-        // their loads first, then the epilogues
+        // at most LQ (row, query) pairs per thread, all with query q: the
+        // loads of four of them first, then their epilogues
         const int pairs = (bd.z - base < ngrp ? bd.z - base : ngrp) * T;
-        EpiLoad ld[kLaneQ];
 #pragma unroll
-        for (int j = 0; j < kLaneQ; j++) {
-          const int p = (int)threadIdx.x + j * kBlock;
-          if (p < pairs && qs.live) ld[j] = epi_load<T>(a, qs, mode, base + p / T, q);
-        }
+        for (int j0 = 0; j0 < LQ; j0 += 4) {
+          if ((int)threadIdx.x + j0 * kBlock >= pairs) break;
+          EpiLoad ld[4];
 #pragma unroll
-        for (int j = 0; j < kLaneQ; j++) {
-          const int p = (int)threadIdx.x + j * kBlock;
-          if (p < pairs) ppr_epilogue<T, DT>(a, qs, mode, base + p / T, q, rs[p], ld[j], red);
+          for (int j = 0; j < 4; j++) {
+            const int p = (int)threadIdx.x + (j0 + j) * kBlock;
+            if (p < pairs && qs.live) ld[j] = epi_load<T>(a, qs, mode, base + p / T, q);
+          }
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int p = (int)threadIdx.x + (j0 + j) * kBlock;
+            if (p < pairs) ppr_epilogue<T, DT>(a, qs, mode, base + p / T, q, rs[p], ld[j], red);
+          }
         }
         __syncthreads();
       }
@@ -489,14 +507,12 @@ void ppr_sweep_kernel(PprArgs a) {
         b = bd.z;
         e = bd.w;
       }
-      const Q4 acc =
+      const QV<LQ> acc =
           block_slices_sum<T>(row_slice_sum<T>(a, yin, y64, m, b + slice, e, kBlock / Q, q0));
       if (threadIdx.x < Q) {
         double *o = rs + q0;
-        o[0] = acc.v0;
-        o[1] = acc.v1;
-        o[2] = acc.v2;
-        o[3] = acc.v3;
+#pragma unroll
+        for (int j = 0; j < LQ; j++) o[j] = acc.v[j];
       }
       __syncthreads();
       if (threadIdx.x < T) {

@@ -57,17 +57,18 @@ inline int lanes_pow2(int want, int cap) {
 }
 
 // Work descriptors of the tile sweep for tile width T (rows already sorted
-// by in-degree).  One edge's T queries take Q = T / kLaneQ lanes; a row gets
+// by in-degree).  One edge's T queries take Q = T / lq lanes (lq queries per
+// lane: tile_lane_queries); a row gets
 // 2^s slices of Q lanes, about four edges per slice.  Runs of rows share a
 // slice count; rows long enough to give every slice of a workgroup four
 // edges get the whole workgroup; rows longer than kHubChunk are split into
 // the graph's hub chunks.
-void plan_tile(const std::vector<int32_t> &rp, int T, const DevGraph &dg,
+void plan_tile(const std::vector<int32_t> &rp, int T, int lq, const DevGraph &dg,
                const std::vector<int32_t> &hub_row, const std::vector<int32_t> &hub_nchunks,
                std::vector<BlockDesc> &desc) {
   desc.clear();
   const int64_t n = (int64_t)rp.size() - 1;
-  const int Q = T / kLaneQ;
+  const int Q = std::max(1, T / lq);
   const int smax = 64 / Q;  // slices of a row within one wave
   const int block_row_min = 4 * (kBlock / Q);
   int h = 0;
@@ -188,7 +189,7 @@ struct PprGraphCache {
   std::vector<int32_t> row_ptr;
   std::vector<uint8_t> dangling;  // new ids, out-degree 0
   std::vector<int32_t> hub_row, hub_nchunks;
-  TilePlan plans[4];
+  TilePlan plans[2][4];  // per delta storage (f32, f16) and tile width
   int tuned_T[2] = {};          // autotuned tile width per delta storage (f32, f16; 0: not
                                 // measured yet)
   double tile_cost[2][4] = {};  // measured ms per query-sweep of widths 8/16/32/64
@@ -232,11 +233,13 @@ gops_status graph_cache(gops_context *ctx, gops_graph *g, PprGraphCache **out) {
 
 int plan_slot(int T) { return T == 8 ? 0 : T == 16 ? 1 : T == 32 ? 2 : 3; }
 
-gops_status tile_plan(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, TilePlan **out) {
-  TilePlan &p = c->plans[plan_slot(T)];
+gops_status tile_plan(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, int delta16,
+                      TilePlan **out) {
+  TilePlan &p = c->plans[delta16 ? 1 : 0][plan_slot(T)];
   if (!p.desc) {
     std::vector<BlockDesc> desc;
-    plan_tile(c->row_ptr, T, g->dev, c->hub_row, c->hub_nchunks, desc);
+    plan_tile(c->row_ptr, T, tile_lane_queries(delta16 != 0), g->dev, c->hub_row,
+              c->hub_nchunks, desc);
     STRY(ctx, hipMalloc((void **)&p.desc, std::max<size_t>(desc.size(), 1) * sizeof(BlockDesc)));
     STRY(ctx, hipMemcpy(p.desc, desc.data(), desc.size() * sizeof(BlockDesc),
                         hipMemcpyHostToDevice));
@@ -484,7 +487,7 @@ int tile_grid(const gops_graph *g) {
 gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, double *cost) {
   *cost = std::numeric_limits<double>::infinity();
   TilePlan *plan = nullptr;
-  gops_status st = tile_plan(ctx, g, c, T, &plan);
+  gops_status st = tile_plan(ctx, g, c, T, ctx->tile_delta == GOPS_TILE_DELTA_F16, &plan);
   if (st != GOPS_OK) return st;
   Tile *t = nullptr;
   st = alloc_tile(ctx, g, T, tile_grid(g), ctx->tile_delta == GOPS_TILE_DELTA_F16, &t);
@@ -576,7 +579,8 @@ namespace gops {
 void free_ppr_cache(gops_graph *g) {
   PprGraphCache *c = g->ppr_cache;
   if (!c) return;
-  for (TilePlan &p : c->plans) hipFree(p.desc);
+  for (auto &row : c->plans)
+    for (TilePlan &p : row) hipFree(p.desc);
   delete c;
   g->ppr_cache = nullptr;
 }
@@ -608,7 +612,7 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
   st = pick_tile(ctx, g, c, (int)std::min<uint32_t>(nq, 1u << 20), &T);
   if (st != GOPS_OK) return st;
   TilePlan *plan = nullptr;
-  st = tile_plan(ctx, g, c, T, &plan);
+  st = tile_plan(ctx, g, c, T, ctx->tile_delta == GOPS_TILE_DELTA_F16, &plan);
   if (st != GOPS_OK) return st;
   Tile *t = nullptr;
   st = alloc_tile(ctx, g, T, tile_grid(g), ctx->tile_delta == GOPS_TILE_DELTA_F16, &t);
@@ -809,7 +813,7 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
   if (cfg && cfg->max_slots) ntiles = std::min<uint32_t>(ntiles, (cfg->max_slots + T - 1) / T);
   if (ntiles == 0) return sfail(ctx, GOPS_OUT_OF_MEMORY, "HBM budget holds no query tile");
   TilePlan *plan = nullptr;
-  st = tile_plan(ctx, g, c, T, &plan);
+  st = tile_plan(ctx, g, c, T, delta16, &plan);
   if (st != GOPS_OK) return st;
   auto *s = new (std::nothrow) gops_server;
   if (!s) return sfail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
@@ -881,7 +885,7 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
   gops_status st = graph_cache(ctx, g, &c);
   if (st != GOPS_OK) return st;
   TilePlan *plan = nullptr;
-  st = tile_plan(ctx, g, c, s->T, &plan);
+  st = tile_plan(ctx, g, c, s->T, s->delta16, &plan);
   if (st != GOPS_OK) return st;
   const int n = (int)g->dev.n;
   // 1. admission (host state is current: no kernel is queued).  With the
