@@ -328,6 +328,18 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
     its = np.array([r.iterations for r in runs])
     qps = nq / dt
     log(f"ppr: {nq} queries in {dt:.3f} s -> {qps:.1f} q/s, iterations mean {its.mean():.2f}")
+    # the same batch with its tile sweeps replayed from captured chains of
+    # four (GOPS_LAUNCH_GRAPH) instead of queued kernel by kernel
+    from graphops_amd import _lib as L
+    dg.ctx.set_option(L.OPT_LAUNCH, L.LAUNCH_GRAPH)
+    try:
+        ga.ppr_batch_run(dg, cfg, off[:65], seeds[:off[64]], with_scores=False)
+        t0 = time.perf_counter()
+        ga.ppr_batch_run(dg, cfg, off, seeds, with_scores=False)
+        gqps = nq / (time.perf_counter() - t0)
+    finally:
+        dg.ctx.set_option(L.OPT_LAUNCH, L.LAUNCH_AUTO)
+    log(f"ppr graph replay: {gqps:.1f} q/s")
     k = plain_sample
     t0 = time.perf_counter()
     _, pruns = ga.ppr_batch_run(dg, cfg, off[:k + 1], seeds[:off[k]], with_scores=False, tile=-1)
@@ -360,6 +372,7 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
                     "degree-proportional, d0.85 tol1e-8",
         "n": n, "m": m, "queries": nq, "queries_per_s": round(qps, 2),
         "ms_per_batch": round(dt * 1e3, 2), "tile": tile,
+        "launch": "queued", "graph_replay_queries_per_s": round(gqps, 2),
         "tile_ms_per_query_sweep": {str(w): round(c, 5) for w, c in costs.items()},
         "iterations_mean": round(float(its.mean()), 3), "iterations_max": int(its.max()),
         "converged": int(sum(r.converged for r in runs)),

@@ -137,6 +137,11 @@ struct Tile {
   std::vector<int32_t> marked_rows;  // rows currently marked in seed_idx
   TileState *ts = nullptr;
   TileState *hts = nullptr;  // pinned host copy
+  // GOPS_LAUNCH_GRAPH: kTileChain sweeps captured as one HIP graph for the
+  // arguments in chain_args (an admission may move the seed arrays; the
+  // chain is captured again when the arguments change)
+  hipGraphExec_t chain = nullptr;
+  PprArgs chain_args{};
   Query slot[kMaxTile];
   bool used[kMaxTile] = {};
   int grid = 0;
@@ -252,6 +257,7 @@ gops_status tile_plan(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T,
 
 void free_tile(Tile *t) {
   if (!t) return;
+  if (t->chain) (void)hipGraphExecDestroy(t->chain);
   hipFree(t->x);
   hipFree(t->y64a);
   hipFree(t->y64b);
@@ -353,6 +359,57 @@ PprArgs tile_args(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, 
   a.num_seed_entries = t->num_seed;
   a.mirror = mirror ? ctx->mirror_dev : nullptr;
   return a;
+}
+
+// Queue `count` sweeps of tile t with arguments a: kernel by kernel, or
+// (GOPS_LAUNCH_GRAPH) as replays of a captured chain of kTileChain sweeps,
+// rounded up to whole chains.  Every sweep of a chain has the same
+// arguments (the device picks buffers, modes and which queries are live),
+// so one capture serves until an admission changes them; sweeps past a
+// query's stop or cap return at once.  Returns the number queued.
+constexpr uint32_t kTileChain = 4;
+
+hipError_t queue_tile_sweeps(gops_context *ctx, Tile *t, const PprArgs &a, uint32_t count,
+                             uint32_t *queued) {
+  *queued = 0;
+  if (ctx->launch != GOPS_LAUNCH_GRAPH) {
+    for (uint32_t k = 0; k < count; k++) {
+      const hipError_t e = ppr_launch_sweep(ctx->stream, t->T, a, t->grid);
+      if (e != hipSuccess) return e;
+      ++*queued;
+    }
+    return hipSuccess;
+  }
+  if (t->chain && std::memcmp(&t->chain_args, &a, sizeof(PprArgs)) != 0) {
+    (void)hipGraphExecDestroy(t->chain);
+    t->chain = nullptr;
+  }
+  if (!t->chain) {
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t ce = hipSuccess;
+    for (uint32_t k = 0; k < kTileChain && ce == hipSuccess; k++)
+      ce = ppr_launch_sweep(ctx->stream, t->T, a, t->grid);
+    e = hipStreamEndCapture(ctx->stream, &graph);
+    if (ce != hipSuccess || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return ce != hipSuccess ? ce : e;
+    }
+    e = hipGraphInstantiate(&t->chain, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      t->chain = nullptr;
+      return e;
+    }
+    t->chain_args = a;
+  }
+  for (uint32_t k = 0; k < count; k += kTileChain) {
+    const hipError_t e = hipGraphLaunch(t->chain, ctx->stream);
+    if (e != hipSuccess) return e;
+    *queued += kTileChain;
+  }
+  return hipSuccess;
 }
 
 // Normalised sparse personalization of one query (ppr.rs:104-109, summed in
@@ -680,8 +737,11 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
     __atomic_store_n(ctx->mirror_host, base << 8, __ATOMIC_RELEASE);
     uint64_t j = 0, seen = 0;
     while (j < cap && st == GOPS_OK) {
-      for (; j < cap && j < seen + kAhead; j++) {
-        hipError_t e = ppr_launch_sweep(ctx->stream, T, a, t->grid);
+      if (j < cap && j < seen + kAhead) {
+        uint32_t queued = 0;
+        const hipError_t e = queue_tile_sweeps(
+            ctx, t, a, (uint32_t)std::min<uint64_t>(cap - j, seen + kAhead - j), &queued);
+        j += queued;
         if (e != hipSuccess) {
           st = sfail(ctx, GOPS_DEVICE_ERROR, hipGetErrorString(e));
           break;
@@ -942,7 +1002,8 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
   for (Tile *t : s->tiles) {
     if (t->hts->all_done) continue;
     PprArgs a = tile_args(ctx, g, t, plan, false);
-    for (uint32_t k = 0; k < sweeps; k++) STRY(ctx, ppr_launch_sweep(ctx->stream, t->T, a, t->grid));
+    uint32_t queued = 0;
+    STRY(ctx, queue_tile_sweeps(ctx, t, a, sweeps, &queued));
   }
   STRY(ctx, hipStreamSynchronize(ctx->stream));
   // 3. harvest finished queries.  With topk: a tile's finished queries are

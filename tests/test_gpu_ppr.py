@@ -137,6 +137,44 @@ def test_batch_refill_same_as_alone(ctx, graph13):
         assert np.array_equal(s1[0], scores[q])
 
 
+def test_batch_graph_replay_same_bits(ctx, graph13):
+    """GOPS_LAUNCH_GRAPH: the tile sweeps replayed from captured chains of
+    four (captured again after each admission) give the bits of the queued
+    kernels, with refills; the server's steps too."""
+    from graphops_amd import _lib as L
+
+    e = graph13
+    nq = 40
+    off, seeds = generators.seed_sets(e, nq, 1, 64, seed=19)
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    cfgs = [ga.PageRankConfig([0.5, 0.85, 0.95][q % 3], 300, [1e-6, 1e-9][q % 2])
+            for q in range(nq)]
+    s1, r1 = ga.ppr_batch_run(dg, cfgs, off, seeds, ctx=ctx, tile=16)
+    ctx.set_option(L.OPT_LAUNCH, L.LAUNCH_GRAPH)
+    try:
+        s2, r2 = ga.ppr_batch_run(dg, cfgs, off, seeds, ctx=ctx, tile=16)
+        srv = ga.Server(dg, tile=8, max_slots=16)
+        tickets = {srv.submit(cfgs[q], seeds[off[q]:off[q + 1]]): q for q in range(12)}
+        served = {}
+        steps = 0
+        while len(served) < 12:
+            srv.step(3)
+            steps += 1
+            while (r := srv.poll()) is not None:
+                served[tickets[r[0]]] = r[1]
+            assert steps < 2000
+        srv.close()
+    finally:
+        ctx.set_option(L.OPT_LAUNCH, L.LAUNCH_AUTO)
+    for q in range(nq):
+        assert r1[q].iterations == r2[q].iterations and r1[q].diff_l1 == r2[q].diff_l1
+        assert np.array_equal(s1[q], s2[q])
+    s8, r8 = ga.ppr_batch_run(dg, cfgs[:12], off[:13], seeds[:off[12]], ctx=ctx, tile=8)
+    for q in range(12):
+        assert served[q].iterations == r8[q].iterations
+        assert np.array_equal(served[q].scores, s8[q])
+
+
 def check_f16(run, scores, ref, tol):
     """The bar of f16 delta storage (GOPS_OPT_TILE_DELTA = F16).  Each
     stored delta rounds to 11 bits and the row carries the rounding into its
