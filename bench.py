@@ -355,7 +355,6 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
     top = [top_agreement(sc[j], refs[j]) for j in sample]
     del sc
     alt = f64_phase_schedule(ga, dg, cfg, off, seeds, sample, refs)
-    half = f16_delta_storage(ga, oracle, e, dg, cfg, off, seeds)
     # CPU: the oracle's push form of personalized_pagerank_run, one thread
     t0 = time.perf_counter()
     for j in range(cpu_queries):
@@ -382,7 +381,6 @@ def ppr_batch_leg(ga, generators, e, dg, nq=1024, plain_sample=32, cpu_queries=3
         "accuracy": {"sample_queries": len(sample), "l1_vs_f64_oracle_max": max(l1),
                      "max_abs": max(mx), "top100_agreement_min": min(top)},
         "f64_phase_schedule": alt,
-        "f16_deltas": half,
         "roofline": {"bound": "hbm", "queries_per_s_at_peak": round(roof_qps, 1),
                      "frac": round(qps / roof_qps, 4),
                      "bytes_per_query_sweep": round(per_qsweep),
@@ -437,65 +435,13 @@ def f64_phase_schedule(ga, dg, cfg, off, seeds, sample, refs):
             "l1_vs_f64_oracle_max": l1}
 
 
-def f16_delta_storage(ga, oracle, e, dg, cfg, off, seeds, n_sample=16):
-    """The same batch with the tiles' gathered deltas stored in f16
-    (GOPS_OPT_TILE_DELTA = F16: scaled per query by a power of two, each
-    row carrying its rounding into its next delta): half the bytes of every
-    gather.  Accuracy against the oracle's f64 runs of n_sample queries:
-    L1, max-abs, top-100 agreement, and how many iteration counts differ
-    (the stored deltas' rounding moves a residual by a few percent)."""
-    nq = len(off) - 1
-    tile, costs = ga.ppr_tile_info(dg, nq, delta="f16")
-    ga.ppr_batch_run(dg, cfg, off[:65], seeds[:off[64]], with_scores=False, delta="f16")
-    t0 = time.perf_counter()
-    _, runs = ga.ppr_batch_run(dg, cfg, off, seeds, with_scores=False, delta="f16")
-    dt = time.perf_counter() - t0
-    qps = nq / dt
-    sc, sruns = ga.ppr_batch_run(dg, cfg, off[:n_sample + 1], seeds[:off[n_sample]],
-                                 with_scores=True, delta="f16")
-    in_off, in_src, _ = oracle.transpose(e.offsets, e.targets)
-    deg = np.diff(e.offsets.astype(np.int64)).astype(np.float64)
-    l1, mx, top, it_diff = [], [], [], 0
-    for j in range(n_sample):
-        p = np.zeros(e.n)
-        p[seeds[off[j]:off[j + 1]]] += 1.0
-        ref = oracle.pull_run(2, in_off, in_src, deg, p=p, damping=cfg.damping,
-                              max_iterations=cfg.max_iterations, tolerance=cfg.tolerance,
-                              threads=16)
-        l1.append(float(np.abs(sc[j] - ref.scores).sum()))
-        mx.append(float(np.abs(sc[j] - ref.scores).max()))
-        top.append(top_agreement(sc[j], ref.scores))
-        it_diff += int(sruns[j].iterations != ref.iterations)
-    del sc
-    log(f"ppr f16 deltas: tile {tile}, {qps:.1f} q/s (L1 max {max(l1):.2e}, "
-        f"{it_diff} of {n_sample} iteration counts differ)")
-    n, m = e.n, e.m
-    per_qsweep = 8 * n + (4 * m + 8 * n) / tile
-    its = float(np.mean([r.iterations for r in runs]))
-    roof_qps = HBM_PEAK_GBS * 1e9 / (its * per_qsweep)
-    return {"storage": "f16 deltas, per-query power-of-two scale, carried rounding",
-            "tile": tile, "tile_ms_per_query_sweep": {str(w): round(c, 5) for w, c in costs.items()},
-            "queries_per_s": round(qps, 2),
-            "roofline": {"bound": "hbm", "frac": round(qps / roof_qps, 4),
-                         "bytes_per_query_sweep": round(per_qsweep),
-                         "traffic_per_tile_sweep": measured_traffic("rmat22-unit ppr tile64", "tile_f16")
-                         if tile == 64 else None},
-            "iterations_mean": round(float(np.mean([r.iterations for r in runs])), 3),
-            "converged": int(sum(r.converged for r in runs)),
-            "accuracy": {"sample_queries": n_sample, "l1_vs_f64_oracle_max": max(l1),
-                         "l1_over_tol_max": max(l1) / cfg.tolerance, "max_abs": max(mx),
-                         "top100_agreement_min": min(top),
-                         "iteration_counts_differing": it_diff}}
-
-
-def serve_once(ga, e, dg, off, seeds, damp, qps_offered, nq, max_slots, sweeps, topk, seed,
-               delta=None):
+def serve_once(ga, e, dg, off, seeds, damp, qps_offered, nq, max_slots, sweeps, topk, seed):
     """One serving run: nq Poisson arrivals at qps_offered, each response the
     query's top-k (node, score) selected on the device.  Latency =
     completion (poll) time - scheduled arrival time."""
     rng = np.random.default_rng(seed)
     arrive = np.cumsum(rng.exponential(1.0 / qps_offered, size=nq))
-    srv = ga.Server(dg, tile=0, max_slots=max_slots, topk=topk, delta=delta)
+    srv = ga.Server(dg, tile=0, max_slots=max_slots, topk=topk)
     st = srv.stats()
     lat = np.zeros(nq)
     its = np.zeros(nq)
@@ -527,12 +473,11 @@ def serve_once(ga, e, dg, off, seeds, damp, qps_offered, nq, max_slots, sweeps, 
     span = time.perf_counter() - t0
     srv.close()
     out = {"offered_qps": qps_offered, "queries": nq, "queries_per_s": round(nq / span, 2),
-           "deltas": delta or "f32",
            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
            "iterations_mean": round(float(its.mean()), 2), "converged": int(conv),
            "slots": st["slots"]}
-    log(f"serving ({delta or 'f32'} deltas) at {qps_offered:.0f} QPS offered: {nq} queries in {span:.3f} s "
+    log(f"serving at {qps_offered:.0f} QPS offered: {nq} queries in {span:.3f} s "
         f"({out['queries_per_s']} q/s), p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms")
     return out
 
@@ -555,13 +500,9 @@ def serving_leg(ga, generators, e, dg, nq=1024, loads=(5000.0, 10000.0, 50000.0)
     low = round(0.5 * capacity, -1)
     n_low = min(nq, max(64, int(2.0 * low)))  # about two seconds of arrivals
     runs.append(serve_once(ga, e, dg, off, seeds, damp, low, n_low, max_slots, sweeps, topk, 41))
-    # the tiles with f16 deltas (GOPS_OPT_TILE_DELTA), at the top offered load
-    runs.append(serve_once(ga, e, dg, off, seeds, damp, loads[-1], nq, max_slots, sweeps, topk,
-                           29 + len(loads) - 1, delta="f16"))
     return {"workload": f"rmat{PPR_GRAPH['scale']}-unit ppr serving, Poisson arrivals, "
                         "|S| log-uniform 1..256, d in {0.5,0.85,0.95}, tol1e-6",
-            "capacity_queries_per_s": capacity,
-            "capacity_queries_per_s_f16_deltas": runs[-1]["queries_per_s"], "loads": runs,
+            "capacity_queries_per_s": capacity, "loads": runs,
             "sweeps_per_step": sweeps, "response": f"top-{topk}"}
 
 

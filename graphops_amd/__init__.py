@@ -501,23 +501,16 @@ def personalized_pagerank_checked(graph, config: PageRankConfig, personalization
     return personalized_pagerank_checked_run(graph, config, personalization, ctx).scores
 
 
-_TILE_DELTA = {"f32": L.TILE_DELTA_F32, "f16": L.TILE_DELTA_F16}
-
-
 def ppr_batch_run(graph, configs, seed_offsets, seeds, seed_weights=None, ctx: Context = None,
-                  with_scores=True, tile=None, delta=None):
+                  with_scores=True, tile=None):
     """Many personalized_pagerank_run calls (ppr.rs:90-156) in one device run.
     Query q's personalization is 1.0 (or seed_weights[j]) on
-    seeds[seed_offsets[q]:seed_offsets[q+1]] and 0 elsewhere.  delta: how
-    the tiles store the deltas they gather, "f32" (the context's default) or
-    "f16" (GOPS_OPT_TILE_DELTA).  Returns (scores[num_queries, n] or None,
-    list of PageRankRun without scores)."""
+    seeds[seed_offsets[q]:seed_offsets[q+1]] and 0 elsewhere.  Returns
+    (scores[num_queries, n] or None, list of PageRankRun without scores)."""
     dg = _device_graph(graph, False, ctx)
     c = dg.ctx.handle
     if tile is not None:  # queries per tile; -1 = one query per device run
         dg.ctx.set_option(L.OPT_PPR_TILE, int(tile))
-    if delta is not None:
-        dg.ctx.set_option(L.OPT_TILE_DELTA, _TILE_DELTA[delta])
     q = len(seed_offsets) - 1
     if isinstance(configs, PageRankConfig):
         configs = [configs] * q
@@ -532,27 +525,21 @@ def ppr_batch_run(graph, configs, seed_offsets, seeds, seed_weights=None, ctx: C
                                     L.ptr(scores, L._f64p), infos)
     if tile is not None:
         dg.ctx.set_option(L.OPT_PPR_TILE, 0)
-    if delta is not None:
-        dg.ctx.set_option(L.OPT_TILE_DELTA, L.TILE_DELTA_F32)
     _check(rc, c, "gops_ppr_batch_run")
     runs = [PageRankRun(None, int(infos[i].iterations), float(infos[i].diff_l1),
                         bool(infos[i].converged)) for i in range(q)]
     return scores, runs
 
 
-def ppr_tile_info(graph, num_queries, ctx: Context = None, delta=None):
+def ppr_tile_info(graph, num_queries, ctx: Context = None):
     """(tile width a batch of num_queries queries would use, {width: measured
     ms per query-sweep}) -- gops_ppr_tile_info; runs the autotuner's trial on
-    first use (per delta storage, "f32" or "f16")."""
+    first use."""
     dg = _device_graph(graph, False, ctx)
     t = ctypes.c_uint32()
     cost = np.zeros(4, dtype=np.float64)
-    if delta is not None:
-        dg.ctx.set_option(L.OPT_TILE_DELTA, _TILE_DELTA[delta])
     rc = L.lib().gops_ppr_tile_info(dg.ctx.handle, dg.handle, int(num_queries), ctypes.byref(t),
                                     L.ptr(cost, L._f64p))
-    if delta is not None:
-        dg.ctx.set_option(L.OPT_TILE_DELTA, L.TILE_DELTA_F32)
     _check(rc, dg.ctx.handle, "gops_ppr_tile_info")
     return int(t.value), {w: float(c) for w, c in zip((8, 16, 32, 64), cost)}
 
@@ -765,17 +752,13 @@ class Server:
     soon as one frees up; step() advances every tile with live queries."""
 
     def __init__(self, graph: DeviceGraph, tile: int = 16, max_slots: int = 0,
-                 hbm_budget_bytes: float = 0.0, topk: int = 0, delta=None):
+                 hbm_budget_bytes: float = 0.0, topk: int = 0):
         self.graph = graph
         self.ctx = graph.ctx
         cfg = L.ServerConfigC(int(max_slots), float(hbm_budget_bytes), int(tile), int(topk))
         h = ctypes.c_void_p()
-        if delta is not None:  # the tiles' delta storage, "f32" or "f16", fixed at creation
-            self.ctx.set_option(L.OPT_TILE_DELTA, _TILE_DELTA[delta])
         rc = L.lib().gops_server_create(self.ctx.handle, graph.handle, ctypes.byref(cfg),
                                         ctypes.byref(h))
-        if delta is not None:
-            self.ctx.set_option(L.OPT_TILE_DELTA, L.TILE_DELTA_F32)
         _check(rc, self.ctx.handle, "gops_server_create")
         self._h = h
         self.topk = int(topk)

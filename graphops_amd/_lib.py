@@ -34,8 +34,6 @@ OPT_TILE_F32_BELOW = 11
 OPT_DELTA_SWEEP = 12
 OPT_LAUNCH = 13
 OPT_ANCHOR_SWEEP = 14
-OPT_TILE_DELTA = 15
-TILE_DELTA_F32, TILE_DELTA_F16 = 0, 1
 ANCHOR_SWEEP_AUTO = 0
 ANCHOR_SWEEP_PULL = 1
 LAUNCH_AUTO, LAUNCH_QUEUED, LAUNCH_GRAPH = 0, 1, 2

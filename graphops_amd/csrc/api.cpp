@@ -585,11 +585,6 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
                     "anchor sweep must be GOPS_ANCHOR_SWEEP_AUTO or _PULL");
       ctx->anchor_sweep = (int)value;
       return GOPS_OK;
-    case GOPS_OPT_TILE_DELTA:
-      if (value != GOPS_TILE_DELTA_F32 && value != GOPS_TILE_DELTA_F16)
-        return fail(ctx, GOPS_INVALID_PARAMETER, "tile delta must be GOPS_TILE_DELTA_F32 or _F16");
-      ctx->tile_delta = (int)value;
-      return GOPS_OK;
     case GOPS_OPT_DELTA_SWEEP:
       if (value != GOPS_DELTA_SWEEP_AUTO && value != GOPS_DELTA_SWEEP_PULL &&
           value != GOPS_DELTA_SWEEP_BLOCKED)

@@ -292,7 +292,6 @@ struct gops_context {
   int normalize = 0;
   int timing = 0;
   int ppr_tile = 0;
-  int tile_delta = GOPS_TILE_DELTA_F32;  // storage of the tiles' gathered deltas
   int graph_build = GOPS_BUILD_DEVICE;
   int delta_sweep = GOPS_DELTA_SWEEP_AUTO;
   int anchor_sweep = GOPS_ANCHOR_SWEEP_AUTO;

@@ -7,15 +7,8 @@
 namespace gops {
 
 constexpr int kMaxTile = 64;  // queries per tile, at most
-constexpr int kLaneQ = 4;     // queries per refill group (and per lane with f32 deltas)
-
-// Queries per lane in the tile sweep.  The host's work descriptors depend
-// on it (Q = T / this lanes per edge).  With f16 deltas 8 per lane (one
-// 16-byte gather, 8 lanes per 64-query edge) measured slower than 4 (0.0624
-// against about 0.058 ms per query-sweep at T = 64, scale 22): it spills
-// registers at five waves per SIMD and halves the rows a pass can spread
-// over.
-constexpr int tile_lane_queries(bool delta16) { return delta16 ? 4 : 4; }
+constexpr int kLaneQ = 4;     // queries per lane in the tile sweep (one 16-byte gather),
+                              // and per refill group
 constexpr int kTileReduce = 64;  // workgroups of the first stage of the per-query sums
 
 // Device state of one tile (T <= kMaxTile queries).  The host writes it while
@@ -42,10 +35,6 @@ struct TileState {
   double diff[kMaxTile];
   double diff_anchor[kMaxTile];
   double x_scale[kMaxTile];
-  // f16 delta storage: the power-of-two scale of the stored deltas the next
-  // sweep gathers (ysc_in) and of the ones it writes (ysc_out); 1 for f32
-  double ysc_in[kMaxTile];
-  double ysc_out[kMaxTile];
   uint64_t iter[kMaxTile];
   uint64_t max_iter[kMaxTile];
   uint64_t since_anchor[kMaxTile];
@@ -64,14 +53,12 @@ struct PprArgs {
   const float *inv32;   // 1/out-degree (0 = dangling), new ids
   const double *inv64;
   double *x;            // [n][T] iterate
-  void *yd0, *yd1;      // [n][T] prescaled deltas, ping-pong: f32, or f16 times the
-                        // query's scale (delta16)
-  float *carry;         // [n][T] what the row's last stored delta rounded off, added
+  float *yd0, *yd1;     // [n][T] prescaled deltas (f32), ping-pong
+  float *carry;         // [n][T] what the row's last f32 delta rounded off, added
                         // to its next one (so the sums the sweeps propagate lag
                         // the f64 deltas by one rounding, not by all of them)
-  const void *yin;      // picked per sweep
-  void *yout;
-  int delta16;          // storage of yd0 / yd1: 0 f32, 1 f16 (GOPS_OPT_TILE_DELTA)
+  const float *yin32;   // picked per sweep
+  float *yout32;
   double *y64a, *y64b;  // [n][T] prescaled iterate for anchor sweeps, ping-pong
   double *partials;     // [grid][T][5] per sweep workgroup, then [kTileReduce][T][5]
                         // after the first reduction stage (at kMaxSweepBlocks * T * 5)
@@ -83,6 +70,8 @@ struct PprArgs {
   const int32_t *seed_q;
   const double *seed_p;     // normalised personalization value (ppr.rs:104-109)
   int num_seed_entries;
+  int zseed_first;          // first seed entry whose row has no in-edges (row >= n_in)
+  int n_in;                 // rows with in-edges: ids [0, n_in) (rows sorted by in-degree)
   uint64_t *mirror;         // mapped host word: (sweeps << 8) | (free groups << 1) | all_done,
                             // a group being kLaneQ consecutive slots with no live query
 };

@@ -21,7 +21,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <type_traits>
 
 #include "internal.h"
 #include "ppr_batch.h"
@@ -63,49 +62,10 @@ struct Red {
 
 }  // namespace
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-
-// Storage of the prescaled deltas a sweep gathers.  f32 as is; f16 times a
-// per-query power of two s (TileState::ysc_in / ysc_out), chosen from the
-// query's previous residual R so that every stored value, at most about
-// 3 R in magnitude, stays below the f16 range, clamped there all the same.
-// What a value rounds off (or loses to the clamp or to f16's underflow) goes
-// into the row's carry, so it reaches the next sweep instead of being lost.
-template <typename DT>
-struct DeltaStore;
-
-template <>
-struct DeltaStore<float> {
-  __device__ static float put(double v, double) { return (float)v; }
-  __device__ static double back(float y, double) { return (double)y; }
-  __device__ static float4 load4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
-  __device__ static void zero4(float *p) { *reinterpret_cast<float4 *>(p) = float4{0.f, 0.f, 0.f, 0.f}; }
-};
-
-template <>
-struct DeltaStore<_Float16> {
-  __device__ static _Float16 put(double v, double s) {
-    const float f = (float)(v * s);  // s is a power of two: exact
-    return (_Float16)fminf(fmaxf(f, -65504.0f), 65504.0f);
-  }
-  __device__ static double back(_Float16 y, double s) { return (double)(float)y / s; }
-  __device__ static float4 load4(const _Float16 *p) {
-    const half4_t h = *reinterpret_cast<const half4_t *>(p);
-    return float4{(float)h.x, (float)h.y, (float)h.z, (float)h.w};
-  }
-  __device__ static void zero4(_Float16 *p) { *reinterpret_cast<uint2 *>(p) = uint2{0u, 0u}; }
-};
-
-// The f16 scale for values bounded by about 3 R: 2^(14 - e) with R < 2^e,
-// so that they stay below 3 * 2^14 = 49152 (f16's largest finite value is
-// 65504) and their relative rounding is f16's 2^-11 down to 2^-28 R.
-__device__ __forceinline__ double f16_scale(double R) {
-  int e = 0;
-  if (!(R > 0.0) || !isfinite(R)) return 16384.0;
-  (void)frexp(R, &e);  // R < 2^e
-  e = e < -900 ? -900 : (e > 900 ? 900 : e);
-  return ldexp(1.0, 14 - e);
-}
+// The prescaled deltas a sweep gathers are stored in f32; what a value
+// rounds off goes into the row's carry, so it reaches the next sweep
+// instead of being lost.
+__device__ __forceinline__ float delta_put(double v) { return (float)v; }
 
 // What a sweep's epilogue needs of its query, read from the tile state once
 // per thread (lane q of the epilogue always handles query q): reloading it
@@ -113,10 +73,9 @@ __device__ __forceinline__ double f16_scale(double R) {
 struct QState {
   bool live, first, f64_phase;
   int c64;  // this sweep's anchors gather y64[c64]; the next gathers the other
-  double d, s_in, s_out, x_scale, dsum_x, dsum_in, mass_in;
+  double d, x_scale, dsum_x, dsum_in, mass_in;
 };
 
-template <typename DT>
 __device__ __forceinline__ QState load_qstate(const TileState *ts, int q) {
   QState z;
   z.live = ts->live[q] != 0;
@@ -124,9 +83,6 @@ __device__ __forceinline__ QState load_qstate(const TileState *ts, int q) {
   z.f64_phase = ts->f64_phase[q] != 0;
   z.c64 = ts->y64_par;
   z.d = ts->damping[q];
-  const bool scaled = !std::is_same<DT, float>::value;
-  z.s_in = scaled ? ts->ysc_in[q] : 1.0;
-  z.s_out = scaled ? ts->ysc_out[q] : 1.0;
   z.x_scale = ts->x_scale[q];
   z.dsum_x = ts->dsum_x[q];
   z.dsum_in = ts->dsum_in[q];
@@ -156,9 +112,8 @@ __device__ __forceinline__ EpiLoad epi_load(const PprArgs &a, const QState &qs, 
 }
 
 // Per (row, query) epilogue.  mode: kModeDelta / kModeDeltaFold / kModeAnchor.
-// acc is the row's gathered sum as stored (delta modes: times ysc_in); ld
-// holds the row's loads (epi_load).
-template <int T, typename DT>
+// acc is the row's gathered sum; ld holds the row's loads (epi_load).
+template <int T>
 __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs, int mode,
                                              int r, int q, double acc, const EpiLoad &ld,
                                              Red &red) {
@@ -172,8 +127,7 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs,
   }
   const size_t at = (size_t)r * T + q;
   const int c64 = qs.c64;
-  DT *yout = static_cast<DT *>(a.yout);
-  const double s_out = qs.s_out;
+  float *yout = a.yout32;
   // every path sets all five contributions; one update of red at the end
   double delta, dang_x = 0.0, sum_x = 0.0;
   bool dangling;
@@ -185,27 +139,25 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs,
     a.x[at] = t;
     const double inv = ld.inv;
     const double v = (d * delta) * inv;  // the delta chain restarts here: no carry
-    const DT yo = DeltaStore<DT>::put(v, s_out);
+    const float yo = delta_put(v);
     yout[at] = yo;
-    a.carry[at] = (float)(v - DeltaStore<DT>::back(yo, s_out));
+    a.carry[at] = (float)(v - (double)yo);
     if (qs.f64_phase) (c64 ? a.y64a : a.y64b)[at] = (d * t) * inv;  // next anchor's input
     dangling = inv == 0.0;
     dang_x = dangling ? t : 0.0;
   } else {
     // delta' = A delta: the dangling and (removed) total sums go to the seeds
-    if (!std::is_same<DT, float>::value) acc /= qs.s_in;  // a power of two: exact
     delta = acc + d * (qs.dsum_in - qs.mass_in) * seed_p;
     const double inv = ld.inv;
     // A query's first sweep (admitted with p-hat as its input delta) starts
-    // from x = p-hat and the carry of p-hat's rounding on its seeds (stored
-    // by the admission at scale ysc_in), 0 elsewhere: neither column is read
-    // (nor cleared at admission) for it.
+    // from x = p-hat and the carry of p-hat's rounding on its seeds, 0
+    // elsewhere: neither column is read (nor cleared at admission) for it.
     const bool first = qs.first;
     double x_old, c_old;
     if (first) {
       x_old = seed_p;
       const double v0 = (d * seed_p) * inv;
-      c_old = v0 - DeltaStore<DT>::back(DeltaStore<DT>::put(v0, qs.s_in), qs.s_in);
+      c_old = v0 - (double)delta_put(v0);
     } else {
       x_old = ld.x;
       c_old = (double)ld.c;
@@ -215,9 +167,9 @@ __device__ __forceinline__ void ppr_epilogue(const PprArgs &a, const QState &qs,
     // the stored delta carries the rounding of the row's previous one:
     // summed over the sweeps, the roundings cancel but for the last
     const double v = (d * delta) * inv + c_old;
-    const DT yo = DeltaStore<DT>::put(v, s_out);
+    const float yo = delta_put(v);
     yout[at] = yo;
-    a.carry[at] = (float)(v - DeltaStore<DT>::back(yo, s_out));
+    a.carry[at] = (float)(v - (double)yo);
     dangling = inv == 0.0;
     if (mode == kModeDeltaFold) {
       (c64 ? a.y64a : a.y64b)[at] = (d * x) * inv;  // the anchor that follows
@@ -262,10 +214,9 @@ __device__ __forceinline__ void tile_block_sum(Red &red, double *out) {
 }
 
 // ---------------------------------------------------------------------------
-// The tile sweep.  Lane layout: LQ (tile_lane_queries) consecutive queries
-// per lane, so Q = T / LQ lanes cover one edge's T queries with one gather
-// each (16 bytes of f32 deltas, 8 of f16); a wave gathers 64 / Q edges per
-// load instruction.  A row gets 2^s slices of Q lanes (slice i sums edges
+// The tile sweep.  Lane layout: LQ (kLaneQ) consecutive queries per lane,
+// so Q = T / LQ lanes cover one edge's T queries with one 16-byte gather
+// each; a wave gathers 64 / Q edges per load instruction.  A row gets 2^s slices of Q lanes (slice i sums edges
 // i, i + 2^s, ...), and the slices are summed by lane shuffles.
 
 // The LQ queries of a lane: sums, modes.
@@ -282,12 +233,12 @@ struct LaneModes {
 };
 
 // LQ consecutive stored deltas, as loaded (one load per lane)
-template <typename DT, int LQ>
-using DeltaVec = DT __attribute__((ext_vector_type(LQ)));
+template <int LQ>
+using DeltaVec = float __attribute__((ext_vector_type(LQ)));
 
-template <int LQ, typename DT>
-__device__ __forceinline__ DeltaVec<DT, LQ> load_deltas(const DT *p) {
-  return *reinterpret_cast<const DeltaVec<DT, LQ> *>(p);
+template <int LQ>
+__device__ __forceinline__ DeltaVec<LQ> load_deltas(const float *p) {
+  return *reinterpret_cast<const DeltaVec<LQ> *>(p);
 }
 
 // acc += f in edge order, or nothing (f belongs to a missing edge)
@@ -309,11 +260,11 @@ __device__ __forceinline__ void add_f64(const double *y64, int c, int q0, QV<LQ>
 }
 
 // queries of the lane in different modes: each takes its own vector
-template <int T, int LQ, typename DT>
-__device__ __forceinline__ void add_mixed(const DT *yin, const double *y64, const LaneModes<LQ> &m,
+template <int T, int LQ>
+__device__ __forceinline__ void add_mixed(const float *yin, const double *y64, const LaneModes<LQ> &m,
                                           int c, int q0, QV<LQ> &acc) {
   const size_t off = (size_t)c * T + q0;
-  const DeltaVec<DT, LQ> f = load_deltas<LQ>(yin + off);
+  const DeltaVec<LQ> f = load_deltas<LQ>(yin + off);
 #pragma unroll
   for (int j = 0; j < LQ; j += 2) {
     const double2 v = *reinterpret_cast<const double2 *>(y64 + off + j);
@@ -325,8 +276,8 @@ __device__ __forceinline__ void add_mixed(const DT *yin, const double *y64, cons
 // Sums of one lane's slice of a row: edges b, b + step, ... < e, added in
 // edge order.  Delta lanes keep four edges' loads in flight, f64 and mixed
 // lanes two.
-template <int T, int LQ, typename DT>
-__device__ __forceinline__ QV<LQ> row_slice_sum(const PprArgs &a, const DT *yin, const double *y64,
+template <int T, int LQ>
+__device__ __forceinline__ QV<LQ> row_slice_sum(const PprArgs &a, const float *yin, const double *y64,
                                                 const LaneModes<LQ> &m, int b, int e, int step,
                                                 int q0) {
   QV<LQ> acc = {};
@@ -342,10 +293,10 @@ __device__ __forceinline__ QV<LQ> row_slice_sum(const PprArgs &a, const DT *yin,
       const bool h1 = k1 < e, h2 = k2 < e, h3 = k3 < e;
       const int c0 = a.col[k];
       const int c1 = a.col[h1 ? k1 : k], c2 = a.col[h2 ? k2 : k], c3 = a.col[h3 ? k3 : k];
-      const DeltaVec<DT, LQ> f0 = load_deltas<LQ>(yin + (size_t)c0 * T + q0);
-      const DeltaVec<DT, LQ> f1 = load_deltas<LQ>(yin + (size_t)c1 * T + q0);
-      const DeltaVec<DT, LQ> f2 = load_deltas<LQ>(yin + (size_t)c2 * T + q0);
-      const DeltaVec<DT, LQ> f3 = load_deltas<LQ>(yin + (size_t)c3 * T + q0);
+      const DeltaVec<LQ> f0 = load_deltas<LQ>(yin + (size_t)c0 * T + q0);
+      const DeltaVec<LQ> f1 = load_deltas<LQ>(yin + (size_t)c1 * T + q0);
+      const DeltaVec<LQ> f2 = load_deltas<LQ>(yin + (size_t)c2 * T + q0);
+      const DeltaVec<LQ> f3 = load_deltas<LQ>(yin + (size_t)c3 * T + q0);
       add_deltas<LQ>(acc, f0, true);
       add_deltas<LQ>(acc, f1, h1);
       add_deltas<LQ>(acc, f2, h2);
@@ -407,19 +358,18 @@ __device__ __forceinline__ QV<LQ> block_slices_sum(QV<LQ> v) {
 // Row sums leave the gather phase in the slice-0 lanes, LQ queries each;
 // they go through LDS so that the epilogue runs one query per lane (lane
 // q = tid % T always handles query q), with one query's state in registers.
-template <int T, typename DT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
-void ppr_sweep_kernel(PprArgs a) {
-  constexpr int LQ = tile_lane_queries(sizeof(DT) == 2);
+template <int T>
+__global__ __launch_bounds__(kBlock) void ppr_sweep_kernel(PprArgs a) {
+  constexpr int LQ = kLaneQ;
   constexpr int Q = T / LQ;  // lanes per edge
   static_assert(Q >= 1 && kBlock % T == 0, "lane layout");
   __shared__ double rs[kBlock * LQ];  // row sums of a pass: rows (<= kBlock / Q) x T
   const TileState *ts = a.ts;
   if (ts->all_done) return;
   const int ycur = ts->yd_cur;
-  a.yin = ycur ? a.yd1 : a.yd0;
-  a.yout = ycur ? a.yd0 : a.yd1;
-  const DT *yin = static_cast<const DT *>(a.yin);
+  a.yin32 = ycur ? a.yd1 : a.yd0;
+  a.yout32 = ycur ? a.yd0 : a.yd1;
+  const float *yin = a.yin32;
   const double *y64 = ts->y64_par ? a.y64b : a.y64a;
   // gather phase: queries q0 .. q0 + LQ - 1 of this lane
   const int qg = threadIdx.x % Q;
@@ -438,7 +388,7 @@ void ppr_sweep_kernel(PprArgs a) {
   // epilogue phase: query q of this lane
   const int q = threadIdx.x % T;
   const int mode = ts->mode[q];
-  const QState qs = load_qstate<DT>(ts, q);
+  const QState qs = load_qstate(ts, q);
   int lq = 0;
   while ((1 << lq) < Q) lq++;
   Red red;
@@ -490,7 +440,7 @@ void ppr_sweep_kernel(PprArgs a) {
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             const int p = (int)threadIdx.x + (j0 + j) * kBlock;
-            if (p < pairs) ppr_epilogue<T, DT>(a, qs, mode, base + p / T, q, rs[p], ld[j], red);
+            if (p < pairs) ppr_epilogue<T>(a, qs, mode, base + p / T, q, rs[p], ld[j], red);
           }
         }
         __syncthreads();
@@ -518,13 +468,28 @@ void ppr_sweep_kernel(PprArgs a) {
       if (threadIdx.x < T) {
         if (r >= 0) {
           const EpiLoad ld = qs.live ? epi_load<T>(a, qs, mode, r, q) : EpiLoad{};
-          ppr_epilogue<T, DT>(a, qs, mode, r, q, rs[q], ld, red);
+          ppr_epilogue<T>(a, qs, mode, r, q, rs[q], ld, red);
         } else {
           const int h = bd.y, chunk = bd.x & (kHubFlag - 1);
           a.hub_partial[(size_t)(a.hub_first[h] + chunk) * T + q] = rs[q];
         }
       }
       __syncthreads();
+    }
+  }
+  // Rows without in-edges (ids from n_in: the rows are sorted by in-degree)
+  // get delta = teleport only, which is 0 for every query they do not seed:
+  // only the ones that seed some query of the tile are swept, from their
+  // seed entries (sorted by row; the first entry of each row takes it).
+  {
+    constexpr int RPB = kBlock / T;  // rows per workgroup pass
+    const int j0 = a.zseed_first;
+    for (int j = j0 + (int)blockIdx.x * RPB + (int)threadIdx.x / T; j < a.num_seed_entries;
+         j += (int)gridDim.x * RPB) {
+      const int r = a.seed_row[j];
+      if (j > j0 && a.seed_row[j - 1] == r) continue;  // the row's first entry sweeps it
+      const EpiLoad ld = qs.live ? epi_load<T>(a, qs, mode, r, q) : EpiLoad{};
+      ppr_epilogue<T>(a, qs, mode, r, q, 0.0, ld, red);
     }
   }
   tile_block_sum<T>(red, a.partials + (size_t)blockIdx.x * T * 5);
@@ -549,16 +514,16 @@ __global__ __launch_bounds__(kBlock) void ppr_reduce_kernel(PprArgs a, int sweep
 // Per-query bookkeeping (pagerank.rs:163-179 for every query) and each
 // query's next mode (the single-vector schedule of pagerank.hip).  One
 // workgroup.
-template <int T, typename DT>
+template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep_blocks) {
   TileState *ts = a.ts;
   if (ts->all_done) return;
   const int ycur = ts->yd_cur;
-  a.yin = ycur ? a.yd1 : a.yd0;
-  a.yout = ycur ? a.yd0 : a.yd1;
+  a.yin32 = ycur ? a.yd1 : a.yd0;
+  a.yout32 = ycur ? a.yd0 : a.yd1;
   const int q = threadIdx.x % T;
   const int mode = ts->mode[q];
-  const QState qs = load_qstate<DT>(ts, q);
+  const QState qs = load_qstate(ts, q);
   Red red;
   // hub rows: (hub, query) pairs; thread t always gets query t % T
   for (int p = threadIdx.x; p < a.num_hubs * T; p += kBlock) {
@@ -567,7 +532,7 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
     double s = 0.0;
     for (int c = 0; c < nch; c++) s += a.hub_partial[(size_t)(first + c) * T + q];
     const EpiLoad ld = qs.live ? epi_load<T>(a, qs, mode, a.hub_row[h], q) : EpiLoad{};
-    ppr_epilogue<T, DT>(a, qs, mode, a.hub_row[h], q, s, ld, red);
+    ppr_epilogue<T>(a, qs, mode, a.hub_row[h], q, s, ld, red);
   }
   const double *stage = a.partials + (size_t)kMaxSweepBlocks * T * 5;
   for (int p = threadIdx.x; p < kTileReduce * T; p += kBlock) {
@@ -616,12 +581,6 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
       next = anchor ? kModeDeltaFold : kModeDelta;
     }
     ts->mode[qq] = next;
-    if (!std::is_same<DT, float>::value) {
-      // the next sweep gathers what this one stored, and stores at the
-      // scale of this sweep's residual
-      ts->ysc_in[qq] = ts->ysc_out[qq];
-      ts->ysc_out[qq] = f16_scale(diff);
-    }
     const uint64_t it = ts->iter[qq] + 1;
     ts->iter[qq] = it;
     ts->diff[qq] = diff;
@@ -656,7 +615,7 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
 // the next sweep gathers, and, for an anchor admission, of x and of the y64
 // buffer the next sweep's anchors gather.  One thread per row and group of four queries;
 // the tile's other queries are not touched.
-template <int T, typename DT>
+template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uint64_t clear_mask) {
   constexpr int Q = T / kLaneQ;
   const int qg = threadIdx.x % Q;
@@ -666,13 +625,41 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
   // first sweep reads only the gathered deltas: it takes x and the carry as
   // p-hat and its rounding (ppr_epilogue), and a fold sweep writes the whole
   // y64 column before any anchor reads it.
+  // Rows without in-edges (r >= n_in) are swept only while they are seeds of
+  // some query of the tile (ppr_sweep_kernel); every other sweep leaves
+  // them alone, so the admitted columns of all their vectors start at 0.
   const bool anchored = a.ts->f32_below > 0.0;
   double *y64 = anchored ? (a.ts->y64_par ? a.y64b : a.y64a) : nullptr;
-  DT *yin = static_cast<DT *>(a.ts->yd_cur ? a.yd1 : a.yd0);
+  float *yin = a.ts->yd_cur ? a.yd1 : a.yd0;
+  float *yout = a.ts->yd_cur ? a.yd0 : a.yd1;
   const int rows_per_block = kBlock / Q;
   for (int r = blockIdx.x * rows_per_block + threadIdx.x / Q; r < n;
        r += gridDim.x * rows_per_block) {
     const size_t at = (size_t)r * T + qg * kLaneQ;
+    if (r >= a.n_in) {
+      if (m4 == 0xFu) {
+        const double2 z = {0.0, 0.0};
+        const float4 zf = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (double *v : {a.x, a.y64a, a.y64b}) {
+          *reinterpret_cast<double2 *>(v + at) = z;
+          *reinterpret_cast<double2 *>(v + at + 2) = z;
+        }
+        for (float *v : {a.carry, yin, yout}) *reinterpret_cast<float4 *>(v + at) = zf;
+      } else {
+#pragma unroll
+        for (int j = 0; j < kLaneQ; j++) {
+          if ((m4 >> j) & 1u) {
+            a.x[at + j] = 0.0;
+            a.carry[at + j] = 0.0f;
+            yin[at + j] = 0.0f;
+            yout[at + j] = 0.0f;
+            a.y64a[at + j] = 0.0;
+            a.y64b[at + j] = 0.0;
+          }
+        }
+      }
+      continue;
+    }
     if (m4 == 0xFu) {
       const double2 z = {0.0, 0.0};
       if (anchored) {
@@ -681,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
         *reinterpret_cast<double2 *>(y64 + at) = z;
         *reinterpret_cast<double2 *>(y64 + at + 2) = z;
       }
-      DeltaStore<DT>::zero4(yin + at);
+      *reinterpret_cast<float4 *>(yin + at) = float4{0.0f, 0.0f, 0.0f, 0.0f};
     } else {
 #pragma unroll
       for (int j = 0; j < kLaneQ; j++) {
@@ -690,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
             a.x[at + j] = 0.0;
             y64[at + j] = 0.0;
           }
-          yin[at + j] = (DT)0.0f;
+          yin[at + j] = 0.0f;
         }
       }
     }
@@ -703,15 +690,12 @@ __global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uin
 // anchor sweep over y64 = d p-hat / deg.  Without, it is a delta sweep whose
 // input delta is p-hat itself: d A p-hat + d (dangling(p-hat) - sum(p-hat))
 // p-hat is T(p-hat) - p-hat, as sum(p-hat) = 1 (ppr.rs:125-137).
-template <int T, typename DT>
+template <int T>
 __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t admit_mask) {
   TileState *ts = a.ts;
   const bool anchored = ts->f32_below > 0.0;
   double *y64 = ts->y64_par ? a.y64b : a.y64a;
-  DT *yin = static_cast<DT *>(ts->yd_cur ? a.yd1 : a.yd0);
-  // p-hat sums to 1: its deltas are at most 1, their first successors at
-  // most 3 d (the scale of a residual of 1)
-  const double s0 = std::is_same<DT, float>::value ? 1.0 : f16_scale(1.0);
+  float *yin = ts->yd_cur ? a.yd1 : a.yd0;
   for (int j = threadIdx.x; j < a.num_seed_entries; j += kBlock) {
     const int qq = a.seed_q[j];
     if (!((admit_mask >> qq) & 1ull)) continue;
@@ -723,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
       a.x[at] = p;  // one entry per (row, query): the host merges duplicate seeds
       y64[at] = v;
     } else {
-      yin[at] = DeltaStore<DT>::put(v, s0);  // x and the carry: ppr_epilogue's first sweep
+      yin[at] = delta_put(v);  // x and the carry: ppr_epilogue's first sweep
     }
   }
   __syncthreads();
@@ -732,8 +716,6 @@ __global__ __launch_bounds__(kBlock) void ppr_admit_kernel(PprArgs a, uint64_t a
     if ((admit_mask >> qq) & 1ull) {
       ts->dsum_x[qq] = ts->init_dsum[qq];
       ts->x_scale[qq] = 1.0;
-      ts->ysc_in[qq] = s0;
-      ts->ysc_out[qq] = s0;
       ts->live[qq] = 1;
       ts->iter[qq] = 0;
       ts->converged[qq] = 0;
@@ -788,54 +770,42 @@ __global__ void seed_mark_kernel(int32_t *seed_idx, const int32_t *rows, int cou
 }
 
 // ---------------------------------------------------------------------------
-template <int T, typename DT>
+template <int T>
 static hipError_t launch_sweep_t(hipStream_t s, const PprArgs &a, int grid) {
-  ppr_sweep_kernel<T, DT><<<grid, kBlock, 0, s>>>(a);
+  ppr_sweep_kernel<T><<<grid, kBlock, 0, s>>>(a);
   ppr_reduce_kernel<T><<<kTileReduce, kBlock, 0, s>>>(a, grid);
-  ppr_finish_kernel<T, DT><<<1, kBlock, 0, s>>>(a, grid);
+  ppr_finish_kernel<T><<<1, kBlock, 0, s>>>(a, grid);
   return hipGetLastError();
 }
 
-template <typename DT>
-static hipError_t launch_sweep_dt(hipStream_t s, int T, const PprArgs &a, int grid) {
+hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid) {
   switch (T) {
-    case 8: return launch_sweep_t<8, DT>(s, a, grid);
-    case 16: return launch_sweep_t<16, DT>(s, a, grid);
-    case 32: return launch_sweep_t<32, DT>(s, a, grid);
-    default: return launch_sweep_t<64, DT>(s, a, grid);
+    case 8: return launch_sweep_t<8>(s, a, grid);
+    case 16: return launch_sweep_t<16>(s, a, grid);
+    case 32: return launch_sweep_t<32>(s, a, grid);
+    default: return launch_sweep_t<64>(s, a, grid);
   }
 }
 
-hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid) {
-  return a.delta16 ? launch_sweep_dt<_Float16>(s, T, a, grid) : launch_sweep_dt<float>(s, T, a, grid);
-}
-
-template <int T, typename DT>
+template <int T>
 static hipError_t launch_admit_t(hipStream_t s, const PprArgs &a, int n, uint64_t clear_mask,
                                  uint64_t admit_mask) {
   const int rows_per_block = kBlock / (T / kLaneQ);
   int grid = (n + rows_per_block - 1) / rows_per_block;
   grid = std::max(1, std::min(grid, 4096));
-  ppr_clear_kernel<T, DT><<<grid, kBlock, 0, s>>>(a, n, clear_mask);
-  ppr_admit_kernel<T, DT><<<1, kBlock, 0, s>>>(a, admit_mask);
+  ppr_clear_kernel<T><<<grid, kBlock, 0, s>>>(a, n, clear_mask);
+  ppr_admit_kernel<T><<<1, kBlock, 0, s>>>(a, admit_mask);
   return hipGetLastError();
-}
-
-template <typename DT>
-static hipError_t launch_admit_dt(hipStream_t s, int T, const PprArgs &a, int n,
-                                  uint64_t clear_mask, uint64_t admit_mask) {
-  switch (T) {
-    case 8: return launch_admit_t<8, DT>(s, a, n, clear_mask, admit_mask);
-    case 16: return launch_admit_t<16, DT>(s, a, n, clear_mask, admit_mask);
-    case 32: return launch_admit_t<32, DT>(s, a, n, clear_mask, admit_mask);
-    default: return launch_admit_t<64, DT>(s, a, n, clear_mask, admit_mask);
-  }
 }
 
 hipError_t ppr_launch_admit(hipStream_t s, int T, const PprArgs &a, int n, uint64_t clear_mask,
                             uint64_t admit_mask) {
-  return a.delta16 ? launch_admit_dt<_Float16>(s, T, a, n, clear_mask, admit_mask)
-                   : launch_admit_dt<float>(s, T, a, n, clear_mask, admit_mask);
+  switch (T) {
+    case 8: return launch_admit_t<8>(s, a, n, clear_mask, admit_mask);
+    case 16: return launch_admit_t<16>(s, a, n, clear_mask, admit_mask);
+    case 32: return launch_admit_t<32>(s, a, n, clear_mask, admit_mask);
+    default: return launch_admit_t<64>(s, a, n, clear_mask, admit_mask);
+  }
 }
 
 hipError_t ppr_launch_gather(hipStream_t s, int T, const PprArgs &a, const uint32_t *rank, int n,

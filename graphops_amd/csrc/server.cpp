@@ -57,18 +57,17 @@ inline int lanes_pow2(int want, int cap) {
 }
 
 // Work descriptors of the tile sweep for tile width T (rows already sorted
-// by in-degree).  One edge's T queries take Q = T / lq lanes (lq queries per
-// lane: tile_lane_queries); a row gets
+// by in-degree).  One edge's T queries take Q = T / kLaneQ lanes; a row gets
 // 2^s slices of Q lanes, about four edges per slice.  Runs of rows share a
 // slice count; rows long enough to give every slice of a workgroup four
 // edges get the whole workgroup; rows longer than kHubChunk are split into
 // the graph's hub chunks.
-void plan_tile(const std::vector<int32_t> &rp, int T, int lq, const DevGraph &dg,
+void plan_tile(const std::vector<int32_t> &rp, int64_t n_in, int T, const DevGraph &dg,
                const std::vector<int32_t> &hub_row, const std::vector<int32_t> &hub_nchunks,
                std::vector<BlockDesc> &desc) {
   desc.clear();
-  const int64_t n = (int64_t)rp.size() - 1;
-  const int Q = std::max(1, T / lq);
+  const int64_t n = std::min<int64_t>((int64_t)rp.size() - 1, n_in);  // rows with in-edges
+  const int Q = T / kLaneQ;
   const int smax = 64 / Q;  // slices of a row within one wave
   const int block_row_min = 4 * (kBlock / Q);
   int h = 0;
@@ -126,14 +125,13 @@ struct Tile {
   size_t n = 0;
   double *x = nullptr, *y64a = nullptr, *y64b = nullptr, *partials = nullptr,
          *hub_partial = nullptr;
-  void *yd0 = nullptr, *yd1 = nullptr;  // f32 or f16 (delta16)
-  float *carry = nullptr;
-  int delta16 = 0;
+  float *yd0 = nullptr, *yd1 = nullptr, *carry = nullptr;
   int32_t *seed_idx = nullptr;
   int32_t *seed_row = nullptr, *seed_q = nullptr;
   double *seed_p = nullptr;
   int seed_cap = 0;
   int num_seed = 0;
+  int zseed_first = 0;  // first seed entry (sorted by row) of a row without in-edges
   std::vector<int32_t> marked_rows;  // rows currently marked in seed_idx
   TileState *ts = nullptr;
   TileState *hts = nullptr;  // pinned host copy
@@ -160,7 +158,6 @@ struct gops_server {
   gops_graph *g = nullptr;
   bool holds_graph = false;  // a reference on g (taken once creation succeeds)
   int T = 16;
-  int delta16 = 0;  // the tiles' delta storage, fixed at creation (GOPS_OPT_TILE_DELTA)
   std::vector<Tile *> tiles;
   std::deque<Query> queue;
   struct Done {
@@ -194,10 +191,10 @@ struct PprGraphCache {
   std::vector<int32_t> row_ptr;
   std::vector<uint8_t> dangling;  // new ids, out-degree 0
   std::vector<int32_t> hub_row, hub_nchunks;
-  TilePlan plans[2][4];  // per delta storage (f32, f16) and tile width
-  int tuned_T[2] = {};          // autotuned tile width per delta storage (f32, f16; 0: not
-                                // measured yet)
-  double tile_cost[2][4] = {};  // measured ms per query-sweep of widths 8/16/32/64
+  int32_t n_in = 0;  // rows with in-edges: [0, n_in) (rows are sorted by in-degree, descending)
+  TilePlan plans[4];  // per tile width
+  int tuned_T = 0;          // autotuned tile width (0: not measured yet)
+  double tile_cost[4] = {}; // measured ms per query-sweep of widths 8/16/32/64
 };
 
 }  // namespace gops
@@ -219,6 +216,15 @@ gops_status graph_cache(gops_context *ctx, gops_graph *g, PprGraphCache **out) {
     e = hipMemcpy(inv.data(), g->dev.inv_deg64, n * sizeof(double), hipMemcpyDeviceToHost);
   c->dangling.resize(n);
   for (size_t i = 0; i < n; i++) c->dangling[i] = inv[i] == 0.0;
+  // the rows without in-edges come last; if they ever did not, every row
+  // is swept (n_in = n)
+  {
+    size_t r = n;
+    while (r > 0 && c->row_ptr[r] == c->row_ptr[r - 1]) r--;
+    bool tail = true;
+    for (size_t i = 0; i < r && tail; i++) tail = c->row_ptr[i + 1] > c->row_ptr[i];
+    c->n_in = (int32_t)(tail ? r : n);
+  }
   c->hub_row.resize(g->dev.num_hubs);
   c->hub_nchunks.resize(g->dev.num_hubs);
   if (e == hipSuccess && g->dev.num_hubs)
@@ -238,13 +244,11 @@ gops_status graph_cache(gops_context *ctx, gops_graph *g, PprGraphCache **out) {
 
 int plan_slot(int T) { return T == 8 ? 0 : T == 16 ? 1 : T == 32 ? 2 : 3; }
 
-gops_status tile_plan(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, int delta16,
-                      TilePlan **out) {
-  TilePlan &p = c->plans[delta16 ? 1 : 0][plan_slot(T)];
+gops_status tile_plan(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, TilePlan **out) {
+  TilePlan &p = c->plans[plan_slot(T)];
   if (!p.desc) {
     std::vector<BlockDesc> desc;
-    plan_tile(c->row_ptr, T, tile_lane_queries(delta16 != 0), g->dev, c->hub_row,
-              c->hub_nchunks, desc);
+    plan_tile(c->row_ptr, c->n_in, T, g->dev, c->hub_row, c->hub_nchunks, desc);
     STRY(ctx, hipMalloc((void **)&p.desc, std::max<size_t>(desc.size(), 1) * sizeof(BlockDesc)));
     STRY(ctx, hipMemcpy(p.desc, desc.data(), desc.size() * sizeof(BlockDesc),
                         hipMemcpyHostToDevice));
@@ -275,21 +279,17 @@ void free_tile(Tile *t) {
   delete t;
 }
 
-// x, y64a, y64b (f64), the two stored deltas (f32 or f16) and the carry (f32)
-// per (node, query), plus seed_idx per node
-size_t tile_bytes(size_t n, int T, int delta16) {
-  return n * (size_t)T * (8 + 8 + 8 + (delta16 ? 2 + 2 : 4 + 4) + 4) + n * 4;
-}
+// x, y64a, y64b (f64), the two stored deltas and the carry (f32) per
+// (node, query), plus seed_idx per node
+size_t tile_bytes(size_t n, int T) { return n * (size_t)T * (8 + 8 + 8 + 4 + 4 + 4) + n * 4; }
 
-gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, int delta16, Tile **out) {
+gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile **out) {
   auto *t = new (std::nothrow) Tile;
   if (!t) return sfail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
   t->T = T;
   t->n = g->dev.n;
   t->grid = grid;
-  t->delta16 = delta16;
   const size_t nt = t->n * (size_t)T;
-  const size_t yb = delta16 ? 2 : 4;  // bytes per stored delta
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t r) {
     if (r != hipSuccess && e == hipSuccess) e = r;
@@ -297,8 +297,8 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, int de
   chk(hipMalloc((void **)&t->x, std::max<size_t>(nt, 1) * 8));
   chk(hipMalloc((void **)&t->y64a, std::max<size_t>(nt, 1) * 8));
   chk(hipMalloc((void **)&t->y64b, std::max<size_t>(nt, 1) * 8));
-  chk(hipMalloc(&t->yd0, std::max<size_t>(nt, 1) * yb));
-  chk(hipMalloc(&t->yd1, std::max<size_t>(nt, 1) * yb));
+  chk(hipMalloc((void **)&t->yd0, std::max<size_t>(nt, 1) * 4));
+  chk(hipMalloc((void **)&t->yd1, std::max<size_t>(nt, 1) * 4));
   chk(hipMalloc((void **)&t->carry, std::max<size_t>(nt, 1) * 4));
   chk(hipMalloc((void **)&t->partials, (size_t)(kMaxSweepBlocks + kTileReduce) * T * 5 * 8));
   chk(hipMalloc((void **)&t->hub_partial, (size_t)std::max(g->dev.num_hub_chunks, 1) * T * 8));
@@ -306,8 +306,8 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, int de
   chk(hipMalloc((void **)&t->ts, sizeof(TileState)));
   chk(hipHostMalloc((void **)&t->hts, sizeof(TileState), hipHostMallocDefault));
   if (e == hipSuccess) chk(hipMemsetAsync(t->seed_idx, 0xff, std::max<size_t>(t->n, 1) * 4, ctx->stream));
-  if (e == hipSuccess) chk(hipMemsetAsync(t->yd0, 0, std::max<size_t>(nt, 1) * yb, ctx->stream));
-  if (e == hipSuccess) chk(hipMemsetAsync(t->yd1, 0, std::max<size_t>(nt, 1) * yb, ctx->stream));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->yd0, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
+  if (e == hipSuccess) chk(hipMemsetAsync(t->yd1, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
   if (e == hipSuccess) chk(hipMemsetAsync(t->carry, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
   if (e == hipSuccess) chk(hipStreamSynchronize(ctx->stream));
   if (e != hipSuccess) {
@@ -342,7 +342,6 @@ PprArgs tile_args(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, 
   a.x = t->x;
   a.yd0 = t->yd0;
   a.yd1 = t->yd1;
-  a.delta16 = t->delta16;
   a.carry = t->carry;
   a.y64a = t->y64a;
   a.y64b = t->y64b;
@@ -357,6 +356,8 @@ PprArgs tile_args(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, 
   a.seed_q = t->seed_q;
   a.seed_p = t->seed_p;
   a.num_seed_entries = t->num_seed;
+  a.zseed_first = t->zseed_first;
+  a.n_in = g->ppr_cache ? g->ppr_cache->n_in : (int)g->dev.n;
   a.mirror = mirror ? ctx->mirror_dev : nullptr;
   return a;
 }
@@ -443,7 +444,7 @@ gops_status make_query(gops_context *ctx, gops_graph *g, PprGraphCache *c,
 }
 
 // Upload the seed entries of every used slot and re-mark seed_idx.
-gops_status upload_seeds(gops_context *ctx, Tile *t) {
+gops_status upload_seeds(gops_context *ctx, Tile *t, int32_t g_n_in) {
   struct E {
     int32_t row, q;
     double p;
@@ -489,6 +490,13 @@ gops_status upload_seeds(gops_context *ctx, Tile *t) {
     STRY(ctx, hipStreamSynchronize(ctx->stream));
   }
   t->num_seed = cnt;
+  t->zseed_first = cnt;
+  const int32_t n_in = g_n_in;
+  for (int j = 0; j < cnt; j++)
+    if (rows[j] >= n_in) {
+      t->zseed_first = j;
+      break;
+    }
   t->marked_rows = rows;
   return GOPS_OK;
 }
@@ -510,10 +518,11 @@ void set_slot_params(Tile *t, int s, const Query &q, double f32_below, uint64_t 
   h->anchor_period = period;
 }
 
-// Admit the queries in `admit` (slot mask) into tile t: fold the live ones,
-// initialise the new ones; the next sweep is an anchor.
+// Admit the queries in `mask` (slot mask) into tile t: upload the tile's
+// seed entries, clear the admitted columns and start the new queries; the
+// running queries keep their state.
 gops_status admit(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, uint64_t mask) {
-  gops_status st = upload_seeds(ctx, t);
+  gops_status st = upload_seeds(ctx, t, g->ppr_cache ? g->ppr_cache->n_in : (int32_t)g->dev.n);
   if (st != GOPS_OK) return st;
   t->hts->all_done = 0;
   STRY(ctx, hipMemcpyAsync(t->ts, t->hts, sizeof(TileState), hipMemcpyHostToDevice, ctx->stream));
@@ -544,10 +553,10 @@ int tile_grid(const gops_graph *g) {
 gops_status trial_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int T, double *cost) {
   *cost = std::numeric_limits<double>::infinity();
   TilePlan *plan = nullptr;
-  gops_status st = tile_plan(ctx, g, c, T, ctx->tile_delta == GOPS_TILE_DELTA_F16, &plan);
+  gops_status st = tile_plan(ctx, g, c, T, &plan);
   if (st != GOPS_OK) return st;
   Tile *t = nullptr;
-  st = alloc_tile(ctx, g, T, tile_grid(g), ctx->tile_delta == GOPS_TILE_DELTA_F16, &t);
+  st = alloc_tile(ctx, g, T, tile_grid(g), &t);
   if (st == GOPS_OUT_OF_MEMORY) {
     ctx->last_error.clear();
     (void)hipGetLastError();  // the failed allocation's error is not a later launch's
@@ -607,25 +616,24 @@ gops_status pick_tile(gops_context *ctx, gops_graph *g, PprGraphCache *c, int nq
     *T = ctx->ppr_tile;
     return GOPS_OK;
   }
-  const int sk = ctx->tile_delta == GOPS_TILE_DELTA_F16;  // widths are tuned per storage
-  if (!c->tuned_T[sk]) {
+  if (!c->tuned_T) {
     double best = std::numeric_limits<double>::infinity();
     int best_T = 0;
     for (int w : {8, 16, 32, 64}) {
       double cost;
       gops_status st = trial_tile(ctx, g, c, w, &cost);
       if (st != GOPS_OK) return st;
-      c->tile_cost[sk][plan_slot(w)] = cost;
+      c->tile_cost[plan_slot(w)] = cost;
       if (cost < best) {
         best = cost;
         best_T = w;
       }
     }
     if (!best_T) return sfail(ctx, GOPS_OUT_OF_MEMORY, "no tile width fits in device memory");
-    c->tuned_T[sk] = best_T;
+    c->tuned_T = best_T;
   }
   int w = 8;
-  while (w < nq && w < c->tuned_T[sk]) w <<= 1;
+  while (w < nq && w < c->tuned_T) w <<= 1;
   *T = w;
   return GOPS_OK;
 }
@@ -636,8 +644,7 @@ namespace gops {
 void free_ppr_cache(gops_graph *g) {
   PprGraphCache *c = g->ppr_cache;
   if (!c) return;
-  for (auto &row : c->plans)
-    for (TilePlan &p : row) hipFree(p.desc);
+  for (TilePlan &p : c->plans) hipFree(p.desc);
   delete c;
   g->ppr_cache = nullptr;
 }
@@ -669,10 +676,10 @@ gops_status gops_ppr_batch_run(gops_context *ctx, gops_graph *g, uint32_t nq,
   st = pick_tile(ctx, g, c, (int)std::min<uint32_t>(nq, 1u << 20), &T);
   if (st != GOPS_OK) return st;
   TilePlan *plan = nullptr;
-  st = tile_plan(ctx, g, c, T, ctx->tile_delta == GOPS_TILE_DELTA_F16, &plan);
+  st = tile_plan(ctx, g, c, T, &plan);
   if (st != GOPS_OK) return st;
   Tile *t = nullptr;
-  st = alloc_tile(ctx, g, T, tile_grid(g), ctx->tile_delta == GOPS_TILE_DELTA_F16, &t);
+  st = alloc_tile(ctx, g, T, tile_grid(g), &t);
   if (st != GOPS_OK) return st;
   double *dout = nullptr;  // one query's scores in the caller's ids
   if (scores && hipMalloc((void **)&dout, n * sizeof(double)) != hipSuccess) {
@@ -838,8 +845,7 @@ gops_status gops_ppr_tile_info(gops_context *ctx, gops_graph *g, uint32_t nq, ui
   if (saved < 0) T = 0;
   if (tile) *tile = (uint32_t)T;
   if (ms_per_query_sweep)
-    for (int i = 0; i < 4; i++)
-      ms_per_query_sweep[i] = c->tile_cost[ctx->tile_delta == GOPS_TILE_DELTA_F16][i];
+    for (int i = 0; i < 4; i++) ms_per_query_sweep[i] = c->tile_cost[i];
   return GOPS_OK;
 }
 
@@ -867,25 +873,23 @@ gops_status gops_server_create(gops_context *ctx, gops_graph *g, const gops_serv
   size_t free_b = 0, total_b = 0;
   STRY(ctx, hipMemGetInfo(&free_b, &total_b));
   double budget = (cfg && cfg->hbm_budget_bytes > 0) ? cfg->hbm_budget_bytes : 0.9 * (double)free_b;
-  const int delta16 = ctx->tile_delta == GOPS_TILE_DELTA_F16;
-  const size_t per_tile = tile_bytes(g->dev.n, T, delta16);
+  const size_t per_tile = tile_bytes(g->dev.n, T);
   uint32_t ntiles = (uint32_t)std::floor(budget / (double)per_tile);
   if (cfg && cfg->max_slots) ntiles = std::min<uint32_t>(ntiles, (cfg->max_slots + T - 1) / T);
   if (ntiles == 0) return sfail(ctx, GOPS_OUT_OF_MEMORY, "HBM budget holds no query tile");
   TilePlan *plan = nullptr;
-  st = tile_plan(ctx, g, c, T, delta16, &plan);
+  st = tile_plan(ctx, g, c, T, &plan);
   if (st != GOPS_OK) return st;
   auto *s = new (std::nothrow) gops_server;
   if (!s) return sfail(ctx, GOPS_OUT_OF_MEMORY, "host allocation failed");
   s->ctx = ctx;
   s->g = g;
   s->T = T;
-  s->delta16 = delta16;
   s->topk = cfg ? cfg->topk : 0;
   s->t0 = std::chrono::steady_clock::now();
   for (uint32_t i = 0; i < ntiles; i++) {
     Tile *t = nullptr;
-    st = alloc_tile(ctx, g, T, tile_grid(g), delta16, &t);
+    st = alloc_tile(ctx, g, T, tile_grid(g), &t);
     if (st != GOPS_OK) {
       if (i == 0) {
         delete s;
@@ -945,7 +949,7 @@ gops_status gops_server_step(gops_server *s, uint32_t sweeps, uint32_t *active) 
   gops_status st = graph_cache(ctx, g, &c);
   if (st != GOPS_OK) return st;
   TilePlan *plan = nullptr;
-  st = tile_plan(ctx, g, c, s->T, s->delta16, &plan);
+  st = tile_plan(ctx, g, c, s->T, &plan);
   if (st != GOPS_OK) return st;
   const int n = (int)g->dev.n;
   // 1. admission (host state is current: no kernel is queued).  With the

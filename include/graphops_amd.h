@@ -122,23 +122,12 @@ enum {
                                    accumulate pass per bin of rows, both streaming,
                                    integer row sums; its layout is built on the graph's
                                    first such run) */,
-    GOPS_OPT_ANCHOR_SWEEP = 14, /* mixed, one vector per run, with blocked delta sweeps:
+    GOPS_OPT_ANCHOR_SWEEP = 14  /* mixed, one vector per run, with blocked delta sweeps:
                                    how the f64 anchor sweeps read the graph:
                                    GOPS_ANCHOR_SWEEP_AUTO (default: blocked as well,
                                    f64 run sums added as 128-bit fixed point) or _PULL
                                    (gathers of the f64 iterate over the CSR) */
-    GOPS_OPT_TILE_DELTA = 15    /* batched and served personalized queries (tiles): how
-                                   the deltas every sweep gathers are stored:
-                                   GOPS_TILE_DELTA_F32 (default: iteration counts and
-                                   residuals those of the f64 reference, L1 near 1e-14)
-                                   or _F16 (half the gathered bytes; f16 scaled per
-                                   query by a power of two, each row carrying its
-                                   rounding into its next delta: L1 about 1e-2 of the
-                                   tolerance, residuals within a few percent, so an
-                                   iteration count may differ by one).  Taken when a
-                                   batch or a server starts */
 };
-enum { GOPS_TILE_DELTA_F32 = 0, GOPS_TILE_DELTA_F16 = 1 };
 enum { GOPS_DELTA_SWEEP_AUTO = 0, GOPS_DELTA_SWEEP_PULL = 1, GOPS_DELTA_SWEEP_BLOCKED = 2 };
 enum { GOPS_ANCHOR_SWEEP_AUTO = 0, GOPS_ANCHOR_SWEEP_PULL = 1 };
 enum { GOPS_LAUNCH_AUTO = 0, GOPS_LAUNCH_QUEUED = 1, GOPS_LAUNCH_GRAPH = 2 };

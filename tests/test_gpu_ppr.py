@@ -137,6 +137,48 @@ def test_batch_refill_same_as_alone(ctx, graph13):
         assert np.array_equal(s1[0], scores[q])
 
 
+def test_batch_zero_in_degree_seeds(oracle, ctx, graph13):
+    """Rows without in-edges are swept only while they seed a query of the
+    tile.  Queries seeded on such nodes (with out-edges, and isolated ones),
+    alone and mixed with ordinary seeds, through refilled slots (a slot's
+    previous query may have seeded other such rows): every result matches
+    the oracle."""
+    e = graph13
+    indeg = np.bincount(np.asarray(e.targets, dtype=np.int64), minlength=e.n)
+    outdeg = np.diff(e.offsets.astype(np.int64))
+    src_only = np.nonzero((indeg == 0) & (outdeg > 0))[0]
+    isolated = np.nonzero((indeg == 0) & (outdeg == 0))[0]
+    assert src_only.size >= 40 and isolated.size >= 2
+    rng = np.random.default_rng(29)
+    sets = []
+    for q in range(30):
+        k = q % 5
+        if k == 0:
+            s = rng.choice(src_only, 3, replace=False)
+        elif k == 1:
+            s = np.array([isolated[q % isolated.size]])
+        elif k == 2:
+            s = np.concatenate([rng.choice(src_only, 2, replace=False),
+                                rng.integers(0, e.n, 3)])
+        elif k == 3:
+            s = np.concatenate([isolated[:2], rng.choice(src_only, 1)])
+        else:
+            s = rng.integers(0, e.n, 4)
+        sets.append(np.asarray(s, dtype=np.uint32))
+    off = np.zeros(len(sets) + 1, np.uint64)
+    off[1:] = np.cumsum([s.size for s in sets])
+    seeds = np.concatenate(sets)
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
+    cfgs = [ga.PageRankConfig([0.5, 0.85, 0.95][q % 3], 400, [1e-8, 1e-10][q % 2])
+            for q in range(len(sets))]
+    scores, runs = ga.ppr_batch_run(dg, cfgs, off, seeds, ctx=ctx, tile=8)
+    for q in range(len(sets)):
+        p = dense_p(e.n, sets[q])
+        ref = oracle_ppr(oracle, e, p, cfgs[q].damping, cfgs[q].tolerance, 400)
+        check(runs[q].iterations, runs[q].converged, runs[q].diff_l1, scores[q], ref,
+              cfgs[q].tolerance)
+
+
 def test_batch_graph_replay_same_bits(ctx, graph13):
     """GOPS_LAUNCH_GRAPH: the tile sweeps replayed from captured chains of
     four (captured again after each admission) give the bits of the queued
@@ -175,71 +217,6 @@ def test_batch_graph_replay_same_bits(ctx, graph13):
         assert np.array_equal(served[q].scores, s8[q])
 
 
-def check_f16(run, scores, ref, tol):
-    """The bar of f16 delta storage (GOPS_OPT_TILE_DELTA = F16).  Each
-    stored delta rounds to 11 bits and the row carries the rounding into its
-    next one, so the iterate stays within 1e-2 to 1e-1 of the tolerance of
-    the f64 oracle (more sweeps, more roundings), while a residual moves by
-    up to a few percent: converged equal; iterations equal, or one apart
-    where the oracle's last two residuals straddle the tolerance within 10 %;
-    residuals within 10 % when the counts agree; L1 within 0.25 tol + 1e-12
-    (counts equal) or 2 tol (one sweep apart: about the last delta's L1)."""
-    assert run.converged == ref.converged
-    if run.iterations != ref.iterations:
-        assert abs(run.iterations - ref.iterations) == 1
-        assert abs(ref.diff_l1 - tol) <= 0.1 * tol or abs(run.diff_l1 - tol) <= 0.1 * tol
-        bound = 2 * tol
-    else:
-        assert abs(run.diff_l1 - ref.diff_l1) <= 0.1 * ref.diff_l1 + 1e-15
-        bound = 0.25 * tol + 1e-12
-    l1 = np.abs(scores - ref.scores).sum()
-    assert l1 <= (bound if ref.converged else 1e-6), l1
-    return l1
-
-
-@pytest.mark.parametrize("tile", [16, 64])
-def test_batch_f16_deltas_match_oracle(oracle, ctx, graph13, tile):
-    """f16 delta storage against the f64 oracle over dampings 0.5/0.85/0.95
-    and tolerances 1e-6/1e-8/1e-10, with refills (more queries than
-    slots)."""
-    e = graph13
-    nq = 3 * tile // 2 + 6
-    off, seeds = generators.seed_sets(e, nq, 1, 64, seed=40 + tile)
-    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
-    cfgs = [ga.PageRankConfig([0.5, 0.85, 0.95][q % 3], 400, [1e-8, 1e-6, 1e-10][(q // 3) % 3])
-            for q in range(nq)]
-    scores, runs = ga.ppr_batch_run(dg, cfgs, off, seeds, ctx=ctx, tile=tile, delta="f16")
-    worst, moved = {}, 0
-    for q in range(nq):
-        p = dense_p(e.n, seeds[off[q]:off[q + 1]])
-        tol = cfgs[q].tolerance
-        ref = oracle_ppr(oracle, e, p, cfgs[q].damping, tol, 400)
-        l1 = check_f16(runs[q], scores[q], ref, tol)
-        if runs[q].iterations == ref.iterations:
-            worst[tol] = max(worst.get(tol, 0.0), l1 / tol)
-        else:
-            moved += 1
-    print(f"f16 deltas, tile {tile}: worst L1 / tol by tol {worst}, {moved} of {nq} counts moved")
-
-
-def test_batch_f16_same_as_alone(ctx, graph13):
-    """With f16 storage too a query's arithmetic (its scale included) depends
-    only on itself: alone and in a refilled mix it gives the same bits."""
-    e = graph13
-    nq = 20
-    off, seeds = generators.seed_sets(e, nq, 1, 32, seed=12)
-    dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
-    cfgs = [ga.PageRankConfig([0.5, 0.95, 0.85][q % 3], 300, [1e-6, 1e-9][q % 2])
-            for q in range(nq)]
-    scores, runs = ga.ppr_batch_run(dg, cfgs, off, seeds, ctx=ctx, tile=8, delta="f16")
-    for q in [0, 5, 11, 19]:
-        s1, r1 = ga.ppr_batch_run(dg, [cfgs[q]], off[q:q + 2] - off[q], seeds[off[q]:off[q + 1]],
-                                  ctx=ctx, tile=8, delta="f16")
-        assert r1[0].iterations == runs[q].iterations
-        assert r1[0].diff_l1 == runs[q].diff_l1
-        assert np.array_equal(s1[0], scores[q])
-
-
 def test_plain_path_matches_tiles(ctx, graph13):
     """tile=-1: one query per device run (the single-vector chain)."""
     e = graph13
@@ -270,14 +247,12 @@ def test_batch_edge_cases(oracle, ctx, graph13):
         ga.ppr_batch_run(dg, cfg, np.array([0, 1], np.uint64), np.array([e.n], np.uint32), ctx=ctx)
 
 
-@pytest.mark.parametrize("delta", ["f32", "f16"])
-def test_server_continuous_batching(oracle, ctx, graph13, delta):
-    """Queries arrive while others run; each result equals the oracle (to the
-    bar of its delta storage) and, bit for bit, the same query served
-    alone."""
+def test_server_continuous_batching(oracle, ctx, graph13):
+    """Queries arrive while others run; each result equals the oracle and,
+    bit for bit, the same query served alone."""
     e = graph13
     dg = ga.DeviceGraph.from_edges(e, ctx=ctx)
-    srv = ga.Server(dg, tile=8, max_slots=16, delta=delta)
+    srv = ga.Server(dg, tile=8, max_slots=16)
     assert srv.stats()["slots"] == 16
     rng = np.random.default_rng(11)
     off, seeds = generators.seed_sets(e, 40, 1, 256, log_uniform=True, seed=17)
@@ -309,12 +284,9 @@ def test_server_continuous_batching(oracle, ctx, graph13, delta):
         p = dense_p(e.n, seeds[off[q]:off[q + 1]])
         ref = oracle_ppr(oracle, e, p, cfgs[q].damping, 1e-6, 500)
         run = results[q]
-        if delta == "f16":
-            check_f16(run, run.scores, ref, 1e-6)
-        else:
-            check(run.iterations, run.converged, run.diff_l1, run.scores, ref, 1e-6)
+        check(run.iterations, run.converged, run.diff_l1, run.scores, ref, 1e-6)
     # alone == in the mix
-    alone = ga.Server(dg, tile=8, max_slots=8, delta=delta)
+    alone = ga.Server(dg, tile=8, max_slots=8)
     for q in (0, 7, 23):
         alone.submit(cfgs[q], seeds[off[q]:off[q + 1]])
         got = None
