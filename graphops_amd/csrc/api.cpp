@@ -323,6 +323,10 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   hs.anchor_min = ppr ? ctx->ppr_anchor_min : 0.0;
   hs.f64_phase = hs.all_f64 || hs.f32_below > 0.0;
   hs.x_scale = 1.0;
+  // global runs with every sweep blocked keep the isolated rows' iterate in
+  // the state (the sweeps' SweepArgs::iso under the same condition)
+  hs.x_iso = hs.inv_n;
+  hs.n_iso = (setup.blocked && setup.anchors_blocked && !ppr && g->has_pb) ? (double)g->pb.n_iso : 0.0;
   hs.sweep_log = rb.sweep_log;
   hs.sweep_log_cap = kSweepLogCap;
   __atomic_store_n(ctx->mirror_host, (uint64_t)0, __ATOMIC_RELEASE);

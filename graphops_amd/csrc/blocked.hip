@@ -55,6 +55,20 @@ int bits_of(uint64_t x) {  // bits that hold every value in [0, x]
 
 // marks[row_ptr[v]] = v for rows with edges (marks zeroed): an inclusive max
 // scan then gives every edge its row
+// count[0] += rows without in-edges and out-edges; count[1] = max over the
+// other rows of (row + 1): the isolated rows are the tail [count[1], n) when
+// count[0] = n - count[1]
+__global__ void pb_iso_kernel(const int32_t *row_ptr, const float *inv_deg, int n,
+                              unsigned long long *count) {
+  unsigned long long iso = 0, last = 0;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    if (row_ptr[r + 1] == row_ptr[r] && inv_deg[r] == 0.0f) iso++;
+    else last = (unsigned long long)r + 1 > last ? (unsigned long long)r + 1 : last;
+  }
+  if (iso) atomicAdd(&count[0], iso);
+  if (last) atomicMax(&count[1], last);
+}
+
 __global__ void pb_mark_kernel(const int32_t *row_ptr, int n, int32_t *marks) {
   for (int v = blockIdx.x * kB + threadIdx.x; v < n; v += gridDim.x * kB)
     if (row_ptr[v] < row_ptr[v + 1]) marks[row_ptr[v]] = v;
@@ -533,7 +547,21 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       pb.rsum = reinterpret_cast<float *>(pb.rsum64);
     }
     if (!pb.seg_off) pb.seg_off = pool.get<int32_t>(1);
+    unsigned long long *iso_count = pool.get<unsigned long long>(2);
     BTRY(pool.err);
+    if (n) {
+      // the renumbering puts the isolated rows last; if they were not the
+      // tail, none is set apart (n_iso = 0)
+      unsigned long long h_count[2] = {0, 0};
+      BTRY(hipMemsetAsync(iso_count, 0, 2 * sizeof(unsigned long long), s));
+      pb_iso_kernel<<<(int)std::min<uint64_t>((n + 255) / 256, 2048), 256, 0, s>>>(
+          dg.row_ptr, dg.inv_deg32, (int)n, iso_count);
+      BTRY(hipGetLastError());
+      BTRY(hipMemcpyAsync(h_count, iso_count, sizeof(h_count), hipMemcpyDeviceToHost, s));
+      BTRY(hipStreamSynchronize(s));
+      pb.n_iso = (h_count[0] == n - h_count[1]) ? h_count[0] : 0;
+    }
+    pool.release(iso_count);
     if (!items1.empty())
       BTRY(hipMemcpyAsync(pb.items1, items1.data(), items1.size() * sizeof(PbItem),
                           hipMemcpyHostToDevice, s));

@@ -12,8 +12,9 @@
 //      a wave per long one;
 //   4. in-degrees: the targets radix-sorted, each id's run measured (no
 //      device-wide atomics: on a chip of several L2s they go to memory);
-//   5. the order: a stable radix sort of ids by descending in-degree, so
-//      that ties keep ascending ids;
+//   5. the order: a stable radix sort of ids by descending in-degree, the
+//      ids without in- and out-edges after the other ids without in-edges,
+//      so that ties keep ascending ids;
 //   6. row pointers: an exclusive scan of the in-degrees in that order;
 //   7. the edges: a stable radix sort of (row, new source id) keys carrying
 //      max(w, 0) in the storage type, so duplicate edges keep adapter order,
@@ -163,13 +164,17 @@ __global__ void runs_kernel(const uint32_t *t, uint64_t m, uint32_t *beg, uint32
   }
 }
 
-// indeg = end - beg; ascending key = descending in-degree; value = the id
-__global__ void relabel_keys_kernel(const uint32_t *beg, const uint32_t *end, uint64_t n,
-                                    uint32_t *indeg, uint32_t *key, uint32_t *val) {
+// indeg = end - beg; ascending key = descending in-degree, and among ids
+// without in-edges the isolated ones (no out-edges either) last: key =
+// 2^32 - 1 - (2 indeg + 1), or 2^32 - 1 for an isolated id (indeg < 2^31);
+// value = the id
+__global__ void relabel_keys_kernel(const uint32_t *beg, const uint32_t *end, const uint64_t *off,
+                                    uint64_t n, uint32_t *indeg, uint32_t *key, uint32_t *val) {
   GRID_STRIDE(v, n) {
     const uint32_t d = end[v] - beg[v];
     indeg[v] = d;
-    key[v] = 0xFFFFFFFFu - d;
+    const bool isolated = d == 0 && off[v + 1] == off[v];
+    key[v] = isolated ? 0xFFFFFFFFu : 0xFFFFFFFFu - (2u * d + 1u);
     val[v] = (uint32_t)v;
   }
 }
@@ -448,7 +453,8 @@ gops_status build_on_device(gops_context *ctx, uint64_t n, const uint64_t *off, 
       DTRY(hipcub::DeviceRadixSort::SortKeys(d_tmp, b, d_nb, d_marks, (int)m, 0, cb, s));
       runs_kernel<<<grid_for(m), kB, 0, s>>>(d_marks, m, d_key32, d_key32b);
     }
-    relabel_keys_kernel<<<grid_for(n), kB, 0, s>>>(d_key32, d_key32b, n, d_indeg, d_key32, d_val32);
+    relabel_keys_kernel<<<grid_for(n), kB, 0, s>>>(d_key32, d_key32b, d_off, n, d_indeg, d_key32,
+                                                   d_val32);
     DTRY(hipGetLastError());
 
     // 5-6: order, rank, row pointers

@@ -3,9 +3,11 @@
 // The reference materialises neighbour lists and out-degree / out-weight sums
 // on every call (pagerank.rs:136-137, 226-234; ppr.rs:112, 133).  Here that
 // happens once per graph: the adapter's neighbour lists are transposed into a
-// CSR of in-edges (pull form), rows are renumbered by in-degree (descending,
-// ties by id) so that rows of similar length sit together, and the per-row
-// work descriptors of the sweep kernels are planned.
+// CSR of in-edges (pull form), rows are renumbered by in-degree (descending;
+// among the rows without in-edges, the ones without out-edges last; ties by
+// id) so that rows of similar length sit together and the isolated rows form
+// the tail, and the per-row work descriptors of the sweep kernels are
+// planned.
 #include <omp.h>
 
 #include <algorithm>
@@ -98,16 +100,21 @@ gops_status build_csrt(uint64_t n, const uint64_t *off, const uint32_t *nb, cons
 #pragma omp parallel for schedule(static)
   for (int64_t j = 0; j < (int64_t)m; j++) __atomic_fetch_add(&indeg[nb[j]], 1u, __ATOMIC_RELAXED);
 
-  // order: in-degree descending, ties by ascending id (counting sort)
+  // order: in-degree descending, then (in-degree 0) nodes with out-edges
+  // before isolated ones, ties by ascending id (counting sort over the key
+  // 2 (maxdeg - indeg) + isolated)
   uint32_t maxdeg = 0;
   for (uint64_t v = 0; v < n; v++) maxdeg = std::max(maxdeg, indeg[v]);
-  std::vector<uint64_t> start((size_t)maxdeg + 2, 0);
-  for (uint64_t v = 0; v < n; v++) start[maxdeg - indeg[v] + 1]++;
+  auto key = [&](uint64_t v) {
+    return 2 * (size_t)(maxdeg - indeg[v]) + ((indeg[v] == 0 && off[v + 1] == off[v]) ? 1 : 0);
+  };
+  std::vector<uint64_t> start(2 * (size_t)maxdeg + 3, 0);
+  for (uint64_t v = 0; v < n; v++) start[key(v) + 1]++;
   for (size_t k = 1; k < start.size(); k++) start[k] += start[k - 1];
   g.order.assign(n, 0);
   g.rank.assign(n, 0);
   for (uint64_t v = 0; v < n; v++) {
-    uint64_t pos = start[maxdeg - indeg[v]]++;
+    uint64_t pos = start[key(v)]++;
     g.order[pos] = (uint32_t)v;
     g.rank[v] = (uint32_t)pos;
   }

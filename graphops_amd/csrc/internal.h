@@ -51,6 +51,11 @@ struct RunState {
   double anchor_min;   // periodic anchors only while diff > anchor_min * tolerance
   int32_t f64_phase;   // still in the leading all-f64 phase
   int32_t pad1;
+  // Global runs in the blocked layout: the isolated rows (no in-edges, no
+  // out-edges) all hold the same iterate, kept here instead of in xbar;
+  // n_iso = 0 when their rows are swept like the others.
+  double x_iso;
+  double n_iso;
   uint64_t iter;
   uint64_t max_iter;
   uint64_t since_anchor;
@@ -226,6 +231,7 @@ struct BlockedLayout {
   uint8_t *bin_flags = nullptr;      // [slot][1 << kPbBinShift] non-finite run sums seen
   double *partials = nullptr;  // [items2 + splits][kMaxReduce] sums of the epilogues
   int num_items1 = 0, num_items2 = 0, num_splits = 0;
+  uint64_t n_iso = 0;  // isolated rows (no in-edges, no out-edges): the tail [n - n_iso, n)
 };
 
 constexpr uint64_t kSweepLogCap = 1 << 16;

@@ -52,9 +52,44 @@ struct SweepArgs {
   int pb_anchor;     // so do the f64 anchor sweeps
   int n;             // rows
   BlockedLayout pb;
+  int n_live;  // rows the blocked epilogues sweep: n, or n - n_iso when the isolated
+               // tail (no in-edges, no out-edges) is kept in RunState::x_iso
 };
 
 constexpr int kReduce = 5;  // |delta|, dangling delta, dangling x, delta, x
+
+// The epilogues of the isolated rows in one go: each would gather 0, add the
+// same uniform term and read and write the same iterate, so their common
+// iterate is updated once and each sum gets n_iso times its term, before
+// finish_sweep reads the sums.  The arithmetic is that of DeltaEpilogue and
+// AnchorEpilogue for a row with acc = 0, inv = 0 and no personalization.
+__device__ void isolated_rows(RunState *st, int mode, double (&s)[kReduce]) {
+  const double n_iso = st->n_iso;
+  if (!(n_iso > 0.0)) return;
+  const double d = st->damping;
+  double delta, x;
+  if (mode == kModeAnchor) {
+    const double sc = st->x_scale;
+    const double dsum = st->dsum_x * sc;
+    const double fill = (1.0 - d) / st->n_f64 + d * dsum / st->n_f64;  // pagerank.rs:150-152
+    const double t = sc * 0.0 + fill;
+    delta = t - sc * st->x_iso;
+    x = t;
+    s[2] += n_iso * t;
+  } else {
+    const double ddsum = d * (st->dsum_in - st->mass_in);
+    delta = 0.0 + ddsum * st->inv_n;
+    x = st->x_iso + delta;
+    if (mode == kModeDeltaFold) {
+      s[2] += n_iso * x;
+      s[4] += n_iso * x;
+    }
+  }
+  st->x_iso = x;
+  s[0] += n_iso * fabs(delta);
+  s[1] += n_iso * delta;
+  s[3] += n_iso * delta;
+}
 
 // ---------------------------------------------------------------------------
 // The sweep chain.  Every launch runs the same two kernels; the mode of the
@@ -359,7 +394,10 @@ __global__ __launch_bounds__(kFinishBlock) void sweep_finish_kernel(SweepArgs a,
   }
   __shared__ double scratch[kReduce * (kFinishBlock / 64)];
   block_sum_waves<kReduce, kFinishBlock / 64>(red, scratch);
-  if (threadIdx.x == 0) finish_sweep(st, mode, red);
+  if (threadIdx.x == 0) {
+    if (a.pb_active && (mode != kModeAnchor || a.pb_anchor)) isolated_rows(st, mode, red);
+    finish_sweep(st, mode, red);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -660,17 +698,18 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
     const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
     const double inv_scale = ldexp(1.0, -k);
     constexpr int R = 4;  // rows per thread with their loads in flight together
-    for (int i0 = threadIdx.x; i0 < rows; i0 += R * kBlock) {
+    const int live = min(rows, a.n_live - row0);  // the isolated tail is the finish kernel's
+    for (int i0 = threadIdx.x; i0 < live; i0 += R * kBlock) {
       DeltaEpilogue::Ops o[R];
 #pragma unroll
       for (int u = 0; u < R; u++) {
         const int i = i0 + u * kBlock;
-        o[u] = epi.load(row0 + (i < rows ? i : i0));
+        o[u] = epi.load(row0 + (i < live ? i : i0));
       }
 #pragma unroll
       for (int u = 0; u < R; u++) {
         const int i = i0 + u * kBlock;
-        if (i < rows) {
+        if (i < live) {
           const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
           epi.apply(row0 + i, pb_row_value((long long)acc[i], fl, inv_scale), o[u], red);
         }
@@ -739,7 +778,8 @@ __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *
       fl |= fp[(size_t)j << kPbBinShift];
     }
     const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
-    epi(row0 + (int)threadIdx.x, pb_row_value(s, fl, inv_scale), red);
+    if (row0 + (int)threadIdx.x < a.n_live)
+      epi(row0 + (int)threadIdx.x, pb_row_value(s, fl, inv_scale), red);
   }
   block_sum<kReduce>(red);
   if (threadIdx.x == 0) {
@@ -1000,7 +1040,8 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
   const int rows = min(kRows, a.n - row0);
   if (it.w < 0) {
     const AnchorEpilogue epi(a, st);
-    for (int i = threadIdx.x; i < rows; i += kAcc64Block) {
+    const int live = min(rows, a.n_live - row0);  // the isolated tail is the finish kernel's
+    for (int i = threadIdx.x; i < live; i += kAcc64Block) {
       const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
       epi(row0 + i, pb_row_value128(acc_lo[i], acc_hi[i], fl), red);
     }
@@ -1060,7 +1101,8 @@ __device__ void pb_combine64(const SweepArgs &a, const RunState *st) {
       fl |= fp[(size_t)j << kPbBinShift];
     }
     const AnchorEpilogue epi(a, st);
-    epi(row0 + (int)threadIdx.x, pb_row_value128(lo, hi, fl), red);
+    if (row0 + (int)threadIdx.x < a.n_live)
+      epi(row0 + (int)threadIdx.x, pb_row_value128(lo, hi, fl), red);
   }
   block_sum<kReduce>(red);
   if (threadIdx.x == 0) {
@@ -1092,8 +1134,12 @@ __global__ __launch_bounds__(kBlock) void init_kernel(SweepArgs a, int n) {
 __global__ __launch_bounds__(kBlock) void finalize_kernel(SweepArgs a, const uint32_t *rank,
                                                           double *out, int n) {
   double red[1] = {0.0};
+  const double x_iso = a.st->x_iso;
+  // the run kept the isolated tail's iterate apart if its state says so
+  const int n_live = a.st->n_iso > 0.0 ? a.n_live : n;
   for (int o = blockIdx.x * kBlock + threadIdx.x; o < n; o += gridDim.x * kBlock) {
-    const double x = a.xbar[rank[o]];
+    const int r = (int)rank[o];
+    const double x = r >= n_live ? x_iso : a.xbar[r];
     out[o] = x;
     red[0] += x;
   }
@@ -1173,6 +1219,9 @@ static SweepArgs make_args(gops_graph *g, const SweepSetup &s) {
   a.pb_anchor = a.pb_active && s.anchors_blocked;
   a.n = (int)dg.n;
   if (a.pb_active) a.pb = g->pb;
+  // isolated rows become one shared iterate when every sweep is blocked and
+  // the teleport is uniform
+  a.n_live = (a.pb_anchor && !s.personalized) ? (int)(dg.n - g->pb.n_iso) : (int)dg.n;
   return a;
 }
 
@@ -1240,6 +1289,7 @@ hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize) {
   SweepSetup s{0, 0, 0, 0};
   SweepArgs a = make_args(g, s);
   const int n = (int)g->dev.n;
+  a.n_live = g->has_pb ? (int)(g->dev.n - g->pb.n_iso) : n;  // if the run's state says so
   const int grid = elementwise_grid(n);
   finalize_kernel<<<grid, kBlock, 0, ctx->stream>>>(a, g->dev.rank, g->run.out, n);
   sum_partials_kernel<<<1, kBlock, 0, ctx->stream>>>(a.partials, grid, &a.st->dsum_x);

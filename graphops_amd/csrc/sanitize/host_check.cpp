@@ -25,7 +25,8 @@ static int failures = 0;
   } while (0)
 
 // The CSR transpose of (off, nb): row r (new id) lists the new ids of the
-// sources of node order[r]'s in-edges, ascending; rows by descending in-degree.
+// sources of node order[r]'s in-edges, ascending; rows by descending in-degree,
+// the isolated ones (no in-, no out-edges) after the other rows without in-edges.
 static void check_csrt(uint64_t n, const uint64_t *off, const uint32_t *nb, const HostCSRT &h) {
   const uint64_t m = off[n];
   CHECK(h.n == n && h.m == m);
@@ -36,6 +37,10 @@ static void check_csrt(uint64_t n, const uint64_t *off, const uint32_t *nb, cons
     CHECK(h.order[r] < n && h.rank[h.order[r]] == r);
     CHECK((uint64_t)(h.row_ptr[r + 1] - h.row_ptr[r]) == indeg[h.order[r]]);
     if (r) CHECK(h.row_ptr[r + 1] - h.row_ptr[r] <= h.row_ptr[r] - h.row_ptr[r - 1]);
+    if (r && indeg[h.order[r]] == 0 && off[h.order[r] + 1] != off[h.order[r]]) {
+      const uint64_t p = h.order[r - 1];  // a row with out-edges follows no isolated row
+      CHECK(!(indeg[p] == 0 && off[p + 1] == off[p]));
+    }
     for (int32_t k = h.row_ptr[r]; k < h.row_ptr[r + 1]; k++) {
       CHECK(h.col[k] >= 0 && (uint64_t)h.col[k] < n);
       if (k > h.row_ptr[r]) CHECK(h.col[k] >= h.col[k - 1]);

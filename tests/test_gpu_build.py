@@ -2,7 +2,8 @@
 independent numpy statement of the same arrays (bit for bit).
 
 The arrays (gops_graph_export): rows are nodes ordered by in-degree,
-descending, ties by id; each row lists the new ids of its sources,
+descending, the nodes without in- and out-edges after the other nodes
+without in-edges, ties by id; each row lists the new ids of its sources,
 ascending, duplicate edges in adapter order (the order in which the
 reference's push loop adds them, pagerank.rs:154-162) with the weight
 max(w, 0) (pagerank.rs:257); 1/out-degree counts duplicate entries
@@ -44,7 +45,8 @@ def numpy_csrt(n, off, nb, ew, storage):
     deg = np.diff(off)
     src = np.repeat(np.arange(n), deg)
     indeg = np.bincount(nb, minlength=n)
-    order = np.lexsort((np.arange(n), -indeg))
+    isolated = (indeg == 0) & (np.diff(off) == 0)  # no in-, no out-edges: the tail
+    order = np.lexsort((np.arange(n), isolated, -indeg))
     rank = np.empty(n, np.int64)
     rank[order] = np.arange(n)
     row, col = rank[nb], rank[src]
