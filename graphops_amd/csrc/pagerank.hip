@@ -659,6 +659,10 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   __shared__ unsigned long long acc[kRows];
   __shared__ uint32_t nonfin[kRows / 4];  // a byte of pb_nonfinite_bits per row
   const PbItem it = a.pb.items2[blockIdx.x];
+  if ((it.x << kPbBinShift) >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
+    if (threadIdx.x < kReduce) a.pb.partials[(size_t)blockIdx.x * kReduce + threadIdx.x] = 0.0;
+    return;
+  }
   for (int i = threadIdx.x; i < kRows; i += kBlock) acc[i] = 0ull;
   for (int i = threadIdx.x; i < kRows / 4; i += kBlock) nonfin[i] = 0u;
   __syncthreads();
@@ -996,6 +1000,10 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
   __shared__ unsigned long long acc_lo[kRows], acc_hi[kRows];
   __shared__ uint32_t nonfin[kRows / 4];
   const PbItem it = a.pb.items2[blockIdx.x];
+  if ((it.x << kPbBinShift) >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
+    if (threadIdx.x < kReduce) a.pb.partials[(size_t)blockIdx.x * kReduce + threadIdx.x] = 0.0;
+    return;
+  }
   for (int i = threadIdx.x; i < kRows; i += kAcc64Block) {
     acc_lo[i] = 0ull;
     acc_hi[i] = 0ull;
