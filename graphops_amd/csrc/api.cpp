@@ -386,17 +386,35 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   // wasted after convergence.
   // A chain replay may run up to kChainSweeps - 1 sweeps past the iteration
   // cap; those return at once (the cap sets the done flag).
+  //
+  // With every sweep blocked, a sweep's kernels are of one kind: the f32
+  // delta kernels or their f64 anchor forms, each returning at once when the
+  // sweep is of the other kind.  The progress word also carries the mode of
+  // the next sweep, and the mode after it follows from that one (an anchor
+  // comes right after a fold sweep and, past a leading f64 phase, is followed
+  // by a delta sweep), so once sweep j - 2 has finished the host knows the
+  // kind of sweep j and launches only its kernels: one sweep runs while the
+  // next is queued, and no empty launches of the other kind remain (about
+  // 14 us per sweep at R-MAT scale 24).
+  const bool by_kind =
+      !use_graph && setup.blocked && setup.anchors_blocked && g->has_pb && !hs.f64_phase;
   const uint64_t step = use_graph ? kChainSweeps : 1;
-  const uint64_t kAhead = use_graph ? kChainAhead * step : 4;
+  const uint64_t kAhead = use_graph ? kChainAhead * step : (by_kind ? 2 : 4);
   uint64_t j = 0, seen = 0;
+  int mode_seen = hs.mode;  // the mode of sweep `seen`, the first not yet finished
   while (j < max_it) {
     for (; j < max_it && j < seen + kAhead; j += step) {
       if (use_graph) {
         HIP_TRY(ctx, hipGraphLaunch(chain, ctx->stream));
         continue;
       }
+      int kinds = 3;
+      if (by_kind) {
+        const bool anchor = j == seen ? mode_seen == kModeAnchor : mode_seen == kModeDeltaFold;
+        kinds = anchor ? 2 : 1;
+      }
       if (j < timed_cap) HIP_TRY(ctx, hipEventRecord(ctx->events[2 + 2 * j], ctx->stream));
-      HIP_TRY(ctx, launch_sweep(ctx, g, setup));
+      HIP_TRY(ctx, launch_sweep(ctx, g, setup, kinds));
       if (j < timed_cap) HIP_TRY(ctx, hipEventRecord(ctx->events[3 + 2 * j], ctx->stream));
     }
     if (j >= max_it) break;
@@ -407,8 +425,9 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
         done = true;
         break;
       }
-      if ((w >> 2) > seen) {
-        seen = w >> 2;
+      if ((w >> 4) > seen) {
+        seen = w >> 4;
+        mode_seen = (int)((w >> 2) & 3);
         break;
       }
       if ((spin & 1023) == 1023) {
@@ -416,9 +435,10 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
         if (q == hipSuccess) {  // queue drained: whatever the word says is final
           const uint64_t w2 = __atomic_load_n(ctx->mirror_host, __ATOMIC_ACQUIRE);
           done = (w2 & 1) != 0;
-          if (!done && (w2 >> 2) <= seen)
+          if (!done && (w2 >> 4) <= seen)
             return fail(ctx, GOPS_DEVICE_ERROR, "sweep chain stalled without progress");
-          seen = w2 >> 2;
+          seen = w2 >> 4;
+          mode_seen = (int)((w2 >> 2) & 3);
           break;
         }
         if (q != hipErrorNotReady) return hip_fail(ctx, q, "hipStreamQuery");
@@ -434,6 +454,11 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
                               hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   RunState last = ctx->host_state[0];
+  if (last.kind_miss) {
+    rb.has_result = false;
+    return fail(ctx, GOPS_DEVICE_ERROR,
+                "sweep " + std::to_string(last.iter + 1) + " was launched for the other kind");
+  }
   if (max_it == 0) {  // unchecked max_iterations == 0 (pagerank.rs:139-141, 182-186)
     info->iterations = 0;
     info->diff_l1 = std::numeric_limits<double>::infinity();

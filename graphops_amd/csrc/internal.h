@@ -67,9 +67,9 @@ struct RunState {
   int32_t yd_cur;      // yd[yd_cur] holds the delta the next sweep gathers
   int32_t y64_cur;     // y64[y64_cur] holds the iterate the next anchor gathers
   uint32_t ticket;
-  uint32_t pad;
+  int32_t kind_miss;   // a sweep found only the other kind's kernels launched (run stopped)
   // host-mapped progress word written after every sweep:
-  // (iterations << 2) | (converged << 1) | done
+  // (iterations << 4) | (mode of the next sweep << 2) | (converged << 1) | done
   uint64_t *mirror;
   uint8_t *sweep_log;  // mode of every sweep (capacity sweep_log_cap)
   uint64_t sweep_log_cap;
@@ -399,7 +399,9 @@ struct SweepSetup {
   int anchors_blocked;    // and the f64 anchor sweeps
 };
 hipError_t launch_init(gops_context *ctx, gops_graph *g, const SweepSetup &s);
-hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s);
+// kinds: the blocked kernels to launch when every sweep is blocked (1 f32
+// delta, 2 f64 anchor, 3 both: each returns at once for the other kind)
+hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s, int kinds = 3);
 hipError_t launch_finalize(gops_context *ctx, gops_graph *g, int normalize);
 hipError_t launch_gather_probe(gops_context *ctx, gops_graph *g);
 

@@ -54,6 +54,8 @@ struct SweepArgs {
   BlockedLayout pb;
   int n_live;  // rows the blocked epilogues sweep: n, or n - n_iso when the isolated
                // tail (no in-edges, no out-edges) is kept in RunState::x_iso
+  int kinds;   // blocked kernels launched for this sweep: 1 the f32 delta kernels,
+               // 2 the f64 anchor kernels, 3 both (each returns at once for the other)
 };
 
 constexpr int kReduce = 5;  // |delta|, dangling delta, dangling x, delta, x
@@ -162,7 +164,8 @@ __device__ void finish_sweep(RunState *st, int mode, const double (&s)[kReduce])
   st->done = done;
   if (it - 1 < st->sweep_log_cap) st->sweep_log[it - 1] = (uint8_t)mode;
   if (st->mirror) {
-    __hip_atomic_store(st->mirror, (it << 2) | (uint64_t)(conv << 1) | (uint64_t)done,
+    __hip_atomic_store(st->mirror,
+                       (it << 4) | ((uint64_t)next << 2) | (uint64_t)(conv << 1) | (uint64_t)done,
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
@@ -369,8 +372,20 @@ __global__ __launch_bounds__(kFinishBlock) void sweep_finish_kernel(SweepArgs a,
                                                                     int sweep_blocks) {
   RunState *st = a.st;
   if (st->done) return;
-  pick_buffers(a, st, yd0, yd1, y640, y641);
   const int mode = st->mode;
+  if (a.pb_anchor && !(a.kinds & (mode == kModeAnchor ? 2 : 1))) {
+    // the host launched the other kind's kernels, which returned at once: stop
+    // the run with an error rather than finish a sweep that did not run
+    if (threadIdx.x == 0) {
+      st->kind_miss = 1;
+      st->done = 1;
+      if (st->mirror)
+        __hip_atomic_store(st->mirror, (st->iter << 4) | 1ull, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return;
+  }
+  pick_buffers(a, st, yd0, yd1, y640, y641);
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
   auto hubs = [&](const auto &epi) {
     for (int h = threadIdx.x; h < a.num_hubs; h += kFinishBlock) {
@@ -1230,6 +1245,7 @@ static SweepArgs make_args(gops_graph *g, const SweepSetup &s) {
   // isolated rows become one shared iterate when every sweep is blocked and
   // the teleport is uniform
   a.n_live = (a.pb_anchor && !s.personalized) ? (int)(dg.n - g->pb.n_iso) : (int)dg.n;
+  a.kinds = 3;
   return a;
 }
 
@@ -1259,11 +1275,14 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
   if (!a.pb_anchor)  // pull anchors (and every sweep when not blocked)
     sweep_kernel<WT><<<grid, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1], rb.y64[0], rb.y64[1]);
   if (a.pb_active) {
-    if (a.pb.num_items1)
-      pb_scatter_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1]);
-    pb_accumulate_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
-                                                                      rb.y64[0], rb.y64[1]);
-    if (a.pb_anchor) {
+    if (a.kinds & 1) {
+      if (a.pb.num_items1)
+        pb_scatter_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.yd[0],
+                                                                                 rb.yd[1]);
+      pb_accumulate_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
+                                                                        rb.y64[0], rb.y64[1]);
+    }
+    if (a.pb_anchor && (a.kinds & 2)) {
       if (a.pb.num_items1)
         pb_scatter64_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.y64[0],
                                                                                    rb.y64[1]);
@@ -1279,8 +1298,11 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
                                                      grid);
 }
 
-hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s) {
+hipError_t launch_sweep(gops_context *ctx, gops_graph *g, const SweepSetup &s, int kinds) {
   SweepArgs a = make_args(g, s);
+  // one kind's kernels only where every sweep is blocked (else the pull takes
+  // the anchors and the blocked kernels run for the deltas)
+  a.kinds = a.pb_anchor ? kinds : 3;
   if (!s.weighted) {
     launch_wt<NoW>(ctx, g, a);
   } else {
