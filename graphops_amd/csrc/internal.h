@@ -229,7 +229,7 @@ struct BlockedLayout {
   long long *bin_partial = nullptr;  // [slot][1 << kPbBinShift] fixed-point row sums, or
                                      // [slot][1 << kPbBinShift][2] (128-bit, anchors)
   uint8_t *bin_flags = nullptr;      // [slot][1 << kPbBinShift] non-finite run sums seen
-  double *partials = nullptr;  // [items2 + splits][kMaxReduce] sums of the epilogues
+  double *partials = nullptr;  // [kReduce][items2 + splits] sums of the epilogues
   int num_items1 = 0, num_items2 = 0, num_splits = 0;
   uint64_t n_iso = 0;  // isolated rows (no in-edges, no out-edges): the tail [n - n_iso, n)
 };

@@ -338,7 +338,8 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(SweepArgs a, float *yd0, 
 }
 
 // The finish kernel's workgroup: enough threads that the partials of a
-// sweep's workgroups (about 4600 blocked, 2048 pulled) load in one round.
+// sweep's workgroups (2048 pulled, about 9500 blocked) load in one or two
+// rounds.
 constexpr int kFinishBlock = 1024;
 
 // red += partials[b] for b = threadIdx.x, + kFinishBlock, ... < count, in
@@ -359,6 +360,40 @@ __device__ __forceinline__ void add_partials(const double *partials, int count,
       if (b0 + u * kFinishBlock < count) {
 #pragma unroll
         for (int i = 0; i < kReduce; i++) red[i] += v[u][i];
+      }
+    }
+  }
+}
+
+// The blocked kernels' partial sums are stored sum-major, [kReduce][parts]
+// with one part per accumulate and combine workgroup, so that a wave of the
+// finish kernel loads 64 consecutive values of one sum (512 bytes) per
+// instruction.
+__device__ __forceinline__ double *pb_partial(const SweepArgs &a, int i, int part) {
+  return a.pb.partials + (size_t)i * (size_t)(a.pb.num_items2 + a.pb.num_splits) + part;
+}
+
+// red[i] += partials[i][b] for b = threadIdx.x, + kFinishBlock, ... < count,
+// in that order, with ten parts' loads of every sum in flight at a time (one
+// round for R-MAT scale 24)
+__device__ __forceinline__ void add_partials_by_sum(const double *partials, int count,
+                                                    double (&red)[kReduce]) {
+  constexpr int U = 10;
+  for (int b0 = threadIdx.x; b0 < count; b0 += U * kFinishBlock) {
+    double v[kReduce][U];
+#pragma unroll
+    for (int i = 0; i < kReduce; i++) {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int b = b0 + u * kFinishBlock;
+        v[i][u] = b < count ? partials[(size_t)i * count + b] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (b0 + u * kFinishBlock < count) {
+#pragma unroll
+        for (int i = 0; i < kReduce; i++) red[i] += v[i][u];
       }
     }
   }
@@ -397,8 +432,7 @@ __global__ __launch_bounds__(kFinishBlock) void sweep_finish_kernel(SweepArgs a,
   };
   if (a.pb_active && (mode != kModeAnchor || a.pb_anchor)) {
     // a blocked sweep: hub rows were summed in their bins
-    const int parts = a.pb.num_items2 + a.pb.num_splits;
-    add_partials(a.pb.partials, parts, red);
+    add_partials_by_sum(a.pb.partials, a.pb.num_items2 + a.pb.num_splits, red);
   } else {
     if (mode == kModeAnchor) {
       hubs(AnchorEpilogue(a, st));
@@ -675,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   __shared__ uint32_t nonfin[kRows / 4];  // a byte of pb_nonfinite_bits per row
   const PbItem it = a.pb.items2[blockIdx.x];
   if ((it.x << kPbBinShift) >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
-    if (threadIdx.x < kReduce) a.pb.partials[(size_t)blockIdx.x * kReduce + threadIdx.x] = 0.0;
+    if (threadIdx.x < kReduce) *pb_partial(a, threadIdx.x, blockIdx.x) = 0.0;
     return;
   }
   for (int i = threadIdx.x; i < kRows; i += kBlock) acc[i] = 0ull;
@@ -746,7 +780,7 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   block_sum<kReduce>(red, reinterpret_cast<double *>(acc));
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < kReduce; i++) a.pb.partials[(size_t)blockIdx.x * kReduce + i] = red[i];
+    for (int i = 0; i < kReduce; i++) *pb_partial(a, i, blockIdx.x) = red[i];
   }
 }
 
@@ -804,7 +838,7 @@ __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < kReduce; q++)
-      a.pb.partials[(size_t)(a.pb.num_items2 + blockIdx.x) * kReduce + q] = red[q];
+      *pb_partial(a, q, a.pb.num_items2 + blockIdx.x) = red[q];
   }
 }
 
@@ -1016,7 +1050,7 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
   __shared__ uint32_t nonfin[kRows / 4];
   const PbItem it = a.pb.items2[blockIdx.x];
   if ((it.x << kPbBinShift) >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
-    if (threadIdx.x < kReduce) a.pb.partials[(size_t)blockIdx.x * kReduce + threadIdx.x] = 0.0;
+    if (threadIdx.x < kReduce) *pb_partial(a, threadIdx.x, blockIdx.x) = 0.0;
     return;
   }
   for (int i = threadIdx.x; i < kRows; i += kAcc64Block) {
@@ -1081,7 +1115,7 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
   block_sum_waves<kReduce, kAcc64Block / 64>(red, reinterpret_cast<double *>(acc_lo));
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < kReduce; i++) a.pb.partials[(size_t)blockIdx.x * kReduce + i] = red[i];
+    for (int i = 0; i < kReduce; i++) *pb_partial(a, i, blockIdx.x) = red[i];
   }
 }
 
@@ -1131,7 +1165,7 @@ __device__ void pb_combine64(const SweepArgs &a, const RunState *st) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < kReduce; q++)
-      a.pb.partials[(size_t)(a.pb.num_items2 + blockIdx.x) * kReduce + q] = red[q];
+      *pb_partial(a, q, a.pb.num_items2 + blockIdx.x) = red[q];
   }
 }
 

@@ -305,3 +305,40 @@ def test_split_bins(oracle, ctx_b, ctx_p):
     dg.close()
     assert pa.iterations == b.iterations
     assert np.abs(pa.scores - b.scores).sum() <= 1e-13
+
+
+def test_sweep_kinds_with_f64_phase_and_timing(oracle, ctx_b):
+    """With every sweep blocked the host launches only the kernels of each
+    sweep's kind (f32 delta or f64 anchor), which it knows from the mode the
+    previous sweep's finish wrote; with a leading all-f64 phase
+    (GOPS_OPT_F32_BELOW > 0) an anchor may follow an anchor, so both kinds
+    are launched while the phase lasts.  Both schedules, and the default with
+    every sweep timed, match the oracle; the timed run gives the bits of the
+    untimed one."""
+    e = generators.rmat(16, 16, seed=21, weights="lognormal", p0=0.0, p1=1.0, rounding="bf16")
+    tol = 1e-12
+    cfg = ga.PageRankConfig(0.85, 300, tol)
+    ref = oracle_pull(oracle, e, 1, tol, 300)
+    dg = ga.DeviceGraph.from_edges(e, weight_storage="bf16", ctx=ctx_b)
+    try:
+        base = ga.pagerank_weighted_run(dg, cfg, ctx=ctx_b)
+        s0 = ctx_b.last_stats()
+        check(base, ref, tol)
+        assert s0["sweeps_blocked"] == base.iterations and s0["sweeps_fp64"] >= 2
+        ctx_b.set_option(L.OPT_TIMING, 1)
+        try:
+            timed = ga.pagerank_weighted_run(dg, cfg, ctx=ctx_b)
+        finally:
+            ctx_b.set_option(L.OPT_TIMING, 0)
+        assert np.array_equal(timed.scores, base.scores) and timed.diff_l1 == base.diff_l1
+        ctx_b.set_option(L.OPT_F32_BELOW, 1e6)
+        try:
+            ph = ga.pagerank_weighted_run(dg, cfg, ctx=ctx_b)
+            s1 = ctx_b.last_stats()
+        finally:
+            ctx_b.set_option(L.OPT_F32_BELOW, 0.0)
+        check(ph, ref, tol)
+        assert s1["sweeps_blocked"] == ph.iterations
+        assert s1["sweeps_fp64"] > s0["sweeps_fp64"] + 2  # the leading phase ran
+    finally:
+        dg.close()
