@@ -77,8 +77,10 @@ struct PprArgs {
 };
 
 hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid);
-hipError_t ppr_launch_admit(hipStream_t s, int T, const PprArgs &a, int n, uint64_t clear_mask,
-                            uint64_t admit_mask);
+// zrows: rows without in-edges where the admitted columns may be nonzero (the
+// departing queries' seed rows there), nz of them
+hipError_t ppr_launch_admit(hipStream_t s, int T, const PprArgs &a, uint64_t clear_mask,
+                            uint64_t admit_mask, const int32_t *zrows, int nz);
 hipError_t ppr_launch_gather(hipStream_t s, int T, const PprArgs &a, const uint32_t *rank, int n,
                              int q, double *out);
 hipError_t ppr_launch_gather_mask(hipStream_t s, int T, const PprArgs &a, const uint32_t *rank,

@@ -613,75 +613,91 @@ __global__ __launch_bounds__(kBlock) void ppr_finish_kernel(PprArgs a, int sweep
 
 // Admission, step 1: clear the admitted queries' columns of the f32 deltas
 // the next sweep gathers, and, for an anchor admission, of x and of the y64
-// buffer the next sweep's anchors gather.  One thread per row and group of four queries;
-// the tile's other queries are not touched.
+// buffer the next sweep's anchors gather.  One thread per row and group of
+// four queries; the tile's other queries are not touched.
+//
+// An anchor admission's first sweep reads x and y64.  A delta admission's
+// first sweep reads only the gathered deltas: it takes x and the carry as
+// p-hat and its rounding (ppr_epilogue), and a fold sweep writes the whole
+// y64 column before any anchor reads it.
+//
+// Rows without in-edges (r >= n_in) are swept only while they are seeds of
+// some query of the tile (ppr_sweep_kernel), and such a sweep leaves a
+// column that is not seeded at the row at 0 (nothing is gathered, and the
+// teleport term is 0 off the query's seeds).  So a column is nonzero there
+// only at the seed rows of the query that held it: the host passes the
+// zero-in seed rows of the departing queries (zrows), and all six vectors of
+// the admitted columns are cleared at those rows alone (the tile starts at 0).
 template <int T>
-__global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, int n, uint64_t clear_mask) {
+__device__ __forceinline__ void clear_row(const PprArgs &a, int r, unsigned m4, int qg,
+                                          bool zero_in, bool anchored, double *y64, float *yin,
+                                          float *yout) {
+  const size_t at = (size_t)r * T + qg * kLaneQ;
+  if (zero_in) {
+    if (m4 == 0xFu) {
+      const double2 z = {0.0, 0.0};
+      const float4 zf = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (double *v : {a.x, a.y64a, a.y64b}) {
+        *reinterpret_cast<double2 *>(v + at) = z;
+        *reinterpret_cast<double2 *>(v + at + 2) = z;
+      }
+      for (float *v : {a.carry, yin, yout}) *reinterpret_cast<float4 *>(v + at) = zf;
+    } else {
+#pragma unroll
+      for (int j = 0; j < kLaneQ; j++) {
+        if ((m4 >> j) & 1u) {
+          a.x[at + j] = 0.0;
+          a.carry[at + j] = 0.0f;
+          yin[at + j] = 0.0f;
+          yout[at + j] = 0.0f;
+          a.y64a[at + j] = 0.0;
+          a.y64b[at + j] = 0.0;
+        }
+      }
+    }
+    return;
+  }
+  if (m4 == 0xFu) {
+    const double2 z = {0.0, 0.0};
+    if (anchored) {
+      *reinterpret_cast<double2 *>(a.x + at) = z;
+      *reinterpret_cast<double2 *>(a.x + at + 2) = z;
+      *reinterpret_cast<double2 *>(y64 + at) = z;
+      *reinterpret_cast<double2 *>(y64 + at + 2) = z;
+    }
+    *reinterpret_cast<float4 *>(yin + at) = float4{0.0f, 0.0f, 0.0f, 0.0f};
+  } else {
+#pragma unroll
+    for (int j = 0; j < kLaneQ; j++) {
+      if ((m4 >> j) & 1u) {
+        if (anchored) {
+          a.x[at + j] = 0.0;
+          y64[at + j] = 0.0;
+        }
+        yin[at + j] = 0.0f;
+      }
+    }
+  }
+}
+
+template <int T>
+__global__ __launch_bounds__(kBlock) void ppr_clear_kernel(PprArgs a, uint64_t clear_mask,
+                                                           const int32_t *zrows, int nz) {
   constexpr int Q = T / kLaneQ;
   const int qg = threadIdx.x % Q;
   const unsigned m4 = (unsigned)(clear_mask >> (qg * kLaneQ)) & 0xFu;
   if (!m4) return;
-  // An anchor admission's first sweep reads x and y64.  A delta admission's
-  // first sweep reads only the gathered deltas: it takes x and the carry as
-  // p-hat and its rounding (ppr_epilogue), and a fold sweep writes the whole
-  // y64 column before any anchor reads it.
-  // Rows without in-edges (r >= n_in) are swept only while they are seeds of
-  // some query of the tile (ppr_sweep_kernel); every other sweep leaves
-  // them alone, so the admitted columns of all their vectors start at 0.
   const bool anchored = a.ts->f32_below > 0.0;
   double *y64 = anchored ? (a.ts->y64_par ? a.y64b : a.y64a) : nullptr;
   float *yin = a.ts->yd_cur ? a.yd1 : a.yd0;
   float *yout = a.ts->yd_cur ? a.yd0 : a.yd1;
   const int rows_per_block = kBlock / Q;
-  for (int r = blockIdx.x * rows_per_block + threadIdx.x / Q; r < n;
-       r += gridDim.x * rows_per_block) {
-    const size_t at = (size_t)r * T + qg * kLaneQ;
-    if (r >= a.n_in) {
-      if (m4 == 0xFu) {
-        const double2 z = {0.0, 0.0};
-        const float4 zf = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (double *v : {a.x, a.y64a, a.y64b}) {
-          *reinterpret_cast<double2 *>(v + at) = z;
-          *reinterpret_cast<double2 *>(v + at + 2) = z;
-        }
-        for (float *v : {a.carry, yin, yout}) *reinterpret_cast<float4 *>(v + at) = zf;
-      } else {
-#pragma unroll
-        for (int j = 0; j < kLaneQ; j++) {
-          if ((m4 >> j) & 1u) {
-            a.x[at + j] = 0.0;
-            a.carry[at + j] = 0.0f;
-            yin[at + j] = 0.0f;
-            yout[at + j] = 0.0f;
-            a.y64a[at + j] = 0.0;
-            a.y64b[at + j] = 0.0;
-          }
-        }
-      }
-      continue;
-    }
-    if (m4 == 0xFu) {
-      const double2 z = {0.0, 0.0};
-      if (anchored) {
-        *reinterpret_cast<double2 *>(a.x + at) = z;
-        *reinterpret_cast<double2 *>(a.x + at + 2) = z;
-        *reinterpret_cast<double2 *>(y64 + at) = z;
-        *reinterpret_cast<double2 *>(y64 + at + 2) = z;
-      }
-      *reinterpret_cast<float4 *>(yin + at) = float4{0.0f, 0.0f, 0.0f, 0.0f};
-    } else {
-#pragma unroll
-      for (int j = 0; j < kLaneQ; j++) {
-        if ((m4 >> j) & 1u) {
-          if (anchored) {
-            a.x[at + j] = 0.0;
-            y64[at + j] = 0.0;
-          }
-          yin[at + j] = 0.0f;
-        }
-      }
-    }
-  }
+  const int first = blockIdx.x * rows_per_block + threadIdx.x / Q;
+  const int stride = gridDim.x * rows_per_block;
+  for (int r = first; r < a.n_in; r += stride)
+    clear_row<T>(a, r, m4, qg, false, anchored, y64, yin, yout);
+  for (int i = first; i < nz; i += stride)
+    clear_row<T>(a, zrows[i], m4, qg, true, anchored, y64, yin, yout);
 }
 
 // Admission, step 2: the admitted queries start from their personalization
@@ -788,23 +804,23 @@ hipError_t ppr_launch_sweep(hipStream_t s, int T, const PprArgs &a, int grid) {
 }
 
 template <int T>
-static hipError_t launch_admit_t(hipStream_t s, const PprArgs &a, int n, uint64_t clear_mask,
-                                 uint64_t admit_mask) {
+static hipError_t launch_admit_t(hipStream_t s, const PprArgs &a, uint64_t clear_mask,
+                                 uint64_t admit_mask, const int32_t *zrows, int nz) {
   const int rows_per_block = kBlock / (T / kLaneQ);
-  int grid = (n + rows_per_block - 1) / rows_per_block;
+  int grid = (std::max(a.n_in, nz) + rows_per_block - 1) / rows_per_block;
   grid = std::max(1, std::min(grid, 4096));
-  ppr_clear_kernel<T><<<grid, kBlock, 0, s>>>(a, n, clear_mask);
+  ppr_clear_kernel<T><<<grid, kBlock, 0, s>>>(a, clear_mask, zrows, nz);
   ppr_admit_kernel<T><<<1, kBlock, 0, s>>>(a, admit_mask);
   return hipGetLastError();
 }
 
-hipError_t ppr_launch_admit(hipStream_t s, int T, const PprArgs &a, int n, uint64_t clear_mask,
-                            uint64_t admit_mask) {
+hipError_t ppr_launch_admit(hipStream_t s, int T, const PprArgs &a, uint64_t clear_mask,
+                            uint64_t admit_mask, const int32_t *zrows, int nz) {
   switch (T) {
-    case 8: return launch_admit_t<8>(s, a, n, clear_mask, admit_mask);
-    case 16: return launch_admit_t<16>(s, a, n, clear_mask, admit_mask);
-    case 32: return launch_admit_t<32>(s, a, n, clear_mask, admit_mask);
-    default: return launch_admit_t<64>(s, a, n, clear_mask, admit_mask);
+    case 8: return launch_admit_t<8>(s, a, clear_mask, admit_mask, zrows, nz);
+    case 16: return launch_admit_t<16>(s, a, clear_mask, admit_mask, zrows, nz);
+    case 32: return launch_admit_t<32>(s, a, clear_mask, admit_mask, zrows, nz);
+    default: return launch_admit_t<64>(s, a, clear_mask, admit_mask, zrows, nz);
   }
 }
 

@@ -142,6 +142,12 @@ struct Tile {
   PprArgs chain_args{};
   Query slot[kMaxTile];
   bool used[kMaxTile] = {};
+  // per slot, the seed rows without in-edges of the last query admitted into
+  // it: its columns may be nonzero there until the slot's next admission
+  std::vector<int32_t> zrows[kMaxTile];
+  std::vector<int32_t> clear_host;  // the rows the last admission cleared
+  int32_t *clear_rows = nullptr;    // device copy
+  int clear_cap = 0;
   int grid = 0;
 };
 
@@ -274,6 +280,7 @@ void free_tile(Tile *t) {
   hipFree(t->seed_row);
   hipFree(t->seed_q);
   hipFree(t->seed_p);
+  hipFree(t->clear_rows);
   hipFree(t->ts);
   hipHostFree(t->hts);
   delete t;
@@ -305,10 +312,13 @@ gops_status alloc_tile(gops_context *ctx, gops_graph *g, int T, int grid, Tile *
   chk(hipMalloc((void **)&t->seed_idx, std::max<size_t>(t->n, 1) * 4));
   chk(hipMalloc((void **)&t->ts, sizeof(TileState)));
   chk(hipHostMalloc((void **)&t->hts, sizeof(TileState), hipHostMallocDefault));
+  // every vector starts at 0: an admission clears the admitted columns only
+  // where an earlier query may have left values (ppr_clear_kernel)
   if (e == hipSuccess) chk(hipMemsetAsync(t->seed_idx, 0xff, std::max<size_t>(t->n, 1) * 4, ctx->stream));
-  if (e == hipSuccess) chk(hipMemsetAsync(t->yd0, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
-  if (e == hipSuccess) chk(hipMemsetAsync(t->yd1, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
-  if (e == hipSuccess) chk(hipMemsetAsync(t->carry, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
+  for (double *v : {t->x, t->y64a, t->y64b})
+    if (e == hipSuccess) chk(hipMemsetAsync(v, 0, std::max<size_t>(nt, 1) * 8, ctx->stream));
+  for (float *v : {t->yd0, t->yd1, t->carry})
+    if (e == hipSuccess) chk(hipMemsetAsync(v, 0, std::max<size_t>(nt, 1) * 4, ctx->stream));
   if (e == hipSuccess) chk(hipStreamSynchronize(ctx->stream));
   if (e != hipSuccess) {
     free_tile(t);
@@ -522,12 +532,37 @@ void set_slot_params(Tile *t, int s, const Query &q, double f32_below, uint64_t 
 // seed entries, clear the admitted columns and start the new queries; the
 // running queries keep their state.
 gops_status admit(gops_context *ctx, gops_graph *g, Tile *t, const TilePlan *p, uint64_t mask) {
-  gops_status st = upload_seeds(ctx, t, g->ppr_cache ? g->ppr_cache->n_in : (int32_t)g->dev.n);
+  const int32_t n_in = g->ppr_cache ? g->ppr_cache->n_in : (int32_t)g->dev.n;
+  gops_status st = upload_seeds(ctx, t, n_in);
   if (st != GOPS_OK) return st;
+  // rows without in-edges to clear in the admitted columns: the departing
+  // queries' seed rows there; then the new queries' take their place
+  t->clear_host.clear();
+  for (int s = 0; s < t->T; s++) {
+    if (!((mask >> s) & 1ull)) continue;
+    t->clear_host.insert(t->clear_host.end(), t->zrows[s].begin(), t->zrows[s].end());
+    t->zrows[s].clear();
+    for (auto &pv : t->slot[s].pers)
+      if ((int32_t)pv.first >= n_in) t->zrows[s].push_back((int32_t)pv.first);
+  }
+  std::sort(t->clear_host.begin(), t->clear_host.end());
+  t->clear_host.erase(std::unique(t->clear_host.begin(), t->clear_host.end()), t->clear_host.end());
+  const int nz = (int)t->clear_host.size();
+  if (nz > t->clear_cap) {
+    hipFree(t->clear_rows);
+    t->clear_rows = nullptr;
+    t->clear_cap = 0;
+    const int cap = std::max(nz, 1024);
+    STRY(ctx, hipMalloc((void **)&t->clear_rows, (size_t)cap * 4));
+    t->clear_cap = cap;
+  }
+  if (nz)
+    STRY(ctx, hipMemcpyAsync(t->clear_rows, t->clear_host.data(), (size_t)nz * 4,
+                             hipMemcpyHostToDevice, ctx->stream));
   t->hts->all_done = 0;
   STRY(ctx, hipMemcpyAsync(t->ts, t->hts, sizeof(TileState), hipMemcpyHostToDevice, ctx->stream));
   PprArgs a = tile_args(ctx, g, t, p, false);
-  STRY(ctx, ppr_launch_admit(ctx->stream, t->T, a, (int)t->n, mask, mask));
+  STRY(ctx, ppr_launch_admit(ctx->stream, t->T, a, mask, mask, t->clear_rows, nz));
   return GOPS_OK;
 }
 
