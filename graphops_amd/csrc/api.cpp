@@ -330,10 +330,11 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   hs.sweep_log = rb.sweep_log;
   hs.sweep_log_cap = kSweepLogCap;
   __atomic_store_n(ctx->mirror_host, (uint64_t)0, __ATOMIC_RELEASE);
-  ctx->host_state[0] = hs;
-  HIP_TRY(ctx, hipMemcpyAsync(rb.state, &ctx->host_state[0], sizeof(RunState),
+  // from pinned slot 1, in stream order before the first kernel; slot 0
+  // takes the state back at the end
+  ctx->host_state[1] = hs;
+  HIP_TRY(ctx, hipMemcpyAsync(rb.state, &ctx->host_state[1], sizeof(RunState),
                               hipMemcpyHostToDevice, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
 
   const uint64_t max_it = cfg->max_iterations;
   const bool timing = ctx->timing != 0;
@@ -356,6 +357,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
     hipGraphExec_t &slot = rb.chain[setup.weighted | (setup.personalized << 1) | (setup.blocked << 2)];
     if (!slot) {
       hipGraph_t graph = nullptr;
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // the state upload first, uncaptured
       HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       hipError_t ce = hipSuccess;
       for (int k = 0; k < kChainSweeps && ce == hipSuccess; k++) ce = launch_sweep(ctx, g, setup);
@@ -452,6 +454,11 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   HIP_TRY(ctx, hipEventRecord(ev_end, ctx->stream));
   HIP_TRY(ctx, hipMemcpyAsync(&ctx->host_state[0], rb.state, sizeof(RunState),
                               hipMemcpyDeviceToHost, ctx->stream));
+  // the mode log of as many sweeps as the run may take, with the state
+  const uint64_t log_n = std::min<uint64_t>(max_it, kSweepLogCap);
+  if (log_n)
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->host_log, rb.sweep_log, log_n, hipMemcpyDeviceToHost,
+                                ctx->stream));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   RunState last = ctx->host_state[0];
   if (last.kind_miss) {
@@ -471,12 +478,9 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   float ms = 0.0f;
   HIP_TRY(ctx, hipEventElapsedTime(&ms, ev_begin, ev_end));
   ctx->stats.device_ms = ms;
-  const uint64_t logged = std::min<uint64_t>(last.iter, kSweepLogCap);
-  std::vector<uint8_t> modes(logged);
-  if (logged)
-    HIP_TRY(ctx, hipMemcpy(modes.data(), rb.sweep_log, logged, hipMemcpyDeviceToHost));
+  const uint64_t logged = std::min<uint64_t>(last.iter, log_n);
   for (uint64_t k = 0; k < logged; k++) {
-    const bool f64 = modes[k] == kModeAnchor;
+    const bool f64 = ctx->host_log[k] == kModeAnchor;
     if (f64) ctx->stats.sweeps_fp64++;
     else ctx->stats.sweeps_fp32++;
     if (setup.blocked && (!f64 || setup.anchors_blocked)) ctx->stats.sweeps_blocked++;
@@ -518,8 +522,15 @@ gops_status gops_context_create(int device, gops_context **out) {
     delete ctx;
     return GOPS_OUT_OF_MEMORY;
   }
+  if (hipHostMalloc((void **)&ctx->host_log, kSweepLogCap, hipHostMallocDefault) != hipSuccess) {
+    hipHostFree(ctx->host_state);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GOPS_OUT_OF_MEMORY;
+  }
   if (hipHostMalloc((void **)&ctx->mirror_host, 64, hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&ctx->mirror_dev, ctx->mirror_host, 0) != hipSuccess) {
+    hipHostFree(ctx->host_log);
     hipHostFree(ctx->host_state);
     hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -551,6 +562,7 @@ void gops_context_destroy(gops_context *ctx) {
   hipFree(ctx->staging);
   free_pinned_chunks(ctx);
   hipHostFree(ctx->host_state);
+  hipHostFree(ctx->host_log);
   hipHostFree(ctx->mirror_host);
   hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -309,6 +309,7 @@ struct gops_context {
   double *staging = nullptr;              // device copy of host scores for gops_top_k
   size_t staging_cap = 0;
   gops::RunState *host_state = nullptr;  // pinned, 2 slots
+  uint8_t *host_log = nullptr;           // pinned, kSweepLogCap: a run's sweep modes
   uint64_t *mirror_host = nullptr;        // mapped pinned progress word
   uint64_t *mirror_dev = nullptr;         // its device address
   std::vector<hipEvent_t> events;         // timing pool
