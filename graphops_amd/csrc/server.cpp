@@ -2,14 +2,15 @@
 // serving loop (gops_server_*), both on resident tiles of T queries
 // (ppr_batch.hip).
 //
-// Serving: queries wait in a host queue; between device steps, a group of
-// slots that one lane serves is refilled once all of its queries have
-// finished, with queries of one damping where the queue has them.  Admission
-// clears the slots' columns and starts the new queries with an f64 anchor
-// sweep while the tile's running queries continue, so new and running queries
-// share every later sweep (their work overlaps on the device).  Every query
-// keeps its own sweep mode, so its arithmetic does not depend on the other
-// queries of its tile.
+// Serving: queries wait in a host queue; between device steps, every slot
+// whose query has finished takes the queue head (with a leading f64 phase, a
+// group of the slots one lane serves is refilled once all of its queries have
+// finished, with queries of one damping where the queue has them).
+// Admission clears the slots' columns and starts the new queries with their
+// personalization as the input delta while the tile's running queries
+// continue, so new and running queries share every later sweep (their work
+// overlaps on the device).  Every query keeps its own sweep mode, so its
+// arithmetic does not depend on the other queries of its tile.
 #include <omp.h>
 
 #include <algorithm>
