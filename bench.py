@@ -697,8 +697,8 @@ def main():
     ap.add_argument("--no-c1c2", action="store_true", help="skip the C1 and C2 legs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is designed, not built)"}))
+        print(json.dumps({"error": "graphops_amd runs on one GPU; --gpus must be 1 "
+                                   "(sharding over several GPUs is not implemented)"}))
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
