@@ -34,6 +34,7 @@ OPT_TILE_F32_BELOW = 11
 OPT_DELTA_SWEEP = 12
 OPT_LAUNCH = 13
 OPT_ANCHOR_SWEEP = 14
+OPT_BLOCKED_BIN_RUNS = 15
 ANCHOR_SWEEP_AUTO = 0
 ANCHOR_SWEEP_PULL = 1
 LAUNCH_AUTO, LAUNCH_QUEUED, LAUNCH_GRAPH = 0, 1, 2
@@ -65,7 +66,8 @@ class RunStatsC(ctypes.Structure):
                 ("device_ms", ctypes.c_double), ("fp32_sweep_ms", ctypes.c_double),
                 ("fp64_sweep_ms", ctypes.c_double), ("other_ms", ctypes.c_double),
                 ("sweeps_blocked", ctypes.c_uint64), ("blocked_fallback", ctypes.c_int32),
-                ("graph_replay", ctypes.c_int32)]
+                ("graph_replay", ctypes.c_int32), ("blocked_bins", ctypes.c_int32),
+                ("blocked_split_bins", ctypes.c_int32)]
 
 
 class RowPlanC(ctypes.Structure):

@@ -10,6 +10,9 @@
 
 #include "internal.h"
 
+// the Python mirror (ctypes) and bindings in other languages lay it out field by field
+static_assert(sizeof(gops_run_stats) == 88, "gops_run_stats layout");
+
 using namespace gops;
 
 namespace {
@@ -274,6 +277,8 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
     } else {
       setup.blocked = 1;
       setup.anchors_blocked = ctx->anchor_sweep != GOPS_ANCHOR_SWEEP_PULL;
+      ctx->stats.blocked_bins = g->pb.num_bins;
+      ctx->stats.blocked_split_bins = g->pb.num_split_bins;
     }
   }
 
@@ -632,6 +637,11 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
         return fail(ctx, GOPS_INVALID_PARAMETER,
                     "delta sweep must be GOPS_DELTA_SWEEP_AUTO, _PULL or _BLOCKED");
       ctx->delta_sweep = (int)value;
+      return GOPS_OK;
+    case GOPS_OPT_BLOCKED_BIN_RUNS:
+      if (!(value == 0 || (value >= 4096 && value <= 1 << 24)))
+        return fail(ctx, GOPS_INVALID_PARAMETER, "blocked bin runs must be 0 or in [4096, 2^24]");
+      ctx->pb_bin_runs = (int)value;
       return GOPS_OK;
     default:
       return fail(ctx, GOPS_INVALID_PARAMETER, "unknown option");

@@ -20,7 +20,10 @@
 //                          that each segment (one chunk into one bin) starts at
 //                          a multiple of kPbLane entries and each chunk at a
 //                          multiple of kPbWin (a window, one wave's entries)
-//   runs, bin-major:       (row >> kPbBinShift, source chunk, row, run number)
+//   runs, bin-major:       (bin of the row, source chunk, row, run number)
+// Bins are ranges of consecutive rows, at most 2^kPbBinShift of them, cut so
+// that the runs of a bin stay within one accumulate workgroup's share
+// (pb_group_marks_kernel bounds every row's runs before the layout exists).
 // A run is a maximal sequence of entries of one row within one window.  A
 // segment's runs are contiguous in both orders, so a run's bin-major place is
 // its chunk-major number plus its segment's offset.  The scatter finds an
@@ -74,6 +77,48 @@ __global__ void pb_mark_kernel(const int32_t *row_ptr, int n, int32_t *marks) {
     if (row_ptr[v] < row_ptr[v + 1]) marks[row_ptr[v]] = v;
 }
 
+// Per edge k of the CSR of in-edges (row[k] its row, sources ascending): 1
+// if k starts a group (the row's edges from one source chunk), 1 if it is a
+// group's second edge, else 0.  A group of c entries is contiguous in the
+// padded layout and meets at most 1 + floor((c + 510) / kPbWin) windows: 1
+// for c = 1, at most 2 + (c - 2) / kPbWin for c >= 2.  So a row's marks plus
+// ceil(edges / kPbWin) bound its runs, whatever the padding.
+__global__ void pb_group_marks_kernel(const int32_t *row_ptr, const int32_t *col,
+                                      const int32_t *row, uint32_t m, int32_t *mark) {
+  for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < m; k += gridDim.x * kB) {
+    const int32_t s = row_ptr[row[k]];
+    const uint32_t c = (uint32_t)col[k] >> kPbChunkShift;
+    const bool head = (int32_t)k == s || ((uint32_t)col[k - 1] >> kPbChunkShift) != c;
+    const bool second =
+        !head && ((int32_t)k - 1 == s || ((uint32_t)col[k - 2] >> kPbChunkShift) != c);
+    mark[k] = (head || second) ? 1 : 0;
+  }
+}
+
+// bound[r] = the runs row r can have (above), from the inclusive sum of marks
+__global__ void pb_row_bound_kernel(const int32_t *row_ptr, const int32_t *msum, int n,
+                                    uint32_t *bound) {
+  for (int r = blockIdx.x * kB + threadIdx.x; r < n; r += gridDim.x * kB) {
+    const int32_t s = row_ptr[r], e = row_ptr[r + 1];
+    bound[r] = e > s ? (uint32_t)(msum[e - 1] - (s > 0 ? msum[s - 1] : 0)) +
+                           (uint32_t)((e - s + kPbWin - 1) / kPbWin)
+                     : 0u;
+  }
+}
+
+// bin_of[r]: the bin whose rows [row0[b], row0[b + 1]) hold r
+__global__ void pb_bin_of_kernel(const int32_t *row0, int bins, int n, int32_t *bin_of) {
+  for (int r = blockIdx.x * kB + threadIdx.x; r < n; r += gridDim.x * kB) {
+    int lo = 0, hi = bins - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) / 2;
+      if (row0[mid] <= r) lo = mid;
+      else hi = mid - 1;
+    }
+    bin_of[r] = lo;
+  }
+}
+
 // chunk-major key of edge k (chunk of the source, row); value = k.  A stable
 // sort keeps the CSR's ascending sources within a row.
 __global__ void pb_keys_kernel(const int32_t *col, const int32_t *row, uint32_t m, uint64_t *keys,
@@ -89,13 +134,14 @@ __device__ __forceinline__ uint32_t key_chunk(uint64_t key) { return (uint32_t)(
 __device__ __forceinline__ uint32_t key_row(uint64_t key) { return (uint32_t)(key & 0x7fffffffu); }
 
 // segment starts among the sorted entries: a new chunk or a new bin of rows
-__global__ void pb_seg_flag_kernel(const uint64_t *keys, uint32_t m, int32_t *flag) {
+__global__ void pb_seg_flag_kernel(const uint64_t *keys, const int32_t *bin_of, uint32_t m,
+                                   int32_t *flag) {
   for (uint32_t j = blockIdx.x * kB + threadIdx.x; j < m; j += gridDim.x * kB) {
     const uint64_t k = keys[j];
     bool f = j == 0;
     if (!f) {
       const uint64_t q = keys[j - 1];
-      f = key_chunk(k) != key_chunk(q) || (key_row(k) >> kPbBinShift) != (key_row(q) >> kPbBinShift);
+      f = key_chunk(k) != key_chunk(q) || bin_of[key_row(k)] != bin_of[key_row(q)];
     }
     flag[j] = f;
   }
@@ -176,16 +222,17 @@ __global__ void pb_is_head_kernel(const uint16_t *src, uint32_t mp, int32_t *fla
 // segment's first run; every window's first run and segment.
 __global__ void pb_run_keys_kernel(const uint64_t *keys, const int32_t *posof, const uint16_t *src,
                                    const int32_t *runid, const int32_t *flag, const int32_t *sg,
-                                   uint32_t m, int sbits, uint64_t *rkey, uint32_t *rval,
-                                   int32_t *seg_run, int32_t *win) {
+                                   const int32_t *bin_of, const int32_t *bin_row0, uint32_t m,
+                                   int sbits, uint64_t *rkey, uint32_t *rval, int32_t *seg_run,
+                                   int32_t *win) {
   for (uint32_t j = blockIdx.x * kB + threadIdx.x; j < m; j += gridDim.x * kB) {
     const int32_t p = posof[j];
     if (!(src[p] & kPbRunHead)) continue;
     const uint32_t r = (uint32_t)runid[p] - 1;
     const uint64_t key = keys[j];
     const uint64_t v = key_row(key), c = key_chunk(key);
-    rkey[r] = ((v >> kPbBinShift) << (sbits + kPbBinShift)) | (c << kPbBinShift) |
-              (v & ((1u << kPbBinShift) - 1));
+    const uint64_t bin = (uint64_t)bin_of[v];
+    rkey[r] = (bin << (sbits + kPbBinShift)) | (c << kPbBinShift) | (v - (uint64_t)bin_row0[bin]);
     rval[r] = r;
     if (flag[j]) seg_run[sg[j] - 1] = (int32_t)r;
     if (p % kPbWin == 0) {
@@ -290,16 +337,16 @@ hipError_t launch_place_and_pad(int storage, const DevGraph &dg, uint32_t m, uin
 }
 
 // ranges [off[j], off[j + 1]) split into items of at most `most` entries;
-// with splits: a split range (a bin) gets one combine entry per kBlock rows,
-// {bin, first slot, slots, first row}
+// with splits: a split range (bin j, of row0[j + 1] - row0[j] rows) gets one
+// combine entry per kBlock rows, {bin, first slot, slots, first row}
 void split_items(const std::vector<uint32_t> &off, uint32_t most, std::vector<PbItem> &items,
-                 std::vector<PbItem> *splits, int *slots) {
+                 std::vector<PbItem> *splits, int *slots, const std::vector<int32_t> *row0) {
   for (size_t j = 0; j + 1 < off.size(); j++) {
     const uint32_t b = off[j], e = off[j + 1];
     const uint32_t pieces = std::max<uint32_t>(1, (e - b + most - 1) / most);
     const bool split = splits && pieces > 1;
     if (split)
-      for (int r = 0; r < (1 << kPbBinShift); r += kBlock)
+      for (int r = 0; r < (*row0)[j + 1] - (*row0)[j]; r += kBlock)
         splits->push_back({(int32_t)j, *slots, (int32_t)pieces, r});
     for (uint32_t p = 0; p < pieces; p++) {
       const uint32_t lo = b + (uint32_t)(((uint64_t)(e - b) * p) / pieces);
@@ -309,13 +356,34 @@ void split_items(const std::vector<uint32_t> &off, uint32_t most, std::vector<Pb
   }
 }
 
+// First row of every bin, then n: a bin ends at 2^kPbBinShift rows, or (cap >
+// 0) before the row whose run bound would take it past cap runs.  A row whose
+// bound alone passes cap gets a bin of its own, split over several
+// accumulate items.
+void cut_bins(const std::vector<uint32_t> &bound, uint64_t n, int cap, std::vector<int32_t> &row0) {
+  row0.assign(1, 0);
+  uint64_t acc = 0;
+  int rows = 0;
+  for (uint64_t r = 0; r < n; r++) {
+    const uint64_t b = bound.empty() ? 0 : bound[r];
+    if (rows == (1 << kPbBinShift) || (cap > 0 && rows > 0 && acc + b > (uint64_t)cap)) {
+      row0.push_back((int32_t)r);
+      acc = 0;
+      rows = 0;
+    }
+    acc += b;
+    rows++;
+  }
+  row0.push_back((int32_t)n);
+}
+
 }  // namespace
 
 void free_blocked(BlockedLayout &pb) {
   for (void *p : {(void *)pb.src, pb.w, (void *)pb.mask, (void *)pb.win, (void *)pb.seg_off,
                   (void *)pb.rrow, (void *)pb.rsum, (void *)pb.items1, (void *)pb.items2,
                   (void *)pb.splits, (void *)pb.bin_partial, (void *)pb.bin_flags,
-                  (void *)pb.partials})
+                  (void *)pb.partials, (void *)pb.bin_row0})
     (void)hipFree(p);
   pb = BlockedLayout{};
 }
@@ -326,13 +394,14 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
   const uint64_t n = dg.n, m = dg.m;
   const hipStream_t s = ctx->stream;
   const int chunks = (int)std::max<uint64_t>(1, (n + (1u << kPbChunkShift) - 1) >> kPbChunkShift);
-  const int bins = (int)std::max<uint64_t>(1, (n + (1u << kPbBinShift) - 1) >> kPbBinShift);
   const int sbits = bits_of((uint64_t)chunks);
   const size_t wb = dg.w ? weight_bytes(dg.weight_storage) : 0;  // weight bytes per entry
   const int storage = dg.w ? dg.weight_storage : GOPS_WEIGHT_NONE;
   Pool pool;
   BlockedLayout pb;
-  std::vector<uint32_t> chunk_off(chunks + 1, 0), bin_off(bins + 1, 0);
+  std::vector<uint32_t> chunk_off(chunks + 1, 0), bin_off;
+  std::vector<int32_t> row0s;  // first row of every bin, then n
+  int bins = 0;
   std::vector<PbItem> items1, items2, splits;
   int slots = 0;
   uint32_t nr = 0, nseg = 0;
@@ -384,6 +453,33 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       b = tmp_bytes;
       BTRY(hipcub::DeviceScan::InclusiveScan(tmp, b, (int32_t *)vals, row, hipcub::Max(), (int)m,
                                              s));
+      // bins of consecutive rows, cut by the rows' run bounds (marks and their
+      // sum in keys_b, free until the sort)
+      int32_t *bin_of = pool.get<int32_t>(n);
+      {
+        int32_t *mark = reinterpret_cast<int32_t *>(keys_b), *msum = mark + m;
+        uint32_t *bound = pool.get<uint32_t>(n);
+        BTRY(pool.err);
+        pb_group_marks_kernel<<<grid_for(m), kB, 0, s>>>(dg.row_ptr, dg.col, row, (uint32_t)m,
+                                                         mark);
+        BTRY(hipGetLastError());
+        b = tmp_bytes;
+        BTRY(hipcub::DeviceScan::InclusiveSum(tmp, b, mark, msum, (int)m, s));
+        pb_row_bound_kernel<<<grid_for(n), kB, 0, s>>>(dg.row_ptr, msum, (int)n, bound);
+        BTRY(hipGetLastError());
+        std::vector<uint32_t> hb(n);
+        BTRY(hipMemcpyAsync(hb.data(), bound, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        BTRY(hipStreamSynchronize(s));
+        pool.release(bound);
+        cut_bins(hb, n, ctx->pb_bin_runs, row0s);
+      }
+      bins = (int)row0s.size() - 1;
+      pb.bin_row0 = pool.get<int32_t>(row0s.size());
+      BTRY(pool.err);
+      BTRY(hipMemcpyAsync(pb.bin_row0, row0s.data(), row0s.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
+      pb_bin_of_kernel<<<grid_for(n), kB, 0, s>>>(pb.bin_row0, bins, (int)n, bin_of);
+      BTRY(hipGetLastError());
       // chunk-major order of the edges
       pb_keys_kernel<<<grid_for(m), kB, 0, s>>>(dg.col, row, (uint32_t)m, keys, vals);
       BTRY(hipGetLastError());
@@ -402,7 +498,7 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       int32_t *flag = pool.get<int32_t>(m);
       int32_t *sg = reinterpret_cast<int32_t *>(vals);  // vals (unsorted) is free now
       BTRY(pool.err);
-      pb_seg_flag_kernel<<<grid_for(m), kB, 0, s>>>(keys_b, (uint32_t)m, flag);
+      pb_seg_flag_kernel<<<grid_for(m), kB, 0, s>>>(keys_b, bin_of, (uint32_t)m, flag);
       BTRY(hipGetLastError());
       b = tmp_bytes;
       BTRY(hipcub::DeviceScan::InclusiveSum(tmp, b, flag, sg, (int)m, s));
@@ -461,7 +557,7 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
         const uint32_t w0 = (uint32_t)(at / kPbWin), w1 = (uint32_t)((at + padded) / kPbWin);
         std::vector<uint32_t> off{w0, w1};
         const size_t before = items1.size();
-        split_items(off, kPbItem1, items1, nullptr, nullptr);
+        split_items(off, kPbItem1, items1, nullptr, nullptr, nullptr);
         for (size_t t = before; t < items1.size(); t++) items1[t] = {nonempty[i], items1[t].y, items1[t].z, 0};
         at += padded;
         if (at >= (1ull << 31)) {
@@ -508,9 +604,11 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       pb.seg_off = pool.get<int32_t>(nseg);
       BTRY(pool.err);
       pb_run_keys_kernel<<<grid_for(m), kB, 0, s>>>(keys_b, posof, pb.src, runid, flag, sg,
-                                                    (uint32_t)m, sbits, rkey, rval, seg_run,
-                                                    pb.win);
+                                                    bin_of, pb.bin_row0, (uint32_t)m, sbits,
+                                                    rkey, rval, seg_run, pb.win);
       BTRY(hipGetLastError());
+      BTRY(hipStreamSynchronize(s));
+      pool.release(bin_of);
       const int bits_bin = bits_of((uint64_t)bins) + sbits + kPbBinShift;
       b = tmp_bytes;
       BTRY(hipcub::DeviceRadixSort::SortPairs(tmp, b, rkey, rkey_b, rval, rval_b, (int)nr, 0,
@@ -522,15 +620,27 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
       BTRY(hipGetLastError());
       pb_seg_off_kernel<<<grid_for(nseg), kB, 0, s>>>(place_of, seg_run, nseg, pb.seg_off);
       BTRY(hipGetLastError());
+      bin_off.assign(bins + 1, 0);
       BTRY(hipMemcpyAsync(bin_off.data(), d_bin_off, (bins + 1) * sizeof(uint32_t),
                           hipMemcpyDeviceToHost, s));
       BTRY(hipStreamSynchronize(s));
+    } else {  // no edges: bins of 2^kPbBinShift rows, no runs
+      cut_bins({}, n, 0, row0s);
+      bins = (int)row0s.size() - 1;
+      bin_off.assign(bins + 1, 0);
+      pb.bin_row0 = pool.get<int32_t>(row0s.size());
+      BTRY(pool.err);
+      BTRY(hipMemcpyAsync(pb.bin_row0, row0s.data(), row0s.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
     }
+    pb.num_bins = bins;
     pb.num_items1 = (int)items1.size();
     pb.items1 = pool.get<PbItem>(items1.size());
     // accumulate items: every bin gets one, also without runs (its rows still
     // take the epilogue)
-    split_items(bin_off, kPbItem2, items2, &splits, &slots);
+    split_items(bin_off, kPbItem2, items2, &splits, &slots, &row0s);
+    for (size_t j = 0; j < splits.size(); j++)
+      pb.num_split_bins += j == 0 || splits[j].x != splits[j - 1].x;
     pb.num_items2 = (int)items2.size();
     pb.num_splits = (int)splits.size();
     pb.items2 = pool.get<PbItem>(items2.size());
@@ -576,7 +686,7 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
   for (void *p : {(void *)pb.src, pb.w, (void *)pb.mask, (void *)pb.win, (void *)pb.seg_off,
                   (void *)pb.rrow, (void *)pb.rsum, (void *)pb.items1, (void *)pb.items2,
                   (void *)pb.splits, (void *)pb.bin_partial, (void *)pb.bin_flags,
-                  (void *)pb.partials})
+                  (void *)pb.partials, (void *)pb.bin_row0})
     if (p) pool.keep(p);
   g->pb = pb;
   g->has_pb = true;

@@ -180,8 +180,13 @@ struct PowerBuffers {
 //               LDS: the chunk's edges are walked in (row, source) order and
 //               the products of each run (consecutive edges of one row) are
 //               summed; one f32 per run goes to its place in rsum
-//   accumulate  per bin of 2^kPbBinShift rows, whose sums sit in LDS as
-//               fixed-point int64: the bin's runs are read in order and added
+//   accumulate  per bin of at most 2^kPbBinShift consecutive rows, whose sums
+//               sit in LDS as fixed-point int64: the bin's runs are read in
+//               order and added.  A bin ends at 2^kPbBinShift rows or where
+//               the next row's bound on its runs would take the bin past a
+//               cap (gops_context::pb_bin_runs), so rows of many runs get
+//               bins of fewer rows; a bin of more than kPbItem2 runs is split
+//               over several workgroups and combined after
 // Entries are the edges of the CSR of in-edges in chunk-major order (chunk of
 // the source, row, source), padded: every segment (the entries of one chunk
 // into one bin) starts at a multiple of kPbLane entries, and every chunk at a
@@ -226,11 +231,12 @@ struct BlockedLayout {
   PbItem *items1 = nullptr;    // scatter: {chunk, first window, end window, 0}
   PbItem *items2 = nullptr;    // accumulate: {bin, first run, end run, slot or -1}
   PbItem *splits = nullptr;    // bins of several items: {bin, first slot, slots, first row}
+  int32_t *bin_row0 = nullptr; // [bins + 1] first row of every bin, then n
   long long *bin_partial = nullptr;  // [slot][1 << kPbBinShift] fixed-point row sums, or
                                      // [slot][1 << kPbBinShift][2] (128-bit, anchors)
   uint8_t *bin_flags = nullptr;      // [slot][1 << kPbBinShift] non-finite run sums seen
   double *partials = nullptr;  // [kReduce][items2 + splits] sums of the epilogues
-  int num_items1 = 0, num_items2 = 0, num_splits = 0;
+  int num_items1 = 0, num_items2 = 0, num_splits = 0, num_bins = 0, num_split_bins = 0;
   uint64_t n_iso = 0;  // isolated rows (no in-edges, no out-edges): the tail [n - n_iso, n)
 };
 
@@ -301,6 +307,10 @@ struct gops_context {
   int graph_build = GOPS_BUILD_DEVICE;
   int delta_sweep = GOPS_DELTA_SWEEP_AUTO;
   int anchor_sweep = GOPS_ANCHOR_SWEEP_AUTO;
+  // run bound that ends a blocked layout's bin (0: 2^12 rows each); bins of up
+  // to twice one accumulate workgroup's runs measured best at R-MAT scale 24
+  // (f32 sweep 491 us, anchor 730 us; kPbItem2: 498 / 731 us; 0: 491 / 740 us)
+  int pb_bin_runs = 2 * gops::kPbItem2;
   int launch = GOPS_LAUNCH_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use

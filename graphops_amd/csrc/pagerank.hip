@@ -708,10 +708,12 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   __shared__ unsigned long long acc[kRows];
   __shared__ uint32_t nonfin[kRows / 4];  // a byte of pb_nonfinite_bits per row
   const PbItem it = a.pb.items2[blockIdx.x];
-  if ((it.x << kPbBinShift) >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
+  const int row0 = a.pb.bin_row0[it.x];
+  if (row0 >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
     if (threadIdx.x < kReduce) *pb_partial(a, threadIdx.x, blockIdx.x) = 0.0;
     return;
   }
+  const int rows = a.pb.bin_row0[it.x + 1] - row0;  // at most kRows
   for (int i = threadIdx.x; i < kRows; i += kBlock) acc[i] = 0ull;
   for (int i = threadIdx.x; i < kRows / 4; i += kBlock) nonfin[i] = 0u;
   __syncthreads();
@@ -745,8 +747,6 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   }
   __syncthreads();
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const int row0 = it.x << kPbBinShift;
-  const int rows = min(kRows, a.n - row0);
   if (it.w < 0) {
     const DeltaEpilogue epi(a, st, st->mode == kModeDeltaFold);
     const double inv_scale = ldexp(1.0, -k);
@@ -800,10 +800,10 @@ __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *
   }
   const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
   const double inv_scale = ldexp(1.0, -pb_exponent(st));
-  const int row0 = (sp.x << kPbBinShift) + sp.w;
+  const int row0 = a.pb.bin_row0[sp.x] + sp.w;
   const int i = sp.w + (int)threadIdx.x;  // row within the bin
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < (1 << kPbBinShift) && row0 + (int)threadIdx.x < a.n) {
+  if (row0 + (int)threadIdx.x < a.pb.bin_row0[sp.x + 1]) {
     // the hottest bins are split into dozens of parts: eight parts' loads in
     // flight at a time (integer sums: the grouping leaves the bits alone)
     const size_t at = ((size_t)sp.y << kPbBinShift) + i;
@@ -1049,10 +1049,12 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
   __shared__ unsigned long long acc_lo[kRows], acc_hi[kRows];
   __shared__ uint32_t nonfin[kRows / 4];
   const PbItem it = a.pb.items2[blockIdx.x];
-  if ((it.x << kPbBinShift) >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
+  const int row0 = a.pb.bin_row0[it.x];
+  if (row0 >= a.n_live) {  // a bin of the isolated tail: no runs, no rows
     if (threadIdx.x < kReduce) *pb_partial(a, threadIdx.x, blockIdx.x) = 0.0;
     return;
   }
+  const int rows = a.pb.bin_row0[it.x + 1] - row0;  // at most kRows
   for (int i = threadIdx.x; i < kRows; i += kAcc64Block) {
     acc_lo[i] = 0ull;
     acc_hi[i] = 0ull;
@@ -1093,8 +1095,6 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
   }
   __syncthreads();
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const int row0 = it.x << kPbBinShift;
-  const int rows = min(kRows, a.n - row0);
   if (it.w < 0) {
     const AnchorEpilogue epi(a, st);
     const int live = min(rows, a.n_live - row0);  // the isolated tail is the finish kernel's
@@ -1123,10 +1123,10 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
 // anchor epilogues.
 __device__ void pb_combine64(const SweepArgs &a, const RunState *st) {
   const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
-  const int row0 = (sp.x << kPbBinShift) + sp.w;
+  const int row0 = a.pb.bin_row0[sp.x] + sp.w;
   const int i = sp.w + (int)threadIdx.x;
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < (1 << kPbBinShift) && row0 + (int)threadIdx.x < a.n) {
+  if (row0 + (int)threadIdx.x < a.pb.bin_row0[sp.x + 1]) {
     const long long *bp = a.pb.bin_partial + ((size_t)sp.y << (kPbBinShift + 1)) + 2 * (size_t)i;
     const uint8_t *fp = a.pb.bin_flags + ((size_t)sp.y << kPbBinShift) + i;
     unsigned long long lo = 0ull, hi = 0ull;

@@ -122,11 +122,21 @@ enum {
                                    accumulate pass per bin of rows, both streaming,
                                    integer row sums; its layout is built on the graph's
                                    first such run) */,
-    GOPS_OPT_ANCHOR_SWEEP = 14  /* mixed, one vector per run, with blocked delta sweeps:
+    GOPS_OPT_ANCHOR_SWEEP = 14, /* mixed, one vector per run, with blocked delta sweeps:
                                    how the f64 anchor sweeps read the graph:
                                    GOPS_ANCHOR_SWEEP_AUTO (default: blocked as well,
                                    f64 run sums added as 128-bit fixed point) or _PULL
                                    (gathers of the f64 iterate over the CSR) */
+    GOPS_OPT_BLOCKED_BIN_RUNS = 15 /* blocked layout, read when a graph's layout is
+                                   built (its first blocked run): the accumulate pass's
+                                   bins of at most 4096 consecutive rows also end before
+                                   a row whose bound on its runs would take the bin
+                                   past this many (default 131072; 0: bins of 4096 rows
+                                   each).  Bins of more than 65536 runs, one
+                                   workgroup's share, are split over several
+                                   workgroups and combined by a further kernel, so at
+                                   65536 none is split.  The results differ only by
+                                   f32 rounding in the run sums */
 };
 enum { GOPS_DELTA_SWEEP_AUTO = 0, GOPS_DELTA_SWEEP_PULL = 1, GOPS_DELTA_SWEEP_BLOCKED = 2 };
 enum { GOPS_ANCHOR_SWEEP_AUTO = 0, GOPS_ANCHOR_SWEEP_PULL = 1 };
@@ -150,6 +160,8 @@ typedef struct gops_run_stats {
                                  did not fit in free device memory, so the f32 delta
                                  sweeps ran as pulls (retried on later runs) */
     int32_t graph_replay;   /* 1: the sweeps ran as replays of a captured HIP graph */
+    int32_t blocked_bins;   /* blocked runs: the layout's bins of rows (accumulate pass) */
+    int32_t blocked_split_bins; /* of those, bins split over several workgroups */
 } gops_run_stats;
 
 /* ---- context ---------------------------------------------------------- */

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts():
     assert ctypes.sizeof(L.PageRankConfigC) == 24
     assert ctypes.sizeof(L.RunInfoC) == 24
-    assert ctypes.sizeof(L.RunStatsC) == 80
+    assert ctypes.sizeof(L.RunStatsC) == 88
     assert ctypes.sizeof(L.RowPlanC) == 32
 
 

@@ -11,8 +11,9 @@ accumulate pass per bin of rows, fixed-point row sums.
   * the f64 anchor sweeps run blocked too (128-bit fixed-point row sums):
     against pull anchors on the same graph, the same iterations and L1
     within f64 rounding;
-  * layouts that exercise every part: bins split over several workgroups
-    (a row of 140 000 in-edges), several chunks and bins with n not a multiple
+  * layouts that exercise every part: bins cut short by their rows' run
+    bounds, and with bins of 4096 rows, bins split over several workgroups,
+    several chunks and bins with n not a multiple
     of either, empty bins and rows, dangling nodes, every weight storage,
     personalization, and graphs without edges.
 """
@@ -265,14 +266,17 @@ def bin_runs(e, chunk=16384, bin_rows=4096):
     return np.bincount(rows // bin_rows)
 
 
-def test_split_bins(oracle, ctx_b, ctx_p):
-    """Bins whose runs exceed one accumulate item (65 536 runs) are split over
-    several workgroups whose partial sums the combine kernel adds, 64-bit in
-    delta sweeps and 128-bit in anchors.  n = 2^19 + 3 (33 chunks): 4096 rows
-    take an in-edge from 33 sources spread over the chunks, which puts 87 203
-    runs in the first bin (counted here in the library's ids); the tolerance
-    makes the run take several anchors.  Against the oracle, the pull, and
-    pull anchors."""
+@pytest.mark.parametrize("bin_cap", [0, 65536])
+def test_split_bins(oracle, ctx_b, ctx_p, bin_cap):
+    """With bins of 4096 rows each (GOPS_OPT_BLOCKED_BIN_RUNS = 0), bins whose
+    runs exceed one accumulate item (65 536 runs) are split over several
+    workgroups whose partial sums the combine kernel adds, 64-bit in delta
+    sweeps and 128-bit in anchors.  n = 2^19 + 3 (33 chunks): 4096 rows take an
+    in-edge from 33 sources spread over the chunks, which puts 87 203 runs in
+    the first bin (counted here in the library's ids); the tolerance makes the
+    run take several anchors.  With a cap of 65 536 runs the layout cuts that
+    bin short by its rows' run bounds, so no bin is split.  Against the
+    oracle, the pull, and pull anchors."""
     n = (1 << 19) + 3
     rng = np.random.default_rng(19)
     hot = np.arange(4096)
@@ -293,16 +297,27 @@ def test_split_bins(oracle, ctx_b, ctx_p):
     assert runs_per_bin.max() > 65536  # at least two accumulate items in one bin
     tol = 1e-12
     ref = oracle_pull(oracle, e, 1, tol, 300)
-    b = runs(ctx_b, ctx_p, e, 1, "bf16", ga.PageRankConfig(0.85, 300, tol))
-    check(b, ref, tol)
-    assert ctx_b.last_stats()["sweeps_fp64"] >= 3
-    dg = ga.DeviceGraph.from_edges(e, weight_storage="bf16", ctx=ctx_b)
-    ctx_b.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_PULL)
+    ctx_b.set_option(L.OPT_BLOCKED_BIN_RUNS, bin_cap)
     try:
-        pa = ga.pagerank_weighted_run(dg, ga.PageRankConfig(0.85, 300, tol), ctx=ctx_b)
+        b = runs(ctx_b, ctx_p, e, 1, "bf16", ga.PageRankConfig(0.85, 300, tol))
+        st = ctx_b.last_stats()
+        dg = ga.DeviceGraph.from_edges(e, weight_storage="bf16", ctx=ctx_b)
+        ctx_b.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_PULL)
+        try:
+            pa = ga.pagerank_weighted_run(dg, ga.PageRankConfig(0.85, 300, tol), ctx=ctx_b)
+        finally:
+            ctx_b.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_AUTO)
+        dg.close()
     finally:
-        ctx_b.set_option(L.OPT_ANCHOR_SWEEP, L.ANCHOR_SWEEP_AUTO)
-    dg.close()
+        ctx_b.set_option(L.OPT_BLOCKED_BIN_RUNS, 131072)
+    check(b, ref, tol)
+    assert st["sweeps_fp64"] >= 3
+    if bin_cap == 0:
+        assert st["blocked_bins"] == (n + 4095) // 4096
+        assert st["blocked_split_bins"] >= 1
+    else:  # the first bin ends early; the rest are 4096 rows each up to a shorter tail
+        assert st["blocked_bins"] >= (n + 4095) // 4096
+        assert st["blocked_split_bins"] == 0
     assert pa.iterations == b.iterations
     assert np.abs(pa.scores - b.scores).sum() <= 1e-13
 
