@@ -643,6 +643,13 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
         return fail(ctx, GOPS_INVALID_PARAMETER, "blocked bin runs must be 0 or in [4096, 2^24]");
       ctx->pb_bin_runs = (int)value;
       return GOPS_OK;
+    case GOPS_OPT_KATZ_SWEEP:
+      if (value != GOPS_KATZ_SWEEP_AUTO && value != GOPS_KATZ_SWEEP_PULL &&
+          value != GOPS_KATZ_SWEEP_BLOCKED)
+        return fail(ctx, GOPS_INVALID_PARAMETER,
+                    "katz sweep must be GOPS_KATZ_SWEEP_AUTO, _PULL or _BLOCKED");
+      ctx->katz_sweep = (int)value;
+      return GOPS_OK;
     default:
       return fail(ctx, GOPS_INVALID_PARAMETER, "unknown option");
   }

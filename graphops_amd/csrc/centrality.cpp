@@ -82,6 +82,35 @@ gops_status alloc_power(gops_context *ctx, gops_graph *g) {
   return GOPS_OK;
 }
 
+// Rows with in-edges (n_in: rows are sorted by in-degree, descending) and
+// the largest in-degree (row 0's), found once per graph from the row
+// pointers by a binary search of single-word copies.
+gops_status find_degrees(gops_context *ctx, gops_graph *g) {
+  PowerBuffers &pb = g->pow;
+  if (pb.have_degrees) return GOPS_OK;
+  const uint64_t n = g->dev.n;
+  auto rp = [&](uint64_t i, int32_t *v) {
+    return hipMemcpy(v, g->dev.row_ptr + i, sizeof(int32_t), hipMemcpyDeviceToHost);
+  };
+  int32_t r0 = 0, r1 = 0;
+  HIP_TRY(ctx, rp(0, &r0));
+  HIP_TRY(ctx, rp(n > 0 ? 1 : 0, &r1));
+  pb.indeg_max = (uint64_t)(r1 - r0);
+  // the first row r with row_ptr[r + 1] == row_ptr[r] (n if none)
+  uint64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    int32_t a = 0, b = 0;
+    HIP_TRY(ctx, rp(mid, &a));
+    HIP_TRY(ctx, rp(mid + 1, &b));
+    if (b > a) lo = mid + 1;
+    else hi = mid;
+  }
+  pb.n_in = lo;
+  pb.have_degrees = true;
+  return GOPS_OK;
+}
+
 // Runs iterations of algo until the device state says done (at most max_it),
 // keeping a few iterations queued beyond the last one seen to finish (as the
 // PageRank chain does), then returns the final state.
@@ -155,12 +184,33 @@ gops_status katz_run(gops_context *ctx, gops_graph *g, const gops_katz_config *c
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if ((st = alloc_power(ctx, g)) != GOPS_OK) return st;
   PowerBuffers &pb = g->pow;
+  // blocked sweeps: when chosen, or by default from 2^21 nodes (as PageRank's
+  // delta sweeps); by default a layout that does not fit keeps the pull
+  pb.katz_blocked = false;
+  if (ctx->katz_sweep == GOPS_KATZ_SWEEP_BLOCKED ||
+      (ctx->katz_sweep == GOPS_KATZ_SWEEP_AUTO && n >= kPbAutoNodes)) {
+    std::string err;
+    st = build_blocked(ctx, g, err);
+    if (st == GOPS_OUT_OF_MEMORY && ctx->katz_sweep == GOPS_KATZ_SWEEP_AUTO) {
+      ctx->stats.blocked_fallback = 1;
+      (void)hipGetLastError();  // the failed allocation's error is not this run's
+    } else if (st != GOPS_OK) {
+      return fail(ctx, st, err);
+    } else {
+      if ((st = find_degrees(ctx, g)) != GOPS_OK) return st;
+      pb.katz_blocked = true;
+      pb.katz_bound_scale = cfg->alpha * (double)pb.indeg_max;
+      ctx->stats.blocked_bins = g->pb.num_bins;
+      ctx->stats.blocked_split_bins = g->pb.num_split_bins;
+    }
+  }
   HIP_TRY(ctx, launch_power_init(ctx, g, kKatz, cfg->beta, cfg->alpha * cfg->beta));
   PowerState init{};
   init.alpha = cfg->alpha;
   init.beta = cfg->beta;
   init.tolerance = cfg->tolerance;
   init.max_iter = cfg->max_iterations;
+  if (pb.katz_blocked) init.fix = katz_initial_fix(cfg->alpha, pb.indeg_max, n, cfg->beta);
   PowerState last{};
   if (cfg->max_iterations == 0) {  // the loop never runs: x = beta, diff = +inf
     *info = gops_run_info{0, std::numeric_limits<double>::infinity(), 0, 0};
@@ -169,6 +219,7 @@ gops_status katz_run(gops_context *ctx, gops_graph *g, const gops_katz_config *c
     *info = gops_run_info{last.iter, last.diff, last.converged, 0};
   }
   ctx->stats.sweeps_fp64 = info->iterations;
+  if (pb.katz_blocked) ctx->stats.sweeps_blocked = info->iterations;
   return download(ctx, g, pb.v[0], scores);
 }
 

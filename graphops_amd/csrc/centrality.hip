@@ -12,6 +12,11 @@
 //   HITS         a = A^T h (in-edge CSR), a' = a / |a|_2; h = A a' (out-edge
 //                CSR), h' = h / |h|_2; residuals |a - a'|_2 and |h - h'|_2.
 //
+// Katz also runs blocked (from 2^21 nodes by default): PageRank's blocked
+// layout of the in-edges (blocked.hip) with unit weights, a scatter pass of
+// f64 run sums of alpha * x and an accumulate pass of 128-bit fixed-point
+// row sums whose units follow a bound on the row sums (below).
+//
 // Every kernel of an iteration first reads PowerState::done and returns at
 // once when the chain has finished, so the host can queue iterations ahead.
 // Reductions are per workgroup and then summed in a fixed order by one
@@ -22,6 +27,7 @@
 
 #include "internal.h"
 #include "sweep_common.h"
+#include "blocked_common.h"
 
 namespace gops {
 
@@ -225,6 +231,149 @@ __global__ __launch_bounds__(kBlock) void permute_kernel(const double *src, cons
 }
 
 // ---------------------------------------------------------------------------
+// Blocked Katz sweeps.  Rows [0, n_in) have in-edges; a row without in-edges
+// keeps x = beta (its sum is 0 every sweep: katz.rs:166-174 gives
+// beta + nothing), so the sweep leaves it and its prescaled values, set to
+// alpha * beta in both buffers at the start, alone; it adds |beta - beta| = 0
+// to the residual.  Row sums are added as 128-bit fixed point in units of
+// 2^-fix: every run sum and row sum is at most alpha * (max in-degree) *
+// max|x| <= alpha * (max in-degree) * sum|x| < 2^(125 - fix), so none wraps,
+// and one unit is at least 2^-100 of that bound's largest row, far below the
+// f64 rounding of the reference's sequential sum.
+struct KatzPbArgs {
+  BlockedLayout pb;
+  int n;
+  int n_in;
+  double *x;          // the iterate
+  double *y0, *y1;    // alpha * x, ping-pong: the sweep gathers y1 when st->par, else y0
+  double *partials;   // [2][items2 + splits]: sum |x - x'| and sum |x'| per workgroup
+  PowerState *st;
+  double bound_scale; // alpha * the largest in-degree (duplicate entries counted)
+};
+
+template <typename WT>
+__global__ __launch_bounds__(kScatterBlock) __attribute__((amdgpu_waves_per_eu(kScatterBlock / 256, kScatterBlock / 256)))
+void katz_pb_scatter_kernel(KatzPbArgs a) {
+  const PowerState *st = a.st;
+  if (st->done) return;
+  __shared__ double z[1 << kPbChunkShift];
+  const PbItem it = a.pb.items1[blockIdx.x];
+  pb_load_chunk64(st->par ? a.y1 : a.y0, a.n, it, z);
+  pb_scatter64_windows<WT>(a.pb, it, z);
+}
+
+// katz.rs:166-174 and 180-185: x' = beta + sum, |x - x'| and |x'|
+__device__ __forceinline__ void katz_pb_row(const KatzPbArgs &a, const PowerState *st, int r,
+                                            double sum, double (&red)[kPowR]) {
+  const double t = st->beta + sum;
+  red[0] += fabs(a.x[r] - t);
+  red[1] += fabs(t);
+  a.x[r] = t;
+  (st->par ? a.y0 : a.y1)[r] = st->alpha * t;
+}
+
+__device__ __forceinline__ double *katz_pb_partial(const KatzPbArgs &a, int i, int part) {
+  return a.partials + (size_t)i * (size_t)(a.pb.num_items2 + a.pb.num_splits) + part;
+}
+
+constexpr int kKatzAccBlock = 1024;
+__global__ __launch_bounds__(kKatzAccBlock) void katz_pb_accumulate_kernel(KatzPbArgs a) {
+  const PowerState *st = a.st;
+  if (st->done) return;
+  constexpr int kRows = 1 << kPbBinShift;
+  __shared__ unsigned long long acc_lo[kRows], acc_hi[kRows];
+  __shared__ uint32_t nonfin[kRows / 4];
+  const PbItem it = a.pb.items2[blockIdx.x];
+  const int row0 = a.pb.bin_row0[it.x];
+  if (row0 >= a.n_in) {  // rows without in-edges: nothing to add, nothing changes
+    if (threadIdx.x < kPowR) *katz_pb_partial(a, threadIdx.x, blockIdx.x) = 0.0;
+    return;
+  }
+  const int rows = a.pb.bin_row0[it.x + 1] - row0;
+  for (int i = threadIdx.x; i < kRows; i += kKatzAccBlock) {
+    acc_lo[i] = 0ull;
+    acc_hi[i] = 0ull;
+  }
+  for (int i = threadIdx.x; i < kRows / 4; i += kKatzAccBlock) nonfin[i] = 0u;
+  __syncthreads();
+  const int fix = st->fix;
+  pb_accumulate128_runs<kKatzAccBlock>(a.pb, it, fix, acc_lo, acc_hi, nonfin);
+  __syncthreads();
+  double red[kPowR] = {0.0, 0.0};
+  if (it.w < 0) {
+    const int live = min(rows, a.n_in - row0);
+    for (int i = threadIdx.x; i < live; i += kKatzAccBlock) {
+      const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
+      katz_pb_row(a, st, row0 + i, pb_row_value128(acc_lo[i], acc_hi[i], fl, fix), red);
+    }
+  } else {  // a part of a split bin: its sums for the combine kernel
+    long long *out = a.pb.bin_partial + ((size_t)it.w << (kPbBinShift + 1));
+    uint8_t *fout = a.pb.bin_flags + ((size_t)it.w << kPbBinShift);
+    for (int i = threadIdx.x; i < rows; i += kKatzAccBlock) {
+      out[2 * i] = (long long)acc_lo[i];
+      out[2 * i + 1] = (long long)acc_hi[i];
+      fout[i] = (uint8_t)((nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu);
+    }
+  }
+  __syncthreads();
+  block_sum_waves<kPowR, kKatzAccBlock / 64>(red, reinterpret_cast<double *>(acc_lo));
+  if (threadIdx.x == 0) {
+    *katz_pb_partial(a, 0, blockIdx.x) = red[0];
+    *katz_pb_partial(a, 1, blockIdx.x) = red[1];
+  }
+}
+
+// kBlock rows of a bin split over several accumulate workgroups
+__global__ __launch_bounds__(kBlock) void katz_pb_combine_kernel(KatzPbArgs a) {
+  const PowerState *st = a.st;
+  if (st->done) return;
+  const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
+  const int r = a.pb.bin_row0[sp.x] + sp.w + (int)threadIdx.x;
+  double red[kPowR] = {0.0, 0.0};
+  if (r < a.pb.bin_row0[sp.x + 1] && r < a.n_in) {
+    unsigned long long lo, hi;
+    uint32_t fl;
+    pb_combine128_row(a.pb, sp, sp.w + (int)threadIdx.x, lo, hi, fl);
+    katz_pb_row(a, st, r, pb_row_value128(lo, hi, fl, st->fix), red);
+  }
+  block_sum<kPowR>(red);
+  if (threadIdx.x == 0) {
+    *katz_pb_partial(a, 0, a.pb.num_items2 + blockIdx.x) = red[0];
+    *katz_pb_partial(a, 1, a.pb.num_items2 + blockIdx.x) = red[1];
+  }
+}
+
+// Units of the row sums of a sweep whose input vector sums to xsum in
+// absolute value: 2^-fix with alpha * (max in-degree) * xsum < 2^(125 - fix).
+__host__ __device__ inline int katz_fix(double bound_scale, double xsum) {
+  const double b = bound_scale * xsum;
+  if (!(b > 0.0) || !(b <= 1.7e308)) return 100;
+  int e = 0;
+  (void)frexp(b, &e);  // b < 2^e
+  const int fix = 125 - e;
+  return fix < -2000 ? -2000 : (fix > 2000 ? 2000 : fix);
+}
+
+// One workgroup: the sweep's partials in a fixed order, the stop rule of
+// katz.rs:180-192, and the next sweep's units (the rows without in-edges add
+// (n - n_in) |beta| to sum |x|).
+__global__ __launch_bounds__(kBlock) void katz_pb_finish_kernel(KatzPbArgs a) {
+  PowerState *st = a.st;
+  if (st->done) return;
+  const int parts = a.pb.num_items2 + a.pb.num_splits;
+  double red[kPowR] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < parts; b += kBlock) {
+    red[0] += *katz_pb_partial(a, 0, b);
+    red[1] += *katz_pb_partial(a, 1, b);
+  }
+  block_sum<kPowR>(red);
+  if (threadIdx.x == 0) {
+    st->fix = katz_fix(a.bound_scale, red[1] + (double)(a.n - a.n_in) * fabs(st->beta));
+    st->par ^= 1;
+    publish(st, red[0], red[0] < st->tolerance);
+  }
+}
+
 static int elementwise_blocks(uint64_t n) {
   uint64_t blocks = (n + kBlock * 8 - 1) / (kBlock * 8);
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 2048));
@@ -273,14 +422,39 @@ hipError_t launch_power_init(gops_context *ctx, gops_graph *g, PowerAlgo algo, d
   const int grid = elementwise_blocks(n);
   PowerBuffers &pb = g->pow;
   fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[0], n, value);
-  if (algo == kKatz) fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[1], n, prescaled);
+  if (algo == kKatz) {
+    fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[1], n, prescaled);
+    // blocked sweeps leave the rows without in-edges in both buffers
+    if (pb.katz_blocked) fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[2], n, prescaled);
+  }
   if (algo == kHits) fill_kernel<<<grid, kBlock, 0, ctx->stream>>>(pb.v[2], n, value);
   return hipGetLastError();
 }
 
+int katz_initial_fix(double alpha, uint64_t indeg_max, uint64_t n, double beta) {
+  return katz_fix(alpha * (double)indeg_max, (double)n * fabs(beta));
+}
+
 hipError_t launch_power_iteration(gops_context *ctx, gops_graph *g, PowerAlgo algo) {
   PowerBuffers &pb = g->pow;
-  if (algo == kKatz) {
+  if (algo == kKatz && pb.katz_blocked) {
+    KatzPbArgs a{};
+    a.pb = g->pb;
+    a.n = (int)g->dev.n;
+    a.n_in = (int)pb.n_in;
+    a.x = pb.v[0];
+    a.y0 = pb.v[1];
+    a.y1 = pb.v[2];
+    a.partials = g->pb.partials;
+    a.st = pb.state;
+    a.bound_scale = pb.katz_bound_scale;
+    if (a.pb.num_items1)
+      katz_pb_scatter_kernel<NoW><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a);
+    katz_pb_accumulate_kernel<<<a.pb.num_items2, kKatzAccBlock, 0, ctx->stream>>>(a);
+    if (a.pb.num_splits)
+      katz_pb_combine_kernel<<<a.pb.num_splits, kBlock, 0, ctx->stream>>>(a);
+    katz_pb_finish_kernel<<<1, kBlock, 0, ctx->stream>>>(a);
+  } else if (algo == kKatz) {
     PowerArgs a = power_args(g, g->dev, nullptr);
     a.x = pb.v[0];
     sweep<kKatz>(ctx, a, pb.v[1], pb.v[2]);

@@ -158,7 +158,7 @@ struct PowerState {
   uint64_t iter, max_iter;
   int32_t done, converged;
   int32_t par;          // Katz: which prescaled buffer the next sweep gathers
-  int32_t pad;
+  int32_t fix;          // blocked Katz: the next sweep's row sums in units of 2^-fix
   // host-mapped progress word: (iterations << 2) | (converged << 1) | done
   uint64_t *mirror;
 };
@@ -172,6 +172,11 @@ struct PowerBuffers {
   double *hub_partial = nullptr;
   size_t hub_cap = 0;
   PowerState *state = nullptr;
+  bool katz_blocked = false;     // the current Katz run sweeps the blocked layout
+  uint64_t n_in = 0;             // rows with in-edges (rows are sorted by in-degree)
+  uint64_t indeg_max = 0;        // in-degree of row 0, the largest
+  bool have_degrees = false;     // n_in and indeg_max found
+  double katz_bound_scale = 0.0; // alpha * indeg_max of the current Katz run
 };
 
 // Layout of the blocked f32 delta sweep (blocked.hip, pagerank.hip).  The
@@ -311,6 +316,7 @@ struct gops_context {
   // to twice one accumulate workgroup's runs measured best at R-MAT scale 24
   // (f32 sweep 491 us, anchor 730 us; kPbItem2: 498 / 731 us; 0: 491 / 740 us)
   int pb_bin_runs = 2 * gops::kPbItem2;
+  int katz_sweep = GOPS_KATZ_SWEEP_AUTO;
   int launch = GOPS_LAUNCH_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
@@ -363,6 +369,8 @@ enum PowerAlgo { kKatz = 0, kEigen = 1, kHits = 2 };
 hipError_t launch_power_init(gops_context *ctx, gops_graph *g, PowerAlgo algo, double value,
                              double prescaled);
 hipError_t launch_power_iteration(gops_context *ctx, gops_graph *g, PowerAlgo algo);
+// units 2^-fix of a blocked Katz run's first sweep (x = beta everywhere)
+int katz_initial_fix(double alpha, uint64_t indeg_max, uint64_t n, double beta);
 hipError_t launch_permute(gops_context *ctx, gops_graph *g, const double *src, double *out);
 
 // topk.hip: top_k (topk.rs:17-42) of n device scores into host ids / values

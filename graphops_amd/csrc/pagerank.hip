@@ -22,6 +22,7 @@
 
 #include "internal.h"
 #include "sweep_common.h"
+#include "blocked_common.h"
 
 namespace gops {
 
@@ -471,87 +472,6 @@ __device__ __forceinline__ int pb_exponent(const RunState *st) {
   return k < -1000 ? -1000 : (k > 1000 ? 1000 : k);
 }
 
-// An f32 DPP move (row_shr / row_bcast), 0 where the control selects no lane.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float pb_dpp(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, true));
-}
-
-// Segmented inclusive sum over a wave: every lane adds the lanes of its run
-// from the run's head (start: the highest lane at or below it that holds a
-// head).  An inclusive scan within rows of 16 lanes (row_shr 1, 2, 4, 8),
-// then across rows (row_bcast 15: rows 1 and 3 add the last lane of the row
-// before; row_bcast 31: rows 2 and 3 add lane 31); a lane adds only lanes of
-// its run.  The order of the additions depends only on the lanes' positions,
-// so a run's sum is the same bits in every sweep.
-__device__ __forceinline__ float pb_seg_scan(float v, int start) {
-  const int lane = threadIdx.x & 63;
-  const int r = lane & 15;
-  float o;
-  o = pb_dpp<0x111, 0xF>(v); if (r >= 1 && lane - 1 >= start) v += o;
-  o = pb_dpp<0x112, 0xF>(v); if (r >= 2 && lane - 2 >= start) v += o;
-  o = pb_dpp<0x114, 0xF>(v); if (r >= 4 && lane - 4 >= start) v += o;
-  o = pb_dpp<0x118, 0xF>(v); if (r >= 8 && lane - 8 >= start) v += o;
-  o = pb_dpp<0x142, 0xA>(v); if (((lane >> 4) & 1) && (lane | 15) - 16 >= start) v += o;
-  o = pb_dpp<0x143, 0xC>(v); if (lane >= 32 && 31 >= start) v += o;
-  return v;
-}
-
-// Inclusive sum over a wave of a 32-bit integer (the same DPP steps, no
-// segments: lanes outside a row read 0).
-template <int CTRL, int ROWS>
-__device__ __forceinline__ uint32_t pb_dppu(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, true);
-}
-__device__ __forceinline__ uint32_t pb_int_scan(uint32_t v) {
-  v += pb_dppu<0x111, 0xF>(v);
-  v += pb_dppu<0x112, 0xF>(v);
-  v += pb_dppu<0x114, 0xF>(v);
-  v += pb_dppu<0x118, 0xF>(v);
-  v += pb_dppu<0x142, 0xA>(v);
-  v += pb_dppu<0x143, 0xC>(v);
-  return v;
-}
-
-typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t pb_u32x2 __attribute__((ext_vector_type(2)));
-
-// A lane's kPbLane consecutive weights (padding entries carry 0), as f32.
-template <typename WT>
-__device__ __forceinline__ void pb_load_w8(const void *w, size_t e, float (&out)[kPbLane]) {
-  static_assert(kPbLane == 8, "the loads below take eight entries");
-  if constexpr (std::is_same<WT, Bf16W>::value) {
-    const pb_u32x4 q = __builtin_nontemporal_load(
-        reinterpret_cast<const pb_u32x4 *>(static_cast<const uint16_t *>(w) + e));
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      out[2 * i] = __uint_as_float(q[i] << 16);
-      out[2 * i + 1] = __uint_as_float(q[i] & 0xffff0000u);
-    }
-  } else if constexpr (std::is_same<WT, float>::value) {
-    const pb_u32x4 *p = reinterpret_cast<const pb_u32x4 *>(static_cast<const float *>(w) + e);
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const pb_u32x4 q = __builtin_nontemporal_load(p + h);
-#pragma unroll
-      for (int i = 0; i < 4; i++) out[4 * h + i] = __uint_as_float(q[i]);
-    }
-  } else if constexpr (std::is_same<WT, double>::value) {
-    const pb_u32x4 *p = reinterpret_cast<const pb_u32x4 *>(static_cast<const double *>(w) + e);
-#pragma unroll
-    for (int h = 0; h < 4; h++) {
-      const pb_u32x4 q = __builtin_nontemporal_load(p + h);
-      out[2 * h] = (float)__hiloint2double((int)q[1], (int)q[0]);
-      out[2 * h + 1] = (float)__hiloint2double((int)q[3], (int)q[2]);
-    }
-  } else {  // unweighted: a byte 1 per edge, 0 per padding entry
-    const pb_u32x2 q = __builtin_nontemporal_load(
-        reinterpret_cast<const pb_u32x2 *>(static_cast<const uint8_t *>(w) + e));
-#pragma unroll
-    for (int i = 0; i < 8; i++) out[i] = (float)((q[i / 4] >> (8 * (i % 4))) & 0xffu);
-  }
-}
-
 // Scatter: one chunk of sources (their f32 deltas in LDS); the item's windows
 // of kPbWin chunk-major entries, one per wave at a time, kPbLane consecutive
 // entries per lane.  A lane sums its entries' products run by run in order;
@@ -567,7 +487,6 @@ __device__ __forceinline__ void pb_load_w8(const void *w, size_t e, float (&out)
 // (measured at R-MAT scale 24: 256 threads 579 us, 512 553 us, 1024 543 us
 // per f32 sweep); the register budget of that occupancy lets all of a wave's
 // windows' loads be in flight at once.
-constexpr int kScatterBlock = 1024;
 template <typename WT>
 __global__ __launch_bounds__(kScatterBlock) __attribute__((amdgpu_waves_per_eu(2 * kScatterBlock / 256, 2 * kScatterBlock / 256)))
 void pb_scatter_kernel(SweepArgs a, float *yd0, float *yd1) {
@@ -853,43 +772,6 @@ __global__ __launch_bounds__(kBlock) void pb_combine_kernel(SweepArgs a, float *
 // x_u, so every row sum is at most d * sum(x) ~ 1, far below the 2^26 that
 // the format holds; a run sum beyond it (a layout or input fault) makes the
 // row NaN rather than wrap.
-constexpr int kAnchorFix = 100;  // fixed-point units of the anchors' row sums: 2^-100
-
-// An f64 DPP move: both halves with the same control (0 where it selects no lane).
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double pb_dpp64(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xF, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xF, true);
-  return __hiloint2double(hi, lo);
-}
-
-// pb_seg_scan in f64 (the same steps and order).
-__device__ __forceinline__ double pb_seg_scan64(double v, int start) {
-  const int lane = threadIdx.x & 63;
-  const int r = lane & 15;
-  double o;
-  o = pb_dpp64<0x111, 0xF>(v); if (r >= 1 && lane - 1 >= start) v += o;
-  o = pb_dpp64<0x112, 0xF>(v); if (r >= 2 && lane - 2 >= start) v += o;
-  o = pb_dpp64<0x114, 0xF>(v); if (r >= 4 && lane - 4 >= start) v += o;
-  o = pb_dpp64<0x118, 0xF>(v); if (r >= 8 && lane - 8 >= start) v += o;
-  o = pb_dpp64<0x142, 0xA>(v); if (((lane >> 4) & 1) && (lane | 15) - 16 >= start) v += o;
-  o = pb_dpp64<0x143, 0xC>(v); if (lane >= 32 && 31 >= start) v += o;
-  return v;
-}
-
-// A lane's kPbLane consecutive f64 weights (for the anchors' products; the
-// other storage types are exact in f32 and take pb_load_w8).
-__device__ __forceinline__ void pb_load_w8(const void *w, size_t e, double (&out)[kPbLane]) {
-  const pb_u32x4 *p = reinterpret_cast<const pb_u32x4 *>(static_cast<const double *>(w) + e);
-#pragma unroll
-  for (int h = 0; h < 4; h++) {
-    const pb_u32x4 q = __builtin_nontemporal_load(p + h);
-    out[2 * h] = __hiloint2double((int)q[1], (int)q[0]);
-    out[2 * h + 1] = __hiloint2double((int)q[3], (int)q[2]);
-  }
-}
-
 // Scatter of an anchor sweep: pb_scatter_kernel's windows and run logic on
 // f64 products (f64 weights as stored, the others converted exactly) with
 // f64 run sums.
@@ -899,141 +781,10 @@ void pb_scatter64_kernel(SweepArgs a, double *y640, double *y641) {
   const RunState *st = a.st;
   if (st->done || st->mode != kModeAnchor) return;
   const double *yin = st->y64_cur ? y641 : y640;
-  constexpr int kChunk = 1 << kPbChunkShift;
-  __shared__ double z[kChunk];
+  __shared__ double z[1 << kPbChunkShift];
   const PbItem it = a.pb.items1[blockIdx.x];  // {chunk, first window, end window}
-  const int base = it.x << kPbChunkShift;
-  const int len = min(kChunk, a.n - base);
-  if (len == kChunk && (base & 1) == 0) {
-    constexpr int V = kChunk / 2 / kScatterBlock;
-    double2 v[V];
-#pragma unroll
-    for (int j = 0; j < V; j++)
-      v[j] = *reinterpret_cast<const double2 *>(yin + base + 2 * ((int)threadIdx.x + j * kScatterBlock));
-#pragma unroll
-    for (int j = 0; j < V; j++)
-      *reinterpret_cast<double2 *>(z + 2 * ((int)threadIdx.x + j * kScatterBlock)) = v[j];
-  } else {
-    for (int i = threadIdx.x; i < len; i += kScatterBlock) z[i] = yin[base + i];
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const uint32_t nruns = (uint32_t)a.pb.num_runs, nseg = (uint32_t)a.pb.num_segs;
-  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-#ifndef GOPS_TUNE_S64U
-#define GOPS_TUNE_S64U 4
-#endif
-  // windows' loads in flight per wave (f64 weights take twice the registers)
-  constexpr int U = std::is_same<WT, double>::value ? 4 : GOPS_TUNE_S64U;
-  constexpr int kWaves = kScatterBlock / 64;
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  for (int w0 = it.y + wave; w0 < it.z; w0 += U * kWaves) {
-    using WV = typename std::conditional<std::is_same<WT, double>::value, double, float>::type;
-    pb_u32x4 sq[U];
-    WV wv[U][kPbLane];
-    int run0[U], seg0[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int wi = min(w0 + u * kWaves, it.z - 1);  // clamped: every load in bounds
-      const size_t e = (size_t)wi * kPbWin + (size_t)lane * kPbLane;
-      sq[u] = __builtin_nontemporal_load(reinterpret_cast<const pb_u32x4 *>(a.pb.src + e));
-      if constexpr (std::is_same<WT, double>::value)
-        pb_load_w8(a.pb.w, e, wv[u]);
-      else
-        pb_load_w8<WT>(std::is_same<WT, NoW>::value ? (const void *)a.pb.mask : a.pb.w, e, wv[u]);
-      run0[u] = a.pb.win[2 * wi];
-      seg0[u] = a.pb.win[2 * wi + 1];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (w0 + u * kWaves >= it.z) break;  // the same in every lane
-      uint32_t sw[kPbLane];
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        sw[2 * i] = sq[u][i] & 0xffffu;
-        sw[2 * i + 1] = sq[u][i] >> 16;
-      }
-      double q[kPbLane];
-      uint32_t hm = 0;
-#pragma unroll
-      for (int j = 0; j < kPbLane; j++) {
-        const double pj = (double)wv[u][j] * z[sw[j] & kPbSrcMask];
-        const bool h = (sw[j] & kPbRunHead) != 0;
-        hm |= (h ? 1u : 0u) << j;
-        q[j] = (j == 0 || h) ? pj : q[j - 1] + pj;
-      }
-      const unsigned long long lheads = __ballot(hm != 0);
-      const int start = 63 - __clzll(lheads & upto);
-      const double S = pb_seg_scan64(q[kPbLane - 1], start);
-      const double X = __shfl_up(S, 1);
-      const uint32_t sh = (sw[0] & kPbSegHead) ? 1u : 0u;
-      const uint32_t cnt = (uint32_t)__popc(hm) | (sh << 16);
-      const uint32_t before = pb_int_scan(cnt) - cnt;
-      const uint32_t sh0 = __builtin_amdgcn_readfirstlane(sh);
-      const uint32_t run = (uint32_t)run0[u] + (before & 0xffffu) + (hm & 1u) - 1u;
-      const uint32_t seg = (uint32_t)seg0[u] + (before >> 16) + sh - sh0;
-      const int32_t off = a.pb.seg_off[seg < nseg ? seg : 0];
-      const unsigned long long first_heads = __ballot((hm & 1u) != 0);
-      const bool next_head = lane == 63 || ((first_heads >> (lane + 1)) & 1ull);
-      uint32_t place = run + (uint32_t)off;
-#pragma unroll
-      for (int j = 0; j < kPbLane; j++) {
-        const bool tail = j + 1 < kPbLane ? ((hm >> (j + 1)) & 1u) != 0 : next_head;
-        const double t = (hm & ((2u << j) - 1u)) ? q[j] : q[j] + X;
-        if (tail) {
-          if (seg < nseg && place < nruns) a.pb.rsum64[place] = t;
-          place++;
-        }
-      }
-    }
-  }
-}
-
-// c * 2^kAnchorFix as a signed 128-bit integer (lo, hi), truncated toward
-// zero.  Returns false for |c| >= 2^26 (beyond the anchors' bound).  Finite c.
-__device__ __forceinline__ bool pb_fixed128(double c, unsigned long long &lo, unsigned long long &hi) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(c);
-  const int ex = (int)((b >> 52) & 0x7ffu);
-  lo = 0ull;
-  hi = 0ull;
-  if (ex == 0) return true;  // zero or subnormal: below one unit
-  const unsigned long long mant = (b & 0xfffffffffffffull) | (1ull << 52);
-  const int sh = ex - 1075 + kAnchorFix;  // c = mant * 2^(ex - 1075)
-  if (sh > 126 - 53) return false;
-  if (sh >= 64) {
-    hi = mant << (sh - 64);
-  } else if (sh > 0) {
-    lo = mant << sh;
-    hi = mant >> (64 - sh);
-  } else if (sh > -53) {
-    lo = mant >> -sh;
-  }
-  if (b >> 63) {  // two's complement
-    lo = ~lo + 1ull;
-    hi = ~hi + (lo == 0ull ? 1ull : 0ull);
-  }
-  return true;
-}
-
-// The 128-bit sum (lo, hi) in units of 2^-kAnchorFix as a double.
-__device__ __forceinline__ double pb_value128(unsigned long long lo, unsigned long long hi) {
-  const bool neg = (long long)hi < 0;
-  if (neg) {
-    lo = ~lo + 1ull;
-    hi = ~hi + (lo == 0ull ? 1ull : 0ull);
-  }
-  const double v = ldexp((double)hi, 64 - kAnchorFix) + ldexp((double)lo, -kAnchorFix);
-  return neg ? -v : v;
-}
-
-__device__ __forceinline__ double pb_row_value128(unsigned long long lo, unsigned long long hi,
-                                                  uint32_t flags) {
-  if (flags) {
-    if ((flags & 1u) || (flags & 6u) == 6u) return __builtin_nan("");
-    return (flags & 2u) ? __builtin_inf() : -__builtin_inf();
-  }
-  return pb_value128(lo, hi);
+  pb_load_chunk64(yin, a.n, it, z);
+  pb_scatter64_windows<WT>(a.pb, it, z);
 }
 
 // Accumulate of an anchor sweep: one bin of rows (or a part of its runs),
@@ -1061,38 +812,7 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
   }
   for (int i = threadIdx.x; i < kRows / 4; i += kAcc64Block) nonfin[i] = 0u;
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int U = 8;
-  if (it.y < it.z) {
-    for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kAcc64Block) {
-      double cv[U];
-      int key[U];
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int f = f0 + u * kAcc64Block + wave * 64 + lane;
-        const bool ok = f >= it.y && f < it.z;
-        const int ff = ok ? f : it.y;
-        cv[u] = __builtin_nontemporal_load(a.pb.rsum64 + ff);
-        const int d = __builtin_nontemporal_load(a.pb.rrow + ff);
-        key[u] = ok ? d : -1;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        if (key[u] < 0) continue;
-        const double c = cv[u];
-        unsigned long long lo, hi;
-        uint32_t bad = 0u;
-        if (!isfinite(c)) bad = isnan(c) ? 1u : (c > 0.0 ? 2u : 4u);
-        else if (!pb_fixed128(c, lo, hi)) bad = 1u;
-        if (bad) {
-          atomicOr(&nonfin[key[u] >> 2], bad << (8 * (key[u] & 3)));
-        } else {
-          const unsigned long long old = atomicAdd(&acc_lo[key[u]], lo);
-          atomicAdd(&acc_hi[key[u]], hi + (old + lo < old ? 1ull : 0ull));
-        }
-      }
-    }
-  }
+  pb_accumulate128_runs<kAcc64Block>(a.pb, it, kAnchorFix, acc_lo, acc_hi, nonfin);
   __syncthreads();
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (it.w < 0) {
@@ -1100,7 +820,7 @@ __global__ __launch_bounds__(kAcc64Block) void pb_accumulate64_kernel(SweepArgs 
     const int live = min(rows, a.n_live - row0);  // the isolated tail is the finish kernel's
     for (int i = threadIdx.x; i < live; i += kAcc64Block) {
       const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
-      epi(row0 + i, pb_row_value128(acc_lo[i], acc_hi[i], fl), red);
+      epi(row0 + i, pb_row_value128(acc_lo[i], acc_hi[i], fl, kAnchorFix), red);
     }
   } else {
     long long *out = a.pb.bin_partial + ((size_t)it.w << (kPbBinShift + 1));
@@ -1127,39 +847,12 @@ __device__ void pb_combine64(const SweepArgs &a, const RunState *st) {
   const int i = sp.w + (int)threadIdx.x;
   double red[kReduce] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (row0 + (int)threadIdx.x < a.pb.bin_row0[sp.x + 1]) {
-    const long long *bp = a.pb.bin_partial + ((size_t)sp.y << (kPbBinShift + 1)) + 2 * (size_t)i;
-    const uint8_t *fp = a.pb.bin_flags + ((size_t)sp.y << kPbBinShift) + i;
-    unsigned long long lo = 0ull, hi = 0ull;
-    uint32_t fl = 0;
-    int j = 0;
-    for (; j + 4 <= sp.z; j += 4) {
-      unsigned long long vl[4], vh[4];
-      uint32_t f[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const size_t at = (size_t)(j + u) << (kPbBinShift + 1);
-        vl[u] = (unsigned long long)bp[at];
-        vh[u] = (unsigned long long)bp[at + 1];
-        f[u] = fp[(size_t)(j + u) << kPbBinShift];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const unsigned long long old = lo;
-        lo += vl[u];
-        hi += vh[u] + (lo < old ? 1ull : 0ull);
-        fl |= f[u];
-      }
-    }
-    for (; j < sp.z; j++) {
-      const size_t at = (size_t)j << (kPbBinShift + 1);
-      const unsigned long long old = lo;
-      lo += (unsigned long long)bp[at];
-      hi += (unsigned long long)bp[at + 1] + (lo < old ? 1ull : 0ull);
-      fl |= fp[(size_t)j << kPbBinShift];
-    }
+    unsigned long long lo, hi;
+    uint32_t fl;
+    pb_combine128_row(a.pb, sp, i, lo, hi, fl);
     const AnchorEpilogue epi(a, st);
     if (row0 + (int)threadIdx.x < a.n_live)
-      epi(row0 + (int)threadIdx.x, pb_row_value128(lo, hi, fl), red);
+      epi(row0 + (int)threadIdx.x, pb_row_value128(lo, hi, fl, kAnchorFix), red);
   }
   block_sum<kReduce>(red);
   if (threadIdx.x == 0) {

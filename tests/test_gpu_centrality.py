@@ -202,3 +202,81 @@ def test_edge_cases_and_checked(ctx):
     star = ga.DeviceGraph.from_lists(5, *lists_from(SMALL["star_out"]), ctx=ctx)
     hub, auth = ga.hits(star, 100, 1e-10, ctx=ctx)
     assert np.all(hub[0] > hub[1:]) and np.all(auth[1:] > auth[0])
+
+
+@pytest.fixture(scope="module")
+def ctx_kb():
+    from graphops_amd import _lib as L
+
+    c = ga.Context(0)
+    c.set_option(L.OPT_KATZ_SWEEP, L.KATZ_SWEEP_BLOCKED)
+    return c
+
+
+def katz_graphs():
+    rng = np.random.default_rng(9)
+    out = {k: lists_from(v) + (len(v),) for k, v in SMALL.items()}
+    e, off, nb = rmat(14, 3)
+    out["rmat14"] = (off.copy(), nb.copy(), e.n)  # e owns the generated arrays
+    # rows of every kind in the blocked layout: a row with 70 000 in-edges
+    # from every chunk, duplicate entries, rows without in-edges that have
+    # out-edges, and n not a multiple of a chunk (16 384) or a bin
+    n = 70001
+    lists = [rng.integers(0, n, size=int(rng.integers(0, 4))).tolist() for _ in range(n)]
+    for u in range(1, 70001):
+        lists[u].append(5)
+    lists[7].extend([9, 9, 9])
+    off, nb = lists_from(lists)
+    out["hub70k"] = (off, nb, n)
+    return out
+
+
+KATZ_GRAPHS = katz_graphs()
+
+
+@pytest.mark.parametrize("name", sorted(KATZ_GRAPHS))
+def test_katz_blocked_against_oracle(oracle, ctx_kb, ctx, name):
+    """Katz over the blocked layout (GOPS_KATZ_SWEEP_BLOCKED: f64 run sums,
+    128-bit fixed-point row sums in units set by a bound on the row sums)
+    against the oracle, after a fixed number of iterations and to
+    convergence, with betas of either sign and of large magnitude; and
+    against the pull on the same graph."""
+    off, nb, n = KATZ_GRAPHS[name]
+    indeg = np.bincount(nb.astype(np.int64), minlength=n).max(initial=0)
+    dgb = ga.DeviceGraph.from_lists(n, off, nb, ctx=ctx_kb)
+    dgp = ga.DeviceGraph.from_lists(n, off, nb, ctx=ctx)
+    maxdeg = max(int(indeg), int(np.diff(off.astype(np.int64)).max(initial=0)), 1)
+    alpha = 0.5 / max(int(indeg), 1)
+    for beta, k in ((1.0, 1), (1.0, 6), (-0.75, 3), (1e200, 2)):
+        rel = 1e-12 + 2 * k * maxdeg * 1.2e-16
+        ref = oracle.katz_run(off, nb, alpha, beta, k, 0.0)
+        r = ga.katz_centrality_run(dgb, ga.KatzConfig(alpha, beta, k, 0.0), ctx=ctx_kb)
+        assert ctx_kb.last_stats()["sweeps_blocked"] == k
+        assert r.iterations == k and not r.converged
+        assert close(r.scores, ref.scores, rel), (name, beta, k)
+        assert residual_close(r.diff_l1, ref.diff_l1, ref.scores, 1e-9)
+        p = ga.katz_centrality_run(dgp, ga.KatzConfig(alpha, beta, k, 0.0), ctx=ctx)
+        assert close(r.scores, p.scores, 2 * rel)
+    tol = 1e-10
+    ref = oracle.katz_run(off, nb, alpha, 1.0, 500, tol)
+    r = ga.katz_centrality_run(dgb, ga.KatzConfig(alpha, 1.0, 500, tol), ctx=ctx_kb)
+    assert r.converged == ref.converged and same_stop(r.iterations, ref, tol)
+    assert close(r.scores, ref.scores, 1e-12 + 2 * r.iterations * maxdeg * 1.2e-16)
+    # a growing run (alpha above 1 / spectral radius on graphs with cycles):
+    # scores far beyond the first sweep's scale
+    ref = oracle.katz_run(off, nb, 0.9, 1.0, 20, 0.0)
+    r = ga.katz_centrality_run(dgb, ga.KatzConfig(0.9, 1.0, 20, 0.0), ctx=ctx_kb)
+    assert np.all(np.isfinite(ref.scores))
+    assert close(r.scores, ref.scores, 1e-11 + 40 * maxdeg * 1.2e-16), name
+    dgb.close()
+    dgp.close()
+
+
+def test_katz_blocked_bitwise_repeatable(ctx_kb):
+    e, off, nb = rmat(13, 4)
+    dg = ga.DeviceGraph.from_edges(e, ctx=ctx_kb)
+    alpha = 0.5 / np.bincount(nb, minlength=e.n).max()
+    a = ga.katz_centrality_run(dg, ga.KatzConfig(alpha, 1.0, 100, 1e-12), ctx=ctx_kb)
+    b = ga.katz_centrality_run(dg, ga.KatzConfig(alpha, 1.0, 100, 1e-12), ctx=ctx_kb)
+    assert a.iterations == b.iterations and a.diff_l1 == b.diff_l1
+    assert np.array_equal(a.scores, b.scores)
