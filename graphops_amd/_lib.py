@@ -35,8 +35,8 @@ OPT_DELTA_SWEEP = 12
 OPT_LAUNCH = 13
 OPT_ANCHOR_SWEEP = 14
 OPT_BLOCKED_BIN_RUNS = 15
-OPT_KATZ_SWEEP = 16
-KATZ_SWEEP_AUTO, KATZ_SWEEP_PULL, KATZ_SWEEP_BLOCKED = 0, 1, 2
+OPT_CENTRALITY_SWEEP = 16
+CENTRALITY_SWEEP_AUTO, CENTRALITY_SWEEP_PULL, CENTRALITY_SWEEP_BLOCKED = 0, 1, 2
 ANCHOR_SWEEP_AUTO = 0
 ANCHOR_SWEEP_PULL = 1
 LAUNCH_AUTO, LAUNCH_QUEUED, LAUNCH_GRAPH = 0, 1, 2

@@ -220,6 +220,7 @@ void release_graph(gops_graph *g) {
   free_dev(g->dev);
   free_dev(g->fwd);
   free_blocked(g->pb);
+  free_blocked(g->pb_fwd);
   delete g;
   gops_context_destroy(ctx);  // the graph's reference
 }
@@ -643,12 +644,12 @@ gops_status gops_context_set_option(gops_context *ctx, int option, double value)
         return fail(ctx, GOPS_INVALID_PARAMETER, "blocked bin runs must be 0 or in [4096, 2^24]");
       ctx->pb_bin_runs = (int)value;
       return GOPS_OK;
-    case GOPS_OPT_KATZ_SWEEP:
-      if (value != GOPS_KATZ_SWEEP_AUTO && value != GOPS_KATZ_SWEEP_PULL &&
-          value != GOPS_KATZ_SWEEP_BLOCKED)
+    case GOPS_OPT_CENTRALITY_SWEEP:
+      if (value != GOPS_CENTRALITY_SWEEP_AUTO && value != GOPS_CENTRALITY_SWEEP_PULL &&
+          value != GOPS_CENTRALITY_SWEEP_BLOCKED)
         return fail(ctx, GOPS_INVALID_PARAMETER,
-                    "katz sweep must be GOPS_KATZ_SWEEP_AUTO, _PULL or _BLOCKED");
-      ctx->katz_sweep = (int)value;
+                    "centrality sweep must be GOPS_CENTRALITY_SWEEP_AUTO, _PULL or _BLOCKED");
+      ctx->centrality_sweep = (int)value;
       return GOPS_OK;
     default:
       return fail(ctx, GOPS_INVALID_PARAMETER, "unknown option");

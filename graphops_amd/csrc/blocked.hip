@@ -388,9 +388,14 @@ void free_blocked(BlockedLayout &pb) {
   pb = BlockedLayout{};
 }
 
-gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
-  if (g->has_pb) return GOPS_OK;
-  const DevGraph &dg = g->dev;
+namespace {
+
+// The blocked layout of dg's rows (the CSR of in-edges, or of out-edges for
+// the power iterations that sum over a node's own neighbours) into out.
+// with_iso: set the isolated tail apart (rows without in-edges and without
+// out-edges, from the in-edge CSR's inverse degrees).
+gops_status build_layout(gops_context *ctx, const DevGraph &dg, bool with_iso, BlockedLayout &out,
+                         std::string &err) {
   const uint64_t n = dg.n, m = dg.m;
   const hipStream_t s = ctx->stream;
   const int chunks = (int)std::max<uint64_t>(1, (n + (1u << kPbChunkShift) - 1) >> kPbChunkShift);
@@ -659,7 +664,7 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
     if (!pb.seg_off) pb.seg_off = pool.get<int32_t>(1);
     unsigned long long *iso_count = pool.get<unsigned long long>(2);
     BTRY(pool.err);
-    if (n) {
+    if (n && with_iso) {
       // the renumbering puts the isolated rows last; if they were not the
       // tail, none is set apart (n_iso = 0)
       unsigned long long h_count[2] = {0, 0};
@@ -688,13 +693,30 @@ gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
                   (void *)pb.splits, (void *)pb.bin_partial, (void *)pb.bin_flags,
                   (void *)pb.partials, (void *)pb.bin_row0})
     if (p) pool.keep(p);
-  g->pb = pb;
-  g->has_pb = true;
+  out = pb;
   return GOPS_OK;
 fail:
   err = std::string("blocked layout: ") + what + ": " + hipGetErrorString(e);
   return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GOPS_OUT_OF_MEMORY
                                                                      : GOPS_DEVICE_ERROR;
+}
+
+}  // namespace
+
+gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err) {
+  if (g->has_pb) return GOPS_OK;
+  const gops_status st = build_layout(ctx, g->dev, true, g->pb, err);
+  g->has_pb = st == GOPS_OK;
+  return st;
+}
+
+gops_status build_blocked_fwd(gops_context *ctx, gops_graph *g, std::string &err) {
+  if (g->has_pb_fwd) return GOPS_OK;
+  gops_status st = build_forward(ctx, g, err);
+  if (st != GOPS_OK) return st;
+  st = build_layout(ctx, g->fwd, false, g->pb_fwd, err);
+  g->has_pb_fwd = st == GOPS_OK;
+  return st;
 }
 
 }  // namespace gops

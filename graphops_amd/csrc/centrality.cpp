@@ -82,6 +82,31 @@ gops_status alloc_power(gops_context *ctx, gops_graph *g) {
   return GOPS_OK;
 }
 
+gops_status find_degrees(gops_context *ctx, gops_graph *g);
+
+// Blocked eigenvector / HITS passes: when chosen, or by default from 2^21
+// nodes; the out-edge layout (and for HITS the in-edge one) built on first
+// use.  By default, layouts that do not fit keep the pulls.
+gops_status choose_power_blocked(gops_context *ctx, gops_graph *g, bool hits) {
+  PowerBuffers &pb = g->pow;
+  pb.power_blocked = false;
+  if (!(ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_BLOCKED ||
+        (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO && g->dev.n >= kPbAutoNodes)))
+    return GOPS_OK;
+  std::string err;
+  gops_status st = build_blocked_fwd(ctx, g, err);
+  if (st == GOPS_OK && hits) st = build_blocked(ctx, g, err);
+  if (st == GOPS_OUT_OF_MEMORY && ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO) {
+    ctx->stats.blocked_fallback = 1;
+    (void)hipGetLastError();
+    return GOPS_OK;
+  }
+  if (st != GOPS_OK) return fail(ctx, st, err);
+  if ((st = find_degrees(ctx, g)) != GOPS_OK) return st;
+  pb.power_blocked = true;
+  return GOPS_OK;
+}
+
 // Rows with in-edges (n_in: rows are sorted by in-degree, descending) and
 // the largest in-degree (row 0's), found once per graph from the row
 // pointers by a binary search of single-word copies.
@@ -107,7 +132,14 @@ gops_status find_degrees(gops_context *ctx, gops_graph *g) {
     else hi = mid;
   }
   pb.n_in = lo;
-  pb.have_degrees = true;
+  if (g->has_fwd) {  // the out-edge CSR's rows are sorted by out-degree, descending
+    int32_t f0 = 0, f1 = 0;
+    HIP_TRY(ctx, hipMemcpy(&f0, g->fwd.row_ptr, sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(ctx, hipMemcpy(&f1, g->fwd.row_ptr + (n > 0 ? 1 : 0), sizeof(int32_t),
+                           hipMemcpyDeviceToHost));
+    pb.outdeg_max = (uint64_t)(f1 - f0);
+  }
+  pb.have_degrees = g->has_fwd;  // again once the out-edge CSR exists
   return GOPS_OK;
 }
 
@@ -187,11 +219,11 @@ gops_status katz_run(gops_context *ctx, gops_graph *g, const gops_katz_config *c
   // blocked sweeps: when chosen, or by default from 2^21 nodes (as PageRank's
   // delta sweeps); by default a layout that does not fit keeps the pull
   pb.katz_blocked = false;
-  if (ctx->katz_sweep == GOPS_KATZ_SWEEP_BLOCKED ||
-      (ctx->katz_sweep == GOPS_KATZ_SWEEP_AUTO && n >= kPbAutoNodes)) {
+  if (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_BLOCKED ||
+      (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO && n >= kPbAutoNodes)) {
     std::string err;
     st = build_blocked(ctx, g, err);
-    if (st == GOPS_OUT_OF_MEMORY && ctx->katz_sweep == GOPS_KATZ_SWEEP_AUTO) {
+    if (st == GOPS_OUT_OF_MEMORY && ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO) {
       ctx->stats.blocked_fallback = 1;
       (void)hipGetLastError();  // the failed allocation's error is not this run's
     } else if (st != GOPS_OK) {
@@ -258,6 +290,7 @@ gops_status gops_eigenvector_centrality_run(gops_context *ctx, gops_graph *g,
   std::string err;
   if ((st = build_forward(ctx, g, err)) != GOPS_OK) return fail(ctx, st, err);
   if ((st = alloc_power(ctx, g)) != GOPS_OK) return st;
+  if ((st = choose_power_blocked(ctx, g, false)) != GOPS_OK) return st;
   const double init_v = 1.0 / std::sqrt((double)n);  // eigenvector.rs:65
   HIP_TRY(ctx, launch_power_init(ctx, g, kEigen, init_v, 0.0));
   PowerState init{};
@@ -271,6 +304,7 @@ gops_status gops_eigenvector_centrality_run(gops_context *ctx, gops_graph *g,
     *info = gops_run_info{last.iter, last.diff, last.converged, 0};
   }
   ctx->stats.sweeps_fp64 = info->iterations;
+  if (g->pow.power_blocked) ctx->stats.sweeps_blocked = info->iterations;
   return download(ctx, g, g->pow.v[0], scores);
 }
 
@@ -289,6 +323,7 @@ gops_status gops_hits(gops_context *ctx, gops_graph *g, uint64_t max_iter, doubl
   std::string err;
   if ((st = build_forward(ctx, g, err)) != GOPS_OK) return fail(ctx, st, err);
   if ((st = alloc_power(ctx, g)) != GOPS_OK) return st;
+  if ((st = choose_power_blocked(ctx, g, true)) != GOPS_OK) return st;
   const double init_v = 1.0 / std::sqrt((double)n);  // centrality.rs:130-132
   HIP_TRY(ctx, launch_power_init(ctx, g, kHits, init_v, 0.0));
   PowerState init{};
@@ -302,6 +337,7 @@ gops_status gops_hits(gops_context *ctx, gops_graph *g, uint64_t max_iter, doubl
     *info = gops_run_info{last.iter, last.diff, last.converged, 0};
   }
   ctx->stats.sweeps_fp64 = 2 * info->iterations;
+  if (g->pow.power_blocked) ctx->stats.sweeps_blocked = 2 * info->iterations;
   if ((st = download(ctx, g, g->pow.v[0], hub)) != GOPS_OK) return st;
   return download(ctx, g, g->pow.v[2], authority);
 }

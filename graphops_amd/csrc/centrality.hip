@@ -374,6 +374,131 @@ __global__ __launch_bounds__(kBlock) void katz_pb_finish_kernel(KatzPbArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Blocked eigenvector and HITS passes: t = the sums of a unit-weight blocked
+// layout (the in-edge layout: HITS authority; the out-edge CSR's: eigenvector
+// and the HITS hub, rows mapped to nodes), the f64 scatter and the 128-bit
+// accumulate of the anchors and of Katz.  The gathered vector has unit L2
+// norm (or 1/sqrt(n) in every node at the start), so no entry exceeds 1 and
+// no row sum exceeds the layout's largest row length: the units 2^-fix are
+// fixed for the run with that bound below 2^(124 - fix).  Every row is
+// written, rows without entries too (their sums are 0).
+struct PowerPbArgs {
+  BlockedLayout pb;
+  int n;
+  const uint32_t *rows;  // the node of each row (out-edge CSR), or null (row = node)
+  const double *yin;     // the gathered vector
+  const double *x;       // eigenvector: the iterate (its own term)
+  double *t;             // the row sums, by node
+  double *partials;      // [2][items2 + splits]
+  PowerState *st;
+  int fix;
+};
+
+template <typename WT>
+__global__ __launch_bounds__(kScatterBlock) __attribute__((amdgpu_waves_per_eu(kScatterBlock / 256, kScatterBlock / 256)))
+void power_pb_scatter_kernel(PowerPbArgs a) {
+  if (a.st->done) return;
+  __shared__ double z[1 << kPbChunkShift];
+  const PbItem it = a.pb.items1[blockIdx.x];
+  pb_load_chunk64(a.yin, a.n, it, z);
+  pb_scatter64_windows<WT>(a.pb, it, z);
+}
+
+// eigenvector.rs:76-84 (t[v] = x[v] + sum; sum t^2, sum t) and
+// centrality.rs:136-152 (t[v] = sum; sum t^2), as EigenEpi and SumEpi
+template <int ALGO>
+__device__ __forceinline__ void power_pb_row(const PowerPbArgs &a, int r, double sum,
+                                             double (&red)[kPowR]) {
+  const int v = a.rows ? (int)a.rows[r] : r;
+  const double t = ALGO == kEigen ? a.x[v] + sum : sum;
+  a.t[v] = t;
+  red[0] += t * t;
+  red[1] += t;
+}
+
+__device__ __forceinline__ double *power_pb_partial(const PowerPbArgs &a, int i, int part) {
+  return a.partials + (size_t)i * (size_t)(a.pb.num_items2 + a.pb.num_splits) + part;
+}
+
+template <int ALGO>
+__global__ __launch_bounds__(kKatzAccBlock) void power_pb_accumulate_kernel(PowerPbArgs a) {
+  if (a.st->done) return;
+  constexpr int kRows = 1 << kPbBinShift;
+  __shared__ unsigned long long acc_lo[kRows], acc_hi[kRows];
+  __shared__ uint32_t nonfin[kRows / 4];
+  const PbItem it = a.pb.items2[blockIdx.x];
+  const int row0 = a.pb.bin_row0[it.x];
+  const int rows = a.pb.bin_row0[it.x + 1] - row0;
+  for (int i = threadIdx.x; i < kRows; i += kKatzAccBlock) {
+    acc_lo[i] = 0ull;
+    acc_hi[i] = 0ull;
+  }
+  for (int i = threadIdx.x; i < kRows / 4; i += kKatzAccBlock) nonfin[i] = 0u;
+  __syncthreads();
+  pb_accumulate128_runs<kKatzAccBlock>(a.pb, it, a.fix, acc_lo, acc_hi, nonfin);
+  __syncthreads();
+  double red[kPowR] = {0.0, 0.0};
+  if (it.w < 0) {
+    for (int i = threadIdx.x; i < rows; i += kKatzAccBlock) {
+      const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
+      power_pb_row<ALGO>(a, row0 + i, pb_row_value128(acc_lo[i], acc_hi[i], fl, a.fix), red);
+    }
+  } else {
+    long long *out = a.pb.bin_partial + ((size_t)it.w << (kPbBinShift + 1));
+    uint8_t *fout = a.pb.bin_flags + ((size_t)it.w << kPbBinShift);
+    for (int i = threadIdx.x; i < rows; i += kKatzAccBlock) {
+      out[2 * i] = (long long)acc_lo[i];
+      out[2 * i + 1] = (long long)acc_hi[i];
+      fout[i] = (uint8_t)((nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu);
+    }
+  }
+  __syncthreads();
+  block_sum_waves<kPowR, kKatzAccBlock / 64>(red, reinterpret_cast<double *>(acc_lo));
+  if (threadIdx.x == 0) {
+    *power_pb_partial(a, 0, blockIdx.x) = red[0];
+    *power_pb_partial(a, 1, blockIdx.x) = red[1];
+  }
+}
+
+template <int ALGO>
+__global__ __launch_bounds__(kBlock) void power_pb_combine_kernel(PowerPbArgs a) {
+  if (a.st->done) return;
+  const PbItem sp = a.pb.splits[blockIdx.x];  // {bin, first slot, slots, first row of the bin}
+  const int r = a.pb.bin_row0[sp.x] + sp.w + (int)threadIdx.x;
+  double red[kPowR] = {0.0, 0.0};
+  if (r < a.pb.bin_row0[sp.x + 1]) {
+    unsigned long long lo, hi;
+    uint32_t fl;
+    pb_combine128_row(a.pb, sp, sp.w + (int)threadIdx.x, lo, hi, fl);
+    power_pb_row<ALGO>(a, r, pb_row_value128(lo, hi, fl, a.fix), red);
+  }
+  block_sum<kPowR>(red);
+  if (threadIdx.x == 0) {
+    *power_pb_partial(a, 0, a.pb.num_items2 + blockIdx.x) = red[0];
+    *power_pb_partial(a, 1, a.pb.num_items2 + blockIdx.x) = red[1];
+  }
+}
+
+// One workgroup: the pass's partials in a fixed order; the norm and sign of t
+// (eigenvector.rs:87-94, centrality.rs:154-155), as power_finish_kernel.
+template <int ALGO>
+__global__ __launch_bounds__(kBlock) void power_pb_finish_kernel(PowerPbArgs a) {
+  PowerState *st = a.st;
+  if (st->done) return;
+  const int parts = a.pb.num_items2 + a.pb.num_splits;
+  double red[kPowR] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < parts; b += kBlock) {
+    red[0] += *power_pb_partial(a, 0, b);
+    red[1] += *power_pb_partial(a, 1, b);
+  }
+  block_sum<kPowR>(red);
+  if (threadIdx.x == 0) {
+    st->norm = sqrt(red[0]);
+    st->sign = (ALGO == kEigen && !(red[1] >= 0.0)) ? -1.0 : 1.0;
+  }
+}
+
 static int elementwise_blocks(uint64_t n) {
   uint64_t blocks = (n + kBlock * 8 - 1) / (kBlock * 8);
   return (int)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 2048));
@@ -402,6 +527,35 @@ static void sweep(gops_context *ctx, const PowerArgs &a, double *y0, double *y1)
   const int grid = std::min(a.num_desc, kMaxSweepBlocks);
   power_sweep_kernel<ALGO><<<grid, kBlock, 0, ctx->stream>>>(a, y0, y1);
   power_finish_kernel<ALGO><<<1, kBlock, 0, ctx->stream>>>(a, y0, y1, grid);
+}
+
+// units of a blocked pass's row sums: entries of at most 1, rows of at most
+// maxdeg entries
+static int unit_fix(uint64_t maxdeg) {
+  int e = 0;
+  (void)frexp((double)std::max<uint64_t>(maxdeg, 1), &e);  // maxdeg < 2^e
+  return 124 - e;
+}
+
+template <int ALGO>
+static void pb_pass(gops_context *ctx, gops_graph *g, const BlockedLayout &lay, uint64_t maxdeg,
+                    const uint32_t *rows, const double *yin, const double *x, double *t) {
+  PowerPbArgs a{};
+  a.pb = lay;
+  a.n = (int)g->dev.n;
+  a.rows = rows;
+  a.yin = yin;
+  a.x = x;
+  a.t = t;
+  a.partials = lay.partials;
+  a.st = g->pow.state;
+  a.fix = unit_fix(maxdeg);
+  if (lay.num_items1)
+    power_pb_scatter_kernel<NoW><<<lay.num_items1, kScatterBlock, 0, ctx->stream>>>(a);
+  power_pb_accumulate_kernel<ALGO><<<lay.num_items2, kKatzAccBlock, 0, ctx->stream>>>(a);
+  if (lay.num_splits)
+    power_pb_combine_kernel<ALGO><<<lay.num_splits, kBlock, 0, ctx->stream>>>(a);
+  power_pb_finish_kernel<ALGO><<<1, kBlock, 0, ctx->stream>>>(a);
 }
 
 template <int ALGO, bool L2>
@@ -458,6 +612,15 @@ hipError_t launch_power_iteration(gops_context *ctx, gops_graph *g, PowerAlgo al
     PowerArgs a = power_args(g, g->dev, nullptr);
     a.x = pb.v[0];
     sweep<kKatz>(ctx, a, pb.v[1], pb.v[2]);
+  } else if (algo == kEigen && pb.power_blocked) {
+    pb_pass<kEigen>(ctx, g, g->pb_fwd, pb.outdeg_max, g->fwd.order, pb.v[0], pb.v[0], pb.v[1]);
+    scale_step<kEigen, false>(ctx, g, pb.v[1], pb.v[0], 0);
+  } else if (algo == kHits && pb.power_blocked) {
+    // authority: in-edges, gathers hub; hub: out-edges, gathers the new authority
+    pb_pass<kHits>(ctx, g, g->pb, pb.indeg_max, nullptr, pb.v[0], nullptr, pb.v[1]);
+    scale_step<kHits, true>(ctx, g, pb.v[1], pb.v[2], 0);
+    pb_pass<kHits>(ctx, g, g->pb_fwd, pb.outdeg_max, g->fwd.order, pb.v[2], nullptr, pb.v[1]);
+    scale_step<kHits, true>(ctx, g, pb.v[1], pb.v[0], 1);
   } else if (algo == kEigen) {
     PowerArgs a = power_args(g, g->fwd, g->fwd.order);
     a.yin = pb.v[0];

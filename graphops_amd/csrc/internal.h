@@ -173,6 +173,8 @@ struct PowerBuffers {
   size_t hub_cap = 0;
   PowerState *state = nullptr;
   bool katz_blocked = false;     // the current Katz run sweeps the blocked layout
+  bool power_blocked = false;    // the current eigenvector / HITS run sweeps blocked layouts
+  uint64_t outdeg_max = 0;       // out-degree of the out-edge CSR's row 0, the largest
   uint64_t n_in = 0;             // rows with in-edges (rows are sorted by in-degree)
   uint64_t indeg_max = 0;        // in-degree of row 0, the largest
   bool have_degrees = false;     // n_in and indeg_max found
@@ -276,6 +278,8 @@ struct gops_graph {
   bool has_fwd = false;
   gops::BlockedLayout pb;  // blocked delta sweep, built on demand
   bool has_pb = false;
+  gops::BlockedLayout pb_fwd;  // blocked layout of fwd (eigenvector, HITS), built on demand
+  bool has_pb_fwd = false;
   gops::RunBuffers run;
   gops::PowerBuffers pow;
   bool weighted = false;   // created with edge weights
@@ -316,7 +320,7 @@ struct gops_context {
   // to twice one accumulate workgroup's runs measured best at R-MAT scale 24
   // (f32 sweep 491 us, anchor 730 us; kPbItem2: 498 / 731 us; 0: 491 / 740 us)
   int pb_bin_runs = 2 * gops::kPbItem2;
-  int katz_sweep = GOPS_KATZ_SWEEP_AUTO;
+  int centrality_sweep = GOPS_CENTRALITY_SWEEP_AUTO;
   int launch = GOPS_LAUNCH_AUTO;
   gops_run_stats stats{};
   gops::TopkScratch *topk = nullptr;     // device top_k scratch (topk.hip), on first use
@@ -426,6 +430,8 @@ hipError_t launch_gather_probe(gops_context *ctx, gops_graph *g);
 
 // blocked.hip: the layout of the blocked delta sweep, built once per graph
 gops_status build_blocked(gops_context *ctx, gops_graph *g, std::string &err);
+// the out-edge CSR (build_forward) and its blocked layout, once per graph
+gops_status build_blocked_fwd(gops_context *ctx, gops_graph *g, std::string &err);
 void free_blocked(BlockedLayout &pb);
 
 // staging.cpp: copies of pageable host memory through pinned chunks on the

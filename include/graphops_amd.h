@@ -137,14 +137,16 @@ enum {
                                    workgroups and combined by a further kernel, so at
                                    65536 none is split.  The results differ only by
                                    f32 rounding in the run sums */,
-    GOPS_OPT_KATZ_SWEEP = 16    /* how Katz sweeps read the graph: GOPS_KATZ_SWEEP_AUTO
-                                   (default: blocked from 2^21 nodes, pull below), _PULL
-                                   (gathers of alpha * x over the CSR of in-edges) or
-                                   _BLOCKED (the blocked layout's scatter and accumulate
-                                   passes, f64 run sums added as 128-bit fixed point;
-                                   the layout is shared with PageRank's blocked sweeps) */
+    GOPS_OPT_CENTRALITY_SWEEP = 16 /* how Katz, eigenvector and HITS sweeps read the
+                                   graph: GOPS_CENTRALITY_SWEEP_AUTO (default: blocked
+                                   from 2^21 nodes, pull below), _PULL (gathers over the
+                                   CSR of in-edges or of out-edges) or _BLOCKED (blocked
+                                   layouts' scatter and accumulate passes, f64 run sums
+                                   added as 128-bit fixed point; the in-edge layout is
+                                   shared with PageRank's blocked sweeps, the out-edge
+                                   one is built for eigenvector and HITS on first use) */
 };
-enum { GOPS_KATZ_SWEEP_AUTO = 0, GOPS_KATZ_SWEEP_PULL = 1, GOPS_KATZ_SWEEP_BLOCKED = 2 };
+enum { GOPS_CENTRALITY_SWEEP_AUTO = 0, GOPS_CENTRALITY_SWEEP_PULL = 1, GOPS_CENTRALITY_SWEEP_BLOCKED = 2 };
 enum { GOPS_DELTA_SWEEP_AUTO = 0, GOPS_DELTA_SWEEP_PULL = 1, GOPS_DELTA_SWEEP_BLOCKED = 2 };
 enum { GOPS_ANCHOR_SWEEP_AUTO = 0, GOPS_ANCHOR_SWEEP_PULL = 1 };
 enum { GOPS_LAUNCH_AUTO = 0, GOPS_LAUNCH_QUEUED = 1, GOPS_LAUNCH_GRAPH = 2 };

@@ -209,7 +209,7 @@ def ctx_kb():
     from graphops_amd import _lib as L
 
     c = ga.Context(0)
-    c.set_option(L.OPT_KATZ_SWEEP, L.KATZ_SWEEP_BLOCKED)
+    c.set_option(L.OPT_CENTRALITY_SWEEP, L.CENTRALITY_SWEEP_BLOCKED)
     return c
 
 
@@ -236,7 +236,7 @@ KATZ_GRAPHS = katz_graphs()
 
 @pytest.mark.parametrize("name", sorted(KATZ_GRAPHS))
 def test_katz_blocked_against_oracle(oracle, ctx_kb, ctx, name):
-    """Katz over the blocked layout (GOPS_KATZ_SWEEP_BLOCKED: f64 run sums,
+    """Katz over the blocked layout (GOPS_CENTRALITY_SWEEP_BLOCKED: f64 run sums,
     128-bit fixed-point row sums in units set by a bound on the row sums)
     against the oracle, after a fixed number of iterations and to
     convergence, with betas of either sign and of large magnitude; and
@@ -280,3 +280,41 @@ def test_katz_blocked_bitwise_repeatable(ctx_kb):
     b = ga.katz_centrality_run(dg, ga.KatzConfig(alpha, 1.0, 100, 1e-12), ctx=ctx_kb)
     assert a.iterations == b.iterations and a.diff_l1 == b.diff_l1
     assert np.array_equal(a.scores, b.scores)
+
+
+@pytest.mark.parametrize("name", sorted(KATZ_GRAPHS))
+def test_eigen_hits_blocked_against_oracle(oracle, ctx_kb, ctx, name):
+    """Eigenvector and HITS over blocked layouts (the out-edge CSR's, and for
+    the HITS authority the in-edge one; unit-norm gathered vectors, 128-bit
+    row sums in units fixed by the longest row) against the oracle and the
+    pulls, after fixed iteration counts and to convergence."""
+    off, nb, n = KATZ_GRAPHS[name]
+    dgb = ga.DeviceGraph.from_lists(n, off, nb, ctx=ctx_kb)
+    dgp = ga.DeviceGraph.from_lists(n, off, nb, ctx=ctx)
+    maxdeg = max(int(np.bincount(nb.astype(np.int64), minlength=n).max(initial=0)),
+                 int(np.diff(off.astype(np.int64)).max(initial=0)), 1)
+    for k in (1, 5):
+        rel = 1e-12 + 2 * k * maxdeg * 1.2e-16
+        ref = oracle.eigenvector_run(off, nb, k, 0.0)
+        r = ga.eigenvector_centrality_run(dgb, k, 0.0, ctx=ctx_kb)
+        assert ctx_kb.last_stats()["sweeps_blocked"] == k
+        assert r.iterations == k and close(r.scores, ref.scores, rel), (name, k)
+        assert residual_close(r.diff_l1, ref.diff_l1, ref.scores, 1e-9)
+        p = ga.eigenvector_centrality_run(dgp, k, 0.0, ctx=ctx)
+        assert close(r.scores, p.scores, 2 * rel)
+        ref = oracle.hits(off, nb, k, 0.0)
+        hub, auth, info = ga.hits_run(dgb, k, 0.0, ctx=ctx_kb)
+        assert ctx_kb.last_stats()["sweeps_blocked"] == 2 * k
+        assert info["iterations"] == k
+        assert close(hub, ref.hub, 2 * rel) and close(auth, ref.auth, 2 * rel), (name, k)
+    ref = oracle.eigenvector_run(off, nb, 1000, 1e-9)
+    r = ga.eigenvector_centrality_run(dgb, 1000, 1e-9, ctx=ctx_kb)
+    assert r.converged == ref.converged and same_stop(r.iterations, ref, 1e-9)
+    assert close(r.scores, ref.scores, 1e-11 + 2 * r.iterations * maxdeg * 1.2e-16)
+    ref = oracle.hits(off, nb, 100, 1e-10)
+    hub, auth, info = ga.hits_run(dgb, 100, 1e-10, ctx=ctx_kb)
+    assert info["iterations"] == ref.iterations, name
+    tolh = 1e-11 + 4 * info["iterations"] * maxdeg * 1.2e-16
+    assert close(hub, ref.hub, tolh) and close(auth, ref.auth, tolh), name
+    dgb.close()
+    dgp.close()
