@@ -618,7 +618,15 @@ __device__ __forceinline__ double pb_row_value(long long s, uint32_t flags, doub
 // Accumulate: one bin of rows (or a part of its runs), fixed-point sums in
 // LDS, one atomic per run; then the rows' epilogues, or the part's sums for
 // the combine kernel.
-__global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, float *yd0, float *yd1,
+// Threads of an f32 accumulate workgroup: with 36 KB of LDS four workgroups
+// fit on a CU, so 512 threads keep 32 waves per CU against 16 at 256
+// (measured at R-MAT scale 24 on one box: f32 sweep 500.8 us at 256 threads,
+// 491.2 us at 512, 548.4 us at 1024, two workgroups per CU).
+#ifndef GOPS_TUNE_ACC_BLOCK
+#define GOPS_TUNE_ACC_BLOCK 512
+#endif
+constexpr int kAccBlock = GOPS_TUNE_ACC_BLOCK;
+__global__ __launch_bounds__(kAccBlock) void pb_accumulate_kernel(SweepArgs a, float *yd0, float *yd1,
                                                                double *y640, double *y641) {
   const RunState *st = a.st;
   if (st->done || st->mode == kModeAnchor) return;
@@ -633,20 +641,20 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
     return;
   }
   const int rows = a.pb.bin_row0[it.x + 1] - row0;  // at most kRows
-  for (int i = threadIdx.x; i < kRows; i += kBlock) acc[i] = 0ull;
-  for (int i = threadIdx.x; i < kRows / 4; i += kBlock) nonfin[i] = 0u;
+  for (int i = threadIdx.x; i < kRows; i += kAccBlock) acc[i] = 0ull;
+  for (int i = threadIdx.x; i < kRows / 4; i += kAccBlock) nonfin[i] = 0u;
   __syncthreads();
   const int k = pb_exponent(st);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   constexpr int U = 16;
   // waves start on a multiple of 64 runs: whole lines per wave
   if (it.y < it.z) {
-    for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kBlock) {
+    for (int f0 = it.y & ~63; f0 < it.z; f0 += U * kAccBlock) {
       float cv[U];
       int key[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int f = f0 + u * kBlock + wave * 64 + lane;  // 64 consecutive runs per wave
+        const int f = f0 + u * kAccBlock + wave * 64 + lane;  // 64 consecutive runs per wave
         const bool ok = f >= it.y && f < it.z;
         const int ff = ok ? f : it.y;
         cv[u] = __builtin_nontemporal_load(a.pb.rsum + ff);
@@ -671,16 +679,16 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
     const double inv_scale = ldexp(1.0, -k);
     constexpr int R = 4;  // rows per thread with their loads in flight together
     const int live = min(rows, a.n_live - row0);  // the isolated tail is the finish kernel's
-    for (int i0 = threadIdx.x; i0 < live; i0 += R * kBlock) {
+    for (int i0 = threadIdx.x; i0 < live; i0 += R * kAccBlock) {
       DeltaEpilogue::Ops o[R];
 #pragma unroll
       for (int u = 0; u < R; u++) {
-        const int i = i0 + u * kBlock;
+        const int i = i0 + u * kAccBlock;
         o[u] = epi.load(row0 + (i < live ? i : i0));
       }
 #pragma unroll
       for (int u = 0; u < R; u++) {
-        const int i = i0 + u * kBlock;
+        const int i = i0 + u * kAccBlock;
         if (i < live) {
           const uint32_t fl = (nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu;
           epi.apply(row0 + i, pb_row_value((long long)acc[i], fl, inv_scale), o[u], red);
@@ -690,13 +698,16 @@ __global__ __launch_bounds__(kBlock) void pb_accumulate_kernel(SweepArgs a, floa
   } else {
     long long *out = a.pb.bin_partial + ((size_t)it.w << kPbBinShift);
     uint8_t *fout = a.pb.bin_flags + ((size_t)it.w << kPbBinShift);
-    for (int i = threadIdx.x; i < rows; i += kBlock) {
+    for (int i = threadIdx.x; i < rows; i += kAccBlock) {
       out[i] = (long long)acc[i];
       fout[i] = (uint8_t)((nonfin[i >> 2] >> (8 * (i & 3))) & 0xffu);
     }
   }
   __syncthreads();
-  block_sum<kReduce>(red, reinterpret_cast<double *>(acc));
+  if constexpr (kAccBlock == kBlock)
+    block_sum<kReduce>(red, reinterpret_cast<double *>(acc));
+  else
+    block_sum_waves<kReduce, kAccBlock / 64>(red, reinterpret_cast<double *>(acc));
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int i = 0; i < kReduce; i++) *pb_partial(a, i, blockIdx.x) = red[i];
@@ -1006,7 +1017,7 @@ static void launch_wt(gops_context *ctx, gops_graph *g, const SweepArgs &a) {
       if (a.pb.num_items1)
         pb_scatter_kernel<WT><<<a.pb.num_items1, kScatterBlock, 0, ctx->stream>>>(a, rb.yd[0],
                                                                                  rb.yd[1]);
-      pb_accumulate_kernel<<<a.pb.num_items2, kBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
+      pb_accumulate_kernel<<<a.pb.num_items2, kAccBlock, 0, ctx->stream>>>(a, rb.yd[0], rb.yd[1],
                                                                         rb.y64[0], rb.y64[1]);
     }
     if (a.pb_anchor && (a.kinds & 2)) {
