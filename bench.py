@@ -232,19 +232,32 @@ def c2_leg(ga, generators, reps=10, push_sweeps=3, find_edge_frac=0.02):
         n32 += r.stats["sweeps_fp32"]
     wall = time.perf_counter() - t0
     scores = ga.pagerank_weighted_run(dg, cfg, ctx=ctx).scores
-    # the row plan's autotuner (gops_graph_tune_rows), then the same runs
-    tune = ga.graph_tune_rows(dg)
-    t_sweeps = 0
-    t_f32 = 0.0
-    t_n32 = 0
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        rt = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
-        t_sweeps += rt.iterations
-        t_f32 += rt.stats["fp32_sweep_ms"]
-        t_n32 += rt.stats["sweeps_fp32"]
-    t_wall = time.perf_counter() - t0
-    tune.update({"gteps": round(m * t_sweeps / t_wall / 1e9, 2),
+    delta_sweep = "blocked" if r.stats["sweeps_blocked"] else "pull"
+    # the pull's row plan and its autotuner (gops_graph_tune_rows): the same
+    # runs as pulls, with the default plan and the tuned one
+    from graphops_amd import _lib as L
+
+    def pull_runs():
+        n_sw, f32, n32_ = 0, 0.0, 0
+        t = time.perf_counter()
+        for _ in range(reps):
+            rp = ga._run("gops_pagerank_weighted_run", dg, cfg, True, with_scores=False)
+            n_sw += rp.iterations
+            f32 += rp.stats["fp32_sweep_ms"]
+            n32_ += rp.stats["sweeps_fp32"]
+        return n_sw, f32, n32_, time.perf_counter() - t
+
+    ctx.set_option(L.OPT_DELTA_SWEEP, L.DELTA_SWEEP_PULL)
+    try:
+        p_sweeps, p_f32, p_n32, p_wall = pull_runs()
+        tune = ga.graph_tune_rows(dg)
+        t_sweeps, t_f32, t_n32, t_wall = pull_runs()
+    finally:
+        ctx.set_option(L.OPT_DELTA_SWEEP, L.DELTA_SWEEP_AUTO)
+    tune.update({"delta_sweep": "pull",
+                 "default_plan_gteps": round(m * p_sweeps / p_wall / 1e9, 2),
+                 "default_plan_f32_sweep_us_in_runs": round(p_f32 / max(p_n32, 1) * 1e3, 2),
+                 "gteps": round(m * t_sweeps / t_wall / 1e9, 2),
                  "f32_sweep_us_in_runs": round(t_f32 / max(t_n32, 1) * 1e3, 2)})
     log(f"c2 row plan: {tune}")
     dg.close()
@@ -278,7 +291,7 @@ def c2_leg(ga, generators, reps=10, push_sweeps=3, find_edge_frac=0.02):
     info = machine_info()
     return {"workload": c["workload"], "n": n, "m": m, "sweeps_per_run": r.iterations,
             "gteps": round(m * sweeps / wall / 1e9, 2), "ms_per_run": round(wall / reps * 1e3, 3),
-            "delta_sweep": "blocked" if r.stats["sweeps_blocked"] else "pull",
+            "delta_sweep": delta_sweep,
             "f32_sweep_us": round(avg_f32 * 1e6, 2),
             "row_plan_autotune": tune,
             "roofline": {"bound": "hbm", "alg_bytes_per_launch": alg,

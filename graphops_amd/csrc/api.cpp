@@ -260,7 +260,7 @@ gops_status run_single(gops_context *ctx, gops_graph *g, const gops_pagerank_con
   RunBuffers &rb = g->run;
   SweepSetup setup{kind == kWeighted ? 1 : 0, kind == kPersonalized ? 1 : 0, 0, 0};
   // f32 delta sweeps in the blocked layout (built once per graph): when chosen,
-  // or by default from 2^21 nodes, where the pull's source vector (8 MB of
+  // or by default from 2^20 nodes, where the pull's source vector (4 MB of
   // f32 and more) outgrows the L2 and the blocked passes measured faster.  By
   // default, a run whose layout does not fit in free device memory keeps the
   // pull and says so in its stats; a later run tries again.

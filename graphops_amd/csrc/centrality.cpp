@@ -91,7 +91,7 @@ gops_status choose_power_blocked(gops_context *ctx, gops_graph *g, bool hits) {
   PowerBuffers &pb = g->pow;
   pb.power_blocked = false;
   if (!(ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_BLOCKED ||
-        (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO && g->dev.n >= kPbAutoNodes)))
+        (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO && g->dev.n >= kPowerAutoNodes)))
     return GOPS_OK;
   std::string err;
   gops_status st = build_blocked_fwd(ctx, g, err);
@@ -216,11 +216,10 @@ gops_status katz_run(gops_context *ctx, gops_graph *g, const gops_katz_config *c
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   if ((st = alloc_power(ctx, g)) != GOPS_OK) return st;
   PowerBuffers &pb = g->pow;
-  // blocked sweeps: when chosen, or by default from 2^21 nodes (as PageRank's
-  // delta sweeps); by default a layout that does not fit keeps the pull
+  // blocked sweeps: when chosen, or by default from 2^21 nodes; by default a layout that does not fit keeps the pull
   pb.katz_blocked = false;
   if (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_BLOCKED ||
-      (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO && n >= kPbAutoNodes)) {
+      (ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO && n >= kPowerAutoNodes)) {
     std::string err;
     st = build_blocked(ctx, g, err);
     if (st == GOPS_OUT_OF_MEMORY && ctx->centrality_sweep == GOPS_CENTRALITY_SWEEP_AUTO) {

@@ -214,8 +214,12 @@ constexpr uint16_t kPbSrcMask = (1u << kPbChunkShift) - 1;
 constexpr uint16_t kPbRunHead = 1u << 14;   // entry starts a run
 constexpr uint16_t kPbSegHead = 1u << 15;   // entry starts a segment (and a run)
 // GOPS_DELTA_SWEEP_AUTO: blocked from here (f32 sweep, bf16 weights, R-MAT:
-// scale 20 pull 105 us / blocked 101 us, scale 21 194 / 119, 22 357 / 172)
-constexpr uint64_t kPbAutoNodes = 1ull << 21;
+// scale 20 pull 105 us / blocked 101 us, scale 21 194 / 119, 22 357 / 172;
+// C2, scale 20 with f32 weights, whole runs: pull 2.73 ms, blocked 2.21 ms)
+constexpr uint64_t kPbAutoNodes = 1ull << 20;
+// GOPS_CENTRALITY_SWEEP_AUTO: blocked Katz, eigenvector and HITS from here
+// (measured at scale 24 only; not below 2^21)
+constexpr uint64_t kPowerAutoNodes = 1ull << 21;
 
 struct PbItem {
   int32_t x, y, z, w;

@@ -115,7 +115,7 @@ enum {
                                    stop); the results are the same bits */
     GOPS_OPT_DELTA_SWEEP = 12   /* mixed, one vector per run: how f32 delta sweeps read
                                    the graph: GOPS_DELTA_SWEEP_AUTO (default: blocked
-                                   from 2^21 nodes, where the f32 source vector outgrows
+                                   from 2^20 nodes, where the f32 source vector outgrows
                                    the L2; pull below), _PULL (gathers over the CSR of
                                    in-edges) or _BLOCKED (a scatter pass per chunk of
                                    sources that sums each row's run of edges, and an
